@@ -1,0 +1,312 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loads ``oracle/liboracle.so`` (the FP64 C restatement in pysic_oracle.c) and,
+when it has been built, ``oracle/_ref/libpysic_ref.so`` (the reference
+Fortran core compiled from its own sources by build_ref.sh).  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use this module;
+the engine in pysic_amd never imports it.
+
+Descriptor formats shared with the engine's C-ABI:
+  potential rows (10 doubles): type(1 exp, 2 charge_self), n_targets, e1, e2,
+      p1, p2, cutoff, soft_cutoff, group, 0
+  bond rows (14 doubles, 'tersoff'): e1 e2 e3 (center, j, k), beta, eta,
+      a, c, d, h, mu, cutoff, soft_cutoff, group, 0
+  ewald (8 doubles): on, real_cutoff, k_cutoff, K1, K2, K3, sigma, epsilon
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libpysic_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+_lib = None
+_ref = None
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_neighbor_list.restype = ctypes.c_long
+        L.orc_neighbor_list.argtypes = [ctypes.c_int, _dp, _dp, _dp, _ip, _dp, ctypes.c_long,
+                                        _ip, _ip, _ip, _lp]
+        L.orc_evaluate.restype = ctypes.c_int
+        L.orc_evaluate.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip,
+                                   _ip, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                   _dp, _dp]
+        L.orc_relax_dynamic.restype = ctypes.c_int
+        L.orc_relax_dynamic.argtypes = [ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip, _ip,
+                                        ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, _dp, _dp]
+        L.orc_pick.restype = ctypes.c_int
+        L.orc_pick.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+class NeighborLists:
+    """CSR neighbor lists: start[N+1], idx[nnz] (0-based), off[nnz,3]."""
+
+    def __init__(self, counts, idx, off):
+        self.counts = np.ascontiguousarray(counts, dtype=np.int32)
+        self.start = np.zeros(len(counts) + 1, dtype=np.int32)
+        np.cumsum(self.counts, out=self.start[1:])
+        self.idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self.off = np.ascontiguousarray(off, dtype=np.int32).reshape(-1, 3)
+
+    def pair_set(self):
+        """Sorted (i, j, o1, o2, o3) rows: the list as a set."""
+        i = np.repeat(np.arange(len(self.counts), dtype=np.int64), self.counts)
+        rows = np.column_stack([i, self.idx.astype(np.int64), self.off.astype(np.int64)])
+        order = np.lexsort(rows.T[::-1])
+        return rows[order]
+
+
+def inverse_of(cell):
+    """Row-major inverse of the matrix M whose columns are the cell vectors,
+    with the bits the reference's caller produces (calculator.py:1557-1561
+    passes numpy's inv(cell).T, i.e. M^-1, in Fortran order)."""
+    return np.ascontiguousarray(np.linalg.inv(np.asarray(cell, dtype=np.float64)).T)
+
+
+def neighbor_list(pos, cell, pbc, cutoffs):
+    L = lib()
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    inv = inverse_of(cell)
+    pbc = np.ascontiguousarray(pbc, dtype=np.int32)
+    cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+    n = len(pos)
+    counts = np.zeros(max(n, 1), dtype=np.int32)
+    cap = max(64, 16 * n)
+    while True:
+        idx = np.zeros(cap, dtype=np.int32)
+        off = np.zeros(3 * cap, dtype=np.int32)
+        need = ctypes.c_long(0)
+        tot = L.orc_neighbor_list(n, _d(pos), _d(cell), _d(inv), _i(pbc), _d(cut), cap,
+                                  _i(counts), _i(idx), _i(off), ctypes.byref(need))
+        if tot == -2:
+            cap = int(need.value)
+            continue
+        if tot < 0:
+            raise RuntimeError("orc_neighbor_list failed: %d" % tot)
+        return NeighborLists(counts[:n], idx[:tot], off[:3 * tot])
+
+
+def evaluate(calc, species, pos, charges, cell, nl, potdesc, bonddesc=None, ewald=None,
+             scaler=None):
+    """calc: 'energy' -> float; 'forces' -> (F[N,3], stress[6]); 'enegs' -> chi[N]."""
+    L = lib()
+    kind = {"energy": 1, "forces": 2, "enegs": 3}[calc]
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = len(pos)
+    q = np.ascontiguousarray(charges if charges is not None else np.zeros(n), dtype=np.float64)
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    inv = inverse_of(cell)
+    pd = np.ascontiguousarray(potdesc, dtype=np.float64).reshape(-1)
+    npot = len(pd) // 10
+    bd = None if bonddesc is None else np.ascontiguousarray(bonddesc, dtype=np.float64).reshape(-1)
+    nbond = 0 if bd is None else len(bd) // 14
+    ew = None if ewald is None else np.ascontiguousarray(ewald, dtype=np.float64)
+    sc = None if scaler is None else np.ascontiguousarray(scaler, dtype=np.float64)
+    e = np.zeros(1)
+    f = np.zeros((n, 3))
+    st = np.zeros(6)
+    x = np.zeros(n)
+    rc = L.orc_evaluate(kind, n, _i(sp), _d(pos), _d(q), _d(cell), _d(inv), _i(nl.start),
+                        _i(nl.idx), _i(nl.off), npot, _d(pd), nbond, _d(bd), _d(ew), _d(sc),
+                        _d(e), _d(f), _d(st), _d(x))
+    if rc != 0:
+        raise RuntimeError("orc_evaluate failed: %d" % rc)
+    if kind == 1:
+        return float(e[0])
+    if kind == 2:
+        return f, st
+    return x
+
+
+def relax_dynamic(species, pos, charges, cell, nl, potdesc, ewald, n_steps, dt, inertia,
+                  friction, tolerance, rates=None, scaler=None):
+    """Damped charge dynamics; returns (charges, rates, steps, error)."""
+    L = lib()
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = len(pos)
+    q = np.array(charges, dtype=np.float64)
+    r = np.zeros(n) if rates is None else np.array(rates, dtype=np.float64)
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    inv = inverse_of(cell)
+    pd = np.ascontiguousarray(potdesc, dtype=np.float64).reshape(-1)
+    ew = None if ewald is None else np.ascontiguousarray(ewald, dtype=np.float64)
+    sc = None if scaler is None else np.ascontiguousarray(scaler, dtype=np.float64)
+    err = np.zeros(1)
+    steps = L.orc_relax_dynamic(n, _i(sp), _d(pos), _d(q), _d(cell), _d(inv), _i(nl.start),
+                                _i(nl.idx), _i(nl.off), len(pd) // 10, _d(pd), _d(ew), _d(sc),
+                                n_steps, dt, inertia, friction, tolerance, _d(r), _d(err))
+    if steps < 0:
+        raise RuntimeError("orc_relax_dynamic failed: %d" % steps)
+    return q, r, steps, float(err[0])
+
+
+# --------------------------------------------------------------------------
+# the reference Fortran core (checker build)
+# --------------------------------------------------------------------------
+
+def _label_code(sym):
+    """Two-integer element code of pysic/utility/f2py.py (str2ints(s, 2))."""
+    table = {}
+    for k, ch in enumerate("abcdefghijklmnopqrstuvwxyz"):
+        table[ch] = k + 1
+    for k, ch in enumerate("ABCDEFGHIJKLMNOPQRSTUVWXYZ"):
+        table[ch] = 101 + k
+    out = [table.get(c, 0) for c in sym]
+    while len(out) < 2:
+        out.append(0)
+    return out[:2]
+
+
+class RefCore:
+    """The reference core, driven in the order of calculator.py:1595-1637.
+
+    The reference core is a process-wide singleton; one RefCore at a time.
+    """
+
+    def __init__(self):
+        global _ref
+        if _ref is None:
+            if not ref_available():
+                raise RuntimeError("reference checker not built (oracle/build_ref.sh)")
+            R = ctypes.CDLL(REF_PATH)
+            R.ref_add_potential.restype = ctypes.c_int
+            R.ref_add_potential.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                            ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int]
+            R.ref_add_tersoff.restype = ctypes.c_int
+            R.ref_add_tersoff.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _ip,
+                                          ctypes.c_int]
+            R.ref_create_system.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
+                                            _dp, _ip]
+            R.ref_update_coordinates.argtypes = [ctypes.c_int, _dp, _dp]
+            R.ref_update_charges.argtypes = [ctypes.c_int, _dp]
+            for name in ("ref_allocate_potentials", "ref_allocate_bond_order_factors"):
+                getattr(R, name).argtypes = [ctypes.c_int]
+            R.ref_allocate_bond_order_storage.argtypes = [ctypes.c_int] * 3
+            R.ref_generate_neighbor_lists.argtypes = [ctypes.c_int, _dp]
+            R.ref_neighbor_count.argtypes = [ctypes.c_int, _ip]
+            R.ref_neighbor_list.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
+            R.ref_set_ewald.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
+                                        ctypes.c_double, ctypes.c_double, _dp]
+            R.ref_energy.argtypes = [_dp]
+            R.ref_forces.argtypes = [ctypes.c_int, _dp, _dp]
+            R.ref_electronegativities.argtypes = [ctypes.c_int, _dp]
+            R.ref_start()
+            _ref = R
+        self.R = _ref
+
+    def setup(self, symbols, pos, cell, pbc, charges=None, masses=None, potentials=(),
+              tersoff=(), ngroups=0, cutoffs=None, ewald=None, scaler=None):
+        """potentials: list of (kind, params, cutoff, soft, symbols_tuple, group) with
+        symbols in target order (all permutations already expanded);
+        tersoff: list of (params7, cutoff, soft, (si, sj, sk), group)."""
+        R = self.R
+        n = len(pos)
+        self.n = n
+        pos_f = np.asfortranarray(np.asarray(pos, dtype=np.float64).T)
+        mom = np.zeros((3, n), order="F")
+        q = np.zeros(n) if charges is None else np.asarray(charges, dtype=np.float64).copy()
+        m = np.ones(n) if masses is None else np.asarray(masses, dtype=np.float64).copy()
+        tags = np.zeros(n, dtype=np.int32)
+        el = np.asfortranarray(np.array([_label_code(s) for s in symbols], dtype=np.int32).T)
+        cellv = np.ascontiguousarray(cell, dtype=np.float64)  # C rows -> Fortran columns
+        inv_f = np.ascontiguousarray(inverse_of(cellv).T)  # Fortran(r,c) = (M^-1)(r,c)
+        pbc_i = np.asarray([1 if p else 0 for p in pbc], dtype=np.int32)
+        R.ref_create_system(n, _d(m), _d(q), _d(pos_f), _d(mom), _i(tags), _i(el), _d(cellv),
+                            _d(inv_f), _i(pbc_i))
+        R.ref_allocate_potentials(len(potentials))
+        for kind, params, cut, soft, syms, group in potentials:
+            p = np.asarray(params, dtype=np.float64)
+            e = np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
+            ok = R.ref_add_potential(kind, len(syms), len(p), _d(p), cut, soft, _i(e), group)
+            if not ok:
+                raise RuntimeError("reference add_potential failed")
+        R.ref_allocate_bond_order_factors(len(tersoff))
+        for params, cut, soft, syms, group in tersoff:
+            p = np.asarray(params, dtype=np.float64)
+            e = np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
+            if not R.ref_add_tersoff(_d(p), cut, soft, _i(e), group):
+                raise RuntimeError("reference add_bond_order_factor failed")
+        R.ref_allocate_bond_order_storage(n, ngroups, len(tersoff))
+        cut = np.asarray(cutoffs, dtype=np.float64)
+        R.ref_generate_neighbor_lists(n, _d(cut))
+        R.ref_create_lists()
+        if ewald is not None:
+            rc, kc, K, sigma, eps = ewald
+            kk = np.asarray(K, dtype=np.int32)
+            sc = np.ones(n) if scaler is None else np.asarray(scaler, dtype=np.float64)
+            R.ref_set_ewald(n, rc, kc, _i(kk), sigma, eps, _d(sc))
+
+    def rebuild(self, pos, cutoffs):
+        pos_f = np.asfortranarray(np.asarray(pos, dtype=np.float64).T)
+        mom = np.zeros((3, self.n), order="F")
+        self.R.ref_update_coordinates(self.n, _d(pos_f), _d(mom))
+        cut = np.asarray(cutoffs, dtype=np.float64)
+        self.R.ref_generate_neighbor_lists(self.n, _d(cut))
+
+    def set_charges(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.R.ref_update_charges(self.n, _d(q))
+
+    def neighbor_lists(self):
+        counts = np.zeros(self.n, dtype=np.int32)
+        idxs, offs = [], []
+        c = ctypes.c_int(0)
+        for i in range(self.n):
+            self.R.ref_neighbor_count(i, ctypes.byref(c))
+            counts[i] = c.value
+            if c.value > 0:
+                nb = np.zeros(c.value, dtype=np.int32)
+                of = np.zeros((c.value, 3), dtype=np.int32)  # Fortran (3,n) == C (n,3)
+                self.R.ref_neighbor_list(i, c.value, _i(nb), _i(of))
+                idxs.append(nb)
+                offs.append(of)
+        idx = np.concatenate(idxs) if idxs else np.zeros(0, np.int32)
+        off = np.concatenate(offs) if offs else np.zeros((0, 3), np.int32)
+        return NeighborLists(counts, idx, off)
+
+    def energy(self):
+        e = np.zeros(1)
+        self.R.ref_energy(_d(e))
+        return float(e[0])
+
+    def forces(self):
+        f = np.zeros((self.n, 3))  # Fortran (3,n) == C (n,3)
+        st = np.zeros(6)
+        self.R.ref_forces(self.n, _d(f), _d(st))
+        return f, st
+
+    def electronegativities(self):
+        x = np.zeros(self.n)
+        self.R.ref_electronegativities(self.n, _d(x))
+        return x

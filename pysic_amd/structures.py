@@ -1,0 +1,218 @@
+"""Seeded synthetic structures for the Tersoff, Ewald and EEM workloads.
+
+The reference obtains geometries from ASE (absent here), so the engine owns
+its own generators.  All randomness comes from numpy's PCG64 generator with
+an explicit seed, so every workload is reproducible bit for bit.
+"""
+import math
+
+import numpy as np
+
+AMU_SI = 28.0855
+AMU_O = 15.999
+AMU_NA = 22.98976928
+AMU_CL = 35.453
+AVOGADRO = 6.02214076e23
+
+DIAMOND_BASIS = np.array([
+    [0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],
+    [0.25, 0.25, 0.25], [0.25, 0.75, 0.75], [0.75, 0.25, 0.75], [0.75, 0.75, 0.25],
+])
+ROCKSALT_BASIS = np.array([
+    [0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],   # cations
+    [0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.0, 0.0, 0.5], [0.5, 0.5, 0.5],   # anions
+])
+
+
+class Structure:
+    """Minimal stand-in for ASE Atoms: what the pysic calculator reads.
+
+    positions (N,3) in Angstrom, cell rows = cell vectors, pbc (3,), symbols,
+    masses (amu), charges (e), momenta (amu*Angstrom/ASE-time).
+    """
+
+    def __init__(self, symbols, positions, cell, pbc=(True, True, True), masses=None,
+                 charges=None, momenta=None):
+        self.symbols = list(symbols)
+        self.positions = np.array(positions, dtype=np.float64).reshape(-1, 3)
+        self.cell = np.array(cell, dtype=np.float64).reshape(3, 3)
+        self.pbc = np.array([bool(p) for p in pbc])
+        n = len(self.positions)
+        if masses is None:
+            table = {"Si": AMU_SI, "O": AMU_O, "Na": AMU_NA, "Cl": AMU_CL}
+            masses = [table.get(s, 1.0) for s in self.symbols]
+        self.masses = np.array(masses, dtype=np.float64)
+        self.charges = np.zeros(n) if charges is None else np.array(charges, dtype=np.float64)
+        self.momenta = np.zeros((n, 3)) if momenta is None else np.array(momenta, dtype=np.float64)
+
+    # ASE-compatible accessors used by the calculator
+    def __len__(self):
+        return len(self.positions)
+
+    def get_number_of_atoms(self):
+        return len(self.positions)
+
+    def get_positions(self):
+        return self.positions.copy()
+
+    def set_positions(self, p):
+        self.positions = np.array(p, dtype=np.float64).reshape(-1, 3)
+
+    def get_cell(self):
+        return self.cell.copy()
+
+    def get_pbc(self):
+        return self.pbc.copy()
+
+    def get_chemical_symbols(self):
+        return list(self.symbols)
+
+    def get_masses(self):
+        return self.masses.copy()
+
+    def get_initial_charges(self):
+        return self.charges.copy()
+
+    def set_initial_charges(self, q):
+        self.charges = np.array(q, dtype=np.float64)
+
+    def get_charges(self):
+        return self.charges.copy()
+
+    def set_charges(self, q):
+        self.set_initial_charges(q)
+
+    def get_momenta(self):
+        return self.momenta.copy()
+
+    def set_momenta(self, p):
+        self.momenta = np.array(p, dtype=np.float64).reshape(-1, 3)
+
+    def get_tags(self):
+        return np.zeros(len(self.positions), dtype=np.int64)
+
+    def get_volume(self):
+        return abs(float(np.linalg.det(self.cell)))
+
+    def copy(self):
+        return Structure(self.symbols, self.positions, self.cell, self.pbc, self.masses,
+                         self.charges, self.momenta)
+
+
+def _lattice(basis, a, reps):
+    nx, ny, nz = reps
+    cells = np.array([[i, j, k] for k in range(nz) for j in range(ny) for i in range(nx)],
+                     dtype=np.float64)
+    frac = (cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    return frac * a
+
+
+def diamond_si(reps=(8, 8, 8), a=5.431, sigma=0.0, seed=0):
+    """Diamond Si, a x reps conventional cells, Gaussian displacements."""
+    pos = _lattice(DIAMOND_BASIS, a, reps)
+    if sigma > 0:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.normal(0.0, sigma, size=pos.shape)
+    cell = np.diag([a * reps[0], a * reps[1], a * reps[2]])
+    return Structure(["Si"] * len(pos), pos, cell)
+
+
+def rocksalt(reps=(4, 4, 4), a=5.64, sigma=0.0, seed=0, cation="Na", anion="Cl"):
+    """Rocksalt with charges +1 / -1, Gaussian displacements."""
+    nx, ny, nz = reps
+    cells = np.array([[i, j, k] for k in range(nz) for j in range(ny) for i in range(nx)],
+                     dtype=np.float64)
+    frac = (cells[:, None, :] + ROCKSALT_BASIS[None, :, :]).reshape(-1, 3)
+    pos = frac * a
+    q = np.tile(np.array([1.0] * 4 + [-1.0] * 4), len(cells))
+    sym = [cation if x > 0 else anion for x in q]
+    if sigma > 0:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.normal(0.0, sigma, size=pos.shape)
+    cell = np.diag([a * nx, a * ny, a * nz])
+    return Structure(sym, pos, cell, charges=q)
+
+
+def random_sequential_addition(n, box, dmin, rng, max_tries=2000000):
+    """Uniform random positions in a cubic periodic box with a hard minimum
+    separation (cell-list accelerated rejection sampling)."""
+    ncell = max(1, int(box // dmin))
+    w = box / ncell
+    grid = {}
+    pos = np.zeros((n, 3))
+    placed = 0
+    tries = 0
+    d2 = dmin * dmin
+    while placed < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("random sequential addition jammed at %d of %d" % (placed, n))
+        p = rng.random(3) * box
+        c = tuple((p // w).astype(int) % ncell)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    key = ((c[0] + dx) % ncell, (c[1] + dy) % ncell, (c[2] + dz) % ncell)
+                    for j in grid.get(key, ()):
+                        d = pos[j] - p
+                        d -= box * np.round(d / box)
+                        if d @ d < d2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+        if ok:
+            pos[placed] = p
+            grid.setdefault(c, []).append(placed)
+            placed += 1
+    return pos
+
+
+def amorphous_si(n, density_gcc, seed, dmin=2.1):
+    """Random-addition a-Si cell: n atoms at the given density (g/cm^3)."""
+    rng = np.random.default_rng(seed)
+    vol = n * AMU_SI / AVOGADRO / density_gcc * 1e24  # Angstrom^3
+    box = vol ** (1.0 / 3.0)
+    pos = random_sequential_addition(n, box, dmin, rng)
+    return Structure(["Si"] * n, pos, np.diag([box] * 3))
+
+
+def batched_amorphous_si(ncells=512, nmin=64, nmax=2048, dmin=2.1, dens=(2.25, 2.35), seed=0):
+    """The batched workload: independent a-Si cells of uneven size."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=ncells)
+    densities = rng.uniform(dens[0], dens[1], size=ncells)
+    seeds = rng.integers(0, 2 ** 31 - 1, size=ncells)
+    return [amorphous_si(int(s), float(d), int(sd), dmin) for s, d, sd in
+            zip(sizes, densities, seeds)]
+
+
+def amorphous_sio2(n_si, density_gcc, seed, dmin=1.5):
+    """Random-addition SiO2 (1:2) cell at the given density."""
+    rng = np.random.default_rng(seed)
+    n = 3 * n_si
+    mass = n_si * (AMU_SI + 2 * AMU_O)
+    vol = mass / AVOGADRO / density_gcc * 1e24
+    box = vol ** (1.0 / 3.0)
+    pos = random_sequential_addition(n, box, dmin, rng)
+    species = np.array(["Si"] * n_si + ["O"] * (2 * n_si))
+    perm = rng.permutation(n)
+    return Structure(list(species[perm]), pos, np.diag([box] * 3))
+
+
+# ASE unit conventions (eV, Angstrom, amu): 1 fs in ASE time units and k_B
+ASE_FS = 1e-15 * math.sqrt(1.602176634e-19 / 1.66053906660e-27) * 1e10
+KB_EV = 8.617333262e-5
+
+
+def maxwell_boltzmann(struct, temperature_k, seed):
+    """Momenta drawn from N(0, sqrt(m kT)) with zero net momentum."""
+    rng = np.random.default_rng(seed)
+    m = struct.masses[:, None]
+    p = rng.normal(size=(len(struct), 3)) * np.sqrt(m * KB_EV * temperature_k)
+    p -= p.sum(axis=0) / len(struct) * (m / m.mean())
+    struct.set_momenta(p)
+    return struct
