@@ -1,0 +1,167 @@
+"""Benchmark: Tersoff Si molecular dynamics on one MI355X.
+
+Workload (the headline configuration): diamond Si, 64x64x64 conventional
+cells = 1,048,576 atoms, a = 5.431 A, Gaussian displacements sigma = 0.1 A
+(seeded), Maxwell-Boltzmann momenta at 600 K (seeded), velocity Verlet with
+dt = 1 fs, neighbor list (cutoff 3.0 + skin 0.5 A) rebuilt every step.
+A "step" is one MD step: drift, cell-list rebuild, Tersoff energy + forces
+(+ virial), second half-kick.  Metric: atom-steps/s = N * steps / wall time
+of the timed steps, with all inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+
+Prints one JSON line.  The cpu_baseline leg times the reference Fortran core
+(oracle/_ref, built from the reference sources) on a bounded sample on the
+host, single core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(reps=64, seed=0):
+    from pysic_amd import structures as st
+    s = st.diamond_si((reps, reps, reps), a=5.431, sigma=0.1, seed=seed)
+    st.maxwell_boltzmann(s, 600.0, seed=seed + 1)
+    return s
+
+
+def tersoff_bytes_per_atom(n_nb):
+    # algorithmic bytes of one Tersoff E+F evaluation per atom: position in
+    # (3 x f64), neighbor list in (4 B per entry), force out (3 x f64),
+    # site energy out (f64)
+    return 24.0 + 4.0 * n_nb + 24.0 + 8.0
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """Reference Fortran core on the host: VV steps (rebuild + forces) on a
+    16x16x16-cell sample (32,768 atoms) of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if not O.ref_available():
+        return None
+    from pysic_amd import structures as st
+    reps = 16
+    s = st.diamond_si((reps, reps, reps), a=5.431, sigma=0.1, seed=0)
+    st.maxwell_boltzmann(s, 600.0, seed=1)
+    n = len(s)
+    T = dict(A=1830.8, l1=2.4799, B=471.18, l2=1.7322)
+    ters = [1.1e-6, 0.78734, 0.0, 100390.0, 16.217, -0.59825, 3.0]
+    pots = [(1, [T["A"], T["l1"]], 3.0, 2.7, ("Si", "Si"), -1),
+            (1, [-T["B"], T["l2"]], 3.0, 2.7, ("Si", "Si"), 0)]
+    cut = np.full(n, 3.5)
+    R = O.RefCore()
+    R.setup(s.symbols, s.positions, s.cell, s.pbc, masses=s.masses, potentials=pots,
+            tersoff=[(ters, 3.0, 2.7, ("Si", "Si", "Si"), 0)], ngroups=1, cutoffs=cut)
+    dt = st.ASE_FS
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    f, _ = R.forces()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        R.rebuild(r, cut)
+        f, _ = R.forces()
+        p = p + 0.5 * dt * f
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_budget or steps >= 50:
+            break
+    return {"value": n * steps / el, "unit": "atom-steps/s", "cores": 1, "kind": "reference",
+            "sample": "reference Fortran core (serial, amdflang -O2), %d atoms (16^3 cells of the "
+                      "same Si workload), %d velocity-Verlet steps with neighbor rebuild + "
+                      "forces each step, %.1f s; its force loop is O(N^2) in N, so this "
+                      "sample overstates its 1M-atom rate" % (n, steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--reps", type=int, default=64)
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
+              "described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+
+    from pysic_amd.engine import Engine
+    from pysic_amd.interactions import build_descriptors, tersoff_si
+    from pysic_amd import autotune, structures as st
+
+    s = build_workload(args.reps)
+    n = len(s)
+    pd, bd, _ = build_descriptors(tersoff_si(), {"Si": 0})
+    eng = Engine(0)
+    eng.create_atoms(np.zeros(n, np.int32), s.positions, masses=s.masses, momenta=s.momenta)
+    eng.create_cell(s.cell, s.pbc)
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(bd)
+    eng.ewald_off()
+    eng.set_neighbor_capacity(16)
+    eng.generate_neighbor_lists(np.full(n, 3.5))
+    counts, _, _ = eng.neighbor_lists()
+    n_nb = float(counts.mean())
+    variant = args.variant if args.variant >= 0 else autotune.best_variant(eng, "tersoff_si")
+    eng.set_variant(variant)
+    dt = st.ASE_FS
+    ew = eng.md_run(args.warmup, dt, rebuild=1, energies=True)
+    _, mom0 = eng.state()
+    ekin0 = float((mom0 ** 2 / (2 * s.masses[:, None])).sum())
+    epot0 = float(ew[-1]) if len(ew) else eng.energy()
+    eng.synchronize()
+    eng.timing(True)
+    t0 = time.perf_counter()
+    energies = eng.md_run(args.steps, dt, rebuild=1, energies=True)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    timers = eng.timing_read()
+    eng.timing(False)
+    _, mom1 = eng.state()
+    ekin1 = float((mom1 ** 2 / (2 * s.masses[:, None])).sum())
+    drift = ((float(energies[-1]) + ekin1) - (epot0 + ekin0)) / n
+    wall = t1 - t0
+    value = n * args.steps / wall
+
+    site_ms, site_calls = timers.get("tersoff_site", (0.0, 1))
+    site_avg_s = site_ms / max(site_calls, 1) / 1e3
+    bytes_site = tersoff_bytes_per_atom(n_nb) * n
+    achieved = bytes_site / site_avg_s / 1e9 if site_avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "tersoff_site", "avg_launch_us": round(site_avg_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": int(bytes_site)}
+    kernels = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "calls": v[1]}
+               for k, v in timers.items()}
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    out = {"metric": "atom-steps/s Tersoff 1M-atom Si", "value": round(value, 1),
+           "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 three-body / f64 accumulate",
+           "data": "synthetic (seeded diamond Si + Gaussian displacements, MB 600 K)",
+           "config": {"workload": "Tersoff Si %dx%dx%d diamond cells, %d atoms, VV dt=1 fs, "
+                                  "rebuild every step" % (args.reps, args.reps, args.reps, n),
+                      "atoms": n, "neighbors_per_atom": round(n_nb, 3),
+                      "kernel_variant": eng.variants()[variant],
+                      "total_energy_drift_eV_per_atom": drift},
+           "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels}
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+/*
+ * pysic_hip.h -- C-ABI of the MI355X pysic force engine (libpysic_hip.so).
+ *
+ * Replaces the reference's f2py module `pysic_interface`
+ * (fortran/PyInterface.f90) for the Tersoff / Ewald / charge-relaxation
+ * path.  Plain pointers and sizes only; all host buffers are copied, none is
+ * retained.  Every call returns 0 on success or a negative PYSIC_E* code,
+ * with a message available from pysic_last_error().  Indices are 0-based,
+ * arrays of 3-vectors are row-major (N,3), cells are given as three vectors
+ * in rows, and species are small integers 0..3 chosen by the caller.
+ *
+ * Potential descriptors (10 doubles per entry, one entry per target
+ * permutation as calculator.py:1166-1192 adds them):
+ *   type (1 'exponential', 2 'charge_self'), n_targets, e1, e2, p1, p2,
+ *   cutoff, soft_cutoff (= cutoff - margin), group (-1 none), 0
+ * Bond-order descriptors ('tersoff', 14 doubles per entry, no permutation as
+ * calculator.py:1372):
+ *   e1 e2 e3 (center, j, k), beta, eta, a, c, d, h, mu, cutoff, soft_cutoff,
+ *   group, 0
+ */
+#ifndef PYSIC_HIP_H
+#define PYSIC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYSIC_OK 0
+#define PYSIC_EINVAL -1
+#define PYSIC_EHIP -2
+#define PYSIC_ESTATE -3
+#define PYSIC_ECAP -4
+#define PYSIC_ENOMEM -5
+#define PYSIC_ERANGE -6
+
+typedef struct pysic_ctx pysic_ctx;
+
+/* start_potentials / start_bond_order_factors / start_mpi / start_rng
+ * (PyInterface.f90:51-195, called at import by pysic/core.py:20-28): one
+ * handle owns one device stream and all device buffers of one system. */
+int pysic_create(pysic_ctx **out, int device);
+/* release (PyInterface.f90:1023) */
+void pysic_destroy(pysic_ctx *h);
+const char *pysic_last_error(pysic_ctx *h);
+
+/* create_atoms (PyInterface.f90:250); species replaces the int-coded labels */
+int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *charges,
+                       const double *positions, const double *momenta, const int *species);
+/* create_cell (PyInterface.f90:208): vectors in rows; inverse = M^-1
+ * row-major where M has the vectors as columns */
+int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse,
+                      const int *pbc);
+/* update_atom_coordinates (PyInterface.f90:277) */
+int pysic_update_atom_coordinates(pysic_ctx *h, int n, const double *positions,
+                                  const double *momenta);
+/* update_atom_charges (PyInterface.f90:295) */
+int pysic_update_atom_charges(pysic_ctx *h, int n, const double *charges);
+/* get_number_of_atoms (PyInterface.f90:311) */
+int pysic_get_number_of_atoms(pysic_ctx *h, int *n);
+
+/* allocate_potentials + add_potential (PyInterface.f90:370, 433) */
+int pysic_set_potentials(pysic_ctx *h, int npot, const double *desc);
+/* allocate_bond_order_factors + add_bond_order_factor (PyInterface.f90:384, 482)
+ * + allocate_bond_order_storage (402) */
+int pysic_set_bond_order_factors(pysic_ctx *h, int nbond, const double *desc);
+/* set_ewald_parameters (PyInterface.f90:1039); scaler may be NULL; on = 0
+ * switches the Coulomb summation off */
+int pysic_set_ewald(pysic_ctx *h, int on, double real_cutoff, double k_cutoff,
+                    const int *kcells, double sigma, double epsilon, const double *scaler);
+
+/* generate_neighbor_lists (PyInterface.f90:1062): per-atom search radii
+ * (cutoff + skin) */
+int pysic_generate_neighbor_lists(pysic_ctx *h, int n, const double *cutoffs);
+/* get_number_of_neighbors_of_atom (PyInterface.f90:1075), for all atoms */
+int pysic_get_neighbor_counts(pysic_ctx *h, int n, int *counts);
+/* get_neighbor_list_of_atom (PyInterface.f90:1085), for all atoms at once:
+ * CSR in atom order, idx[nnz], offsets[3*nnz] */
+int pysic_get_neighbor_lists(pysic_ctx *h, long nnz, int *idx, int *offsets);
+
+/* calculate_energy (PyInterface.f90:652) */
+int pysic_calculate_energy(pysic_ctx *h, double *energy);
+/* calculate_forces (PyInterface.f90:668): forces (N,3), stress(6) =
+ * sum r (x) f in the order xx yy zz yz xz xy */
+int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress);
+/* calculate_electronegativities (PyInterface.f90:684) */
+int pysic_calculate_electronegativities(pysic_ctx *h, int n, double *enegs);
+
+/* ChargeRelaxation.charge_relaxation 'dynamic' (charges/relaxation.py:417-450)
+ * run on the device; charges/rates in and out (original order).  Returns the
+ * step count in *steps and the final max(dchi) in *error. */
+int pysic_relax_charges(pysic_ctx *h, int n, double *charges, double *rates, int n_steps,
+                        double timestep, double inertia, double friction, double tolerance,
+                        int *steps, double *error);
+
+/* ASE VelocityVerlet (usage at doc/run examples.rst:177-268) run on the
+ * device for nsteps with the neighbor list rebuilt every `rebuild` steps;
+ * energies (nsteps, may be NULL) receives the potential energy after each
+ * step. */
+int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energies);
+/* positions and momenta back in original order (after pysic_md_run) */
+int pysic_get_state(pysic_ctx *h, int n, double *positions, double *momenta);
+
+/* tuning and measurement */
+int pysic_set_kernel_variant(pysic_ctx *h, int variant);
+int pysic_kernel_variant_count(void);
+const char *pysic_kernel_variant_name(int variant);
+int pysic_set_neighbor_capacity(pysic_ctx *h, int cap);
+int pysic_timing(pysic_ctx *h, int on);
+/* per kernel family: name, accumulated ms, launch count; returns the count */
+int pysic_timing_read(pysic_ctx *h, int max, char *names, int name_len, double *ms, long *calls);
+int pysic_synchronize(pysic_ctx *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,733 @@
+// capi.hip -- the C-ABI of libpysic_hip.so (declared in include/pysic_hip.h).
+//
+// One pysic_ctx owns one System (device buffers in cell-sorted order) and the
+// host-side copy of the interaction model.  Calls mirror the f2py entry
+// points of fortran/PyInterface.f90; errors are returned as codes with a
+// message, never by exiting the process (the reference stops or exits,
+// Core.f90:4909-4911, MPI.f90:58-70).
+#include "system.h"
+#include "ewald.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../include/pysic_hip.h"
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+void KernelTimer::begin(const char *name, hipStream_t s, hipEvent_t *a)
+{
+    if (!on) return;
+    HIP_TRY(hipEventCreate(a));
+    HIP_TRY(hipEventRecord(*a, s));
+    (void)name;
+}
+void KernelTimer::end(const char *name, hipStream_t s, hipEvent_t a)
+{
+    if (!on) return;
+    hipEvent_t b;
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(b, s));
+    pending.push_back({name, a, b});
+    if (pending.size() > 4096) collect();
+}
+void KernelTimer::collect()
+{
+    for (auto &r : pending) {
+        HIP_TRY(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        bool found = false;
+        for (size_t k = 0; k < total.size(); ++k)
+            if (total[k].first == r.name) {
+                total[k].second += ms;
+                calls[k].second += 1;
+                found = true;
+            }
+        if (!found) {
+            total.push_back({r.name, ms});
+            calls.push_back({r.name, 1});
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    pending.clear();
+}
+void KernelTimer::reset()
+{
+    collect();
+    total.clear();
+    calls.clear();
+}
+
+void System::release_all()
+{
+    DBuf<double4> *b4[] = {&pos, &pos2, &mom, &mom2, &frc, &frc2, &out4};
+    for (auto *b : b4) b->release();
+    DBuf<double> *bd[] = {&q, &q2, &cut, &cut2, &partial, &totals, &host_stage};
+    for (auto *b : bd) b->release();
+    DBuf<int> *bi[] = {&id, &id2, &key, &wrap, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
+                       &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt};
+    for (auto *b : bi) b->release();
+    dU.release();
+    dmodel.release();
+}
+
+// ---------------------------------------------------------------------------
+// order-restoring helpers
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_gather4(int n, const int *__restrict__ id, const double4 *__restrict__ src,
+                          double4 *__restrict__ dst)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) dst[s] = src[id[s]];
+}
+__global__ void k_gather1(int n, const int *__restrict__ id, const double *__restrict__ src,
+                          double *__restrict__ dst)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) dst[s] = src[id[s]];
+}
+__global__ void k_scatter1(int n, const int *__restrict__ id, const double *__restrict__ src,
+                           double *__restrict__ dst)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) dst[id[s]] = src[s];
+}
+__global__ void k_iota(int n, int *__restrict__ id)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) id[s] = s;
+}
+__global__ void k_zero4(int n, double4 *__restrict__ a)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) a[s] = make_double4(0, 0, 0, 0);
+}
+__global__ void k_copy_total(const double *__restrict__ totals, double *__restrict__ dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = totals[0];
+}
+}  // namespace
+
+struct pysic_ctx {
+    int device = 0;
+    System S;
+    std::string err;
+    std::vector<int> species;     // original order
+    std::vector<double> masses;   // original order
+    std::vector<double> potdesc, bonddesc;
+    int variant = 0;
+    bool atoms_ready = false, cell_ready = false;
+    bool lists_valid = false;
+    bool forces_valid = false;
+    EwaldParams ewald;
+    EwaldState ew;
+};
+
+namespace {
+
+int fail(pysic_ctx *h, const PysicError &e)
+{
+    if (h) h->err = e.msg;
+    return e.code;
+}
+
+#define API_BEGIN(h)                                                                         \
+    if (!(h)) return PYSIC_EINVAL;                                                           \
+    try {                                                                                    \
+        HIP_TRY(hipSetDevice((h)->device));
+#define API_END(h)                                                                           \
+    }                                                                                        \
+    catch (const PysicError &e) { return fail(h, e); }                                       \
+    catch (const std::exception &e) { return fail(h, PysicError{PYSIC_ENOMEM, e.what()}); }  \
+    return PYSIC_OK;
+
+// Compile the descriptor rows into species tables (Core.f90:1034-1125 builds
+// the per-atom lists the same way: position-1 element match).
+void compile_model(pysic_ctx *h)
+{
+    Model M;
+    std::memset(&M, 0, sizeof(M));
+    int ns = 1;
+    for (int s : h->species) ns = std::max(ns, s + 1);
+    PYSIC_REQUIRE(ns <= PYSIC_MAXS, PYSIC_EINVAL, "at most 4 species per system");
+    M.nspecies = ns;
+    M.group = -1;
+    auto use_group = [&](int g) {
+        if (g < 0) return;
+        PYSIC_REQUIRE(M.group < 0 || M.group == g, PYSIC_EINVAL,
+                      "one bond-order group (coordinator) per system is supported");
+        M.group = g;
+    };
+    size_t np = h->potdesc.size() / 10;
+    for (size_t k = 0; k < np; ++k) {
+        const double *d = h->potdesc.data() + 10 * k;
+        int type = (int)d[0], nt = (int)d[1], e1 = (int)d[2], e2 = (int)d[3];
+        PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && (nt < 2 || (e2 >= 0 && e2 < PYSIC_MAXS)),
+                      PYSIC_EINVAL, "potential species out of range");
+        if (e1 < ns) M.maxpot[e1] = std::max(M.maxpot[e1], d[6]);
+        if (type == 1) {
+            PYSIC_REQUIRE(nt == 2, PYSIC_EINVAL, "'exponential' takes 2 targets");
+            int &c = M.npair[e1][e2];
+            PYSIC_REQUIRE(c < PYSIC_MAXP, PYSIC_EINVAL, "too many exponential terms for one pair");
+            PairTerm &t = M.pair[e1][e2][c++];
+            t.eps = d[4];
+            t.zeta = d[5];
+            t.cut = d[6];
+            // soft-cutoff rule of create_potential (Potentials.f90:2382-2388)
+            if (d[7] > 0.0 && d[7] < d[6]) { t.soft = d[7]; t.smooth = 1; }
+            else { t.soft = d[6]; t.smooth = 0; }
+            t.group = (int)d[8];
+            use_group(t.group);
+            M.has_pairs = 1;
+        } else if (type == 2) {
+            PYSIC_REQUIRE(nt == 1, PYSIC_EINVAL, "'charge_self' takes 1 target");
+            int nexp = (int)d[5];
+            PYSIC_REQUIRE(nexp >= 0 && nexp <= 4, PYSIC_EINVAL, "charge_self exponent must be 0..4");
+            M.self_eps[e1][nexp] += d[4];
+            M.has_self = 1;
+        } else {
+            throw PysicError{PYSIC_EINVAL, "unsupported potential type"};
+        }
+    }
+    size_t nb = h->bonddesc.size() / 14;
+    for (size_t k = 0; k < nb; ++k) {
+        const double *d = h->bonddesc.data() + 14 * k;
+        int e1 = (int)d[0], e2 = (int)d[1], e3 = (int)d[2];
+        PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && e2 >= 0 && e2 < PYSIC_MAXS && e3 >= 0 &&
+                          e3 < PYSIC_MAXS,
+                      PYSIC_EINVAL, "bond-order species out of range");
+        BondTerm &b = M.bond[e1][e2][e3];
+        PYSIC_REQUIRE(!b.valid, PYSIC_EINVAL, "duplicate tersoff entry for one species triplet");
+        b.beta = d[3]; b.eta = d[4]; b.a = d[5]; b.c = d[6]; b.d = d[7]; b.h = d[8];
+        b.mu = d[9]; b.cut = d[10]; b.soft = d[11];
+        b.valid = 1;
+        b.a_zero = (d[5] == 0.0 && d[9] > 0.0) ? 1 : 0;
+        use_group((int)d[12]);
+        M.maxbond[e1] = std::max(M.maxbond[e1], d[10]);
+        if (!M.post[e1].valid) M.post[e1] = b;  // first factor of the center species
+    }
+    double rs = 0.0;
+    for (int s = 0; s < PYSIC_MAXS; ++s) {
+        double mpair = 0.0;
+        for (int t = 0; t < PYSIC_MAXS; ++t)
+            for (int c = 0; c < M.npair[s][t]; ++c) mpair = std::max(mpair, M.pair[s][t][c].cut);
+        rs = std::max(rs, std::min(mpair, M.maxpot[s]));
+        rs = std::max(rs, M.maxbond[s]);
+    }
+    M.rshort = rs;
+    h->S.model = M;
+    h->S.dmodel.ensure(1);
+    HIP_TRY(hipMemcpyAsync(h->S.dmodel.p, &h->S.model, sizeof(Model), hipMemcpyHostToDevice,
+                           h->S.stream));
+    HIP_TRY(hipStreamSynchronize(h->S.stream));
+    h->S.model_dirty = false;
+}
+
+void upload_state(pysic_ctx *h, const double *positions, const double *momenta, bool sorted_by_id)
+{
+    System &S = h->S;
+    int n = S.n;
+    std::vector<double4> hp(n), hm(n);
+    for (int i = 0; i < n; ++i) {
+        hp[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2],
+                             (double)h->species[i]);
+        double px = momenta ? momenta[3 * i] : 0.0, py = momenta ? momenta[3 * i + 1] : 0.0,
+               pz = momenta ? momenta[3 * i + 2] : 0.0;
+        hm[i] = make_double4(px, py, pz, h->masses[i]);
+    }
+    if (!sorted_by_id) {
+        HIP_TRY(hipMemcpyAsync(S.pos.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
+        HIP_TRY(hipMemcpyAsync(S.mom.p, hm.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
+    } else {
+        S.out4.ensure(n);
+        HIP_TRY(hipMemcpyAsync(S.out4.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
+        k_gather4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.out4.p, S.pos.p);
+        HIP_TRY(hipMemcpyAsync(S.out4.p, hm.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
+        k_gather4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.out4.p, S.mom.p);
+    }
+    HIP_TRY(hipStreamSynchronize(S.stream));
+}
+
+void upload_scalar(System &S, const double *host, DBuf<double> &dst)
+{
+    int n = S.n;
+    S.host_stage.ensure(n);
+    HIP_TRY(hipMemcpyAsync(S.host_stage.p, host, sizeof(double) * n, hipMemcpyHostToDevice, S.stream));
+    k_gather1<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.host_stage.p, dst.p);
+    HIP_TRY(hipStreamSynchronize(S.stream));
+}
+
+void download_scalar(System &S, const double *src, double *host)
+{
+    int n = S.n;
+    S.host_stage.ensure(n);
+    k_scatter1<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, src, S.host_stage.p);
+    HIP_TRY(hipMemcpyAsync(host, S.host_stage.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+}
+
+void download_vec3(System &S, const double4 *src, double *host)
+{
+    int n = S.n;
+    S.out4.ensure(n);
+    sys_scatter_original(S, src, S.out4.p);
+    std::vector<double4> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), S.out4.p, sizeof(double4) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    for (int i = 0; i < n; ++i) {
+        host[3 * i] = tmp[i].x;
+        host[3 * i + 1] = tmp[i].y;
+        host[3 * i + 2] = tmp[i].z;
+    }
+}
+
+void require_ready(pysic_ctx *h)
+{
+    PYSIC_REQUIRE(h->atoms_ready, PYSIC_ESTATE, "atoms not created");
+    PYSIC_REQUIRE(h->cell_ready, PYSIC_ESTATE, "cell not created");
+    PYSIC_REQUIRE(h->lists_valid, PYSIC_ESTATE, "neighbor lists not generated");
+    if (h->S.model_dirty) compile_model(h);
+}
+
+void check_flags(pysic_ctx *h, bool lists_only)
+{
+    int f[4];
+    nb_read_flags(h->S, f);
+    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+    if (!lists_only) {
+        PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
+                      "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
+        PYSIC_REQUIRE(f[3] == 0, PYSIC_EINVAL, "neighbor list is not symmetric");
+    }
+}
+
+// Neighbor lists for the current positions; grows the capacity on overflow.
+void build_lists(pysic_ctx *h)
+{
+    System &S = h->S;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
+        nb_sort_atoms(S);
+        nb_build_lists(S);
+        int f[4];
+        nb_read_flags(S, f);
+        PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+        if (f[0] <= S.cap) {
+            h->lists_valid = true;
+            return;
+        }
+        S.cap = f[0] + 8;
+    }
+    throw PysicError{PYSIC_ECAP, "neighbor list capacity could not be established"};
+}
+
+void tersoff_forces(pysic_ctx *h, bool kick, double half_dt)
+{
+    System &S = h->S;
+    ts_site(S, h->variant);
+    ts_gather(S, half_dt, kick ? 1 : 0);
+    ts_reduce(S);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int pysic_create(pysic_ctx **out, int device)
+{
+    if (!out) return PYSIC_EINVAL;
+    pysic_ctx *h = new pysic_ctx();
+    h->device = device;
+    try {
+        int count = 0;
+        HIP_TRY(hipGetDeviceCount(&count));
+        PYSIC_REQUIRE(device >= 0 && device < count, PYSIC_EINVAL, "no such HIP device");
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreate(&h->S.stream));
+        h->S.flags.ensure(4);
+        HIP_TRY(hipMemset(h->S.flags.p, 0, sizeof(int) * 4));
+        h->S.cap = 16;
+    } catch (const PysicError &e) {
+        delete h;
+        *out = nullptr;
+        return e.code;
+    }
+    *out = h;
+    return PYSIC_OK;
+}
+
+void pysic_destroy(pysic_ctx *h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->S.stream) (void)hipStreamSynchronize(h->S.stream);
+    h->S.timer.reset();
+    h->S.release_all();
+    ewald_release(h->ew);
+    if (h->S.stream) (void)hipStreamDestroy(h->S.stream);
+    delete h;
+}
+
+const char *pysic_last_error(pysic_ctx *h) { return h ? h->err.c_str() : "null handle"; }
+
+int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *charges,
+                       const double *positions, const double *momenta, const int *species)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(n > 0 && positions && species, PYSIC_EINVAL, "create_atoms: bad arguments");
+    System &S = h->S;
+    S.n = n;
+    h->species.assign(species, species + n);
+    for (int s : h->species)
+        PYSIC_REQUIRE(s >= 0 && s < PYSIC_MAXS, PYSIC_EINVAL, "species must be 0..3");
+    h->masses.assign(n, 1.0);
+    if (masses) h->masses.assign(masses, masses + n);
+    S.pos.ensure(n); S.mom.ensure(n); S.frc.ensure(n); S.q.ensure(n); S.cut.ensure(n); S.id.ensure(n);
+    k_iota<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p);
+    k_zero4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.frc.p);
+    upload_state(h, positions, momenta, false);
+    std::vector<double> q(n, 0.0);
+    if (charges) q.assign(charges, charges + n);
+    HIP_TRY(hipMemcpyAsync(S.q.p, q.data(), sizeof(double) * n, hipMemcpyHostToDevice, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    h->atoms_ready = true;
+    h->lists_valid = false;
+    h->forces_valid = false;
+    S.model_dirty = true;
+    API_END(h)
+}
+
+int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse, const int *pbc)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(vectors && inverse && pbc, PYSIC_EINVAL, "create_cell: bad arguments");
+    std::memcpy(h->S.geom.v, vectors, sizeof(double) * 9);
+    std::memcpy(h->S.geom.inv, inverse, sizeof(double) * 9);
+    for (int a = 0; a < 3; ++a) h->S.geom.pbc[a] = pbc[a] ? 1 : 0;
+    h->cell_ready = true;
+    h->lists_valid = false;
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_update_atom_coordinates(pysic_ctx *h, int n, const double *positions, const double *momenta)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->atoms_ready && n == h->S.n && positions, PYSIC_EINVAL,
+                  "update_atom_coordinates: atom count does not match");
+    upload_state(h, positions, momenta, true);
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_update_atom_charges(pysic_ctx *h, int n, const double *charges)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->atoms_ready && n == h->S.n && charges, PYSIC_EINVAL,
+                  "update_atom_charges: atom count does not match");
+    upload_scalar(h->S, charges, h->S.q);
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_get_number_of_atoms(pysic_ctx *h, int *n)
+{
+    if (!h || !n) return PYSIC_EINVAL;
+    *n = h->S.n;
+    return PYSIC_OK;
+}
+
+int pysic_set_potentials(pysic_ctx *h, int npot, const double *desc)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(npot >= 0 && (npot == 0 || desc), PYSIC_EINVAL, "set_potentials: bad arguments");
+    h->potdesc.assign(desc, desc + 10 * (size_t)npot);
+    h->S.model_dirty = true;
+    h->forces_valid = false;
+    if (h->atoms_ready) compile_model(h);
+    API_END(h)
+}
+
+int pysic_set_bond_order_factors(pysic_ctx *h, int nbond, const double *desc)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(nbond >= 0 && (nbond == 0 || desc), PYSIC_EINVAL, "set_bond_order_factors: bad arguments");
+    h->bonddesc.assign(desc, desc + 14 * (size_t)nbond);
+    h->S.model_dirty = true;
+    h->forces_valid = false;
+    if (h->atoms_ready) compile_model(h);
+    API_END(h)
+}
+
+int pysic_set_ewald(pysic_ctx *h, int on, double real_cutoff, double k_cutoff, const int *kcells,
+                    double sigma, double epsilon, const double *scaler)
+{
+    API_BEGIN(h)
+    EwaldParams &p = h->ewald;
+    p.on = on ? 1 : 0;
+    if (p.on) {
+        PYSIC_REQUIRE(kcells && sigma > 0 && epsilon > 0 && real_cutoff > 0 && k_cutoff > 0,
+                      PYSIC_EINVAL, "set_ewald: bad parameters");
+        p.rc = real_cutoff;
+        p.kc = k_cutoff;
+        for (int a = 0; a < 3; ++a) p.K[a] = kcells[a];
+        p.sigma = sigma;
+        p.eps = epsilon;
+        p.has_scaler = scaler != nullptr;
+        p.scaler.assign(h->S.n, 1.0);
+        if (scaler) p.scaler.assign(scaler, scaler + h->S.n);
+    }
+    h->ew.dirty = true;
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_generate_neighbor_lists(pysic_ctx *h, int n, const double *cutoffs)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->atoms_ready && h->cell_ready, PYSIC_ESTATE, "atoms and cell must be created first");
+    PYSIC_REQUIRE(n == h->S.n && cutoffs, PYSIC_EINVAL, "generate_neighbor_lists: bad arguments");
+    double mc = 0.0;
+    for (int i = 0; i < n; ++i) mc = std::max(mc, cutoffs[i]);
+    upload_scalar(h->S, cutoffs, h->S.cut);
+    nb_configure_grid(h->S, mc);
+    build_lists(h);
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_get_neighbor_counts(pysic_ctx *h, int n, int *counts)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->lists_valid && n == h->S.n && counts, PYSIC_ESTATE, "no neighbor lists");
+    System &S = h->S;
+    std::vector<int> cnt(n), id(n);
+    HIP_TRY(hipMemcpy(cnt.data(), S.ncnt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(id.data(), S.id.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; ++s) counts[id[s]] = cnt[s];
+    API_END(h)
+}
+
+int pysic_get_neighbor_lists(pysic_ctx *h, long nnz, int *idx, int *offsets)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->lists_valid && idx && offsets, PYSIC_ESTATE, "no neighbor lists");
+    System &S = h->S;
+    int n = S.n;
+    std::vector<int> cnt(n), id(n), nbr((size_t)S.cap * n), img((size_t)S.cap * n);
+    HIP_TRY(hipMemcpy(cnt.data(), S.ncnt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(id.data(), S.id.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nbr.data(), S.nbr.p, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(img.data(), S.nimg.p, sizeof(int) * img.size(), hipMemcpyDeviceToHost));
+    std::vector<int> slot(n);
+    for (int s = 0; s < n; ++s) slot[id[s]] = s;
+    long total = 0;
+    for (int s = 0; s < n; ++s) total += cnt[s];
+    PYSIC_REQUIRE(total == nnz, PYSIC_EINVAL, "get_neighbor_lists: nnz does not match the counts");
+    long e = 0;
+    for (int a = 0; a < n; ++a) {
+        int s = slot[a];
+        for (int k = 0; k < cnt[s]; ++k, ++e) {
+            size_t o = (size_t)k * n + s;
+            idx[e] = id[nbr[o]];
+            offsets[3 * e] = img_x(img[o]);
+            offsets[3 * e + 1] = img_y(img[o]);
+            offsets[3 * e + 2] = img_z(img[o]);
+        }
+    }
+    API_END(h)
+}
+
+int pysic_calculate_energy(pysic_ctx *h, double *energy)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(energy, PYSIC_EINVAL, "calculate_energy: null output");
+    require_ready(h);
+    System &S = h->S;
+    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
+    double e = 0.0;
+    if (S.model.has_pairs) {
+        tersoff_forces(h, false, 0.0);
+        double t[8];
+        HIP_TRY(hipMemcpyAsync(t, S.totals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));
+        check_flags(h, false);
+        e += t[0];
+    }
+    if (S.model.has_self || h->ewald.on) e += ewald_eem_energy(S, h->ewald, h->ew);
+    *energy = e;
+    API_END(h)
+}
+
+int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    System &S = h->S;
+    PYSIC_REQUIRE(n == S.n && forces && stress, PYSIC_EINVAL, "calculate_forces: bad arguments");
+    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (S.model.has_pairs) {
+        tersoff_forces(h, false, 0.0);
+        HIP_TRY(hipMemcpyAsync(t, S.totals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));
+        check_flags(h, false);
+    } else {
+        k_zero4<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.frc.p);
+    }
+    for (int q = 0; q < 6; ++q) stress[q] = t[1 + q];
+    if (h->ewald.on) {
+        double st[6];
+        ewald_add_forces(S, h->ewald, h->ew, st);  // adds into S.frc
+        for (int q = 0; q < 6; ++q) stress[q] += st[q];
+    }
+    download_vec3(S, S.frc.p, forces);
+    h->forces_valid = true;
+    API_END(h)
+}
+
+int pysic_calculate_electronegativities(pysic_ctx *h, int n, double *enegs)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    PYSIC_REQUIRE(n == h->S.n && enegs, PYSIC_EINVAL, "calculate_electronegativities: bad arguments");
+    DBuf<double> &chi = ewald_eem_chi(h->S, h->ewald, h->ew);
+    download_scalar(h->S, chi.p, enegs);
+    API_END(h)
+}
+
+int pysic_relax_charges(pysic_ctx *h, int n, double *charges, double *rates, int n_steps,
+                        double timestep, double inertia, double friction, double tolerance,
+                        int *steps, double *error)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    System &S = h->S;
+    PYSIC_REQUIRE(n == S.n && charges && rates && steps && error, PYSIC_EINVAL,
+                  "relax_charges: bad arguments");
+    upload_scalar(S, charges, S.q);
+    DBuf<double> dr;
+    dr.ensure(n);
+    upload_scalar(S, rates, dr);
+    ewald_relax(S, h->ewald, h->ew, dr.p, n_steps, timestep, inertia, friction, tolerance, steps, error);
+    download_scalar(S, S.q.p, charges);
+    download_scalar(S, dr.p, rates);
+    dr.release();
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energies)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    System &S = h->S;
+    PYSIC_REQUIRE(nsteps >= 0 && rebuild >= 0, PYSIC_EINVAL, "md_run: bad arguments");
+    PYSIC_REQUIRE(!h->ewald.on && S.model.has_pairs, PYSIC_EINVAL,
+                  "md_run drives the Tersoff/pair engine only");
+    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
+    DBuf<double> ebuf;
+    if (energies) ebuf.ensure(nsteps > 0 ? nsteps : 1);
+    if (!h->forces_valid) tersoff_forces(h, false, 0.0);
+    const double half = 0.5 * dt;
+    for (int step = 0; step < nsteps; ++step) {
+        md_drift(S, dt, half);
+        if (rebuild > 0 && (step + 1) % rebuild == 0) {
+            nb_sort_atoms(S);
+            nb_build_lists(S);
+        }
+        tersoff_forces(h, true, half);
+        if (energies)
+            k_copy_total<<<1, 64, 0, S.stream>>>(S.totals.p, ebuf.p + step);
+    }
+    HIP_TRY(hipGetLastError());
+    if (energies && nsteps > 0)
+        HIP_TRY(hipMemcpyAsync(energies, ebuf.p, sizeof(double) * nsteps, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    ebuf.release();
+    int f[4];
+    nb_read_flags(S, f);
+    PYSIC_REQUIRE(f[0] <= S.cap, PYSIC_ECAP,
+                  "neighbor list overflow during MD (" + std::to_string(f[0]) +
+                      "); raise the capacity and rerun");
+    check_flags(h, false);
+    h->forces_valid = true;
+    API_END(h)
+}
+
+int pysic_get_state(pysic_ctx *h, int n, double *positions, double *momenta)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(h->atoms_ready && n == h->S.n, PYSIC_EINVAL, "get_state: bad arguments");
+    if (positions) download_vec3(h->S, h->S.pos.p, positions);
+    if (momenta) download_vec3(h->S, h->S.mom.p, momenta);
+    API_END(h)
+}
+
+int pysic_set_kernel_variant(pysic_ctx *h, int variant)
+{
+    if (!h) return PYSIC_EINVAL;
+    if (variant < 0 || variant >= ts_variant_count()) {
+        h->err = "no such kernel variant";
+        return PYSIC_EINVAL;
+    }
+    h->variant = variant;
+    return PYSIC_OK;
+}
+
+int pysic_kernel_variant_count(void) { return ts_variant_count(); }
+const char *pysic_kernel_variant_name(int v) { return ts_variant_name(v); }
+
+int pysic_set_neighbor_capacity(pysic_ctx *h, int cap)
+{
+    if (!h || cap < 1) return PYSIC_EINVAL;
+    h->S.cap = cap;
+    h->lists_valid = false;
+    return PYSIC_OK;
+}
+
+int pysic_timing(pysic_ctx *h, int on)
+{
+    API_BEGIN(h)
+    h->S.timer.reset();
+    h->S.timer.on = on != 0;
+    API_END(h)
+}
+
+int pysic_timing_read(pysic_ctx *h, int max, char *names, int name_len, double *ms, long *calls)
+{
+    if (!h) return PYSIC_EINVAL;
+    try {
+        HIP_TRY(hipSetDevice(h->device));
+        h->S.timer.collect();
+    } catch (const PysicError &e) {
+        return fail(h, e);
+    }
+    int k = 0;
+    for (; k < (int)h->S.timer.total.size() && k < max; ++k) {
+        std::strncpy(names + (size_t)k * name_len, h->S.timer.total[k].first.c_str(), name_len - 1);
+        names[(size_t)k * name_len + name_len - 1] = 0;
+        ms[k] = h->S.timer.total[k].second;
+        calls[k] = h->S.timer.calls[k].second;
+    }
+    return k;
+}
+
+int pysic_synchronize(pysic_ctx *h)
+{
+    API_BEGIN(h)
+    HIP_TRY(hipStreamSynchronize(h->S.stream));
+    API_END(h)
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// common.h -- shared types of the MI355X pysic engine (host + device).
+//
+// Everything here is plain data: the compiled interaction model (tables
+// indexed by species), the device-resident system in cell-sorted order, and
+// the error/launch helpers used by every translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#define PYSIC_MAXS 4      // species per system
+#define PYSIC_MAXP 4      // exponential terms per ordered species pair
+
+// ---------------------------------------------------------------------------
+// interaction model, compiled from the pysic potential / bond-order entries
+// ---------------------------------------------------------------------------
+
+// One exponential pair term v(r) = eps*exp(-zeta*r) (Potentials.f90:3643-3677)
+// with the create_potential soft-cutoff rule (Potentials.f90:2382-2388).
+struct PairTerm {
+    double eps, zeta, cut, soft;
+    int smooth;   // 1 when 0 < soft < cut
+    int group;    // -1 plain, >= 0 coordinated by the Tersoff group
+};
+
+// One 'tersoff' bond-order parameter set (Potentials.f90:5685-5961).
+struct BondTerm {
+    double beta, eta, a, c, d, h, mu, cut, soft;
+    int valid;
+    int a_zero;   // a == 0: exp((a dr)^mu) == 1 and its derivative vanishes
+};
+
+struct Model {
+    int nspecies;
+    int npair[PYSIC_MAXS][PYSIC_MAXS];
+    PairTerm pair[PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXP];
+    BondTerm bond[PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXS];  // [center][j][k]
+    BondTerm post[PYSIC_MAXS];                           // b~ post-processing per center
+    double maxpot[PYSIC_MAXS];    // max_potential_radius per species (Core.f90:1073-1125)
+    double maxbond[PYSIC_MAXS];   // max_bond_radius per species (Core.f90:1034-1070)
+    double rshort;                // largest cutoff any Tersoff-kernel term uses
+    int group;                    // the single bond-order group (-1: none)
+    int has_pairs;                // any exponential terms at all
+    // charge_self singlets: sum_n eps_n q^n per species, n in {1,2,3,4}
+    double self_eps[PYSIC_MAXS][5];
+    int has_self;
+};
+
+// ---------------------------------------------------------------------------
+// packed periodic image offsets: three signed 10-bit fields
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int pack_img(int a, int b, int c)
+{
+    return ((a & 1023)) | ((b & 1023) << 10) | ((c & 1023) << 20);
+}
+__host__ __device__ inline int img_x(int p) { return (p << 22) >> 22; }
+__host__ __device__ inline int img_y(int p) { return (p << 12) >> 22; }
+__host__ __device__ inline int img_z(int p) { return (p << 2) >> 22; }
+__host__ __device__ inline int neg_img(int p)
+{
+    return pack_img(-img_x(p), -img_y(p), -img_z(p));
+}
+#define PYSIC_IMG_MAX 511
+
+// separation_vector (Geometry.f90:449-466): r2 - r1 + sum_i o_i v_i in that
+// order, no contraction, so distances match the reference's bits.
+__device__ __forceinline__ double3 separation64(const double4 &ri, const double4 &rj, int img,
+                                                const double *v)
+{
+#pragma clang fp contract(off)
+    double ox = (double)img_x(img), oy = (double)img_y(img), oz = (double)img_z(img);
+    double x = rj.x - ri.x, y = rj.y - ri.y, z = rj.z - ri.z;
+    x = x + ox * v[0]; y = y + ox * v[1]; z = z + ox * v[2];
+    x = x + oy * v[3]; y = y + oy * v[4]; z = z + oy * v[5];
+    x = x + oz * v[6]; y = y + oz * v[7]; z = z + oz * v[8];
+    return make_double3(x, y, z);
+}
+// .o. (Quaternions.f90:475-483), no contraction
+__device__ __forceinline__ double dot64(const double3 &a)
+{
+#pragma clang fp contract(off)
+    return a.x * a.x + a.y * a.y + a.z * a.z;
+}
+
+// ---------------------------------------------------------------------------
+// cell geometry passed by value to kernels
+// ---------------------------------------------------------------------------
+struct CellGeom {
+    double v[9];     // cell vectors in rows
+    double inv[9];   // M^-1 row-major (fractional = inv * r)
+    int pbc[3];
+    int nc[3];       // cell-list bins per axis (interior)
+    int dim[3];      // bins per axis incl. padding on open axes
+    int ncell;       // dim[0]*dim[1]*dim[2]
+};
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+#define PYSIC_OK 0
+#define PYSIC_EINVAL -1
+#define PYSIC_EHIP -2
+#define PYSIC_ESTATE -3
+#define PYSIC_ECAP -4
+#define PYSIC_ENOMEM -5
+#define PYSIC_ERANGE -6
+
+struct PysicError {
+    int code;
+    std::string msg;
+};
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            throw PysicError{PYSIC_EHIP, std::string(#expr) + ": " +               \
+                                             hipGetErrorString(_e)};               \
+    } while (0)
+
+#define PYSIC_REQUIRE(cond, code, text)                                            \
+    do {                                                                           \
+        if (!(cond)) throw PysicError{code, text};                                 \
+    } while (0)
+
+static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,40 @@
+// ewald.h -- Ewald summation and electronegativity equalization on device.
+#pragma once
+
+#include "system.h"
+
+#include <vector>
+
+struct EwaldParams {
+    int on = 0;
+    double rc = 0, kc = 0, sigma = 0, eps = 0;
+    int K[3] = {0, 0, 0};
+    int has_scaler = 0;
+    std::vector<double> scaler;  // original order
+};
+
+struct EwaldState {
+    bool dirty = true;
+    int nk = 0;
+    DBuf<double4> kvec;     // kx, ky, kz, coefficient c_k
+    DBuf<double2> sk;       // sum q s cos(k.r), sum q s sin(k.r)
+    DBuf<double> scaler;    // current (sorted) order
+    DBuf<double> chi;       // electronegativities (sorted order)
+    DBuf<double> part;      // block partials
+    DBuf<double> red;       // reduced scalars
+    DBuf<double2> skpart;   // structure-factor partials over atom chunks
+    double volume = 0;
+    int skchunks = 0;
+};
+
+void ewald_release(EwaldState &E);
+// Energy of the charge_self singlets + Ewald (real, reciprocal, self, background).
+double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E);
+// Adds Ewald forces into S.frc; stress[6] receives the Ewald part.
+void ewald_add_forces(System &S, const EwaldParams &P, EwaldState &E, double *stress);
+// Electronegativities (charge_self + Ewald) into E.chi, sorted order.
+DBuf<double> &ewald_eem_chi(System &S, const EwaldParams &P, EwaldState &E);
+// Damped charge dynamics on device; S.q and rates (sorted order) updated.
+void ewald_relax(System &S, const EwaldParams &P, EwaldState &E, double *rates, int n_steps,
+                 double dt, double inertia, double friction, double tolerance, int *steps,
+                 double *error);

@@ -1,0 +1,106 @@
+// system.h -- device-resident system and the host launchers of each kernel
+// family.  One System lives inside each pysic handle; every buffer is owned
+// by it and sized for the current atom count (grown, never shrunk).
+#pragma once
+
+#include "common.h"
+
+#include <vector>
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;  // elements
+    void ensure(size_t k)
+    {
+        if (k <= cap && p) return;
+        if (p) HIP_TRY(hipFree(p));
+        p = nullptr;
+        size_t want = k < 1 ? 1 : k;
+        HIP_TRY(hipMalloc((void **)&p, want * sizeof(T)));
+        cap = want;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Kernel timing: events recorded around every launch of a kernel family on
+// the system's stream, accumulated only while timing is enabled.
+struct KernelTimer {
+    bool on = false;
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<std::pair<std::string, double>> total;  // name -> ms
+    std::vector<std::pair<std::string, long>> calls;
+    void begin(const char *name, hipStream_t s, hipEvent_t *a);
+    void end(const char *name, hipStream_t s, hipEvent_t a);
+    void collect();
+    void reset();
+};
+
+struct System {
+    hipStream_t stream = nullptr;
+    int n = 0;
+    CellGeom geom{};
+
+    // atom data in the current (cell-sorted) order; *2 are the swap buffers
+    DBuf<double4> pos, pos2;    // x, y, z, species (as a double)
+    DBuf<double4> mom, mom2;    // px, py, pz, mass
+    DBuf<double4> frc, frc2;    // fx, fy, fz, 0
+    DBuf<double> q, q2;         // charges
+    DBuf<double> cut, cut2;     // neighbor-search radius per atom (cutoff + skin)
+    DBuf<int> id, id2;          // original index of the atom in this slot
+    DBuf<int> key;              // cell of each atom (current order)
+    DBuf<int> wrap;             // packed wrap offsets (current order)
+    DBuf<int> perm;             // new slot -> old slot
+    DBuf<int> cellcount, cellstart, cellfill, scanblk;
+    double max_cut = 0.0;
+
+    // neighbor lists (column-major: entry e of atom i at e*n + i)
+    int cap = 0;
+    DBuf<int> nbr, nimg, ncnt;
+    DBuf<int> flags;            // [0] max list length, [1] max short length, [2] wrap range, [3] unpaired bond
+
+    // Tersoff short lists and per-bond gradients
+    int scap = 0;
+    DBuf<int> sj, simg, scnt;
+    DBuf<float4> dU;            // dU_i / dr_ij per short-list entry
+    DBuf<double> partial;       // per-block energy + virial partials
+    int partial_blocks = 0;
+    DBuf<double> totals;        // reduced energy + virial (7 doubles)
+
+    DBuf<double> host_stage;    // scratch for original-order transfers
+    DBuf<double4> out4;
+
+    Model model{};
+    DBuf<Model> dmodel;
+    bool model_dirty = true;
+
+    KernelTimer timer;
+
+    void release_all();
+};
+
+// neighbor.hip
+void nb_configure_grid(System &S, double max_cut);
+void nb_sort_atoms(System &S);                // bin + counting sort + permute
+void nb_build_lists(System &S);               // cell-list search, reference semantics
+int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
+
+// tersoff.hip
+void ts_site(System &S, int variant);         // site energies, dU/dr_ij, virial partials
+void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
+void ts_reduce(System &S);                    // totals[0..6] = E, W(6)
+int ts_variant_count();
+const char *ts_variant_name(int v);
+
+// md.hip
+void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m
+void sys_scatter_original(System &S, const double4 *src, double4 *dst);  // dst[id[s]] = src[s]

@@ -1,0 +1,260 @@
+"""ctypes binding of libpysic_hip.so and a thin object over one handle.
+
+This is the only place the Python side touches native code.  It plays the
+role of the reference's f2py module (`pysic.pysic_fortran.pysic_interface`,
+imported at pysic/calculator.py:16): every method maps one-to-one onto an
+entry point of include/pysic_hip.h.  There is no fallback: if the library is
+missing or cannot find a GPU, constructing an Engine raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpysic_hip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+# exported symbols and their signatures (restype, argtypes)
+SIGNATURES = {
+    "pysic_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    "pysic_destroy": (None, [_vp]),
+    "pysic_last_error": (ctypes.c_char_p, [_vp]),
+    "pysic_create_atoms": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "pysic_create_cell": (ctypes.c_int, [_vp, _dp, _dp, _ip]),
+    "pysic_update_atom_coordinates": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "pysic_update_atom_charges": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_get_number_of_atoms": (ctypes.c_int, [_vp, _ip]),
+    "pysic_set_potentials": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_set_bond_order_factors": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_set_ewald": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
+                                       ctypes.c_double, ctypes.c_double, _dp]),
+    "pysic_generate_neighbor_lists": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_get_neighbor_counts": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "pysic_get_neighbor_lists": (ctypes.c_int, [_vp, ctypes.c_long, _ip, _ip]),
+    "pysic_calculate_energy": (ctypes.c_int, [_vp, _dp]),
+    "pysic_calculate_forces": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "pysic_calculate_electronegativities": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_relax_charges": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, _ip, _dp]),
+    "pysic_md_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+    "pysic_get_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "pysic_set_kernel_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pysic_kernel_variant_count": (ctypes.c_int, []),
+    "pysic_kernel_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "pysic_set_neighbor_capacity": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pysic_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pysic_timing_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
+                                         _lp]),
+    "pysic_synchronize": (ctypes.c_int, [_vp]),
+}
+
+
+class PysicEngineError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PysicEngineError(
+                "libpysic_hip.so is not built; run __graft_entry__.build() or "
+                "python -m pysic_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Engine:
+    """One device-resident system (one pysic handle)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = _vp()
+        rc = self.L.pysic_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise PysicEngineError("pysic_create failed (%d): no usable HIP device %d" % (rc, device))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pysic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.pysic_last_error(self.h)
+            raise PysicEngineError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    # --- system ----------------------------------------------------------
+    def create_atoms(self, species, positions, masses=None, charges=None, momenta=None):
+        pos = _f64(positions, (-1, 3))
+        n = len(pos)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        m = None if masses is None else _f64(masses)
+        q = None if charges is None else _f64(charges)
+        p = None if momenta is None else _f64(momenta, (-1, 3))
+        self._check(self.L.pysic_create_atoms(self.h, n, _d(m), _d(q), _d(pos), _d(p), _i(sp)),
+                    "create_atoms")
+        self.n = n
+
+    def create_cell(self, cell, pbc=(True, True, True)):
+        cell = _f64(cell, (3, 3))
+        inv = np.ascontiguousarray(np.linalg.inv(cell).T)  # M^-1, M = vectors as columns
+        pb = np.array([1 if p else 0 for p in pbc], dtype=np.int32)
+        self._check(self.L.pysic_create_cell(self.h, _d(cell), _d(inv), _i(pb)), "create_cell")
+
+    def update_coordinates(self, positions, momenta=None):
+        pos = _f64(positions, (-1, 3))
+        p = None if momenta is None else _f64(momenta, (-1, 3))
+        self._check(self.L.pysic_update_atom_coordinates(self.h, len(pos), _d(pos), _d(p)),
+                    "update_atom_coordinates")
+
+    def update_charges(self, charges):
+        q = _f64(charges)
+        self._check(self.L.pysic_update_atom_charges(self.h, len(q), _d(q)), "update_atom_charges")
+
+    def set_potentials(self, potdesc):
+        pd = _f64(potdesc).reshape(-1)
+        self._check(self.L.pysic_set_potentials(self.h, len(pd) // 10, _d(pd)), "set_potentials")
+
+    def set_bond_order_factors(self, bonddesc):
+        bd = _f64(bonddesc).reshape(-1)
+        self._check(self.L.pysic_set_bond_order_factors(self.h, len(bd) // 14, _d(bd)),
+                    "set_bond_order_factors")
+
+    def set_ewald(self, real_cutoff, k_cutoff, kcells, sigma, epsilon, scaler=None, on=True):
+        K = np.ascontiguousarray(kcells, dtype=np.int32)
+        sc = None if scaler is None else _f64(scaler)
+        self._check(self.L.pysic_set_ewald(self.h, 1 if on else 0, real_cutoff, k_cutoff, _i(K),
+                                           sigma, epsilon, _d(sc)), "set_ewald")
+
+    def ewald_off(self):
+        K = np.zeros(3, dtype=np.int32)
+        self._check(self.L.pysic_set_ewald(self.h, 0, 0.0, 0.0, _i(K), 0.0, 0.0, None), "set_ewald")
+
+    # --- neighbor lists --------------------------------------------------
+    def generate_neighbor_lists(self, cutoffs):
+        c = _f64(cutoffs)
+        self._check(self.L.pysic_generate_neighbor_lists(self.h, len(c), _d(c)),
+                    "generate_neighbor_lists")
+
+    def neighbor_lists(self):
+        """(counts[N], idx[nnz], offsets[nnz,3]) in atom order."""
+        counts = np.zeros(self.n, dtype=np.int32)
+        self._check(self.L.pysic_get_neighbor_counts(self.h, self.n, _i(counts)),
+                    "get_neighbor_counts")
+        nnz = int(counts.sum())
+        idx = np.zeros(max(nnz, 1), dtype=np.int32)
+        off = np.zeros((max(nnz, 1), 3), dtype=np.int32)
+        self._check(self.L.pysic_get_neighbor_lists(self.h, nnz, _i(idx), _i(off)),
+                    "get_neighbor_lists")
+        return counts, idx[:nnz], off[:nnz]
+
+    def set_neighbor_capacity(self, cap):
+        self._check(self.L.pysic_set_neighbor_capacity(self.h, int(cap)), "set_neighbor_capacity")
+
+    # --- evaluation ------------------------------------------------------
+    def energy(self):
+        e = np.zeros(1)
+        self._check(self.L.pysic_calculate_energy(self.h, _d(e)), "calculate_energy")
+        return float(e[0])
+
+    def forces(self):
+        f = np.zeros((self.n, 3))
+        st = np.zeros(6)
+        self._check(self.L.pysic_calculate_forces(self.h, self.n, _d(f), _d(st)), "calculate_forces")
+        return f, st
+
+    def electronegativities(self):
+        x = np.zeros(self.n)
+        self._check(self.L.pysic_calculate_electronegativities(self.h, self.n, _d(x)),
+                    "calculate_electronegativities")
+        return x
+
+    def relax_charges(self, charges, rates, n_steps, timestep, inertia, friction, tolerance):
+        q = np.array(charges, dtype=np.float64)
+        r = np.array(rates, dtype=np.float64)
+        steps = ctypes.c_int(0)
+        err = np.zeros(1)
+        self._check(self.L.pysic_relax_charges(self.h, self.n, _d(q), _d(r), int(n_steps),
+                                               timestep, inertia, friction, tolerance,
+                                               ctypes.byref(steps), _d(err)), "relax_charges")
+        return q, r, steps.value, float(err[0])
+
+    def md_run(self, nsteps, dt, rebuild=1, energies=False):
+        e = np.zeros(max(nsteps, 1)) if energies else None
+        self._check(self.L.pysic_md_run(self.h, int(nsteps), float(dt), int(rebuild), _d(e)),
+                    "md_run")
+        return None if e is None else e[:nsteps]
+
+    def state(self):
+        pos = np.zeros((self.n, 3))
+        mom = np.zeros((self.n, 3))
+        self._check(self.L.pysic_get_state(self.h, self.n, _d(pos), _d(mom)), "get_state")
+        return pos, mom
+
+    # --- tuning / measurement -------------------------------------------
+    def set_variant(self, v):
+        self._check(self.L.pysic_set_kernel_variant(self.h, int(v)), "set_kernel_variant")
+
+    def variants(self):
+        return [self.L.pysic_kernel_variant_name(v).decode()
+                for v in range(self.L.pysic_kernel_variant_count())]
+
+    def timing(self, on):
+        self._check(self.L.pysic_timing(self.h, 1 if on else 0), "timing")
+
+    def timing_read(self):
+        k = 32
+        names = ctypes.create_string_buffer(k * 32)
+        ms = np.zeros(k)
+        calls = np.zeros(k, dtype=np.int64)
+        cnt = self.L.pysic_timing_read(self.h, k, names, 32, _d(ms),
+                                       calls.ctypes.data_as(_lp))
+        if cnt < 0:
+            self._check(cnt, "timing_read")
+        out = {}
+        raw = names.raw
+        for i in range(cnt):
+            nm = raw[i * 32:(i + 1) * 32].split(b"\0")[0].decode()
+            out[nm] = (float(ms[i]), int(calls[i]))
+        return out
+
+    def synchronize(self):
+        self._check(self.L.pysic_synchronize(self.h), "synchronize")

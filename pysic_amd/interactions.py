@@ -1,0 +1,352 @@
+"""Interaction objects of the pysic API: Potential, BondOrderParameters,
+Coordinator, and their translation into engine descriptors.
+
+Mirrors pysic/interactions/local.py (Potential, local.py:284-776) and
+pysic/interactions/bondorder.py (BondOrderParameters :7-375, Coordinator
+:379-532) for the potential and bond-order types on the Tersoff / EEM path:
+'exponential' and 'charge_self' potentials and the 'tersoff' bond-order
+factor.  Parameter names and orders follow the reference's descriptor
+registry (Potentials.f90:3612-3677, 4312-4372, 5685-5736).
+"""
+from itertools import permutations
+
+import numpy as np
+
+
+class InvalidPotentialError(Exception):
+    """Raised for unknown potential types or bad targets (utility/error.py)."""
+
+
+class InvalidParametersError(Exception):
+    """Raised for bad parameter lists (utility/error.py)."""
+
+
+# name -> (n_targets, parameter names)  (Potentials.f90 characterizers)
+POTENTIAL_TYPES = {
+    "exponential": (2, ["epsilon", "zeta"]),
+    "charge_self": (1, ["epsilon", "n"]),
+}
+POTENTIAL_CODES = {"exponential": 1, "charge_self": 2}
+
+# name -> (n_targets, level, [[scaling params], [local params]])
+BOND_ORDER_TYPES = {
+    "tersoff": (3, 2, [["beta", "eta"], ["a", "c", "d", "h", "mu"]]),
+}
+
+
+def is_potential(name):
+    return name in POTENTIAL_TYPES
+
+
+def is_bond_order_factor(name):
+    return name in BOND_ORDER_TYPES
+
+
+def _normalise_targets(symbols, n_targets):
+    if symbols is None:
+        return None
+    if isinstance(symbols, str):
+        symbols = [[symbols]]
+    out = []
+    for t in symbols:
+        if isinstance(t, str):
+            t = [t]
+        t = list(t)
+        if len(t) != n_targets:
+            raise InvalidPotentialError("target list %s does not have %d symbols" % (t, n_targets))
+        out.append(t)
+    return out
+
+
+class BondOrderParameters:
+    """Bond-order factor parameters (bondorder.py:7-375).
+
+    parameters: [[beta, eta], [a, c, d, h, mu]] for 'tersoff'.
+    """
+
+    def __init__(self, bond_order_type, cutoff=0.0, cutoff_margin=0.0, parameters=None,
+                 symbols=None):
+        if not is_bond_order_factor(bond_order_type):
+            raise InvalidParametersError("unknown bond order factor type: %s" % bond_order_type)
+        self.bond_order_type = bond_order_type
+        self.n_targets, self.level, self.names = BOND_ORDER_TYPES[bond_order_type]
+        self.n_params = [len(x) for x in self.names]
+        self.cutoff = float(cutoff)
+        self.cutoff_margin = float(cutoff_margin)
+        self.parameters = None
+        self.set_parameters(parameters if parameters is not None else
+                            [[0.0] * k for k in self.n_params])
+        self.symbols = _normalise_targets(symbols, self.n_targets)
+
+    def accepts_parameters(self, params):
+        if len(params) != len(self.n_params):
+            return False
+        return all(len(params[i]) == self.n_params[i] for i in range(len(self.n_params)))
+
+    def set_parameters(self, params):
+        if not self.accepts_parameters(params):
+            raise InvalidParametersError("bond order parameters must have shape %s" % self.n_params)
+        self.parameters = [list(map(float, p)) for p in params]
+
+    def get_parameters_as_list(self):
+        """Single-element parameters first, then pair parameters
+        (bondorder.py:135-145)."""
+        return [x for p in self.parameters for x in p]
+
+    def set_parameter_value(self, name, value):
+        for i, group in enumerate(self.names):
+            if name in group:
+                self.parameters[i][group.index(name)] = float(value)
+                return
+        raise InvalidParametersError("no parameter %s" % name)
+
+    def get_parameter_value(self, name):
+        for i, group in enumerate(self.names):
+            if name in group:
+                return self.parameters[i][group.index(name)]
+        raise InvalidParametersError("no parameter %s" % name)
+
+    def get_symbols(self):
+        return self.symbols
+
+    def set_symbols(self, symbols):
+        self.symbols = _normalise_targets(symbols, self.n_targets)
+
+    def get_cutoff(self):
+        return self.cutoff
+
+    def set_cutoff(self, c):
+        self.cutoff = float(c)
+
+    def get_cutoff_margin(self):
+        return self.cutoff_margin
+
+    def set_cutoff_margin(self, m):
+        self.cutoff_margin = float(m)
+
+    def get_soft_cutoff(self):
+        """cutoff - margin (bondorder.py:241-243)."""
+        return self.cutoff - self.cutoff_margin
+
+    def get_level(self):
+        return self.level
+
+    def get_number_of_targets(self):
+        return self.n_targets
+
+    def get_bond_order_type(self):
+        return self.bond_order_type
+
+    def __eq__(self, other):
+        return (isinstance(other, BondOrderParameters)
+                and self.bond_order_type == other.bond_order_type
+                and self.parameters == other.parameters and self.symbols == other.symbols
+                and self.cutoff == other.cutoff and self.cutoff_margin == other.cutoff_margin)
+
+    def __repr__(self):
+        return "BondOrderParameters('%s',symbols=%s,parameters=%s,cutoff=%s,cutoff_margin=%s)" % (
+            self.bond_order_type, self.symbols, self.parameters, self.cutoff, self.cutoff_margin)
+
+
+class Coordinator:
+    """Holds the bond-order parameters that scale a potential
+    (bondorder.py:379-532)."""
+
+    def __init__(self, bond_order_parameters=None):
+        if bond_order_parameters is None:
+            bond_order_parameters = []
+        if isinstance(bond_order_parameters, BondOrderParameters):
+            bond_order_parameters = [bond_order_parameters]
+        self.bond_order_params = list(bond_order_parameters)
+        self.group_index = -1
+
+    def get_bond_order_parameters(self):
+        return self.bond_order_params
+
+    def set_bond_order_parameters(self, p):
+        self.bond_order_params = [p] if isinstance(p, BondOrderParameters) else list(p)
+
+    def add_bond_order_parameters(self, p):
+        self.bond_order_params.append(p)
+
+    def set_group_index(self, g):
+        self.group_index = g
+
+    def get_group_index(self):
+        return self.group_index
+
+    def __eq__(self, other):
+        return isinstance(other, Coordinator) and \
+            self.bond_order_params == other.bond_order_params
+
+
+class Potential:
+    """A potential acting on groups of atoms (local.py:284-776)."""
+
+    def __init__(self, potential_type, symbols=None, tags=None, indices=None, parameters=None,
+                 cutoff=0.0, cutoff_margin=0.0, coordinator=None):
+        if not is_potential(potential_type):
+            raise InvalidPotentialError("There is no potential called '%s'." % potential_type)
+        if tags is not None or indices is not None:
+            raise InvalidPotentialError("only element targets are supported by this engine")
+        self.potential_type = potential_type
+        self.n_targets, self.names = POTENTIAL_TYPES[potential_type]
+        self.symbols = _normalise_targets(symbols, self.n_targets)
+        self.parameters = [0.0] * len(self.names)
+        if parameters is not None:
+            self.set_parameter_values(parameters)
+        self.cutoff = float(cutoff)
+        self.cutoff_margin = float(cutoff_margin)
+        self.coordinator = coordinator
+
+    def set_parameter_values(self, values):
+        if len(values) != len(self.names):
+            raise InvalidParametersError("'%s' takes %d parameters" % (self.potential_type,
+                                                                     len(self.names)))
+        self.parameters = [float(v) for v in values]
+
+    def get_parameter_values(self):
+        return list(self.parameters)
+
+    def set_parameter_value(self, name, value):
+        self.parameters[self.names.index(name)] = float(value)
+
+    def get_parameter_value(self, name):
+        return self.parameters[self.names.index(name)]
+
+    def get_parameter_names(self):
+        return list(self.names)
+
+    def get_symbols(self):
+        return self.symbols
+
+    def set_symbols(self, s):
+        self.symbols = _normalise_targets(s, self.n_targets)
+
+    def get_different_symbols(self):
+        out = []
+        for t in (self.symbols or []):
+            for s in t:
+                if s not in out:
+                    out.append(s)
+        return out
+
+    def get_cutoff(self):
+        return self.cutoff
+
+    def set_cutoff(self, c):
+        self.cutoff = float(c)
+
+    def get_cutoff_margin(self):
+        return self.cutoff_margin
+
+    def set_cutoff_margin(self, m):
+        self.cutoff_margin = float(m)
+
+    def get_soft_cutoff(self):
+        """cutoff - margin (local.py:478-480)."""
+        return self.cutoff - self.cutoff_margin
+
+    def get_coordinator(self):
+        return self.coordinator
+
+    def set_coordinator(self, c):
+        self.coordinator = c
+
+    def get_number_of_targets(self):
+        return self.n_targets
+
+    def get_potential_type(self):
+        return self.potential_type
+
+    def __eq__(self, other):
+        return (isinstance(other, Potential) and self.potential_type == other.potential_type
+                and self.symbols == other.symbols and self.parameters == other.parameters
+                and self.cutoff == other.cutoff and self.cutoff_margin == other.cutoff_margin
+                and self.coordinator == other.coordinator)
+
+    def __repr__(self):
+        return "Potential('%s',symbols=%s,parameters=%s,cutoff=%s,cutoff_margin=%s)" % (
+            self.potential_type, self.symbols, self.parameters, self.cutoff, self.cutoff_margin)
+
+
+def build_descriptors(potentials, species_of):
+    """Translate potentials into the engine's descriptor rows, the way
+    update_core_potentials does for the Fortran core (calculator.py:1063-1446):
+
+    * every potential is added once per distinct permutation of each target
+      tuple (calculator.py:1166-1192);
+    * a coordinator's group index is the running index of the first
+      coordinated potential with equal bond parameters (1227-1233);
+    * bond-order factors are added per target tuple without permutation
+      (permutate = False, 1372).
+
+    species_of maps element symbols to engine species ids; potentials naming
+    elements absent from the system are dropped (they can act on no atom).
+    Returns (potdesc (k,10), bonddesc (m,14), n_groups).
+    """
+    pot_rows, bond_rows = [], []
+    coord_list = []  # (coordinator, group index)
+    pot_index = 0
+    for pot in potentials or []:
+        coord = pot.get_coordinator()
+        if coord is not None:
+            if not any(c is coord or c == coord for c, _ in coord_list):
+                coord_list.append((coord, pot_index))
+            pot_index += 1
+    for pot in potentials or []:
+        group = -1
+        coord = pot.get_coordinator()
+        if coord is not None:
+            for c, g in coord_list:
+                if c.get_bond_order_parameters() == coord.get_bond_order_parameters():
+                    group = g
+                    break
+            coord.set_group_index(group)
+        code = POTENTIAL_CODES[pot.get_potential_type()]
+        p = pot.get_parameter_values()
+        for targets in pot.get_symbols() or []:
+            for perm in sorted(set(permutations(targets))):
+                if any(s not in species_of for s in perm):
+                    continue
+                e = [species_of[s] for s in perm] + [0] * (2 - len(perm))
+                pot_rows.append([code, pot.get_number_of_targets(), e[0], e[1], p[0], p[1],
+                                 pot.get_cutoff(), pot.get_soft_cutoff(), group, 0.0])
+    for coord, g in coord_list:
+        for bond in coord.get_bond_order_parameters():
+            if bond.get_bond_order_type() != "tersoff":
+                raise InvalidParametersError("only 'tersoff' bond order factors are supported")
+            for targets in bond.get_symbols() or []:
+                if any(s not in species_of for s in targets):
+                    continue
+                e = [species_of[s] for s in targets]
+                bond_rows.append(e + bond.get_parameters_as_list() +
+                                 [bond.get_cutoff(), bond.get_soft_cutoff(), g, 0.0])
+    pd = np.array(pot_rows, dtype=np.float64).reshape(-1, 10)
+    bd = np.array(bond_rows, dtype=np.float64).reshape(-1, 14)
+    return pd, bd, len(coord_list)
+
+
+# Tersoff (1988) silicon, written with pysic's building blocks: a repulsive
+# exponential, an attractive exponential (epsilon = -B) scaled by a
+# coordinator holding the 'tersoff' bond-order factor
+# (the construction of interactions/comb.py:668-731).
+TERSOFF_SI = dict(A=1830.8, lambda1=2.4799, B=471.18, lambda2=1.7322, beta=1.1e-6,
+                  eta=0.78734, a=0.0, c=100390.0, d=16.217, h=-0.59825, mu=3.0,
+                  cutoff=3.0, margin=0.3)
+
+
+def tersoff_si(params=None):
+    P = dict(TERSOFF_SI)
+    if params:
+        P.update(params)
+    bond = BondOrderParameters("tersoff", cutoff=P["cutoff"], cutoff_margin=P["margin"],
+                               parameters=[[P["beta"], P["eta"]],
+                                           [P["a"], P["c"], P["d"], P["h"], P["mu"]]],
+                               symbols=[["Si", "Si", "Si"]])
+    rep = Potential("exponential", symbols=[["Si", "Si"]], parameters=[P["A"], P["lambda1"]],
+                    cutoff=P["cutoff"], cutoff_margin=P["margin"])
+    att = Potential("exponential", symbols=[["Si", "Si"]], parameters=[-P["B"], P["lambda2"]],
+                    cutoff=P["cutoff"], cutoff_margin=P["margin"],
+                    coordinator=Coordinator([bond]))
+    return [rep, att]

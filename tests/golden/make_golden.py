@@ -1,0 +1,122 @@
+"""Writes the golden vectors in tests/golden/ from the reference Fortran core.
+
+Runs only in a container that has the reference sources: build the checker
+first with oracle/build_ref.sh, then
+
+    python tests/golden/make_golden.py
+
+Every fixture is data only (inputs and the reference's outputs).  The
+system set-up replays pysic/calculator.py:1595-1637 through the reference's
+own f2py interface routines (see oracle/ref_driver.f90).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from pysic_amd import structures as st  # noqa: E402
+
+T = dict(A=1830.8, l1=2.4799, B=471.18, l2=1.7322)
+TERS = [1.1e-6, 0.78734, 0.0, 100390.0, 16.217, -0.59825, 3.0]
+
+
+def tersoff_setup(struct, ters=TERS, cut=3.0, margin=0.3, skin=0.5):
+    pots = [(1, [T["A"], T["l1"]], cut, cut - margin, ("Si", "Si"), -1),
+            (1, [-T["B"], T["l2"]], cut, cut - margin, ("Si", "Si"), 0)]
+    bonds = [(ters, cut, cut - margin, ("Si", "Si", "Si"), 0)]
+    return pots, bonds, np.full(len(struct), cut + skin)
+
+
+def kcells(cell, kc):
+    rec = 2 * math.pi * np.linalg.inv(cell).T
+    vol = rec[0] @ np.cross(rec[1], rec[2])
+    return [int(kc * np.linalg.norm(np.cross(rec[(a + 1) % 3], rec[(a + 2) % 3])) / vol + 0.5)
+            for a in range(3)]
+
+
+def run_tersoff(name, struct, ters=TERS, with_lists=True):
+    R = O.RefCore()
+    pots, bonds, cuts = tersoff_setup(struct, ters)
+    R.setup(struct.symbols, struct.positions, struct.cell, struct.pbc, potentials=pots,
+            tersoff=bonds, ngroups=1, cutoffs=cuts)
+    e = R.energy()
+    f, s = R.forces()
+    out = dict(positions=struct.positions, cell=struct.cell, pbc=struct.pbc.astype(np.int32),
+               cutoffs=cuts, tersoff=np.array(ters), energy=e, forces=f, stress=s)
+    if with_lists:
+        nl = R.neighbor_lists()
+        out.update(nl_counts=nl.counts, nl_idx=nl.idx, nl_off=nl.off)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, len(struct), "E = %.12f" % e)
+
+
+def run_ewald(name, struct, rc, accuracy_scale=3.0, charge_self=None, scaler=None):
+    sigma = rc / accuracy_scale
+    kc = accuracy_scale / sigma
+    eps = 0.00552635
+    K = kcells(struct.cell, kc)
+    pots = []
+    if charge_self:
+        for sym, (chi0, J) in charge_self.items():
+            pots.append((2, [chi0, 1.0], 0.0, 0.0, (sym,), -1))
+            pots.append((2, [0.5 * J, 2.0], 0.0, 0.0, (sym,), -1))
+    R = O.RefCore()
+    R.setup(struct.symbols, struct.positions, struct.cell, struct.pbc, charges=struct.charges,
+            potentials=pots, tersoff=[], ngroups=0, cutoffs=np.full(len(struct), rc + 0.5),
+            ewald=(rc, kc, K, sigma, eps), scaler=scaler)
+    e = R.energy()
+    f, s = R.forces()
+    x = R.electronegativities()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), positions=struct.positions,
+                        cell=struct.cell, charges=struct.charges,
+                        symbols=np.array(struct.symbols), ewald=np.array([rc, kc, sigma, eps]),
+                        kcells=np.array(K), energy=e, forces=f, stress=s, enegs=x,
+                        scaler=np.ones(len(struct)) if scaler is None else np.asarray(scaler),
+                        charge_self=np.array([[chi0, J] for chi0, J in (charge_self or {}).values()])
+                        if charge_self else np.zeros((0, 2)),
+                        charge_self_symbols=np.array(list((charge_self or {}).keys())))
+    print(name, len(struct), "E = %.12f" % e, "K", K)
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("build the reference checker first: oracle/build_ref.sh")
+    # Tersoff Si, the parity configuration at reduced and full size
+    run_tersoff("si_diamond_2x2x2_s005", st.diamond_si((2, 2, 2), sigma=0.05, seed=1))
+    run_tersoff("si_diamond_4x4x4_s005", st.diamond_si((4, 4, 4), sigma=0.05, seed=0))
+    run_tersoff("si_diamond_8x8x8_s005_seed0", st.diamond_si((8, 8, 8), sigma=0.05, seed=0),
+                with_lists=False)
+    run_tersoff("si_diamond_3x3x3_perfect", st.diamond_si((3, 3, 3)))
+    # amorphous Si: neighbors inside the smooth-cutoff shell
+    run_tersoff("si_amorphous_216", st.amorphous_si(216, 2.3, seed=7))
+    # a != 0 exercises the exp((a (r_ij - r_ik))^mu) branch
+    ters_a = list(TERS)
+    ters_a[2] = 1.3
+    run_tersoff("si_amorphous_128_a13", st.amorphous_si(128, 2.3, seed=11), ters=ters_a)
+    # open boundaries: padding subcells
+    clus = st.diamond_si((3, 3, 3), sigma=0.05, seed=5)
+    clus.pbc[:] = [True, False, False]
+    clus.cell = np.diag([16.293, 20.0, 21.0])
+    run_tersoff("si_slab_open_yz", clus)
+    # Ewald: rocksalt with displacements
+    run_ewald("nacl_2x2x2_s008_rc5", st.rocksalt((2, 2, 2), sigma=0.08, seed=3), rc=5.0)
+    run_ewald("nacl_3x3x3_s008_rc7", st.rocksalt((3, 3, 3), sigma=0.08, seed=4), rc=7.0)
+    # EEM: small random SiO2 with charge_self + Ewald and random charges
+    sio2 = st.amorphous_sio2(16, 2.2, seed=2)
+    rng = np.random.default_rng(8)
+    q = rng.normal(0.0, 0.5, len(sio2))
+    q -= q.mean()
+    sio2.set_initial_charges(q)
+    run_ewald("sio2_48_eem_rc5", sio2, rc=5.0,
+              charge_self={"Si": (4.77, 8.0), "O": (8.74, 13.4)})
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""GPU parity of the Tersoff path: HIP engine vs the oracle and the golden
+vectors of the reference core.
+
+Contract (tolerances written here, checked per case):
+  neighbor lists   bit-exact as sorted (i, j, offset) sets
+  energy           |E - E_ref| / N <= 1e-7 eV/atom
+  forces           max|F - F_ref| / max|F_ref| <= 1e-5
+  stress (virial)  max|W - W_ref| <= 1e-5 max|W_ref| (+1e-6 absolute)
+"""
+import numpy as np
+import pytest
+
+from conftest import TERSOFF_CASES, load_golden
+from helpers import rel_maxnorm, tersoff_descriptors
+from pysic_amd import structures as st
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from pysic_amd.engine import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def load_system(eng, pos, cell, pbc, pd, bd, cutoffs, species=None, masses=None, momenta=None):
+    n = len(pos)
+    eng.create_atoms(np.zeros(n, np.int32) if species is None else species, pos, masses=masses,
+                     momenta=momenta)
+    eng.create_cell(cell, pbc)
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(bd)
+    eng.ewald_off()
+    eng.generate_neighbor_lists(cutoffs)
+
+
+def pair_set(counts, idx, off):
+    i = np.repeat(np.arange(len(counts), dtype=np.int64), counts)
+    rows = np.column_stack([i, idx.astype(np.int64), off.astype(np.int64)])
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def check_tersoff(eng, oracle, pos, cell, pbc, pd, bd, cutoffs, ref=None):
+    n = len(pos)
+    print("case: n=%d" % n, flush=True)
+    load_system(eng, pos, cell, pbc, pd, bd, cutoffs)
+    nl = oracle.neighbor_list(pos, cell, pbc, cutoffs)
+    counts, idx, off = eng.neighbor_lists()
+    assert np.array_equal(pair_set(counts, idx, off), nl.pair_set())
+    sp = np.zeros(n, np.int32)
+    if ref is None:
+        e_ref = oracle.evaluate("energy", sp, pos, None, cell, nl, pd, bd)
+        f_ref, s_ref = oracle.evaluate("forces", sp, pos, None, cell, nl, pd, bd)
+    else:
+        e_ref, f_ref, s_ref = ref
+    e = eng.energy()
+    f, s = eng.forces()
+    assert abs(e - e_ref) / n <= 1e-7, (e, e_ref)
+    assert rel_maxnorm(f, f_ref) <= 1e-5
+    assert np.abs(s - s_ref).max() <= 1e-5 * np.abs(s_ref).max() + 1e-6
+    return e, f, s
+
+
+@pytest.mark.parametrize("name", TERSOFF_CASES + ["si_diamond_8x8x8_s005_seed0"])
+def test_golden_cases(eng, oracle, name):
+    g = load_golden(name)
+    pd, bd = tersoff_descriptors(g["tersoff"])
+    ref = (float(g["energy"]), g["forces"], g["stress"])
+    check_tersoff(eng, oracle, g["positions"], g["cell"], g["pbc"], pd, bd, g["cutoffs"], ref)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config1_seeds(eng, oracle, seed):
+    # Tersoff Si 8x8x8 (4096 atoms), Gaussian displacements 0.05 A
+    s = st.diamond_si((8, 8, 8), sigma=0.05, seed=seed)
+    pd, bd = tersoff_descriptors()
+    check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+
+
+def test_perfect_lattice_closed_form(eng, oracle):
+    s = st.diamond_si((8, 8, 8))
+    pd, bd = tersoff_descriptors()
+    load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+    e = eng.energy()
+    f, _ = eng.forces()
+    assert abs(e / len(s) - (-4.629593127667)) <= 1e-7
+    assert np.abs(f).max() < 1e-4  # zero by symmetry; FP32 three-body terms
+
+
+def test_all_kernel_variants_agree(eng, oracle):
+    s = st.amorphous_si(512, 2.3, seed=5)
+    pd, bd = tersoff_descriptors()
+    ref = None
+    for v in range(len(eng.variants())):
+        eng.set_variant(v)
+        out = check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd,
+                            np.full(len(s), 3.5))
+        if ref is None:
+            ref = out
+    eng.set_variant(0)
+
+
+def test_bitwise_deterministic(eng):
+    s = st.diamond_si((6, 6, 6), sigma=0.1, seed=9)
+    pd, bd = tersoff_descriptors()
+    outs = []
+    for _ in range(2):
+        load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+        e = eng.energy()
+        f, w = eng.forces()
+        outs.append((e, f, w))
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_neighbor_capacity_growth(eng, oracle):
+    # a list capacity far below the real count is grown and the lists rebuilt
+    s = st.amorphous_si(300, 2.3, seed=2)
+    pd, bd = tersoff_descriptors()
+    eng.set_neighbor_capacity(2)
+    check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+
+
+def test_md_matches_oracle_velocity_verlet(eng, oracle):
+    # ASE VelocityVerlet semantics, list rebuilt every step; 20 steps of 1 fs
+    s = st.diamond_si((4, 4, 4), sigma=0.1, seed=3)
+    st.maxwell_boltzmann(s, 600.0, seed=4)
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    cut = np.full(n, 3.5)
+    load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, cut, masses=s.masses,
+                momenta=s.momenta)
+    dt = st.ASE_FS
+    steps = 20
+    e_gpu = eng.md_run(steps, dt, rebuild=1, energies=True)
+    pos_g, mom_g = eng.state()
+    # oracle VV
+    sp = np.zeros(n, np.int32)
+    m = s.masses[:, None]
+    r = s.positions.copy()
+    p = s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+    e_or = []
+    for _ in range(steps):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+        p = p + 0.5 * dt * f
+        e_or.append(oracle.evaluate("energy", sp, r, None, s.cell, nl, pd, bd))
+    assert np.abs(pos_g - r).max() < 1e-7
+    assert np.abs(mom_g - p).max() < 1e-5 * np.abs(p).max()
+    assert np.abs(np.array(e_gpu) - np.array(e_or)).max() / n <= 1e-7

@@ -1,0 +1,262 @@
+"""CPU tests that pin the oracle (oracle/pysic_oracle.c).
+
+Each check is chosen so that a plausible mistake in the restatement (a
+dropped term, a wrong sign or index, a transposed operand) fails it:
+golden vectors written by the reference Fortran core, closed forms, finite
+differences of the energy, brute-force enumeration and invariances.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import EWALD_CASES, TERSOFF_CASES, load_golden
+from helpers import ewald_descriptors, rel_maxnorm, tersoff_descriptors
+from pysic_amd import structures as st
+
+
+def _tersoff_case(O, name):
+    g = load_golden(name)
+    pos, cell, pbc = g["positions"], g["cell"], g["pbc"]
+    nl = O.neighbor_list(pos, cell, pbc, g["cutoffs"])
+    pd, bd = tersoff_descriptors(g["tersoff"])
+    sp = np.zeros(len(pos), np.int32)
+    return g, nl, pd, bd, sp
+
+
+@pytest.mark.parametrize("name", TERSOFF_CASES)
+def test_tersoff_matches_reference_core(oracle, name):
+    g, nl, pd, bd, sp = _tersoff_case(oracle, name)
+    if "nl_idx" in g:
+        # the reference's own insertion order is reproduced exactly
+        assert np.array_equal(nl.counts, g["nl_counts"])
+        assert np.array_equal(nl.idx, g["nl_idx"])
+        assert np.array_equal(nl.off, g["nl_off"])
+    e = oracle.evaluate("energy", sp, g["positions"], None, g["cell"], nl, pd, bd)
+    f, s = oracle.evaluate("forces", sp, g["positions"], None, g["cell"], nl, pd, bd)
+    assert abs(e - float(g["energy"])) <= 1e-12 * abs(float(g["energy"]))
+    assert np.abs(f - g["forces"]).max() <= 1e-12 * max(1.0, np.abs(g["forces"]).max())
+    assert np.abs(s - g["stress"]).max() <= 1e-11 * max(1.0, np.abs(g["stress"]).max())
+
+
+def test_tersoff_8x8x8_matches_reference_core(oracle):
+    g, nl, pd, bd, sp = _tersoff_case(oracle, "si_diamond_8x8x8_s005_seed0")
+    e = oracle.evaluate("energy", sp, g["positions"], None, g["cell"], nl, pd, bd)
+    f, s = oracle.evaluate("forces", sp, g["positions"], None, g["cell"], nl, pd, bd)
+    assert abs(e - float(g["energy"])) < 1e-9
+    assert np.abs(f - g["forces"]).max() < 1e-11
+    assert np.abs(s - g["stress"]).max() < 1e-9
+
+
+def test_perfect_diamond_closed_form(oracle):
+    # r1 = sqrt(3)/4 a; zeta = 3 g(-1/3); b = (1+(beta zeta)^eta)^(-1/2eta)
+    # E/atom = 2 [A e^{-l1 r1} - B e^{-l2 r1} b]  (4 bonds per atom, 1/2 each)
+    a = 5.431
+    r1 = math.sqrt(3) / 4 * a
+    c, d, h = 100390.0, 16.217, -0.59825
+    gth = 1 + c * c / (d * d) - c * c / (d * d + (h + 1.0 / 3.0) ** 2)
+    zeta = 3 * gth
+    b = (1 + (1.1e-6 * zeta) ** 0.78734) ** (-1 / (2 * 0.78734))
+    e_closed = 2 * (1830.8 * math.exp(-2.4799 * r1) - 471.18 * math.exp(-1.7322 * r1) * b)
+    assert abs(e_closed - (-4.629593127667)) < 1e-11
+    s = st.diamond_si((3, 3, 3))
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), 3.5))
+    pd, bd = tersoff_descriptors()
+    sp = np.zeros(len(s), np.int32)
+    e = oracle.evaluate("energy", sp, s.positions, None, s.cell, nl, pd, bd)
+    f, _ = oracle.evaluate("forces", sp, s.positions, None, s.cell, nl, pd, bd)
+    assert abs(e / len(s) - e_closed) < 1e-11
+    assert np.abs(f).max() < 1e-10
+
+
+def test_tersoff_forces_are_minus_gradient(oracle):
+    # central differences, h = 1e-4 as Pysic.get_numerical_energy_gradient
+    # (calculator.py:1639-1677); amorphous cell exercises the smooth shells
+    g, nl, pd, bd, sp = _tersoff_case(oracle, "si_amorphous_128_a13")
+    pos = g["positions"].copy()
+    f, _ = oracle.evaluate("forces", sp, pos, None, g["cell"], nl, pd, bd)
+    h = 1e-4
+    for i in (0, 17, 63, 127):
+        for c in range(3):
+            p1 = pos.copy(); p1[i, c] += h
+            p2 = pos.copy(); p2[i, c] -= h
+            e1 = oracle.evaluate("energy", sp, p1, None, g["cell"], nl, pd, bd)
+            e2 = oracle.evaluate("energy", sp, p2, None, g["cell"], nl, pd, bd)
+            assert abs(-(e1 - e2) / (2 * h) - f[i, c]) < 2e-5 * max(1.0, abs(f[i, c]))
+
+
+def test_tersoff_stress_is_strain_derivative(oracle):
+    # sum r (x) f  equals  -dE/d(eps) under a homogeneous strain
+    g, nl, pd, bd, sp = _tersoff_case(oracle, "si_amorphous_216")
+    pos, cell = g["positions"], g["cell"]
+    _, s = oracle.evaluate("forces", sp, pos, None, cell, nl, pd, bd)
+    h = 1e-6
+    comps = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for q, (a, b) in enumerate(comps):
+        vals = []
+        for sgn in (1, -1):
+            eps = np.eye(3)
+            eps[a, b] += sgn * h
+            if a != b:
+                eps[b, a] += sgn * h
+            p = pos @ eps.T
+            cl = cell @ eps.T
+            vals.append(oracle.evaluate("energy", sp, p, None, cl, nl, pd, bd))
+        dE = (vals[0] - vals[1]) / (2 * h)
+        if a != b:
+            dE /= 2.0
+        assert abs(-dE - s[q]) < 1e-5 * max(1.0, np.abs(s).max())
+
+
+def _brute_force_pairs(pos, cell, pbc, rc):
+    """All (i, j, o) with |r_j + o.V - r_i| < rc by direct image enumeration."""
+    rows = []
+    cell = np.asarray(cell)
+    heights = [abs(np.dot(cell[a], np.cross(cell[(a + 1) % 3], cell[(a + 2) % 3]))) /
+               np.linalg.norm(np.cross(cell[(a + 1) % 3], cell[(a + 2) % 3])) for a in range(3)]
+    ext = [int(math.ceil(rc / heights[a])) + 2 if pbc[a] else 0 for a in range(3)]
+    for i in range(len(pos)):
+        for o1 in range(-ext[0], ext[0] + 1):
+            for o2 in range(-ext[1], ext[1] + 1):
+                for o3 in range(-ext[2], ext[2] + 1):
+                    sh = o1 * cell[0] + o2 * cell[1] + o3 * cell[2]
+                    d = pos + sh - pos[i]
+                    r2 = (d * d).sum(1)
+                    for j in np.nonzero(r2 < rc * rc)[0]:
+                        if j == i and (o1, o2, o3) == (0, 0, 0):
+                            continue
+                        rows.append((i, j, o1, o2, o3))
+    rows = np.array(sorted(rows), dtype=np.int64).reshape(-1, 5)
+    return rows
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_neighbor_lists_brute_force_triclinic(oracle, seed):
+    rng = np.random.default_rng(seed)
+    cell = np.array([[9.0, 0.0, 0.0], [2.5, 8.5, 0.0], [1.0, -1.5, 10.0]])
+    frac = rng.random((60, 3)) * 1.6 - 0.3  # some atoms outside the cell
+    pos = frac @ cell
+    nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(60, 3.2))
+    got = nl.pair_set()
+    want = _brute_force_pairs(pos, cell, [1, 1, 1], 3.2)
+    # offsets relate raw positions; brute force uses raw positions too
+    assert np.array_equal(got, want)
+
+
+def test_neighbor_lists_symmetric(oracle):
+    s = st.amorphous_si(200, 2.3, seed=3)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), 3.5))
+    rows = nl.pair_set()
+    rev = rows[:, [1, 0, 2, 3, 4]].copy()
+    rev[:, 2:] *= -1
+    rev = rev[np.lexsort(rev.T[::-1])]
+    assert np.array_equal(rows, rev)
+
+
+@pytest.mark.parametrize("name", EWALD_CASES)
+def test_ewald_matches_reference_core(oracle, name):
+    g = load_golden(name)
+    sp, pd, ew = ewald_descriptors(g)
+    rc = float(g["ewald"][0])
+    nl = oracle.neighbor_list(g["positions"], g["cell"], [1, 1, 1], np.full(len(sp), rc + 0.5))
+    e = oracle.evaluate("energy", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
+    f, s = oracle.evaluate("forces", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
+    x = oracle.evaluate("enegs", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
+    assert abs(e - float(g["energy"])) <= 1e-11 * max(1.0, abs(float(g["energy"])))
+    assert rel_maxnorm(f, g["forces"]) < 1e-11
+    assert np.abs(s - g["stress"]).max() < 1e-10 * max(1.0, np.abs(g["stress"]).max())
+    assert rel_maxnorm(x, g["enegs"]) < 1e-11
+
+
+def _rocksalt_energy(oracle, s, rc, scale):
+    sigma = rc / scale
+    kc = scale / sigma
+    rec = 2 * math.pi * np.linalg.inv(s.cell).T
+    vol = rec[0] @ np.cross(rec[1], rec[2])
+    K = [int(kc * np.linalg.norm(np.cross(rec[(a + 1) % 3], rec[(a + 2) % 3])) / vol + 0.5)
+         for a in range(3)]
+    ew = np.array([1, rc, kc, K[0], K[1], K[2], sigma, 0.00552635])
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), rc + 0.5))
+    sp = (s.charges < 0).astype(np.int32)
+    return oracle.evaluate("energy", sp, s.positions, s.charges, s.cell, nl,
+                           np.zeros((0, 10)), None, ew)
+
+
+def test_madelung_energy_and_sigma_invariance(oracle):
+    # E/ion = -(1/2) M k_e / r0, r0 = a/2, M = 1.747564595, k_e = 1/(4 pi eps0)
+    s = st.rocksalt((2, 2, 2))
+    ke = 1.0 / (4 * math.pi * 0.00552635)
+    e_mad = -1.747564594633 * ke / 5.64
+    assert abs(e_mad - (-4.461756683)) < 1e-8
+    e1 = _rocksalt_energy(oracle, s, 5.5, 5.0) / len(s)
+    e2 = _rocksalt_energy(oracle, s, 5.5, 5.5) / len(s)
+    assert abs(e1 - e_mad) < 1e-5
+    assert abs(e1 - e2) < 1e-5
+
+
+def test_ewald_forces_and_chi_are_gradients(oracle):
+    g = load_golden("sio2_48_eem_rc5")
+    sp, pd, ew = ewald_descriptors(g)
+    pos, q, cell = g["positions"], g["charges"], g["cell"]
+    nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(len(sp), 5.5))
+    f, _ = oracle.evaluate("forces", sp, pos, q, cell, nl, pd, None, ew)
+    x = oracle.evaluate("enegs", sp, pos, q, cell, nl, pd, None, ew)
+    h = 1e-5
+    for i in (0, 9, 30):
+        for c in range(3):
+            p1 = pos.copy(); p1[i, c] += h
+            p2 = pos.copy(); p2[i, c] -= h
+            e1 = oracle.evaluate("energy", sp, p1, q, cell, nl, pd, None, ew)
+            e2 = oracle.evaluate("energy", sp, p2, q, cell, nl, pd, None, ew)
+            assert abs(-(e1 - e2) / (2 * h) - f[i, c]) < 1e-5 * max(1.0, abs(f[i, c]))
+        q1 = q.copy(); q1[i] += h
+        q2 = q.copy(); q2[i] -= h
+        e1 = oracle.evaluate("energy", sp, pos, q1, cell, nl, pd, None, ew)
+        e2 = oracle.evaluate("energy", sp, pos, q2, cell, nl, pd, None, ew)
+        # chi = -dE/dq (calculator.py:1782-1832)
+        assert abs(-(e1 - e2) / (2 * h) - x[i]) < 1e-5 * max(1.0, abs(x[i]))
+
+
+def test_relaxation_without_coulomb_reaches_eem_solution(oracle):
+    # chi_i = -(chi0_i + J_i q_i); the fixed point equalises chi with sum q
+    # conserved: q_i = (nu - chi0_i)/J_i, nu = (Q + sum chi0/J)/(sum 1/J)
+    s = st.amorphous_sio2(8, 2.2, seed=1)
+    sp = np.array([0 if x == "Si" else 1 for x in s.symbols], np.int32)
+    chi0 = {0: 4.77, 1: 8.74}
+    J = {0: 8.0, 1: 13.4}
+    rows = []
+    for k in (0, 1):
+        rows.append([2, 1, k, 0, chi0[k], 1.0, 0.0, 0.0, -1, 0])
+        rows.append([2, 1, k, 0, 0.5 * J[k], 2.0, 0.0, 0.0, -1, 0])
+    pd = np.array(rows, float)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), 1.0))
+    c0 = np.array([chi0[k] for k in sp])
+    Jv = np.array([J[k] for k in sp])
+    nu = (0.0 + (c0 / Jv).sum()) / (1.0 / Jv).sum()
+    q_star = (nu - c0) / Jv
+    q, r, steps, err = oracle.relax_dynamic(sp, s.positions, np.zeros(len(s)), s.cell, nl, pd,
+                                            None, 4000, 0.05, 1.0, 0.5, 1e-12)
+    assert err <= 1e-12 and steps < 4000
+    assert np.abs(q - q_star).max() < 1e-11
+    assert abs(q.sum()) < 1e-12
+
+
+def test_relaxation_single_mode_is_damped_oscillator(oracle):
+    # two atoms, no Coulomb, chi = -2 q: the charge difference obeys
+    # q'' + g q' + (2/M) q = 0 in the small-timestep limit of the damped
+    # dynamics (relaxation.py:426-448); compare with the analytic solution
+    sp = np.array([0, 0], np.int32)
+    pd = np.array([[2, 1, 0, 0, 1.0, 2.0, 0, 0, -1, 0]], float)  # E = q^2, chi = -2q
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 5.0, 5.0]])
+    cell = np.eye(3) * 10.0
+    nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(2, 1.0))
+    dt, M, g, T = 0.002, 2.0, 0.3, 2.5
+    nstep = int(round(T / dt))
+    q, r, steps, err = oracle.relax_dynamic(sp, pos, np.array([0.3, -0.3]), cell, nl, pd, None,
+                                            nstep, dt, M, g, 1e-300)
+    w0 = math.sqrt(2.0 / M)
+    w = math.sqrt(w0 * w0 - g * g / 4)
+    x = 0.3 * math.exp(-g * T / 2) * (math.cos(w * T) + g / (2 * w) * math.sin(w * T))
+    assert steps == nstep
+    assert abs(q[0] - x) < 2e-3 * 0.3
+    assert abs(q[0] + q[1]) < 1e-14  # total charge is conserved
