@@ -102,7 +102,7 @@ struct SiteShared {
     int img[APB][SCAP];
     int spc[APB][SCAP];
     float P[APB][SCAP];       // 1/2 v_A f_c db~/dzeta for the bond (i,j)
-    float dpair[APB][SCAP];   // d/dr of 1/2 f_c (v_R + v_A b~) at fixed b~
+    double dpair[APB][SCAP];  // d/dr of 1/2 f_c (v_R + v_A b~) at fixed b~
     double red[APB][8];
 };
 
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
         }
         e_lane += eh;
         sh.P[a][jj] = (float)P;
-        sh.dpair[a][jj] = (float)dp;
+        sh.dpair[a][jj] = dp;
     }
     __syncthreads();
 
@@ -231,8 +231,7 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
         float inv_r1 = 1.0f / r1;
         float Pj = sh.P[a][jj];
         int sjp = sh.spc[a][jj];
-        float dp = sh.dpair[a][jj];
-        float cu = dp * inv_r1;  // coefficient of u
+        float cu = 0.f;  // three-body coefficient of u
         float dcx = 0.f, dcy = 0.f, dcz = 0.f;  // accumulated coefficient * dcos/du
         if (grp >= 0) {
             const float mb = (float)maxbond_i;
@@ -294,18 +293,22 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
                 cu += coef_u * inv_r1;
             }
         }
-        float Dx = cu * ux + dcx, Dy = cu * uy + dcy, Dz = cu * uz + dcz;
+        // pair part in FP64, three-body part from FP32
+        double cp = sh.dpair[a][jj] / ud.w;
+        double Dx = cp * ud.x + (double)(cu * ux + dcx);
+        double Dy = cp * ud.y + (double)(cu * uy + dcy);
+        double Dz = cp * ud.z + (double)(cu * uz + dcz);
         size_t o = (size_t)jj * n + i;
-        dU[o] = make_float4(Dx, Dy, Dz, 0.f);
+        dU[o] = make_float4((float)Dx, (float)Dy, (float)Dz, 0.f);
         sj[o] = sh.jdx[a][jj];
         simg[o] = sh.img[a][jj];
         // virial W = -sum r_ij (x) D_ij ; order xx yy zz yz xz xy
-        w6[0] -= ud.x * (double)Dx;
-        w6[1] -= ud.y * (double)Dy;
-        w6[2] -= ud.z * (double)Dz;
-        w6[3] -= ud.y * (double)Dz;
-        w6[4] -= ud.x * (double)Dz;
-        w6[5] -= ud.x * (double)Dy;
+        w6[0] -= ud.x * Dx;
+        w6[1] -= ud.y * Dy;
+        w6[2] -= ud.z * Dz;
+        w6[3] -= ud.y * Dz;
+        w6[4] -= ud.x * Dz;
+        w6[5] -= ud.x * Dy;
     }
     if (valid && l == 0) scnt[i] = ns;
 

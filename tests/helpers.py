@@ -32,5 +32,6 @@ def ewald_descriptors(g):
     return sp, pd, ew
 
 
-def rel_maxnorm(a, b):
-    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+def rel_maxnorm(a, b, floor=1e-300):
+    """max|a-b| / max(max|b|, floor)"""
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), floor))
