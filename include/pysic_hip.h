@@ -105,6 +105,11 @@ int pysic_set_kernel_variant(pysic_ctx *h, int variant);
 int pysic_kernel_variant_count(void);
 const char *pysic_kernel_variant_name(int variant);
 int pysic_set_neighbor_capacity(pysic_ctx *h, int cap);
+/* neighbor scan: 0 auto, 1 per-atom scan from global memory, 2 LDS-staged
+ * bricks, 3 bricks with no LDS room (the global-memory fallback of 2),
+ * 4 lane-parallel scan over x-contiguous bin columns, 5 per-atom scan with an
+ * FP32 prefilter and the exact FP64 test on survivors (0 selects 5) */
+int pysic_set_neighbor_method(pysic_ctx *h, int method);
 int pysic_timing(pysic_ctx *h, int on);
 /* per kernel family: name, accumulated ms, launch count; returns the count */
 int pysic_timing_read(pysic_ctx *h, int max, char *names, int name_len, double *ms, long *calls);

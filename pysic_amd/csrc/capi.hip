@@ -70,10 +70,11 @@ void System::release_all()
     for (auto *b : b4) b->release();
     DBuf<double> *bd[] = {&q, &q2, &cut, &cut2, &partial, &totals, &host_stage};
     for (auto *b : bd) b->release();
-    DBuf<int> *bi[] = {&id, &id2, &key, &wrap, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
+    DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
                        &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt};
     for (auto *b : bi) b->release();
     dU.release();
+    wpos.release();
     dmodel.release();
 }
 
@@ -328,12 +329,40 @@ void build_lists(pysic_ctx *h)
     throw PysicError{PYSIC_ECAP, "neighbor list capacity could not be established"};
 }
 
+// forces (+ optional momentum kick) and the virial into totals[1..6]
 void tersoff_forces(pysic_ctx *h, bool kick, double half_dt)
 {
     System &S = h->S;
-    ts_site(S, h->variant);
+    ts_site(S, h->variant, SITE_FORCES);
     ts_gather(S, half_dt, kick ? 1 : 0);
     ts_reduce(S);
+}
+
+// energy into totals[0]
+void tersoff_energy(pysic_ctx *h)
+{
+    System &S = h->S;
+    ts_site(S, h->variant, SITE_ENERGY);
+    ts_reduce(S);
+}
+
+// One evaluation for the API path: if an atom had more short neighbors than
+// the variant holds, switch to a variant that holds them and run again.
+void tersoff_checked(pysic_ctx *h, int mode)
+{
+    System &S = h->S;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
+        if (mode == SITE_ENERGY) tersoff_energy(h);
+        else tersoff_forces(h, false, 0.0);
+        int f[4];
+        nb_read_flags(S, f);
+        if (f[1] == 0) break;
+        int v = ts_variant_for_capacity(f[1]);
+        PYSIC_REQUIRE(v >= 0, PYSIC_ECAP,
+                      "more short-range neighbors (" + std::to_string(f[1]) + ") than any kernel holds");
+        h->variant = v;
+    }
 }
 
 }  // namespace
@@ -554,10 +583,9 @@ int pysic_calculate_energy(pysic_ctx *h, double *energy)
     PYSIC_REQUIRE(energy, PYSIC_EINVAL, "calculate_energy: null output");
     require_ready(h);
     System &S = h->S;
-    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     double e = 0.0;
     if (S.model.has_pairs) {
-        tersoff_forces(h, false, 0.0);
+        tersoff_checked(h, SITE_ENERGY);
         double t[8];
         HIP_TRY(hipMemcpyAsync(t, S.totals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));
@@ -575,10 +603,9 @@ int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress)
     require_ready(h);
     System &S = h->S;
     PYSIC_REQUIRE(n == S.n && forces && stress, PYSIC_EINVAL, "calculate_forces: bad arguments");
-    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (S.model.has_pairs) {
-        tersoff_forces(h, false, 0.0);
+        tersoff_checked(h, SITE_FORCES);
         HIP_TRY(hipMemcpyAsync(t, S.totals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));
         check_flags(h, false);
@@ -638,17 +665,21 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
     HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     DBuf<double> ebuf;
     if (energies) ebuf.ensure(nsteps > 0 ? nsteps : 1);
-    if (!h->forces_valid) tersoff_forces(h, false, 0.0);
+    if (!h->forces_valid) tersoff_checked(h, SITE_FORCES);
+    HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     const double half = 0.5 * dt;
     for (int step = 0; step < nsteps; ++step) {
-        md_drift(S, dt, half);
         if (rebuild > 0 && (step + 1) % rebuild == 0) {
-            nb_sort_atoms(S);
+            nb_sort_atoms(S, true, dt, half);  // drift fused with binning
             nb_build_lists(S);
+        } else {
+            md_drift(S, dt, half);
         }
         tersoff_forces(h, true, half);
-        if (energies)
+        if (energies) {
+            tersoff_energy(h);  // an observer: not part of the step itself
             k_copy_total<<<1, 64, 0, S.stream>>>(S.totals.p, ebuf.p + step);
+        }
     }
     HIP_TRY(hipGetLastError());
     if (energies && nsteps > 0)
@@ -692,6 +723,14 @@ int pysic_set_neighbor_capacity(pysic_ctx *h, int cap)
 {
     if (!h || cap < 1) return PYSIC_EINVAL;
     h->S.cap = cap;
+    h->lists_valid = false;
+    return PYSIC_OK;
+}
+
+int pysic_set_neighbor_method(pysic_ctx *h, int method)
+{
+    if (!h || method < 0 || method > 5) return PYSIC_EINVAL;
+    h->S.nb_method = method;
     h->lists_valid = false;
     return PYSIC_OK;
 }

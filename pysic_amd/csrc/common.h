@@ -76,6 +76,17 @@ __device__ __forceinline__ double3 separation64(const double4 &ri, const double4
     x = x + oz * v[6]; y = y + oz * v[7]; z = z + oz * v[8];
     return make_double3(x, y, z);
 }
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remap so that XCD x runs one contiguous range of logical blocks:
+// spatially adjacent atoms (the cell-sorted order) then share an L2.
+__device__ __forceinline__ int xcd_block(int nblocks)
+{
+    const int b = blockIdx.x;
+    const int x = b & 7, q = b >> 3;
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 // .o. (Quaternions.f90:475-483), no contraction
 __device__ __forceinline__ double dot64(const double3 &a)
 {

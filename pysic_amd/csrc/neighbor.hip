@@ -32,20 +32,17 @@ __device__ __forceinline__ void frac_of(const CellGeom &g, double x, double y, d
     rel[2] = g.inv[6] * x + g.inv[7] * y + g.inv[8] * z;
 }
 
-// Bin every atom: wrap offset -floor(frac) on periodic axes
+// Bin one atom: wrap offset -floor(frac) on periodic axes
 // (Geometry.f90:597-619) and the bin of the wrapped fractional coordinate.
-__global__ void k_bin(int n, CellGeom g, const double4 *__restrict__ pos, int *__restrict__ key,
-                      int *__restrict__ wrap, int *__restrict__ cellcount, int *__restrict__ flags)
+// Non-finite or far-away coordinates are flagged and binned at 0.
+__device__ __forceinline__ void bin_atom(const CellGeom &g, const double4 &p, int *key, int *wrap,
+                                         bool *bad_out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 p = pos[i];
     double rel[3];
     frac_of(g, p.x, p.y, p.z, rel);
     int w[3], c[3];
     bool bad = false;
     for (int a = 0; a < 3; ++a) {
-        // non-finite or far-away coordinates are flagged and binned at 0
         if (g.pbc[a]) {
             double fl = floor(rel[a]);
             bool ok = fabs(fl) < (double)(PYSIC_IMG_MAX / 2);
@@ -63,10 +60,52 @@ __global__ void k_bin(int n, CellGeom g, const double4 *__restrict__ pos, int *_
             c[a] = (int)fb;
         }
     }
+    *key = c[0] + g.dim[0] * (c[1] + g.dim[1] * c[2]);
+    *wrap = pack_img(w[0], w[1], w[2]);
+    *bad_out = bad;
+}
+
+__global__ void k_bin(int n, CellGeom g, const double4 *__restrict__ pos, int *__restrict__ key,
+                      int *__restrict__ wrap, int *__restrict__ cellcount, int *__restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k, w;
+    bool bad;
+    bin_atom(g, pos[i], &k, &w, &bad);
     if (bad) atomicOr(flags + 2, 1);
-    int k = c[0] + g.dim[0] * (c[1] + g.dim[1] * c[2]);
     key[i] = k;
-    wrap[i] = pack_img(w[0], w[1], w[2]);
+    wrap[i] = w;
+    atomicAdd(cellcount + k, 1);
+}
+
+// velocity-Verlet first half (p += dt/2 F; r += dt p/m) fused with binning
+__global__ void k_drift_bin(int n, CellGeom g, double dt, double half_dt,
+                            const double4 *__restrict__ frc, double4 *__restrict__ mom,
+                            double4 *__restrict__ pos, int *__restrict__ key,
+                            int *__restrict__ wrap, int *__restrict__ cellcount,
+                            int *__restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 f = frc[i];
+    double4 p = mom[i];
+    p.x += half_dt * f.x;
+    p.y += half_dt * f.y;
+    p.z += half_dt * f.z;
+    mom[i] = p;
+    double4 r = pos[i];
+    double s = dt / p.w;  // p.w carries the mass
+    r.x += s * p.x;
+    r.y += s * p.y;
+    r.z += s * p.z;
+    pos[i] = r;
+    int k, w;
+    bool bad;
+    bin_atom(g, r, &k, &w, &bad);
+    if (bad) atomicOr(flags + 2, 1);
+    key[i] = k;
+    wrap[i] = w;
     atomicAdd(cellcount + k, 1);
 }
 
@@ -165,26 +204,120 @@ __global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__rest
     }
 }
 
+// Forces are not carried: after the drift they are dead until recomputed.
 __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__restrict__ pos,
-                          const double4 *__restrict__ mom, const double4 *__restrict__ frc,
-                          const double *__restrict__ q, const double *__restrict__ cut,
-                          const int *__restrict__ id, const int *__restrict__ key,
-                          const int *__restrict__ wrap, double4 *__restrict__ pos2,
-                          double4 *__restrict__ mom2, double4 *__restrict__ frc2,
+                          const double4 *__restrict__ mom, const double *__restrict__ q,
+                          const double *__restrict__ cut, const int *__restrict__ id,
+                          const int *__restrict__ key, const int *__restrict__ wrap,
+                          double4 *__restrict__ pos2, double4 *__restrict__ mom2,
                           double *__restrict__ q2, double *__restrict__ cut2,
-                          int *__restrict__ id2, int *__restrict__ key2, int *__restrict__ wrap2)
+                          int *__restrict__ id2, int *__restrict__ key2, int *__restrict__ wrap2,
+                          CellGeom g, float4 *__restrict__ wpos)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     int o = perm[s];
-    pos2[s] = pos[o];
+    double4 p = pos[o];
+    int w = wrap[o];
+    pos2[s] = p;
     mom2[s] = mom[o];
-    frc2[s] = frc[o];
     q2[s] = q[o];
     cut2[s] = cut[o];
     id2[s] = id[o];
     key2[s] = key[o];
-    wrap2[s] = wrap[o];
+    wrap2[s] = w;
+    // wrapped position in FP32 for the neighbor prefilter
+    double wx = img_x(w), wy = img_y(w), wz = img_z(w);
+    wpos[s] = make_float4((float)(p.x + wx * g.v[0] + wy * g.v[3] + wz * g.v[6]),
+                          (float)(p.y + wx * g.v[1] + wy * g.v[4] + wz * g.v[7]),
+                          (float)(p.z + wx * g.v[2] + wy * g.v[5] + wz * g.v[8]), 0.f);
+}
+
+// Per-atom scan with an FP32 prefilter: candidates are first tested on
+// FP32 wrapped positions (16 B per candidate) against cutoff^2 + margin2,
+// where margin2 bounds the FP32 rounding error of the squared distance; only
+// the survivors take the exact FP64 test in the reference's operation order,
+// so the accepted set is the exact one.  The x-adjacent bins of a column are
+// consecutive in memory, so each column costs four bin-start loads.
+__global__ __launch_bounds__(256) void k_neighbors_pf(int n, CellGeom g,
+                                                      const double4 *__restrict__ pos,
+                                                      const float4 *__restrict__ wpos,
+                                                      const int *__restrict__ key,
+                                                      const int *__restrict__ wrap,
+                                                      const int *__restrict__ start,
+                                                      const double *__restrict__ cut, int cap,
+                                                      float margin2, int *__restrict__ nbr,
+                                                      int *__restrict__ nimg,
+                                                      int *__restrict__ ncnt,
+                                                      int *__restrict__ flags)
+{
+    int s = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double4 ps = pos[s];
+    const float4 qs = wpos[s];
+    const int k = key[s];
+    const int c0 = k % g.dim[0];
+    const int c1 = (k / g.dim[0]) % g.dim[1];
+    const int c2 = k / (g.dim[0] * g.dim[1]);
+    const int ws = wrap[s];
+    const int wsx = img_x(ws), wsy = img_y(ws), wsz = img_z(ws);
+    const double cs = cut[s];
+    const double c2lim = cs * cs;
+    const float c2f = (float)c2lim + margin2;
+    const float v0x = (float)g.v[0], v0y = (float)g.v[1], v0z = (float)g.v[2];
+    const float v1x = (float)g.v[3], v1y = (float)g.v[4], v1z = (float)g.v[5];
+    const float v2x = (float)g.v[6], v2y = (float)g.v[7], v2z = (float)g.v[8];
+    int count = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int bz = c2 + dz, sz = 0;
+        if (g.pbc[2]) {
+            if (bz < 0) { bz += g.nc[2]; sz = -1; }
+            else if (bz >= g.nc[2]) { bz -= g.nc[2]; sz = 1; }
+        } else if (bz < 0 || bz >= g.dim[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int by = c1 + dy, sy = 0;
+            if (g.pbc[1]) {
+                if (by < 0) { by += g.nc[1]; sy = -1; }
+                else if (by >= g.nc[1]) { by -= g.nc[1]; sy = 1; }
+            } else if (by < 0 || by >= g.dim[1]) continue;
+            const int row = g.dim[0] * (by + g.dim[1] * bz);
+            const float syf = (float)sy, szf = (float)sz;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int bx = c0 + dx, sx = 0;
+                if (g.pbc[0]) {
+                    if (bx < 0) { bx += g.nc[0]; sx = -1; }
+                    else if (bx >= g.nc[0]) { bx -= g.nc[0]; sx = 1; }
+                } else if (bx < 0 || bx >= g.dim[0]) continue;
+                const float sxf = (float)sx;
+                // image shift of this bin, FP32
+                const float shx = sxf * v0x + syf * v1x + szf * v2x - qs.x;
+                const float shy = sxf * v0y + syf * v1y + szf * v2y - qs.y;
+                const float shz = sxf * v0z + syf * v1z + szf * v2z - qs.z;
+                const int t1 = start[row + bx + 1];
+                for (int t = start[row + bx]; t < t1; ++t) {
+                    float4 qt = wpos[t];
+                    float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
+                    if (!(fx * fx + fy * fy + fz * fz < c2f)) continue;
+                    int wt = wrap[t];
+                    int ox = sx + img_x(wt) - wsx;
+                    int oy = sy + img_y(wt) - wsy;
+                    int oz = sz + img_z(wt) - wsz;
+                    if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
+                    int im = pack_img(ox, oy, oz);
+                    double d2 = dot64(separation64(ps, pos[t], im, g.v));
+                    if (d2 < c2lim) {
+                        if (count < cap) {
+                            nbr[(size_t)count * n + s] = t;
+                            nimg[(size_t)count * n + s] = im;
+                        }
+                        ++count;
+                    }
+                }
+            }
+        }
+    }
+    ncnt[s] = count < cap ? count : cap;
+    if (count > cap) atomicMax(flags, count);
 }
 
 // Per-atom scan of the 27 adjacent bins (Core.f90:8182-8252 as a full list).
@@ -196,7 +329,7 @@ __global__ __launch_bounds__(256) void k_neighbors(int n, CellGeom g, const doub
                                                    int *__restrict__ nbr, int *__restrict__ nimg,
                                                    int *__restrict__ ncnt, int *__restrict__ flags)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double4 ps = pos[s];
     int k = key[s];
@@ -251,6 +384,312 @@ __global__ __launch_bounds__(256) void k_neighbors(int n, CellGeom g, const doub
     if (count > cap) atomicMax(flags, count);
 }
 
+
+// ---------------------------------------------------------------------------
+// LDS-staged brick scan.  One workgroup owns a brick of Bx*By*Bz bins; the
+// positions of every atom in the brick and its one-bin halo are loaded into
+// LDS once (coalesced, each halo atom ~3x instead of once per neighbor
+// candidate), then every atom of the brick scans its 27 bins out of LDS.
+// Bricks whose halo exceeds the LDS capacity take the same scan from global
+// memory, so the result never depends on the capacity.
+// ---------------------------------------------------------------------------
+struct BrickCfg {
+    int B[3];       // bins per brick edge
+    int nb[3];      // bricks per axis
+    int lcap;       // halo atoms held in LDS
+};
+
+__device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__device__ __forceinline__ int upper_bound_idx(const int *arr, int len, int x)
+{
+    // largest h in [0, len) with arr[h] <= x (arr ascending, arr[0] = 0)
+    int lo = 0, hi = len;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (arr[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ int block_scan_inplace(int *arr, int len, int *lds_tmp)
+{
+    // exclusive scan of arr[0..len) in place; returns the total; blockDim 256
+    int carry = 0;
+    for (int base = 0; base < len; base += SCAN_T) {
+        int t = base + threadIdx.x;
+        int v = t < len ? arr[t] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, lds_tmp, &tot);
+        if (t < len) arr[t] = ex + carry;
+        carry += tot;
+        __syncthreads();
+    }
+    return carry;
+}
+
+__global__ __launch_bounds__(256) void k_neighbors_brick(int n, CellGeom g, BrickCfg bc,
+                                                         const double4 *__restrict__ pos,
+                                                         const int *__restrict__ wrap,
+                                                         const int *__restrict__ start,
+                                                         const double *__restrict__ cut, int cap,
+                                                         int *__restrict__ nbr,
+                                                         int *__restrict__ nimg,
+                                                         int *__restrict__ ncnt,
+                                                         int *__restrict__ flags)
+{
+    extern __shared__ double4 smem4[];
+    const int Hx = bc.B[0] + 2, Hy = bc.B[1] + 2, Hz = bc.B[2] + 2;
+    const int H = Hx * Hy * Hz;
+    const int C = bc.B[0] * bc.B[1] * bc.B[2];
+    double4 *s_pos = smem4;                              // [lcap]
+    int *s_tag = (int *)(s_pos + bc.lcap);               // [lcap] shift + wrap, packed
+    int *s_gid = s_tag + bc.lcap;                        // [lcap]
+    int *s_hstart = s_gid + bc.lcap;                     // [H]
+    int *s_hcnt = s_hstart + H;                          // [H]
+    int *s_hloc = s_hcnt + H;                            // [H + 1]
+    int *s_hshift = s_hloc + H + 1;                      // [H]
+    int *s_cloc = s_hshift + H;                          // [C + 1]
+    __shared__ int s_tmp[SCAN_T];
+
+    int b = blockIdx.x;
+    int b0 = b % bc.nb[0], b1 = (b / bc.nb[0]) % bc.nb[1], b2 = b / (bc.nb[0] * bc.nb[1]);
+    int o0 = b0 * bc.B[0], o1 = b1 * bc.B[1], o2 = b2 * bc.B[2];
+
+    // ---- halo bins: global bin, image shift, atom range -------------------------
+    for (int h = threadIdx.x; h < H; h += blockDim.x) {
+        int hx = h % Hx, hy = (h / Hx) % Hy, hz = h / (Hx * Hy);
+        int gc[3] = {o0 + hx - 1, o1 + hy - 1, o2 + hz - 1};
+        int sh[3] = {0, 0, 0};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            if (g.pbc[a]) {
+                sh[a] = floor_div(gc[a], g.nc[a]);
+                gc[a] -= sh[a] * g.nc[a];
+            } else if (gc[a] < 0 || gc[a] >= g.dim[a]) {
+                ok = false;
+            }
+        }
+        int cnt = 0, st0 = 0;
+        if (ok) {
+            int cell = gc[0] + g.dim[0] * (gc[1] + g.dim[1] * gc[2]);
+            st0 = start[cell];
+            cnt = start[cell + 1] - st0;
+        }
+        s_hstart[h] = st0;
+        s_hcnt[h] = cnt;
+        s_hloc[h] = cnt;
+        s_hshift[h] = pack_img(sh[0], sh[1], sh[2]);
+    }
+    // center bins: real only inside the grid (no wrap)
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        int cx = c % bc.B[0], cy = (c / bc.B[0]) % bc.B[1], cz = c / (bc.B[0] * bc.B[1]);
+        bool real = o0 + cx < g.dim[0] && o1 + cy < g.dim[1] && o2 + cz < g.dim[2];
+        int h = (cx + 1) + Hx * ((cy + 1) + Hy * (cz + 1));
+        s_cloc[c] = real ? 0 : -1;  // marker, filled below
+        (void)h;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        int cx = c % bc.B[0], cy = (c / bc.B[0]) % bc.B[1], cz = c / (bc.B[0] * bc.B[1]);
+        int h = (cx + 1) + Hx * ((cy + 1) + Hy * (cz + 1));
+        s_cloc[c] = s_cloc[c] < 0 ? 0 : s_hcnt[h];
+    }
+    __syncthreads();
+    int T = block_scan_inplace(s_hloc, H, s_tmp);
+    int Cn = block_scan_inplace(s_cloc, C, s_tmp);
+    if (threadIdx.x == 0) {
+        s_hloc[H] = T;
+        s_cloc[C] = Cn;
+    }
+    __syncthreads();
+    const bool in_lds = T <= bc.lcap;
+
+    // ---- stage the halo atoms ---------------------------------------------------
+    if (in_lds) {
+        for (int x = threadIdx.x; x < T; x += blockDim.x) {
+            int h = upper_bound_idx(s_hloc, H, x);
+            // skip empty bins that share the same offset
+            while (h + 1 < H && s_hloc[h + 1] <= x) ++h;
+            int t = s_hstart[h] + (x - s_hloc[h]);
+            int sh = s_hshift[h], wt = wrap[t];
+            s_pos[x] = pos[t];
+            s_tag[x] = pack_img(img_x(sh) + img_x(wt), img_y(sh) + img_y(wt), img_z(sh) + img_z(wt));
+            s_gid[x] = t;
+        }
+    }
+    __syncthreads();
+
+    // ---- every atom of the brick scans its 27 bins ------------------------------
+    for (int x = threadIdx.x; x < Cn; x += blockDim.x) {
+        int c = upper_bound_idx(s_cloc, C, x);
+        while (c + 1 < C && s_cloc[c + 1] <= x) ++c;
+        int cx = c % bc.B[0], cy = (c / bc.B[0]) % bc.B[1], cz = c / (bc.B[0] * bc.B[1]);
+        int h = (cx + 1) + Hx * ((cy + 1) + Hy * (cz + 1));
+        int q = x - s_cloc[c];
+        int s = s_hstart[h] + q;
+        double4 ps = in_lds ? s_pos[s_hloc[h] + q] : pos[s];
+        int ws = wrap[s];
+        int wsx = img_x(ws), wsy = img_y(ws), wsz = img_z(ws);
+        double cs = cut[s];
+        double c2lim = cs * cs;
+        int count = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int hn = h + dx + Hx * (dy + Hy * dz);
+                    int m = s_hcnt[hn];
+                    if (in_lds) {
+                        int l0 = s_hloc[hn];
+                        for (int u = 0; u < m; ++u) {
+                            int tag = s_tag[l0 + u];
+                            int t = s_gid[l0 + u];
+                            int ox = img_x(tag) - wsx, oy = img_y(tag) - wsy, oz = img_z(tag) - wsz;
+                            if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
+                            int im = pack_img(ox, oy, oz);
+                            double d2 = dot64(separation64(ps, s_pos[l0 + u], im, g.v));
+                            if (d2 < c2lim) {
+                                if (count < cap) {
+                                    nbr[(size_t)count * n + s] = t;
+                                    nimg[(size_t)count * n + s] = im;
+                                }
+                                ++count;
+                            }
+                        }
+                    } else {
+                        int sh = s_hshift[hn];
+                        int t0 = s_hstart[hn];
+                        for (int u = 0; u < m; ++u) {
+                            int t = t0 + u;
+                            int wt = wrap[t];
+                            int ox = img_x(sh) + img_x(wt) - wsx;
+                            int oy = img_y(sh) + img_y(wt) - wsy;
+                            int oz = img_z(sh) + img_z(wt) - wsz;
+                            if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
+                            int im = pack_img(ox, oy, oz);
+                            double d2 = dot64(separation64(ps, pos[t], im, g.v));
+                            if (d2 < c2lim) {
+                                if (count < cap) {
+                                    nbr[(size_t)count * n + s] = t;
+                                    nimg[(size_t)count * n + s] = im;
+                                }
+                                ++count;
+                            }
+                        }
+                    }
+                }
+        ncnt[s] = count < cap ? count : cap;
+        if (count > cap) atomicMax(flags, count);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lane-parallel scan: G lanes per atom.  The three bins of each (dy, dz)
+// column are consecutive in the cell-sorted atom array (bin index runs x
+// fastest), so every column is at most three contiguous segments; the lanes
+// of a group test consecutive candidates of a segment in parallel and the
+// hits are compacted with a ballot, which keeps the sequential (dz, dy, dx,
+// atom) order of the per-atom scan.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ __launch_bounds__(256) void k_neighbors_group(int n, CellGeom g,
+                                                         const double4 *__restrict__ pos,
+                                                         const int *__restrict__ key,
+                                                         const int *__restrict__ wrap,
+                                                         const int *__restrict__ start,
+                                                         const double *__restrict__ cut, int cap,
+                                                         int *__restrict__ nbr,
+                                                         int *__restrict__ nimg,
+                                                         int *__restrict__ ncnt,
+                                                         int *__restrict__ flags)
+{
+    constexpr int APB = 256 / G;
+    const int a = threadIdx.x / G;
+    const int l = threadIdx.x % G;
+    const int s = blockIdx.x * APB + a;
+    const bool valid = s < n;
+    const int gbase = ((threadIdx.x & 63) / G) * G;
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    const unsigned long long below = (1ull << l) - 1ull;
+
+    double4 ps = valid ? pos[s] : make_double4(0, 0, 0, 0);
+    int k = valid ? key[s] : 0;
+    int c0 = k % g.dim[0];
+    int c1 = (k / g.dim[0]) % g.dim[1];
+    int c2 = k / (g.dim[0] * g.dim[1]);
+    int ws = valid ? wrap[s] : 0;
+    int wsx = img_x(ws), wsy = img_y(ws), wsz = img_z(ws);
+    double csq = valid ? cut[s] : 0.0;
+    csq = csq * csq;
+    int count = 0;
+    for (int dz = valid ? -1 : 2; dz <= 1; ++dz) {
+        int bz = c2 + dz, sz = 0;
+        if (g.pbc[2]) {
+            if (bz < 0) { bz += g.nc[2]; sz = -1; }
+            else if (bz >= g.nc[2]) { bz -= g.nc[2]; sz = 1; }
+        } else if (bz < 0 || bz >= g.dim[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int by = c1 + dy, sy = 0;
+            if (g.pbc[1]) {
+                if (by < 0) { by += g.nc[1]; sy = -1; }
+                else if (by >= g.nc[1]) { by -= g.nc[1]; sy = 1; }
+            } else if (by < 0 || by >= g.dim[1]) continue;
+            int row = g.dim[0] * (by + g.dim[1] * bz);
+            // up to three segments of consecutive atoms sharing one x shift
+            int seg_b[3], seg_e[3], seg_s[3], nseg = 0;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int bx = c0 + dx, sx = 0;
+                if (g.pbc[0]) {
+                    if (bx < 0) { bx += g.nc[0]; sx = -1; }
+                    else if (bx >= g.nc[0]) { bx -= g.nc[0]; sx = 1; }
+                } else if (bx < 0 || bx >= g.dim[0]) continue;
+                int b0 = start[row + bx], b1 = start[row + bx + 1];
+                if (nseg > 0 && seg_e[nseg - 1] == b0 && seg_s[nseg - 1] == sx) {
+                    seg_e[nseg - 1] = b1;
+                } else {
+                    seg_b[nseg] = b0;
+                    seg_e[nseg] = b1;
+                    seg_s[nseg] = sx;
+                    ++nseg;
+                }
+            }
+            for (int q = 0; q < nseg; ++q) {
+                int e1 = seg_e[q];
+                int sxq = seg_s[q];
+                for (int t0 = seg_b[q]; t0 < e1; t0 += G) {
+                    int t = t0 + l;
+                    bool hit = false;
+                    int im = 0;
+                    if (t < e1) {
+                        int wt = wrap[t];
+                        int ox = sxq + img_x(wt) - wsx;
+                        int oy = sy + img_y(wt) - wsy;
+                        int oz = sz + img_z(wt) - wsz;
+                        im = pack_img(ox, oy, oz);
+                        if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
+                            double d2 = dot64(separation64(ps, pos[t], im, g.v));
+                            hit = d2 < csq;
+                        }
+                    }
+                    unsigned long long b = __ballot(hit);
+                    unsigned long long gb = (b >> gbase) & gmask;
+                    int slot = count + __popcll(gb & below);
+                    if (hit && slot < cap) {
+                        nbr[(size_t)slot * n + s] = t;
+                        nimg[(size_t)slot * n + s] = im;
+                    }
+                    count += __popcll(gb);
+                }
+            }
+        }
+    }
+    if (valid && l == 0) {
+        ncnt[s] = count < cap ? count : cap;
+        if (count > cap) atomicMax(flags, count);
+    }
+}
+
 }  // namespace
 
 // Bins per axis: floor(perpendicular width / max_cut) as in
@@ -287,65 +726,131 @@ static void swap_bufs(System &S)
 {
     std::swap(S.pos, S.pos2);
     std::swap(S.mom, S.mom2);
-    std::swap(S.frc, S.frc2);
     std::swap(S.q, S.q2);
     std::swap(S.cut, S.cut2);
     std::swap(S.id, S.id2);
+    std::swap(S.key, S.key2);
+    std::swap(S.wrap, S.wrap2);
 }
 
-void nb_sort_atoms(System &S)
+void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
 {
     const int n = S.n;
     CellGeom &g = S.geom;
     hipStream_t st = S.stream;
-    S.key.ensure(2 * (size_t)n);  // second half: keys after the permutation
-    S.wrap.ensure(2 * (size_t)n);
+    S.key.ensure(n); S.key2.ensure(n);
+    S.wrap.ensure(n); S.wrap2.ensure(n);
     S.perm.ensure(n);
     S.cellcount.ensure(g.ncell);
     S.cellfill.ensure(g.ncell);
     S.cellstart.ensure((size_t)g.ncell + 1);
     int nb = div_up(g.ncell, SCAN_BLK);
     S.scanblk.ensure(nb);
-    S.pos2.ensure(n); S.mom2.ensure(n); S.frc2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
+    S.pos2.ensure(n); S.mom2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
     S.id2.ensure(n);
+    S.wpos.ensure(n);
     S.flags.ensure(4);
 
     hipEvent_t ev;
-    S.timer.begin("bin_sort", st, &ev);
+    S.timer.begin(drift ? "drift_bin_sort" : "bin_sort", st, &ev);
     HIP_TRY(hipMemsetAsync(S.cellcount.p, 0, sizeof(int) * g.ncell, st));
     HIP_TRY(hipMemsetAsync(S.cellfill.p, 0, sizeof(int) * g.ncell, st));
-    int *key_old = S.key.p, *key_new = S.key.p + n;
-    int *wrap_old = S.wrap.p, *wrap_new = S.wrap.p + n;
-    k_bin<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, key_old, wrap_old, S.cellcount.p, S.flags.p);
+    if (drift)
+        k_drift_bin<<<div_up(n, 256), 256, 0, st>>>(n, g, dt, half_dt, S.frc.p, S.mom.p, S.pos.p,
+                                                    S.key.p, S.wrap.p, S.cellcount.p, S.flags.p);
+    else
+        k_bin<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.key.p, S.wrap.p, S.cellcount.p,
+                                              S.flags.p);
     k_scan_local<<<nb, SCAN_T, 0, st>>>(g.ncell, S.cellcount.p, S.cellstart.p, S.scanblk.p);
     k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, g.ncell);
     k_scan_add<<<nb, SCAN_T, 0, st>>>(g.ncell, S.cellstart.p, S.scanblk.p);
-    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, key_old, S.cellstart.p, S.cellfill.p, S.perm.p);
+    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.cellstart.p, S.cellfill.p, S.perm.p);
     k_cellsort<<<div_up(g.ncell, 256), 256, 0, st>>>(g.ncell, S.cellstart.p, S.perm.p, S.id.p);
-    k_permute<<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.frc.p, S.q.p, S.cut.p,
-                                              S.id.p, key_old, wrap_old, S.pos2.p, S.mom2.p,
-                                              S.frc2.p, S.q2.p, S.cut2.p, S.id2.p, key_new,
-                                              wrap_new);
+    k_permute<<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p, S.cut.p, S.id.p,
+                                              S.key.p, S.wrap.p, S.pos2.p, S.mom2.p, S.q2.p,
+                                              S.cut2.p, S.id2.p, S.key2.p, S.wrap2.p, g, S.wpos.p);
     HIP_TRY(hipGetLastError());
-    S.timer.end("bin_sort", st, ev);
+    S.timer.end(drift ? "drift_bin_sort" : "bin_sort", st, ev);
     swap_bufs(S);
-    // keep the sorted keys/wraps in the first half
-    HIP_TRY(hipMemcpyAsync(key_old, key_new, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(wrap_old, wrap_new, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
 }
 
 void nb_build_lists(System &S)
 {
     const int n = S.n;
+    const CellGeom &g = S.geom;
     hipStream_t st = S.stream;
     S.nbr.ensure((size_t)S.cap * n);
     S.nimg.ensure((size_t)S.cap * n);
     S.ncnt.ensure(n);
+    double interior = (double)g.nc[0] * g.nc[1] * g.nc[2];
+    double apb = n / interior;  // atoms per bin
+    bool brick = S.nb_method == 2 || S.nb_method == 3;
     hipEvent_t ev;
     S.timer.begin("neighbors", st, &ev);
-    k_neighbors<<<div_up(n, 256), 256, 0, st>>>(n, S.geom, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p,
-                                                 S.cut.p, S.cap, S.nbr.p, S.nimg.p, S.ncnt.p,
-                                                 S.flags.p);
+    if (S.nb_method == 0 || S.nb_method == 5) {
+        // FP32 prefilter margin: bound on the rounding of the FP32 squared
+        // distance of wrapped positions (|error| per component <= ~1.2e-6 L,
+        // taken 2.5x larger); open axes leave positions unbounded: no prefilter
+        float margin2 = 3.0e38f;
+        if (g.pbc[0] && g.pbc[1] && g.pbc[2]) {
+            double L = 0.0;
+            for (int a = 0; a < 3; ++a)
+                L += std::sqrt(g.v[3 * a] * g.v[3 * a] + g.v[3 * a + 1] * g.v[3 * a + 1] +
+                               g.v[3 * a + 2] * g.v[3 * a + 2]);
+            double delta = 3.0e-6 * L;
+            margin2 = (float)(4.0 * std::sqrt(3.0) * S.max_cut * delta + 12.0 * delta * delta);
+        }
+        k_neighbors_pf<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.wpos.p, S.key.p, S.wrap.p,
+                                                       S.cellstart.p, S.cut.p, S.cap, margin2,
+                                                       S.nbr.p, S.nimg.p, S.ncnt.p, S.flags.p);
+    } else if (S.nb_method == 4) {
+        // lane-parallel scan; wider groups when the bins are dense
+        if (apb * 3.0 <= 10.0)
+            k_neighbors_group<8><<<div_up(n, 256 / 8), 256, 0, st>>>(
+                n, g, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap, S.nbr.p, S.nimg.p,
+                S.ncnt.p, S.flags.p);
+        else
+            k_neighbors_group<32><<<div_up(n, 256 / 32), 256, 0, st>>>(
+                n, g, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap, S.nbr.p, S.nimg.p,
+                S.ncnt.p, S.flags.p);
+    } else if (!brick) {
+        k_neighbors<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p,
+                                                     S.cut.p, S.cap, S.nbr.p, S.nimg.p, S.ncnt.p,
+                                                     S.flags.p);
+    } else {
+        // grow the brick one bin-edge at a time (smallest edge first) while
+        // its expected atom count stays within one pass of 256 threads
+        BrickCfg bc;
+        int B[3] = {1, 1, 1};
+        for (;;) {
+            int a = 0;
+            for (int q = 1; q < 3; ++q)
+                if (B[q] < B[a]) a = q;
+            if (B[a] >= g.dim[a] || B[a] >= 8) break;
+            double cexp = (double)B[0] * B[1] * B[2] / B[a] * (B[a] + 1) * apb * 1.15;
+            if (cexp > 256.0) break;
+            B[a] += 1;
+        }
+        for (int a = 0; a < 3; ++a) {
+            bc.B[a] = B[a];
+            bc.nb[a] = div_up(g.dim[a], bc.B[a]);
+        }
+        int H = (bc.B[0] + 2) * (bc.B[1] + 2) * (bc.B[2] + 2);
+        int C = bc.B[0] * bc.B[1] * bc.B[2];
+        size_t fixed = (size_t)(4 * H + C + 2) * sizeof(int);
+        size_t per_atom = sizeof(double4) + 2 * sizeof(int);
+        long room = (long)((60 * 1024 - (long)fixed) / (long)per_atom);  // stay under 64 KB
+        int lcap = (int)(H * apb * 1.6) + 64;
+        lcap = ((lcap + 63) / 64) * 64;
+        lcap = (int)std::max(0L, std::min((long)lcap, room));
+        if (S.nb_method == 3) lcap = 8;  // exercise the global-memory fallback
+        bc.lcap = lcap;
+        size_t smem = (size_t)lcap * per_atom + fixed;
+        int nbricks = bc.nb[0] * bc.nb[1] * bc.nb[2];
+        k_neighbors_brick<<<nbricks, 256, smem, st>>>(n, g, bc, S.pos.p, S.wrap.p, S.cellstart.p,
+                                                      S.cut.p, S.cap, S.nbr.p, S.nimg.p, S.ncnt.p,
+                                                      S.flags.p);
+    }
     HIP_TRY(hipGetLastError());
     S.timer.end("neighbors", st, ev);
 }

@@ -57,14 +57,17 @@ struct System {
     DBuf<double> q, q2;         // charges
     DBuf<double> cut, cut2;     // neighbor-search radius per atom (cutoff + skin)
     DBuf<int> id, id2;          // original index of the atom in this slot
-    DBuf<int> key;              // cell of each atom (current order)
-    DBuf<int> wrap;             // packed wrap offsets (current order)
+    DBuf<int> key, key2;        // cell of each atom (current order)
+    DBuf<int> wrap, wrap2;      // packed wrap offsets (current order)
     DBuf<int> perm;             // new slot -> old slot
+    DBuf<float4> wpos;          // wrapped positions, FP32 (neighbor prefilter)
     DBuf<int> cellcount, cellstart, cellfill, scanblk;
     double max_cut = 0.0;
 
     // neighbor lists (column-major: entry e of atom i at e*n + i)
     int cap = 0;
+    int nb_method = 0;          // 0 auto (= 5), 1 global scan, 2 LDS bricks, 3 bricks w/o LDS room,
+                                // 4 lane groups, 5 FP32-prefiltered scan
     DBuf<int> nbr, nimg, ncnt;
     DBuf<int> flags;            // [0] max list length, [1] max short length, [2] wrap range, [3] unpaired bond
 
@@ -90,17 +93,23 @@ struct System {
 
 // neighbor.hip
 void nb_configure_grid(System &S, double max_cut);
-void nb_sort_atoms(System &S);                // bin + counting sort + permute
+void nb_sort_atoms(System &S, bool drift = false, double dt = 0.0,
+                   double half_dt = 0.0);  // [drift +] bin + counting sort + permute
 void nb_build_lists(System &S);               // cell-list search, reference semantics
 int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
 
 // tersoff.hip
-void ts_site(System &S, int variant);         // site energies, dU/dr_ij, virial partials
+// SITE_ENERGY: energy only (FP64); SITE_FORCES: dU/dr_ij + virial (the
+// multi-species kernel computes all three in either mode)
+enum { SITE_ENERGY = 1, SITE_FORCES = 2 };
+void ts_site(System &S, int variant, int mode);
 void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
 void ts_reduce(System &S);                    // totals[0..6] = E, W(6)
 int ts_variant_count();
 const char *ts_variant_name(int v);
+int ts_variant_capacity(int v);
+int ts_variant_for_capacity(int need);
 
 // md.hip
-void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m
+void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m (no rebuild)
 void sys_scatter_original(System &S, const double4 *src, double4 *dst);  // dst[id[s]] = src[s]

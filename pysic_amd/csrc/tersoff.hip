@@ -97,12 +97,13 @@ __device__ __forceinline__ bool pick_dev(int i1, int i2, int img)
 template <int G, int SCAP>
 struct SiteShared {
     static constexpr int APB = BLOCK / G;
-    double4 sep[APB][SCAP];   // r_ij (FP64) and |r_ij|
-    int jdx[APB][SCAP];
-    int img[APB][SCAP];
-    int spc[APB][SCAP];
-    float P[APB][SCAP];       // 1/2 v_A f_c db~/dzeta for the bond (i,j)
-    double dpair[APB][SCAP];  // d/dr of 1/2 f_c (v_R + v_A b~) at fixed b~
+    // [slot][atom]: the atoms of a wave sit on consecutive LDS banks
+    double4 sep[SCAP][APB];   // r_ij (FP64) and |r_ij|
+    int jdx[SCAP][APB];
+    int img[SCAP][APB];
+    int spc[SCAP][APB];
+    float P[SCAP][APB];       // 1/2 v_A f_c db~/dzeta for the bond (i,j)
+    double dpair[SCAP][APB];  // d/dr of 1/2 f_c (v_R + v_A b~) at fixed b~
     double red[APB][8];
 };
 
@@ -123,7 +124,8 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
     __shared__ SH sh;
     const int a = threadIdx.x / G;
     const int l = threadIdx.x % G;
-    const int i = blockIdx.x * APB + a;
+    const int blk = xcd_block(gridDim.x);
+    const int i = blk * APB + a;
     const bool valid = i < n;
     const int wl = threadIdx.x & 63;
     const int gbase = (wl / G) * G;
@@ -157,10 +159,10 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
         int rank = __popcll(gb & ((1ull << l) - 1ull));
         int slot = cnt + rank;
         if (keep && slot < SCAP) {
-            sh.sep[a][slot] = d;
-            sh.jdx[a][slot] = j;
-            sh.img[a][slot] = im;
-            sh.spc[a][slot] = sp;
+            sh.sep[slot][a] = d;
+            sh.jdx[slot][a] = j;
+            sh.img[slot][a] = im;
+            sh.spc[slot][a] = sp;
         }
         cnt += __popcll(gb);
     }
@@ -172,15 +174,15 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
     double e_lane = 0.0;
     const int grp = M->group;
     for (int jj = l; jj < ns; jj += G) {
-        double4 u = sh.sep[a][jj];
-        int sjp = sh.spc[a][jj];
+        double4 u = sh.sep[jj][a];
+        int sjp = sh.spc[jj][a];
         double zeta = 0.0;
         if (grp >= 0) {
             for (int k = 0; k < ns; ++k) {
                 if (k == jj) continue;
-                double4 v = sh.sep[a][k];
+                double4 v = sh.sep[k][a];
                 if (!(v.w < maxbond_i)) continue;
-                const BondTerm &bt = M->bond[si][sjp][sh.spc[a][k]];
+                const BondTerm &bt = M->bond[si][sjp][sh.spc[k][a]];
                 if (bt.valid) zeta += zeta_term64(u, v, bt);
             }
         }
@@ -194,8 +196,8 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
         // the picked atom1, and r < the term's own cutoff
         double mp = M->maxpot[si];
         if (M->maxpot[sjp] != mp) {
-            int idi = id[i], idj = id[sh.jdx[a][jj]];
-            if (!pick_dev(idi, idj, sh.img[a][jj])) mp = M->maxpot[sjp];
+            int idi = id[i], idj = id[sh.jdx[jj][a]];
+            if (!pick_dev(idi, idj, sh.img[jj][a])) mp = M->maxpot[sjp];
         }
         double eh = 0.0, dp = 0.0, P = 0.0;
         if (u.w < mp) {
@@ -218,29 +220,29 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
             }
         }
         e_lane += eh;
-        sh.P[a][jj] = (float)P;
-        sh.dpair[a][jj] = dp;
+        sh.P[jj][a] = (float)P;
+        sh.dpair[jj][a] = dp;
     }
     __syncthreads();
 
     // ---- 3. D_ij = dU_i/dr_ij (FP32 three-body arithmetic) -------------------
     double w6[6] = {0, 0, 0, 0, 0, 0};
     for (int jj = l; jj < ns; jj += G) {
-        double4 ud = sh.sep[a][jj];
+        double4 ud = sh.sep[jj][a];
         float ux = (float)ud.x, uy = (float)ud.y, uz = (float)ud.z, r1 = (float)ud.w;
         float inv_r1 = 1.0f / r1;
-        float Pj = sh.P[a][jj];
-        int sjp = sh.spc[a][jj];
+        float Pj = sh.P[jj][a];
+        int sjp = sh.spc[jj][a];
         float cu = 0.f;  // three-body coefficient of u
         float dcx = 0.f, dcy = 0.f, dcz = 0.f;  // accumulated coefficient * dcos/du
         if (grp >= 0) {
             const float mb = (float)maxbond_i;
             for (int k = 0; k < ns; ++k) {
                 if (k == jj) continue;
-                double4 vd = sh.sep[a][k];
+                double4 vd = sh.sep[k][a];
                 float vx = (float)vd.x, vy = (float)vd.y, vz = (float)vd.z, r2 = (float)vd.w;
-                float Pk = sh.P[a][k];
-                int skp = sh.spc[a][k];
+                float Pk = sh.P[k][a];
+                int skp = sh.spc[k][a];
                 float inv_r12 = inv_r1 / r2;
                 float cs = (ux * vx + uy * vy + uz * vz) * inv_r12;
                 float coef_u = 0.f, coef_c = 0.f;
@@ -294,14 +296,14 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
             }
         }
         // pair part in FP64, three-body part from FP32
-        double cp = sh.dpair[a][jj] / ud.w;
+        double cp = sh.dpair[jj][a] / ud.w;
         double Dx = cp * ud.x + (double)(cu * ux + dcx);
         double Dy = cp * ud.y + (double)(cu * uy + dcy);
         double Dz = cp * ud.z + (double)(cu * uz + dcz);
         size_t o = (size_t)jj * n + i;
         dU[o] = make_float4((float)Dx, (float)Dy, (float)Dz, 0.f);
-        sj[o] = sh.jdx[a][jj];
-        simg[o] = sh.img[a][jj];
+        sj[o] = sh.jdx[jj][a];
+        simg[o] = sh.img[jj][a];
         // virial W = -sum r_ij (x) D_ij ; order xx yy zz yz xz xy
         w6[0] -= ud.x * Dx;
         w6[1] -= ud.y * Dy;
@@ -325,7 +327,321 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
     if (threadIdx.x < 7) {
         double s = 0.0;
         for (int t = 0; t < APB; ++t) s += sh.red[t][threadIdx.x];
-        partial[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
+        partial[(size_t)blk * 8 + threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Single-species fast path.  Every parameter is uniform (scalar registers)
+// and each neighbor's cutoff functions and pair terms are evaluated once by
+// the lane that loads it, not once per triplet.  Two kernels split the work
+// the way callers need it:
+//   k_site1e  energy only: zeta, b~ and the pair energies in FP64 (the
+//             1e-7 eV/atom contract); no gradients;
+//   k_site1f  forces + virial: zeta, b~, b~' and the three-body gradient in
+//             FP32 (the force path's precision form), pair terms, D and the
+//             virial in FP64.  A velocity-Verlet step needs exactly this, as
+//             calculate_forces returns forces and stress (PyInterface.f90:668).
+// ---------------------------------------------------------------------------
+template <int G, int SCAP>
+__global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const double4 *__restrict__ pos,
+                                                  const int *__restrict__ nbr,
+                                                  const int *__restrict__ nimg,
+                                                  const int *__restrict__ ncnt,
+                                                  const Model *__restrict__ M,
+                                                  double *__restrict__ partial,
+                                                  int *__restrict__ flags)
+{
+    constexpr int APB = BLOCK / G;
+    constexpr int PER = (SCAP + G - 1) / G;
+    // [slot][atom]: the atoms of a wave sit on consecutive LDS banks
+    __shared__ double4 s_sep[SCAP][APB];      // x, y, z, r
+    __shared__ double2 s_fi[SCAP][APB];       // bond f_c(r) (0 at r >= max_bond_radius), 1/r
+    __shared__ double s_red[APB];
+
+    const int a = threadIdx.x / G;
+    const int l = threadIdx.x % G;
+    const int blk = xcd_block(gridDim.x);
+    const int i = blk * APB + a;
+    const bool valid = i < n;
+    const int gbase = ((threadIdx.x & 63) / G) * G;
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+
+    const BondTerm B = M->bond[0][0][0];
+    const bool bond_on = M->group >= 0 && B.valid;
+    const double maxbond = M->maxbond[0], maxpot = M->maxpot[0], rshort = M->rshort;
+    const int npair = M->npair[0][0];
+    const double dd = B.d * B.d, cd2 = B.c * B.c / dd;
+
+    double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    int nf = valid ? ncnt[i] : 0;
+    int cnt = 0;
+    for (int base = 0; base < nf; base += G) {
+        int e = base + l;
+        bool keep = false;
+        double4 d = make_double4(0, 0, 0, 0);
+        if (e < nf) {
+            int j = nbr[(size_t)e * n + i];
+            int im = nimg[(size_t)e * n + i];
+            double3 sp3 = separation64(pi, pos[j], im, g.v);
+            double r = sqrt(dot64(sp3));
+            d = make_double4(sp3.x, sp3.y, sp3.z, r);
+            keep = r < rshort;
+        }
+        unsigned long long b = __ballot(keep);
+        unsigned long long gb = (b >> gbase) & gmask;
+        int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
+        if (keep && slot < SCAP) {
+            bool inb = bond_on && d.w < maxbond;
+            s_sep[slot][a] = d;
+            s_fi[slot][a] = make_double2(inb ? fc64(d.w, B.cut, B.soft) : 0.0, 1.0 / d.w);
+        }
+        cnt += __popcll(gb);
+    }
+    if (cnt > SCAP && l == 0) atomicMax(flags + 1, cnt);
+    const int ns = cnt < SCAP ? cnt : SCAP;
+    __syncthreads();
+
+    double e_lane = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int jj = l + q * G;
+        if (jj >= ns) continue;
+        double4 u = s_sep[jj][a];
+        double ir1 = s_fi[jj][a].y;
+        double zeta = 0.0;
+        if (bond_on) {
+            for (int k = 0; k < ns; ++k) {
+                if (k == jj) continue;
+                double2 fk = s_fi[k][a];
+                if (fk.x == 0.0) continue;
+                double4 v = s_sep[k][a];
+                double cosine = (u.x * v.x + u.y * v.y + u.z * v.z) * ir1 * fk.y;
+                double t = B.h - cosine;
+                double t2 = t * t;
+                double xi = fk.x;
+                if (!B.a_zero) xi *= exp(pow(B.a * (u.w - v.w), B.mu));
+                zeta += xi * (1.0 + cd2 * (t2 / (dd + t2)));
+            }
+        }
+        double bt = 1.0;
+        if (bond_on) bt = pow(1.0 + pow(B.beta * zeta, B.eta), -1.0 / (2.0 * B.eta));
+        if (u.w < maxpot) {
+            for (int t = 0; t < npair; ++t) {
+                const PairTerm pt = M->pair[0][0][t];
+                if (!(pt.cut > u.w)) continue;
+                double v = pt.eps * exp(-pt.zeta * u.w);
+                double f = pt.smooth ? fc64(u.w, pt.cut, pt.soft) : 1.0;
+                e_lane += 0.5 * v * f * (pt.group >= 0 ? bt : 1.0);
+            }
+        }
+    }
+    for (int off = G / 2; off > 0; off >>= 1) e_lane += __shfl_xor(e_lane, off, G);
+    if (l == 0) s_red[a] = valid ? e_lane : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sacc = 0.0;
+        for (int t = 0; t < APB; ++t) sacc += s_red[t];
+        partial[(size_t)blk * 8] = sacc;
+        for (int c = 1; c < 7; ++c) partial[(size_t)blk * 8 + c] = 0.0;
+    }
+}
+
+template <int G, int SCAP>
+__global__ __launch_bounds__(BLOCK) void k_site1f(int n, CellGeom g, const double4 *__restrict__ pos,
+                                                  const int *__restrict__ nbr,
+                                                  const int *__restrict__ nimg,
+                                                  const int *__restrict__ ncnt,
+                                                  const Model *__restrict__ M,
+                                                  int *__restrict__ sj, int *__restrict__ simg,
+                                                  int *__restrict__ scnt, float4 *__restrict__ dU,
+                                                  double *__restrict__ partial,
+                                                  int *__restrict__ flags)
+{
+    constexpr int APB = BLOCK / G;
+    constexpr int PER = (SCAP + G - 1) / G;
+    // [slot][atom]: the atoms of a wave sit on consecutive LDS banks
+    __shared__ double4 s_sep[SCAP][APB];      // x, y, z, r (FP64)
+    __shared__ float4 s_aux[SCAP][APB];       // 1/r, f_c(r), f_c'(r), A (then P)
+    __shared__ double2 s_pair[SCAP][APB];     // 1/2 d(f v_R)/dr, 1/2 d(f v_A)/dr
+    __shared__ double s_red[APB][8];
+
+    const int a = threadIdx.x / G;
+    const int l = threadIdx.x % G;
+    const int blk = xcd_block(gridDim.x);
+    const int i = blk * APB + a;
+    const bool valid = i < n;
+    const int gbase = ((threadIdx.x & 63) / G) * G;
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+
+    const BondTerm B = M->bond[0][0][0];
+    const bool bond_on = M->group >= 0 && B.valid;
+    const double maxbond = M->maxbond[0], maxpot = M->maxpot[0], rshort = M->rshort;
+    const int npair = M->npair[0][0];
+    const float hf = (float)B.h, ddf = (float)(B.d * B.d), ccf = (float)(B.c * B.c);
+    const float cd2f = (float)(B.c * B.c / (B.d * B.d));
+    const float betaf = (float)B.beta, etaf = (float)B.eta;
+
+    double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+
+    // ---- 1. short list; per neighbor: cutoff functions and pair terms -------
+    int nf = valid ? ncnt[i] : 0;
+    int cnt = 0;
+    for (int base = 0; base < nf; base += G) {
+        int e = base + l;
+        bool keep = false;
+        double3 sp3 = make_double3(0, 0, 0);
+        double r = 0.0;
+        int j = 0, im = 0;
+        if (e < nf) {
+            j = nbr[(size_t)e * n + i];
+            im = nimg[(size_t)e * n + i];
+            sp3 = separation64(pi, pos[j], im, g.v);
+            r = sqrt(dot64(sp3));
+            keep = r < rshort;
+        }
+        unsigned long long b = __ballot(keep);
+        unsigned long long gb = (b >> gbase) & gmask;
+        int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
+        if (keep && slot < SCAP) {
+            bool inb = bond_on && r < maxbond;
+            double dR = 0.0, dA = 0.0, A = 0.0;
+            if (r < maxpot) {
+                for (int t = 0; t < npair; ++t) {
+                    const PairTerm pt = M->pair[0][0][t];
+                    if (!(pt.cut > r)) continue;
+                    double v = pt.eps * exp(-pt.zeta * r);
+                    double f = pt.smooth ? fc64(r, pt.cut, pt.soft) : 1.0;
+                    double df = pt.smooth ? dfc64(r, pt.cut, pt.soft) : 0.0;
+                    double dv = 0.5 * (-pt.zeta * v * f + v * df);
+                    if (pt.group >= 0) { dA += dv; A += 0.5 * v * f; }
+                    else dR += dv;
+                }
+            }
+            s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
+            s_aux[slot][a] = make_float4((float)(1.0 / r), inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
+                                         inb ? (float)dfc64(r, B.cut, B.soft) : 0.f, (float)A);
+            s_pair[slot][a] = make_double2(dR, dA);
+            size_t o = (size_t)slot * n + i;
+            sj[o] = j;
+            simg[o] = im;
+        }
+        cnt += __popcll(gb);
+    }
+    if (cnt > SCAP && l == 0) atomicMax(flags + 1, cnt);
+    const int ns = cnt < SCAP ? cnt : SCAP;
+    __syncthreads();
+
+    // ---- 2. zeta, b~, b~' in FP32 -------------------------------------------------
+    double cpair[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        cpair[q] = 0.0;
+        int jj = l + q * G;
+        if (jj >= ns) continue;
+        double4 ud = s_sep[jj][a];
+        float ux = (float)ud.x, uy = (float)ud.y, uz = (float)ud.z, r1 = (float)ud.w;
+        float4 au = s_aux[jj][a];
+        float zeta = 0.f;
+        if (bond_on) {
+            for (int k = 0; k < ns; ++k) {
+                if (k == jj) continue;
+                float4 ak = s_aux[k][a];
+                if (ak.y == 0.f) continue;
+                double4 vd = s_sep[k][a];
+                float cs = (ux * (float)vd.x + uy * (float)vd.y + uz * (float)vd.z) * au.x * ak.x;
+                float t = hf - cs;
+                float t2 = t * t;
+                float xi = ak.y;
+                if (!B.a_zero) xi *= __expf(powf((float)B.a * (r1 - (float)vd.w), (float)B.mu));
+                zeta += xi * (1.0f + cd2f * t2 * __frcp_rn(ddf + t2));
+            }
+        }
+        float bt = 1.f, dbt = 0.f;
+        if (bond_on) {
+            float be = betaf * zeta;
+            float pw = __powf(be, etaf);
+            float inv_eta = -0.5f / etaf;
+            bt = __powf(1.0f + pw, inv_eta);
+            if (!(etaf == 0.f || (zeta == 0.f && etaf <= 1.f)))
+                dbt = inv_eta * (bt / (1.0f + pw)) * etaf * (pw / be) * betaf;
+        }
+        double2 pp = s_pair[jj][a];
+        cpair[q] = pp.x + pp.y * (double)bt;  // d/dr of 1/2 f (v_R + v_A b~) at fixed b~
+        s_aux[jj][a].w = au.w * dbt;           // P = 1/2 f v_A db~/dzeta
+    }
+    __syncthreads();
+
+    // ---- 3. D_ij: FP64 pair part + FP32 three-body part; virial in FP64 ------
+    double w6[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int jj = l + q * G;
+        if (jj >= ns) continue;
+        double4 ud = s_sep[jj][a];
+        float ux = (float)ud.x, uy = (float)ud.y, uz = (float)ud.z, r1 = (float)ud.w;
+        float4 aj = s_aux[jj][a];
+        float ir1 = aj.x, fj = aj.y, dfj = aj.z, Pj = aj.w;
+        float cu = 0.f, dcx = 0.f, dcy = 0.f, dcz = 0.f;
+        if (bond_on) {
+            for (int k = 0; k < ns; ++k) {
+                if (k == jj) continue;
+                float4 ak = s_aux[k][a];
+                float fk = ak.y, Pk = ak.w;
+                if (fk == 0.f && fj == 0.f) continue;
+                double4 vd = s_sep[k][a];
+                float vx = (float)vd.x, vy = (float)vd.y, vz = (float)vd.z;
+                float ir12 = ir1 * ak.x;
+                float cs = (ux * vx + uy * vy + uz * vz) * ir12;
+                float t = hf - cs;
+                float iden = __frcp_rn(ddf + t * t);
+                float gee = 1.0f + cd2f * t * t * iden;
+                float gp = -2.0f * ccf * t * iden * iden;
+                float coef_u, coef_c;
+                if (B.a_zero) {
+                    coef_u = Pk * gee * dfj;
+                    coef_c = gp * (Pj * fk + Pk * fj);
+                } else {
+                    float av = (float)B.a, mu = (float)B.mu, r2 = (float)vd.w;
+                    float x1 = av * (r1 - r2), x2 = -x1;
+                    float p1 = powf(x1, mu - 1.0f), p2 = powf(x2, mu - 1.0f);
+                    float e1 = __expf(x1 * p1), e2 = __expf(x2 * p2);
+                    float d1 = e1 * av * mu * p1, d2 = e2 * av * mu * p2;
+                    coef_u = Pj * gee * fk * d1 + Pk * gee * (dfj * e2 - fj * d2);
+                    coef_c = gp * (Pj * fk * e1 + Pk * fj * e2);
+                }
+                float cu2 = cs * ir1 * ir1;
+                dcx += coef_c * (vx * ir12 - cu2 * ux);
+                dcy += coef_c * (vy * ir12 - cu2 * uy);
+                dcz += coef_c * (vz * ir12 - cu2 * uz);
+                cu += coef_u * ir1;
+            }
+        }
+        double cp = cpair[q] / ud.w;
+        double Dx = cp * ud.x + (double)(cu * ux + dcx);
+        double Dy = cp * ud.y + (double)(cu * uy + dcy);
+        double Dz = cp * ud.z + (double)(cu * uz + dcz);
+        dU[(size_t)jj * n + i] = make_float4((float)Dx, (float)Dy, (float)Dz, 0.f);
+        w6[0] -= ud.x * Dx;
+        w6[1] -= ud.y * Dy;
+        w6[2] -= ud.z * Dz;
+        w6[3] -= ud.y * Dz;
+        w6[4] -= ud.x * Dz;
+        w6[5] -= ud.x * Dy;
+    }
+    if (valid && l == 0) scnt[i] = ns;
+
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        for (int off = G / 2; off > 0; off >>= 1) w6[q] += __shfl_xor(w6[q], off, G);
+    if (l == 0)
+        for (int q = 0; q < 6; ++q) s_red[a][1 + q] = valid ? w6[q] : 0.0;
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double sacc = 0.0;
+        if (threadIdx.x > 0)
+            for (int t = 0; t < APB; ++t) sacc += s_red[t][threadIdx.x];
+        partial[(size_t)blk * 8 + threadIdx.x] = sacc;  // energy slot 0 left at 0
     }
 }
 
@@ -338,7 +654,7 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ s
                                                 double4 *__restrict__ mom, double half_dt,
                                                 int kick, int *__restrict__ flags)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int ns = scnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -375,49 +691,66 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ s
     }
 }
 
-__global__ void k_reduce_partials(int nb, const double *__restrict__ partial,
-                                  double *__restrict__ totals)
+// one block per component: block c sums partial[b*8 + c] over b (fixed order)
+__global__ __launch_bounds__(1024) void k_reduce_partials(int nb, const double *__restrict__ partial,
+                                                          double *__restrict__ totals)
 {
-    __shared__ double lds[256];
-    for (int q = 0; q < 7; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += 256) s += partial[(size_t)b * 8 + q];
-        lds[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) totals[q] = lds[0];
+    __shared__ double lds[1024];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 1024) s += partial[(size_t)b * 8 + q];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) totals[q] = lds[0];
 }
 
 struct Variant {
     const char *name;
     int G, SCAP;
 };
-const Variant kVariants[] = {{"g4_s16", 4, 16}, {"g8_s16", 8, 16}, {"g16_s32", 16, 32},
-                             {"g2_s16", 2, 16}, {"g8_s32", 8, 32}};
+const Variant kVariants[] = {{"g4_s8", 4, 8},   {"g4_s16", 4, 16}, {"g8_s8", 8, 8},
+                             {"g8_s16", 8, 16}, {"g2_s8", 2, 8},   {"g16_s32", 16, 32}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 template <int G, int SCAP>
-void launch_site(System &S)
+void launch_site(System &S, int mode)
 {
     constexpr int APB = BLOCK / G;
     int nb = div_up(S.n, APB);
     S.partial.ensure((size_t)nb * 8);
-    k_site<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.id.p, S.nbr.p, S.nimg.p,
-                                               S.ncnt.p, S.dmodel.p, S.sj.p, S.simg.p, S.scnt.p,
-                                               S.dU.p, S.partial.p, S.flags.p);
+    if (S.model.nspecies == 1 && mode == SITE_ENERGY)
+        k_site1e<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+                                                     S.ncnt.p, S.dmodel.p, S.partial.p, S.flags.p);
+    else if (S.model.nspecies == 1 && mode == SITE_FORCES)
+        k_site1f<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+                                                     S.ncnt.p, S.dmodel.p, S.sj.p, S.simg.p,
+                                                     S.scnt.p, S.dU.p, S.partial.p, S.flags.p);
+    else
+        k_site<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.id.p, S.nbr.p, S.nimg.p,
+                                                   S.ncnt.p, S.dmodel.p, S.sj.p, S.simg.p, S.scnt.p,
+                                                   S.dU.p, S.partial.p, S.flags.p);
 }
 
 }  // namespace
 
 int ts_variant_count() { return kNumVariants; }
 const char *ts_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].name : ""; }
+int ts_variant_capacity(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].SCAP : 0; }
+// smallest-G variant whose short-list capacity holds `need` neighbors
+int ts_variant_for_capacity(int need)
+{
+    int best = -1;
+    for (int v = 0; v < kNumVariants; ++v)
+        if (kVariants[v].SCAP >= need && (best < 0 || kVariants[v].SCAP < kVariants[best].SCAP))
+            best = v;
+    return best;
+}
 
-void ts_site(System &S, int variant)
+void ts_site(System &S, int variant, int mode)
 {
     PYSIC_REQUIRE(variant >= 0 && variant < kNumVariants, PYSIC_EINVAL, "bad Tersoff kernel variant");
     S.scap = kVariants[variant].SCAP;
@@ -426,16 +759,18 @@ void ts_site(System &S, int variant)
     S.scnt.ensure(S.n);
     S.dU.ensure((size_t)S.scap * S.n);
     hipEvent_t ev;
-    S.timer.begin("tersoff_site", S.stream, &ev);
+    const char *name = mode == SITE_ENERGY ? "tersoff_energy" : "tersoff_site";
+    S.timer.begin(name, S.stream, &ev);
     switch (variant) {
-    case 0: launch_site<4, 16>(S); break;
-    case 1: launch_site<8, 16>(S); break;
-    case 2: launch_site<16, 32>(S); break;
-    case 3: launch_site<2, 16>(S); break;
-    case 4: launch_site<8, 32>(S); break;
+    case 0: launch_site<4, 8>(S, mode); break;
+    case 1: launch_site<4, 16>(S, mode); break;
+    case 2: launch_site<8, 8>(S, mode); break;
+    case 3: launch_site<8, 16>(S, mode); break;
+    case 4: launch_site<2, 8>(S, mode); break;
+    case 5: launch_site<16, 32>(S, mode); break;
     }
     HIP_TRY(hipGetLastError());
-    S.timer.end("tersoff_site", S.stream, ev);
+    S.timer.end(name, S.stream, ev);
     S.partial_blocks = div_up(S.n, BLOCK / kVariants[variant].G);
 }
 
@@ -452,6 +787,6 @@ void ts_gather(System &S, double half_dt, int kick)
 void ts_reduce(System &S)
 {
     S.totals.ensure(8);
-    k_reduce_partials<<<1, 256, 0, S.stream>>>(S.partial_blocks, S.partial.p, S.totals.p);
+    k_reduce_partials<<<7, 1024, 0, S.stream>>>(S.partial_blocks, S.partial.p, S.totals.p);
     HIP_TRY(hipGetLastError());
 }

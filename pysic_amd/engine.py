@@ -50,6 +50,7 @@ SIGNATURES = {
     "pysic_kernel_variant_count": (ctypes.c_int, []),
     "pysic_kernel_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
     "pysic_set_neighbor_capacity": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pysic_set_neighbor_method": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pysic_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pysic_timing_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                          _lp]),
@@ -185,6 +186,9 @@ class Engine:
         self._check(self.L.pysic_get_neighbor_lists(self.h, nnz, _i(idx), _i(off)),
                     "get_neighbor_lists")
         return counts, idx[:nnz], off[:nnz]
+
+    def set_neighbor_method(self, method):
+        self._check(self.L.pysic_set_neighbor_method(self.h, int(method)), "set_neighbor_method")
 
     def set_neighbor_capacity(self, cap):
         self._check(self.L.pysic_set_neighbor_capacity(self.h, int(cap)), "set_neighbor_capacity")

@@ -163,3 +163,17 @@ def test_md_matches_oracle_velocity_verlet(eng, oracle):
     assert np.abs(pos_g - r).max() < 1e-7
     assert np.abs(mom_g - p).max() < 1e-5 * np.abs(p).max()
     assert np.abs(np.array(e_gpu) - np.array(e_or)).max() / n <= 1e-7
+
+
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
+def test_neighbor_methods_agree(eng, oracle, method):
+    # global per-atom scan, LDS-staged bricks, and the bricks' global fallback
+    s = st.amorphous_si(700, 2.3, seed=13)
+    pd, bd = tersoff_descriptors()
+    eng.set_neighbor_method(method)
+    try:
+        check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+        t = st.diamond_si((5, 4, 3), sigma=0.2, seed=21)  # uneven bins, wrapped halo
+        check_tersoff(eng, oracle, t.positions, t.cell, [1, 1, 1], pd, bd, np.full(len(t), 3.5))
+    finally:
+        eng.set_neighbor_method(0)
