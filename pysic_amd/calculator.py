@@ -1,0 +1,412 @@
+"""The Pysic calculator and FastNeighborList on the MI355X engine.
+
+Mirrors pysic/calculator.py (Pysic :192-1832, FastNeighborList :29-189): the
+ASE-calculator methods get_potential_energy / get_forces / get_stress /
+get_electronegativities with the same caching and units, over one device
+engine per calculator (the reference keeps one process-global Fortran core,
+Core.f90:19-97; here every calculator owns its own handle).
+"""
+import math
+
+import numpy as np
+
+from .coulomb import CoulombSummation
+from .engine import Engine
+from .interactions import build_descriptors
+
+
+class MissingAtomsError(Exception):
+    pass
+
+
+class LockedCoreError(Exception):
+    pass
+
+
+def _symbols(atoms):
+    return list(atoms.get_chemical_symbols())
+
+
+def _charges(atoms):
+    try:
+        return np.array(atoms.get_initial_charges(), dtype=np.float64)
+    except AttributeError:
+        return np.array(atoms.get_charges(), dtype=np.float64)
+
+
+class FastNeighborList:
+    """Neighbor lists built by the engine's cell list (calculator.py:29-189).
+
+    cutoffs are per atom; the search radius is cutoff + skin, and the lists
+    are rebuilt when an atom has moved more than skin since the last build
+    (the ASE NeighborList rule the reference inherits).
+    """
+
+    neighbor_marginal = 0.5
+
+    def __init__(self, cutoffs, skin=None):
+        self.skin = FastNeighborList.neighbor_marginal if skin is None else float(skin)
+        self.cutoffs = np.asarray(cutoffs, dtype=np.float64) + self.skin
+        self.positions = None
+        self.nupdates = 0
+        self._counts = None
+        self._start = None
+        self._idx = None
+        self._off = None
+        self._engine = None
+
+    def _attach(self, engine):
+        self._engine = engine
+
+    def update(self, atoms):
+        pos = atoms.get_positions()
+        if (self.positions is None or len(pos) != len(self.positions)
+                or ((pos - self.positions) ** 2).sum(1).max() > self.skin ** 2):
+            self.build(atoms)
+            return True
+        return False
+
+    def build(self, atoms):
+        if self._engine is None:
+            raise MissingAtomsError("neighbor list is not attached to a calculator")
+        self.positions = atoms.get_positions().copy()
+        self._engine.generate_neighbor_lists(self.cutoffs)
+        self._counts = None
+        self.nupdates += 1
+
+    def _fetch(self):
+        if self._counts is None:
+            c, i, o = self._engine.neighbor_lists()
+            self._counts = c
+            self._start = np.concatenate([[0], np.cumsum(c)])
+            self._idx = i
+            self._off = o
+
+    def get_neighbors(self, index, atoms=None, sort=False):
+        self._fetch()
+        a, b = self._start[index], self._start[index + 1]
+        nb, off = self._idx[a:b].copy(), self._off[a:b].copy()
+        if sort and atoms is not None and len(nb):
+            d = self.get_neighbor_distances(index, atoms)
+            o = np.argsort(d, kind="stable")
+            nb, off = nb[o], off[o]
+        return nb, off
+
+    def get_neighbor_separations(self, index, atoms, sort=False):
+        nb, off = self.get_neighbors(index)
+        pos = atoms.get_positions()
+        sep = pos[nb] + off @ atoms.get_cell() - pos[index]
+        if sort:
+            sep = sep[np.argsort(np.sqrt((sep * sep).sum(1)), kind="stable")]
+        return sep
+
+    def get_neighbor_distances(self, index, atoms, sort=False):
+        sep = self.get_neighbor_separations(index, atoms)
+        d = np.sqrt((sep * sep).sum(1))
+        return np.sort(d) if sort else d
+
+
+class Pysic:
+    """ASE calculator over the MI355X engine (calculator.py:192-1832)."""
+
+    def __init__(self, atoms=None, potentials=None, charge_relaxation=None, coulomb=None,
+                 full_initialization=False, device=0):
+        self.engine = Engine(device)
+        self.structure = None
+        self.potentials = None
+        self.charge_relaxation = None
+        self.coulomb = None
+        self.neighbor_list = None
+        self.force_core_initialization = full_initialization
+        self.extra_calculators = []
+        self.forces = self.stress = self.energy = self.electronegativities = None
+        self._core = None  # what the engine currently holds
+        self.set_atoms(atoms)
+        self.set_potentials(potentials)
+        self.set_charge_relaxation(charge_relaxation)
+        self.set_coulomb_summation(coulomb)
+
+    # --- configuration (calculator.py:537-820) -------------------------------
+    def _invalidate(self):
+        self.forces = self.stress = self.energy = self.electronegativities = None
+
+    def set_atoms(self, atoms=None):
+        if atoms is None:
+            return
+        if self.structure is not None and self._same_structure(atoms):
+            return
+        self.structure = atoms.copy() if hasattr(atoms, "copy") else atoms
+        self._invalidate()
+
+    def _same_structure(self, atoms):
+        s = self.structure
+        try:
+            return (len(s) == len(atoms) and _symbols(s) == _symbols(atoms)
+                    and np.array_equal(s.get_positions(), atoms.get_positions())
+                    and np.array_equal(s.get_cell(), atoms.get_cell())
+                    and np.array_equal(s.get_pbc(), atoms.get_pbc())
+                    and np.array_equal(_charges(s), _charges(atoms))
+                    and np.array_equal(s.get_momenta(), atoms.get_momenta()))
+        except Exception:
+            return False
+
+    def get_atoms(self):
+        return self.structure
+
+    def set_potentials(self, potentials):
+        self.potentials = None if potentials is None else list(potentials)
+        self._core = None
+        self._invalidate()
+
+    def add_potential(self, potential):
+        if self.potentials is None:
+            self.potentials = []
+        self.potentials.append(potential)
+        self._core = None
+        self._invalidate()
+
+    def get_potentials(self):
+        return self.potentials
+
+    def set_coulomb_summation(self, coulomb):
+        self.coulomb = coulomb
+        self._core = None
+        self._invalidate()
+
+    def get_coulomb_summation(self):
+        return self.coulomb
+
+    def set_charge_relaxation(self, charge_relaxation):
+        self.charge_relaxation = charge_relaxation
+        if charge_relaxation is not None:
+            charge_relaxation.set_calculator(self)
+        self._invalidate()
+
+    def get_charge_relaxation(self):
+        return self.charge_relaxation
+
+    def add_calculator(self, calc):
+        self.extra_calculators.append(calc)
+
+    def get_calculators(self):
+        return self.extra_calculators
+
+    def get_neighbor_lists(self):
+        return self.neighbor_list
+
+    def get_individual_cutoffs(self, scaler=1.0):
+        """Per-atom maximum cutoff over the potentials, bond factors and the
+        Coulomb real-space cutoff that target the atom (calculator.py:860-915)."""
+        if self.structure is None:
+            return None
+        base = self.coulomb.get_realspace_cutoff() if self.coulomb is not None else 0.0
+        cuts = []
+        for sym in _symbols(self.structure):
+            c = base
+            for pot in self.potentials or []:
+                if sym in pot.get_different_symbols():
+                    c = max(c, pot.get_cutoff())
+                coord = pot.get_coordinator()
+                if coord is not None:
+                    for bond in coord.get_bond_order_parameters():
+                        if any(sym == t[0] for t in (bond.get_symbols() or [])) or \
+                                any(sym in t for t in (bond.get_symbols() or [])):
+                            c = max(c, bond.get_cutoff())
+            cuts.append(c * scaler)
+        return cuts
+
+    def create_neighbor_lists(self, cutoffs=None, marginal=None):
+        if cutoffs is None:
+            cutoffs = self.get_individual_cutoffs(1.0)
+        self.neighbor_list = FastNeighborList(cutoffs, skin=marginal)
+        self.neighbor_list._attach(self.engine)
+
+    # --- core synchronisation (calculator.py:998-1046, 1450-1637) -----------
+    def _species_map(self):
+        names = sorted(set(_symbols(self.structure)))
+        if len(names) > 4:
+            raise LockedCoreError("at most 4 elements per system")
+        return {s: k for k, s in enumerate(names)}
+
+    def _kcells(self):
+        kc = self.coulomb.parameters["k_cutoff"]
+        rec = 2.0 * math.pi * np.linalg.inv(self.structure.get_cell()).T
+        vol = np.dot(rec[0], np.cross(rec[1], rec[2]))
+        return [int(kc * np.linalg.norm(np.cross(rec[(a + 1) % 3], rec[(a + 2) % 3])) / vol + 0.5)
+                for a in range(3)]
+
+    def _full_init(self):
+        s = self.structure
+        if s is None:
+            raise MissingAtomsError("no atoms assigned to the calculator")
+        spmap = self._species_map()
+        species = np.array([spmap[x] for x in _symbols(s)], dtype=np.int32)
+        eng = self.engine
+        eng.create_atoms(species, s.get_positions(), masses=s.get_masses(), charges=_charges(s),
+                         momenta=s.get_momenta())
+        eng.create_cell(s.get_cell(), s.get_pbc())
+        pd, bd, _ = build_descriptors(self.potentials or [], spmap)
+        eng.set_potentials(pd)
+        eng.set_bond_order_factors(bd)
+        if self.coulomb is not None:
+            p = self.coulomb.parameters
+            sc = self.coulomb.get_scaling_factors()
+            if sc is not None and len(sc) != len(s):
+                raise ValueError("length of the scaling factor vector does not match the atoms")
+            eng.set_ewald(p["real_cutoff"], p["k_cutoff"], self._kcells(), p["sigma"], p["epsilon"],
+                          sc)
+        else:
+            eng.ewald_off()
+        self.create_neighbor_lists()
+        self.neighbor_list.build(s)
+        self._core = dict(n=len(s), symbols=_symbols(s), cell=s.get_cell().copy(),
+                          pbc=np.array(s.get_pbc()).copy(), positions=s.get_positions().copy(),
+                          charges=_charges(s).copy(), momenta=s.get_momenta().copy())
+
+    def set_core(self):
+        s = self.structure
+        if s is None:
+            raise MissingAtomsError("no atoms assigned to the calculator")
+        c = self._core
+        if (self.force_core_initialization or c is None or c["n"] != len(s)
+                or c["symbols"] != _symbols(s) or not np.array_equal(c["cell"], s.get_cell())
+                or not np.array_equal(c["pbc"], np.array(s.get_pbc()))):
+            self._full_init()
+            return
+        if not (np.array_equal(c["positions"], s.get_positions())
+                and np.array_equal(c["momenta"], s.get_momenta())):
+            self.engine.update_coordinates(s.get_positions(), s.get_momenta())
+            c["positions"] = s.get_positions().copy()
+            c["momenta"] = s.get_momenta().copy()
+            self.neighbor_list.update(s)
+        if not np.array_equal(c["charges"], _charges(s)):
+            q = _charges(s)
+            self.engine.update_charges(q)
+            c["charges"] = q.copy()
+
+    # --- calculations (calculator.py:315-534, 918-990) ---------------------
+    def calculation_required(self, atoms=None, quantities=("forces", "energy", "stress",
+                                                           "electronegativities")):
+        if isinstance(quantities, str):
+            quantities = [quantities]
+        table = {"energy": self.energy, "forces": self.forces, "stress": self.stress,
+                 "electronegativities": self.electronegativities}
+        return any(table.get(q, 0) is None for q in quantities)
+
+    def relax_charges(self, charges, rates, n_steps, timestep, inertia, friction, tolerance):
+        self.set_core()
+        out = self.engine.relax_charges(charges, rates, n_steps, timestep, inertia, friction,
+                                        tolerance)
+        self._core["charges"] = out[0].copy()
+        return out
+
+    def _relax_if_needed(self, skip):
+        if self.charge_relaxation is not None and not skip:
+            self.charge_relaxation.charge_relaxation()
+            self._invalidate()
+
+    def calculate_forces(self, skip_charge_relaxation=False):
+        self.set_core()
+        self._relax_if_needed(skip_charge_relaxation)
+        self.set_core()
+        f, st = self.engine.forces()
+        for calc in self.extra_calculators:
+            f = f + calc.get_forces(self.structure)
+            st = st + calc.get_stress(self.structure)
+        self.forces, self.stress = f, st
+
+    def calculate_energy(self, skip_charge_relaxation=False):
+        self.set_core()
+        self._relax_if_needed(skip_charge_relaxation)
+        self.set_core()
+        e = self.engine.energy()
+        for calc in self.extra_calculators:
+            e += calc.get_potential_energy(self.structure)
+        self.energy = e
+
+    def calculate_stress(self, skip_charge_relaxation=False):
+        self.calculate_forces(skip_charge_relaxation)
+
+    def calculate_electronegativities(self):
+        self.set_core()
+        self.electronegativities = self.engine.electronegativities()
+
+    def get_forces(self, atoms=None, skip_charge_relaxation=False):
+        self.set_atoms(atoms)
+        if self.calculation_required(atoms, "forces"):
+            self.calculate_forces(skip_charge_relaxation)
+        return np.copy(self.forces)
+
+    def get_potential_energy(self, atoms=None, force_consistent=False,
+                             skip_charge_relaxation=False):
+        self.set_atoms(atoms)
+        if self.calculation_required(atoms, "energy"):
+            self.calculate_energy(skip_charge_relaxation)
+        return self.energy
+
+    def get_stress(self, atoms=None, skip_charge_relaxation=False):
+        """-(sum m v (x) v + sum r (x) f) / V in the order xx yy zz yz xz xy
+        (calculator.py:468-534)."""
+        self.set_atoms(atoms)
+        if self.calculation_required(atoms, "stress"):
+            self.calculate_stress(skip_charge_relaxation)
+        s = self.structure
+        p = s.get_momenta()
+        m = s.get_masses()
+        v = p / m[:, None]
+        kin = np.array([p[:, 0] @ v[:, 0], p[:, 1] @ v[:, 1], p[:, 2] @ v[:, 2],
+                        p[:, 1] @ v[:, 2], p[:, 0] @ v[:, 2], p[:, 0] @ v[:, 1]])
+        return np.copy(-(kin + self.stress) / s.get_volume())
+
+    def get_electronegativities(self, atoms=None):
+        self.set_atoms(atoms)
+        if self.calculation_required(atoms, "electronegativities"):
+            self.calculate_electronegativities()
+        return np.copy(self.electronegativities)
+
+    def get_electronegativity_differences(self, atoms=None):
+        x = self.get_electronegativities(atoms)
+        return x - x.sum() / len(x)
+
+    # --- finite-difference debug helpers (calculator.py:1639-1832) ----------
+    def get_numerical_energy_gradient(self, atom_index, shift=0.0001, atoms=None):
+        s = (atoms if atoms is not None else self.structure).copy()
+        out = np.zeros(3)
+        for c in range(3):
+            e = []
+            for sg in (1, -1):
+                t = s.copy()
+                p = t.get_positions()
+                p[atom_index, c] += sg * shift
+                t.set_positions(p)
+                e.append(self.get_potential_energy(t, skip_charge_relaxation=True))
+            out[c] = -(e[0] - e[1]) / (2 * shift)
+        self.set_atoms(s)
+        return out
+
+    def get_numerical_electronegativity(self, atom_index, shift=0.001, atoms=None):
+        s = (atoms if atoms is not None else self.structure).copy()
+        e = []
+        for sg in (1, -1):
+            t = s.copy()
+            q = _charges(t)
+            q[atom_index] += sg * shift
+            t.set_initial_charges(q)
+            e.append(self.get_potential_energy(t, skip_charge_relaxation=True))
+        self.set_atoms(s)
+        return -(e[0] - e[1]) / (2 * shift)
+
+    # --- device-resident dynamics -----------------------------------------
+    def run_md(self, steps, dt, rebuild=1, energies=False):
+        """Velocity Verlet on the device (ASE VelocityVerlet semantics);
+        updates the calculator's structure with the final state."""
+        self.set_core()
+        e = self.engine.md_run(steps, dt, rebuild=rebuild, energies=energies)
+        pos, mom = self.engine.state()
+        self.structure.set_positions(pos)
+        self.structure.set_momenta(mom)
+        self._core["positions"] = pos.copy()
+        self._core["momenta"] = mom.copy()
+        self.neighbor_list.positions = pos.copy()
+        self._invalidate()
+        return e

@@ -1,0 +1,146 @@
+"""The pysic calculator API over the GPU engine, used the way pysic scripts use
+it (doc/run examples.rst), checked against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import rel_maxnorm, tersoff_descriptors
+from pysic_amd import (ChargeRelaxation, CoulombSummation, Potential, structures as st,
+                       tersoff_si)
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_tersoff(oracle, s):
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    nl = oracle.neighbor_list(s.get_positions(), s.get_cell(), [1, 1, 1], np.full(n, 3.5))
+    sp = np.zeros(n, np.int32)
+    e = oracle.evaluate("energy", sp, s.get_positions(), None, s.get_cell(), nl, pd, bd)
+    f, w = oracle.evaluate("forces", sp, s.get_positions(), None, s.get_cell(), nl, pd, bd)
+    return e, f, w
+
+
+def test_tersoff_through_the_calculator(oracle):
+    from pysic_amd import Pysic
+    s = st.diamond_si((4, 4, 4), sigma=0.05, seed=3)
+    st.maxwell_boltzmann(s, 300.0, seed=1)
+    calc = Pysic(s, tersoff_si())
+    e = calc.get_potential_energy()
+    f = calc.get_forces()
+    stress = calc.get_stress()
+    e_ref, f_ref, w_ref = oracle_tersoff(oracle, s)
+    n = len(s)
+    assert abs(e - e_ref) / n <= 1e-7
+    assert rel_maxnorm(f, f_ref) <= 1e-5
+    # stress = -(sum m v v + sum r f) / V (calculator.py:517-534)
+    p, m = s.get_momenta(), s.get_masses()
+    v = p / m[:, None]
+    kin = np.array([p[:, 0] @ v[:, 0], p[:, 1] @ v[:, 1], p[:, 2] @ v[:, 2], p[:, 1] @ v[:, 2],
+                    p[:, 0] @ v[:, 2], p[:, 0] @ v[:, 1]])
+    want = -(kin + w_ref) / s.get_volume()
+    assert np.abs(stress - want).max() * s.get_volume() / n <= 1e-6
+    # cached: nothing to compute for the same structure
+    assert not calc.calculation_required(s, ["energy", "forces", "stress"])
+
+
+def test_small_moves_keep_the_list_and_stay_exact(oracle):
+    from pysic_amd import Pysic
+    s = st.amorphous_si(300, 2.3, seed=4)
+    calc = Pysic(s, tersoff_si())
+    calc.get_forces()
+    builds = calc.get_neighbor_lists().nupdates
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        t = calc.get_atoms().copy()
+        t.set_positions(t.get_positions() + rng.normal(0, 0.01, (len(t), 3)))
+        f = calc.get_forces(t)
+        e = calc.get_potential_energy(t)
+        e_ref, f_ref, _ = oracle_tersoff(oracle, t)
+        assert rel_maxnorm(f, f_ref) <= 1e-5
+        assert abs(e - e_ref) / len(t) <= 1e-7
+    assert calc.get_neighbor_lists().nupdates == builds  # moves < skin: no rebuild
+    t = calc.get_atoms().copy()
+    p = t.get_positions()
+    p[0] += 0.6
+    t.set_positions(p)
+    calc.get_forces(t)
+    assert calc.get_neighbor_lists().nupdates == builds + 1
+
+
+def test_neighbor_list_accessors(oracle):
+    from pysic_amd import Pysic
+    s = st.diamond_si((3, 3, 3), sigma=0.05, seed=2)
+    calc = Pysic(s, tersoff_si())
+    calc.get_forces()
+    nlist = calc.get_neighbor_lists()
+    nb, off = nlist.get_neighbors(5)
+    assert len(nb) == 4
+    d = nlist.get_neighbor_distances(5, s, sort=True)
+    assert np.all(np.diff(d) >= 0) and d.max() < 3.5
+
+
+def test_numerical_gradient_helper_matches_forces():
+    from pysic_amd import Pysic
+    s = st.amorphous_si(64, 2.3, seed=8)
+    calc = Pysic(s, tersoff_si())
+    f = calc.get_forces()
+    g = calc.get_numerical_energy_gradient(7)
+    assert np.abs(g - f[7]).max() < 1e-4 * max(1.0, np.abs(f[7]).max())
+
+
+def test_charge_relaxation_through_the_calculator(oracle):
+    from pysic_amd import Pysic
+    s = st.amorphous_sio2(32, 2.2, seed=6)
+    pots = []
+    for sym, (chi0, J) in {"Si": (4.77, 16.0), "O": (8.74, 20.0)}.items():
+        pots.append(Potential("charge_self", symbols=[[sym]], parameters=[chi0, 1]))
+        pots.append(Potential("charge_self", symbols=[[sym]], parameters=[0.5 * J, 2]))
+    rc = 6.0
+    coul = CoulombSummation(parameters={"real_cutoff": rc, "k_cutoff": 1.5, "sigma": 2.0,
+                                        "epsilon": 0.00552635})
+    relax = ChargeRelaxation("dynamic", parameters={"n_steps": 1000, "timestep": 0.1,
+                                                    "inertia": 1.0, "friction": 4.0,
+                                                    "tolerance": 1e-8})
+    calc = Pysic(s, pots, coulomb=coul)
+    calc.set_charge_relaxation(relax)
+    calc.get_potential_energy()
+    q = calc.get_atoms().get_initial_charges()
+    assert relax.last_steps < 1000 and relax.last_error <= 1e-8
+    assert abs(q.sum()) < 1e-9
+    # converged: electronegativities are equalised
+    dchi = calc.get_electronegativity_differences()
+    assert np.abs(dchi).max() < 1e-6
+    # and agree with the oracle's electronegativities at those charges
+    names = ["O", "Si"]
+    sp = np.array([names.index(x) for x in s.symbols], np.int32)
+    rows = []
+    for k, sym in enumerate(names):
+        chi0, J = {"Si": (4.77, 16.0), "O": (8.74, 20.0)}[sym]
+        rows += [[2, 1, k, 0, chi0, 1.0, 0, 0, -1, 0], [2, 1, k, 0, 0.5 * J, 2.0, 0, 0, -1, 0]]
+    rec = 2 * math.pi * np.linalg.inv(s.cell).T
+    vol = rec[0] @ np.cross(rec[1], rec[2])
+    K = [int(1.5 * np.linalg.norm(np.cross(rec[(a + 1) % 3], rec[(a + 2) % 3])) / vol + 0.5)
+         for a in range(3)]
+    ew = np.array([1, rc, 1.5, K[0], K[1], K[2], 2.0, 0.00552635])
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), rc + 0.5))
+    x_ref = oracle.evaluate("enegs", sp, s.positions, q, s.cell, nl, np.array(rows, float), None,
+                            ew)
+    assert rel_maxnorm(calc.get_electronegativities(), x_ref) < 1e-9
+
+
+def test_device_md_through_the_calculator(oracle):
+    from pysic_amd import Pysic
+    s = st.diamond_si((3, 3, 3), sigma=0.1, seed=7)
+    st.maxwell_boltzmann(s, 600.0, seed=8)
+    calc = Pysic(s, tersoff_si())
+    e0 = calc.get_potential_energy()
+    p0 = s.get_momenta()
+    k0 = (p0 ** 2 / (2 * s.get_masses()[:, None])).sum()
+    calc.run_md(50, 0.5 * st.ASE_FS)
+    t = calc.get_atoms()
+    e1 = calc.get_potential_energy()
+    k1 = (t.get_momenta() ** 2 / (2 * t.get_masses()[:, None])).sum()
+    # NVE at 0.5 fs conserves the total energy well
+    assert abs((e1 + k1) - (e0 + k0)) / len(s) < 2e-4
