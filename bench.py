@@ -44,6 +44,46 @@ def tersoff_bytes_per_atom(n_nb):
     return 24.0 + 4.0 * n_nb + 24.0 + 8.0
 
 
+def measured_traffic(cells, variant_name):
+    """HBM bytes per tersoff_site launch, measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by what the same counters read for a
+    plain device copy of known size) and recorded in profiles/.  Returns None
+    unless the recorded workload and kernel variant are the ones run here."""
+    path = os.path.join(ROOT, "profiles", "tersoff_site_si1m_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    rec = {}
+    with open(path) as fh:
+        for line in fh:
+            if ":" in line and line.split(":", 1)[0].strip() in ("cells", "kernel_variant",
+                                                                  "traffic_bytes_per_launch"):
+                k, v = line.split(":", 1)
+                rec[k.strip()] = v.strip()
+    try:
+        if [int(x) for x in rec["cells"].split()] != list(cells):
+            return None
+        if rec["kernel_variant"] != variant_name:
+            return None
+        return int(rec["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes=256 << 20):
+    """One device-to-device copy of nbytes (HIP's blit copy kernel): the
+    reference reading for FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(seconds_budget=20.0):
     """Reference Fortran core on the host: VV steps (rebuild + forces) on a
     16x16x16-cell sample (32,768 atoms) of the same workload."""
@@ -95,6 +135,8 @@ def main():
     ap.add_argument("--cells", type=int, nargs=3, default=[64, 64, 32])
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device copy (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding is "
@@ -146,12 +188,16 @@ def main():
     site_avg_s = site_ms / max(site_calls, 1) / 1e3
     bytes_site = tersoff_bytes_per_atom(n_nb) * n
     achieved = bytes_site / site_avg_s / 1e9 if site_avg_s > 0 else 0.0
+    variant_name = eng.variants()[variant]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.cells, variant_name),
                 "kernel": "tersoff_site", "avg_launch_us": round(site_avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": int(bytes_site)}
     kernels = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "calls": v[1]}
                for k, v in timers.items()}
+    if args.copy_calibration:
+        copy_calibration()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {"metric": "atom-steps/s Tersoff 1M-atom Si", "value": round(value, 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -161,7 +207,7 @@ def main():
            "config": {"workload": "Tersoff Si %dx%dx%d diamond cells, %d atoms, VV dt=1 fs, "
                                   "rebuild every step" % (tuple(args.cells) + (n,)),
                       "atoms": n, "neighbors_per_atom": round(n_nb, 3),
-                      "kernel_variant": eng.variants()[variant],
+                      "kernel_variant": variant_name,
                       "total_energy_drift_eV_per_atom": drift},
            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels}
     print(json.dumps(out))
