@@ -84,9 +84,10 @@ class ChargeRelaxation:
     charge dynamics, relaxation.py:417-450) runs entirely on the device; the
     rates are carried across calls as in the reference (relaxation.py:431).
 
-    An optional initial-guess predictor (a callable atoms -> charges, such as
-    pysic_amd.learned.ChargePredictor) warm-starts the relaxation; the
-    unchanged dynamics then runs to the same tolerance.
+    An optional initial guess (a callable atoms -> charges, or a
+    pysic_amd.learned.ChargePredictor, which predicts on the calculator's
+    device-resident system) warm-starts the relaxation; the unchanged
+    dynamics then runs to the same tolerance.
     """
 
     relaxation_modes = ["dynamic", "potentiostat", "optimize"]
@@ -168,7 +169,11 @@ class ChargeRelaxation:
         atoms = calc.get_atoms()
         charges = np.array(atoms.get_initial_charges(), dtype=np.float64)
         if self.initial_guess is not None:
-            charges = np.array(self.initial_guess(atoms), dtype=np.float64)
+            g = self.initial_guess
+            if hasattr(g, "predict_for_calculator"):
+                charges = np.array(g.predict_for_calculator(calc), dtype=np.float64)
+            else:
+                charges = np.array(g(atoms), dtype=np.float64)
         if self.charge_rates is None or len(self.charge_rates) != len(charges):
             self.charge_rates = np.zeros(len(charges))
         p = self.parameters

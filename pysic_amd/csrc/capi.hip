@@ -7,6 +7,7 @@
 // Core.f90:4909-4911, MPI.f90:58-70).
 #include "system.h"
 #include "ewald.h"
+#include "learned.h"
 
 #include <algorithm>
 #include <cmath>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "../../include/pysic_hip.h"
+#include "../../include/pysic_learned.h"
 
 // ---------------------------------------------------------------------------
 // kernel timing
@@ -129,6 +131,8 @@ struct pysic_ctx {
     bool forces_valid = false;
     EwaldParams ewald;
     EwaldState ew;
+    double min_cut = 0.0;         // smallest per-atom list radius of the current lists
+    DBuf<double> rates;           // charge rates of relax_charges
 };
 
 namespace {
@@ -403,6 +407,7 @@ void pysic_destroy(pysic_ctx *h)
     h->S.timer.reset();
     h->S.release_all();
     ewald_release(h->ew);
+    h->rates.release();
     if (h->S.stream) (void)hipStreamDestroy(h->S.stream);
     delete h;
 }
@@ -526,8 +531,12 @@ int pysic_generate_neighbor_lists(pysic_ctx *h, int n, const double *cutoffs)
     API_BEGIN(h)
     PYSIC_REQUIRE(h->atoms_ready && h->cell_ready, PYSIC_ESTATE, "atoms and cell must be created first");
     PYSIC_REQUIRE(n == h->S.n && cutoffs, PYSIC_EINVAL, "generate_neighbor_lists: bad arguments");
-    double mc = 0.0;
-    for (int i = 0; i < n; ++i) mc = std::max(mc, cutoffs[i]);
+    double mc = 0.0, lo = cutoffs[0];
+    for (int i = 0; i < n; ++i) {
+        mc = std::max(mc, cutoffs[i]);
+        lo = std::min(lo, cutoffs[i]);
+    }
+    h->min_cut = lo;
     upload_scalar(h->S, cutoffs, h->S.cut);
     nb_configure_grid(h->S, mc);
     build_lists(h);
@@ -643,13 +652,12 @@ int pysic_relax_charges(pysic_ctx *h, int n, double *charges, double *rates, int
     PYSIC_REQUIRE(n == S.n && charges && rates && steps && error, PYSIC_EINVAL,
                   "relax_charges: bad arguments");
     upload_scalar(S, charges, S.q);
-    DBuf<double> dr;
-    dr.ensure(n);
-    upload_scalar(S, rates, dr);
-    ewald_relax(S, h->ewald, h->ew, dr.p, n_steps, timestep, inertia, friction, tolerance, steps, error);
+    h->rates.ensure(n);
+    upload_scalar(S, rates, h->rates);
+    ewald_relax(S, h->ewald, h->ew, h->rates.p, n_steps, timestep, inertia, friction, tolerance,
+                steps, error);
     download_scalar(S, S.q.p, charges);
-    download_scalar(S, dr.p, rates);
-    dr.release();
+    download_scalar(S, h->rates.p, rates);
     h->forces_valid = false;
     API_END(h)
 }
@@ -767,6 +775,286 @@ int pysic_synchronize(pysic_ctx *h)
     API_BEGIN(h)
     HIP_TRY(hipStreamSynchronize(h->S.stream));
     API_END(h)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned charge warm start (include/pysic_learned.h)
+// ---------------------------------------------------------------------------
+struct pysic_mlp {
+    MlpState M;
+    std::string err;
+};
+
+namespace {
+
+int fail_m(pysic_mlp *m, const PysicError &e)
+{
+    if (m) m->err = e.msg;
+    return e.code;
+}
+
+#define MLP_BEGIN(m)                                                                         \
+    if (!(m)) return PYSIC_EINVAL;                                                           \
+    try {                                                                                    \
+        HIP_TRY(hipSetDevice((m)->M.device));
+#define MLP_END(m)                                                                           \
+    }                                                                                        \
+    catch (const PysicError &e) { return fail_m(m, e); }                                     \
+    catch (const std::exception &e) { return fail_m(m, PysicError{PYSIC_ENOMEM, e.what()}); } \
+    return PYSIC_OK;
+
+void mlp_require_params(const MlpState &M)
+{
+    PYSIC_REQUIRE(M.configured, PYSIC_ESTATE, "network not configured");
+    PYSIC_REQUIRE(M.has_params, PYSIC_ESTATE, "network parameters not set");
+}
+
+// sample indices are used as addresses on the device: check every one
+void mlp_upload_samples(MlpState &M, long count, const int *samples)
+{
+    PYSIC_REQUIRE(samples && count > 0, PYSIC_EINVAL, "empty sample list");
+    PYSIC_REQUIRE(M.ntrain > 0, PYSIC_ESTATE, "no training data");
+    for (long k = 0; k < count; ++k)
+        PYSIC_REQUIRE(samples[k] >= 0 && samples[k] < M.ntrain, PYSIC_ERANGE,
+                      "sample index out of range: " + std::to_string(samples[k]));
+    M.order.ensure(count);
+    HIP_TRY(hipMemcpyAsync(M.order.p, samples, sizeof(int) * count, hipMemcpyHostToDevice,
+                           M.stream));
+}
+
+// descriptors of h's system into M.sfeat / M.sspec (current order), on h's stream
+void mlp_system_descriptors(MlpState &M, pysic_ctx *h)
+{
+    PYSIC_REQUIRE(M.configured, PYSIC_ESTATE, "network not configured");
+    PYSIC_REQUIRE(h->device == M.device, PYSIC_EINVAL, "network and system live on different GPUs");
+    require_ready(h);
+    PYSIC_REQUIRE(h->min_cut >= M.dp.rc, PYSIC_EINVAL,
+                  "neighbor lists are shorter than the descriptor cutoff");
+    for (int s : h->species)
+        PYSIC_REQUIRE(s >= 0 && s < M.nspecies, PYSIC_EINVAL,
+                      "system has more species than the network was configured for");
+    System &S = h->S;
+    M.sfeat.ensure((size_t)S.n * MLP_F);
+    M.sspec.ensure(S.n);
+    mlp_descriptors(S, M, S.stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pysic_mlp_param_count(void) { return MLP_NP; }
+
+int pysic_mlp_create(pysic_mlp **out, int device)
+{
+    if (!out) return PYSIC_EINVAL;
+    *out = nullptr;
+    pysic_mlp *m = new pysic_mlp();
+    m->M.device = device;
+    try {
+        int ndev = 0;
+        HIP_TRY(hipGetDeviceCount(&ndev));
+        PYSIC_REQUIRE(device >= 0 && device < ndev, PYSIC_EINVAL, "no such GPU");
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreate(&m->M.stream));
+        m->M.P.ensure(MLP_NP);
+        m->M.W.ensure(MLP_NPK);
+        m->M.m1.ensure(MLP_NP);
+        m->M.m2.ensure(MLP_NP);
+        m->M.grad.ensure(MLP_NP);
+        m->M.norm.ensure(2 * MLP_F);
+    } catch (const PysicError &e) {
+        m->M.release_all();
+        delete m;
+        return e.code;
+    }
+    *out = m;
+    return PYSIC_OK;
+}
+
+void pysic_mlp_destroy(pysic_mlp *m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->M.device);
+    if (m->M.stream) (void)hipStreamSynchronize(m->M.stream);
+    m->M.release_all();
+    if (m->M.stream) (void)hipStreamDestroy(m->M.stream);
+    delete m;
+}
+
+const char *pysic_mlp_last_error(pysic_mlp *m) { return m ? m->err.c_str() : "null handle"; }
+
+int pysic_mlp_configure(pysic_mlp *m, int nspecies, double rc, const double *mu, const double *eta,
+                        const float *mean, const float *inv_scale)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(nspecies == 1 || nspecies == 2, PYSIC_EINVAL, "descriptors take 1 or 2 species");
+    PYSIC_REQUIRE(rc > 0.0 && mu && eta && mean && inv_scale, PYSIC_EINVAL,
+                  "configure: bad arguments");
+    MlpState &M = m->M;
+    M.nspecies = nspecies;
+    M.dp.rc = (float)rc;
+    for (int g = 0; g < MLP_G; ++g) {
+        M.dp.mu[g] = (float)mu[g];
+        M.dp.eta[g] = (float)eta[g];
+    }
+    float nb[2 * MLP_F];
+    std::memcpy(nb, mean, sizeof(float) * MLP_F);
+    std::memcpy(nb + MLP_F, inv_scale, sizeof(float) * MLP_F);
+    HIP_TRY(hipMemcpyAsync(M.norm.p, nb, sizeof(nb), hipMemcpyHostToDevice, M.stream));
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    M.configured = true;
+    MLP_END(m)
+}
+
+int pysic_mlp_set_params(pysic_mlp *m, const float *params)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(params, PYSIC_EINVAL, "set_params: null");
+    MlpState &M = m->M;
+    HIP_TRY(hipMemcpyAsync(M.P.p, params, sizeof(float) * MLP_NP, hipMemcpyHostToDevice, M.stream));
+    HIP_TRY(hipMemsetAsync(M.m1.p, 0, sizeof(float) * MLP_NP, M.stream));
+    HIP_TRY(hipMemsetAsync(M.m2.p, 0, sizeof(float) * MLP_NP, M.stream));
+    M.adam_t = 0;
+    mlp_pack(M);
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    M.has_params = true;
+    MLP_END(m)
+}
+
+int pysic_mlp_get_params(pysic_mlp *m, float *params)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(params, PYSIC_EINVAL, "get_params: null");
+    mlp_require_params(m->M);
+    HIP_TRY(hipMemcpyAsync(params, m->M.P.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost,
+                           m->M.stream));
+    HIP_TRY(hipStreamSynchronize(m->M.stream));
+    MLP_END(m)
+}
+
+int pysic_mlp_descriptors(pysic_mlp *m, pysic_ctx *h, int n, float *feat, int *species)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(h && feat && species && n == h->S.n, PYSIC_EINVAL, "descriptors: bad arguments");
+    MlpState &M = m->M;
+    try {
+        mlp_system_descriptors(M, h);
+    } catch (const PysicError &e) {
+        h->err = e.msg;
+        throw;
+    }
+    System &S = h->S;
+    std::vector<float> f((size_t)n * MLP_F);
+    std::vector<int> sp(n), id(n);
+    HIP_TRY(hipMemcpyAsync(f.data(), M.sfeat.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipMemcpyAsync(sp.data(), M.sspec.p, sizeof(int) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipMemcpyAsync(id.data(), S.id.p, sizeof(int) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    for (int s = 0; s < n; ++s) {
+        std::memcpy(feat + (size_t)id[s] * MLP_F, f.data() + (size_t)s * MLP_F, sizeof(float) * MLP_F);
+        species[id[s]] = sp[s];
+    }
+    MLP_END(m)
+}
+
+int pysic_mlp_predict_charges(pysic_mlp *m, pysic_ctx *h, int n, int neutralize, double *charges)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(h && charges && n == h->S.n, PYSIC_EINVAL, "predict_charges: bad arguments");
+    MlpState &M = m->M;
+    mlp_require_params(M);
+    try {
+        mlp_system_descriptors(M, h);
+    } catch (const PysicError &e) {
+        h->err = e.msg;
+        throw;
+    }
+    System &S = h->S;
+    M.y.ensure(n);
+    M.q.ensure(n);
+    mlp_forward(M, n, nullptr, M.sfeat.p, M.sspec.p, M.y.p, S.stream);
+    mlp_scatter_charges(S, M.y.p, M.q.p, S.stream);
+    HIP_TRY(hipMemcpyAsync(charges, M.q.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    if (neutralize) {
+        double mean = 0.0;
+        for (int i = 0; i < n; ++i) mean += charges[i];
+        mean /= n;
+        for (int i = 0; i < n; ++i) charges[i] -= mean;
+    }
+    MLP_END(m)
+}
+
+int pysic_mlp_set_training_data(pysic_mlp *m, int n, const float *feat, const int *species,
+                                const float *targets)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(n > 0 && feat && species && targets, PYSIC_EINVAL, "training data: bad arguments");
+    for (int i = 0; i < n; ++i)
+        PYSIC_REQUIRE(species[i] >= 0 && species[i] < MLP_S, PYSIC_EINVAL,
+                      "training species out of range");
+    MlpState &M = m->M;
+    M.feat.ensure((size_t)n * MLP_F);
+    M.spec.ensure(n);
+    M.targ.ensure(n);
+    HIP_TRY(hipMemcpyAsync(M.feat.p, feat, sizeof(float) * (size_t)n * MLP_F, hipMemcpyHostToDevice, M.stream));
+    HIP_TRY(hipMemcpyAsync(M.spec.p, species, sizeof(int) * n, hipMemcpyHostToDevice, M.stream));
+    HIP_TRY(hipMemcpyAsync(M.targ.p, targets, sizeof(float) * n, hipMemcpyHostToDevice, M.stream));
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    M.ntrain = n;
+    MLP_END(m)
+}
+
+int pysic_mlp_evaluate(pysic_mlp *m, int count, const int *samples, float *y)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(y, PYSIC_EINVAL, "evaluate: null output");
+    MlpState &M = m->M;
+    mlp_require_params(M);
+    mlp_upload_samples(M, count, samples);
+    M.y.ensure(count);
+    mlp_forward(M, count, M.order.p, M.feat.p, M.spec.p, M.y.p, M.stream);
+    HIP_TRY(hipMemcpyAsync(y, M.y.p, sizeof(float) * count, hipMemcpyDeviceToHost, M.stream));
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    MLP_END(m)
+}
+
+int pysic_mlp_gradients(pysic_mlp *m, int count, const int *samples, float *grad, float *loss)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(grad && loss, PYSIC_EINVAL, "gradients: null output");
+    PYSIC_REQUIRE(count <= (1 << 18), PYSIC_EINVAL, "gradients: at most 262144 samples per call");
+    MlpState &M = m->M;
+    mlp_require_params(M);
+    mlp_upload_samples(M, count, samples);
+    M.losses.ensure(1);
+    mlp_gradient(M, count, M.order.p, M.losses.p);
+    HIP_TRY(hipMemcpyAsync(grad, M.grad.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost, M.stream));
+    HIP_TRY(hipMemcpyAsync(loss, M.losses.p, sizeof(float), hipMemcpyDeviceToHost, M.stream));
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    MLP_END(m)
+}
+
+int pysic_mlp_train(pysic_mlp *m, int nsteps, int batch, const int *order, float lr, float beta1,
+                    float beta2, float eps, float *losses)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(nsteps > 0 && batch > 0 && batch <= (1 << 18) && losses, PYSIC_EINVAL,
+                  "train: bad arguments");
+    MlpState &M = m->M;
+    mlp_require_params(M);
+    mlp_upload_samples(M, (long)nsteps * batch, order);
+    M.losses.ensure(nsteps);
+    for (int k = 0; k < nsteps; ++k) {
+        mlp_gradient(M, batch, M.order.p + (size_t)k * batch, M.losses.p + k);
+        mlp_adam(M, lr, beta1, beta2, eps);
+    }
+    HIP_TRY(hipMemcpyAsync(losses, M.losses.p, sizeof(float) * nsteps, hipMemcpyDeviceToHost, M.stream));
+    HIP_TRY(hipStreamSynchronize(M.stream));
+    MLP_END(m)
 }
 
 }  // extern "C"

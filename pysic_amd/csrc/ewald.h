@@ -23,6 +23,7 @@ struct EwaldState {
     DBuf<double> part;      // block partials
     DBuf<double> red;       // reduced scalars
     DBuf<double2> skpart;   // structure-factor partials over atom chunks
+    DBuf<double4> rpart;    // per-atom reciprocal partials over k chunks
     double volume = 0;
     int skchunks = 0;
 };

@@ -37,7 +37,8 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
                                              const Model *__restrict__ M, int ewald_on, double rc,
                                              double inv_eps_4pi, double inv_sig_sqrt2,
                                              double inv_sig_sqrt2pi, double inv_sig_sq2,
-                                             double bg_chi, int mode, double *__restrict__ out1,
+                                             double bg_chi, const double *__restrict__ bg_dev,
+                                             int mode, double *__restrict__ out1,
                                              double4 *__restrict__ frc, double *__restrict__ part)
 {
     __shared__ double red[EB][7];
@@ -62,7 +63,8 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
         if (ewald_on) {
             double qs = qi * sc[i];
             // self (Potentials.f90:2126) and background (2166) electronegativity
-            chi += 2.0 * qs * inv_eps_4pi * inv_sig_sqrt2pi + sc[i] * bg_chi;
+            const double bg = bg_dev ? bg_dev[0] : bg_chi;
+            chi += 2.0 * qs * inv_eps_4pi * inv_sig_sqrt2pi + sc[i] * bg;
             const double inv2perpi = 2.0 * inv_sig_sqrt2pi;
             int nf = ncnt[i];
             for (int t = 0; t < nf; ++t) {
@@ -293,6 +295,14 @@ __global__ void k_charge_sums(int n, const double *__restrict__ q, const double 
     }
 }
 
+// background electronegativity from the reduced charge sum (same operations
+// as the host form - inv_eps_2v * (-sigma^2 Q), Potentials.f90:2166-2167)
+__global__ void k_background(const double *__restrict__ qsum, double *__restrict__ out,
+                             double inv_eps_2v, double sigma)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = -inv_eps_2v * (-sigma * sigma * qsum[0]);
+}
+
 // Deterministic sum of `count` strided partials (component c of stride 8).
 __global__ void k_sum_partials(int count, int stride, int ncomp, const double *__restrict__ part,
                                double *__restrict__ out)
@@ -489,7 +499,7 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
         if (stress6) for (int c = 0; c < 6; ++c) stress6[c] = 0.0;
         return;
     }
-    DBuf<double4> part;
+    DBuf<double4> &part = E.rpart;
     part.ensure((size_t)nchunk * n);
     E.part.ensure((size_t)atiles * 8);
     hipEvent_t ev;
@@ -506,10 +516,7 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
         HIP_TRY(hipMemcpyAsync(h, E.red.p, sizeof(double) * 6, hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));
         for (int c = 0; c < 6; ++c) stress6[c] = h[c];
-    } else {
-        HIP_TRY(hipStreamSynchronize(S.stream));
     }
-    part.release();
 }
 
 // q sums: out[0] = sum q s, out[1] = sum (q s)^2
@@ -525,7 +532,7 @@ void charge_sums(System &S, EwaldState &E, double *out)
 
 // real-space + singlet pass; returns the energy (mode 1) or stress (mode 2)
 void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double bg_chi,
-               double *out7)
+               double *out7, const double *bg_dev = nullptr)
 {
     const int n = S.n;
     Consts c = consts_of(P, E.volume);
@@ -535,7 +542,8 @@ void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double 
     S.timer.begin("ewald_real", S.stream, &ev);
     k_real<<<nb, EB, 0, S.stream>>>(n, S.geom, S.pos.p, S.q.p, E.scaler.p, S.nbr.p, S.nimg.p, S.ncnt.p,
                                     S.dmodel.p, P.on, P.rc, c.inv_eps_4pi, c.inv_sig_sqrt2,
-                                    c.inv_sig_sqrt2pi, c.inv_sig_sq2, bg_chi, mode, E.chi.p, S.frc.p,
+                                    c.inv_sig_sqrt2pi, c.inv_sig_sq2, bg_chi, bg_dev, mode, E.chi.p,
+                                    S.frc.p,
                                     E.part.p);
     HIP_TRY(hipGetLastError());
     S.timer.end("ewald_real", S.stream, ev);
@@ -546,17 +554,23 @@ void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double 
     }
 }
 
+// Electronegativities into E.chi without a host round trip: the charge sum
+// and the background term stay on the device.
 void chi_eval(System &S, const EwaldParams &P, EwaldState &E)
 {
-    double bg = 0.0;
+    const double *bg_dev = nullptr;
     if (P.on) {
-        double qs[2];
-        charge_sums(S, E, qs);
+        int nb = div_up(S.n, EB);
+        E.part.ensure((size_t)nb * 8);
+        k_charge_sums<<<nb, EB, 0, S.stream>>>(S.n, S.q.p, E.scaler.p, E.part.p);
+        k_sum_partials<<<1, EB, 0, S.stream>>>(nb, 8, 2, E.part.p, E.red.p);
         Consts c = consts_of(P, E.volume);
         // - s inv_eps_2v (-sigma^2 Q)  (Potentials.f90:2166-2167)
-        bg = -c.inv_eps_2v * (-P.sigma * P.sigma * qs[0]);
+        k_background<<<1, 64, 0, S.stream>>>(E.red.p, E.red.p + 12, c.inv_eps_2v, P.sigma);
+        HIP_TRY(hipGetLastError());
+        bg_dev = E.red.p + 12;
     }
-    real_pass(S, P, E, 3, bg, nullptr);
+    real_pass(S, P, E, 3, 0.0, nullptr, bg_dev);
     if (P.on && E.nk > 0) {
         structure_factors(S, E);
         recip_atoms(S, E, 3, nullptr);
@@ -574,6 +588,7 @@ void ewald_release(EwaldState &E)
     E.part.release();
     E.red.release();
     E.skpart.release();
+    E.rpart.release();
 }
 
 double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
@@ -653,13 +668,27 @@ void ewald_relax(System &S, const EwaldParams &P, EwaldState &E, double *rates, 
         HIP_TRY(hipStreamSynchronize(S.stream));
         return err;
     };
-    dchi_of_q(0);
-    double err = 2.0 * tolerance;
+    double err = dchi_of_q(0);
     int step = 0;
+    // the reference tests the signed max(dchi) (relaxation.py:449); a
+    // non-finite value means the dynamics ran away (time step too long for
+    // the stiffest charge mode, or an indefinite charge Hessian)
+    auto check = [&]() {
+        if (!std::isfinite(err)) {
+            dchi.release();
+            mx.release();
+            throw PysicError{PYSIC_ERANGE, "charge relaxation diverged at step " +
+                                               std::to_string(step) +
+                                               " (reduce the time step or raise the inertia)"};
+        }
+    };
+    check();
+    err = 2.0 * tolerance;
     while (err > tolerance && step < n_steps) {
         k_relax_update<<<nb, EB, 0, S.stream>>>(n, S.q.p, rates, dchi.p, dt, dt2, inv_mq, friction);
         err = dchi_of_q(1);
         ++step;
+        check();
     }
     *steps = step;
     *error = err;

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpysic_hip.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_long)
+_fp = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 _lib = None
@@ -55,6 +56,22 @@ SIGNATURES = {
     "pysic_timing_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                          _lp]),
     "pysic_synchronize": (ctypes.c_int, [_vp]),
+    # learned charge warm start (include/pysic_learned.h)
+    "pysic_mlp_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    "pysic_mlp_destroy": (None, [_vp]),
+    "pysic_mlp_last_error": (ctypes.c_char_p, [_vp]),
+    "pysic_mlp_param_count": (ctypes.c_int, []),
+    "pysic_mlp_configure": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _fp,
+                                           _fp]),
+    "pysic_mlp_set_params": (ctypes.c_int, [_vp, _fp]),
+    "pysic_mlp_get_params": (ctypes.c_int, [_vp, _fp]),
+    "pysic_mlp_descriptors": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _fp, _ip]),
+    "pysic_mlp_predict_charges": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "pysic_mlp_set_training_data": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _ip, _fp]),
+    "pysic_mlp_evaluate": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _fp]),
+    "pysic_mlp_gradients": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _fp, _fp]),
+    "pysic_mlp_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]),
 }
 
 
@@ -85,6 +102,10 @@ def _d(a):
 
 def _i(a):
     return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_fp)
 
 
 def _f64(a, shape=None):
@@ -262,3 +283,96 @@ class Engine:
 
     def synchronize(self):
         self._check(self.L.pysic_synchronize(self.h), "synchronize")
+
+
+class Mlp:
+    """The charge warm-start network on one device (include/pysic_learned.h)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        m = _vp()
+        rc = self.L.pysic_mlp_create(ctypes.byref(m), device)
+        if rc != 0:
+            raise PysicEngineError("pysic_mlp_create failed (%d): no usable HIP device %d"
+                                   % (rc, device))
+        self.m = m
+        self.nparams = self.L.pysic_mlp_param_count()
+
+    def close(self):
+        if getattr(self, "m", None):
+            self.L.pysic_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.pysic_mlp_last_error(self.m)
+            raise PysicEngineError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def configure(self, nspecies, rc, mu, eta, mean, inv_scale):
+        mu = _f64(mu)
+        eta = _f64(eta)
+        mean = np.ascontiguousarray(mean, dtype=np.float32)
+        isc = np.ascontiguousarray(inv_scale, dtype=np.float32)
+        assert len(mu) == 16 and len(eta) == 16 and len(mean) == 32 and len(isc) == 32
+        self._check(self.L.pysic_mlp_configure(self.m, int(nspecies), float(rc), _d(mu), _d(eta),
+                                               _f(mean), _f(isc)), "mlp_configure")
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.nparams
+        self._check(self.L.pysic_mlp_set_params(self.m, _f(p)), "mlp_set_params")
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self.L.pysic_mlp_get_params(self.m, _f(p)), "mlp_get_params")
+        return p
+
+    def descriptors(self, eng):
+        feat = np.zeros((eng.n, 32), dtype=np.float32)
+        sp = np.zeros(eng.n, dtype=np.int32)
+        self._check(self.L.pysic_mlp_descriptors(self.m, eng.h, eng.n, _f(feat), _i(sp)),
+                    "mlp_descriptors")
+        return feat, sp
+
+    def predict_charges(self, eng, neutralize=True):
+        q = np.zeros(eng.n)
+        self._check(self.L.pysic_mlp_predict_charges(self.m, eng.h, eng.n, 1 if neutralize else 0,
+                                                     _d(q)), "mlp_predict_charges")
+        return q
+
+    def set_training_data(self, feat, species, targets):
+        f = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 32)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        assert len(f) == len(sp) == len(t)
+        self._check(self.L.pysic_mlp_set_training_data(self.m, len(f), _f(f), _i(sp), _f(t)),
+                    "mlp_set_training_data")
+
+    def evaluate(self, samples):
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        y = np.zeros(len(s), dtype=np.float32)
+        self._check(self.L.pysic_mlp_evaluate(self.m, len(s), _i(s), _f(y)), "mlp_evaluate")
+        return y
+
+    def gradients(self, samples):
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.L.pysic_mlp_gradients(self.m, len(s), _i(s), _f(g), _f(loss)),
+                    "mlp_gradients")
+        return g, float(loss[0])
+
+    def train(self, order, batch, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        nsteps = len(o) // batch
+        assert nsteps >= 1
+        losses = np.zeros(nsteps, dtype=np.float32)
+        self._check(self.L.pysic_mlp_train(self.m, nsteps, int(batch), _i(o[:nsteps * batch]),
+                                           lr, beta1, beta2, eps, _f(losses)), "mlp_train")
+        return losses

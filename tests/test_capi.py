@@ -9,12 +9,12 @@ import pytest
 
 from conftest import ROOT
 
-HEADER = os.path.join(ROOT, "include", "pysic_hip.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pysic_hip.h", "pysic_learned.h")]
 LIB = os.path.join(ROOT, "pysic_amd", "libpysic_hip.so")
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(pysic_[a-z_]+)\s*\(", text)))
 
@@ -25,7 +25,8 @@ def test_header_declares_the_boundary():
               "pysic_set_bond_order_factors", "pysic_set_ewald",
               "pysic_generate_neighbor_lists", "pysic_calculate_energy",
               "pysic_calculate_forces", "pysic_calculate_electronegativities",
-              "pysic_relax_charges", "pysic_md_run"):
+              "pysic_relax_charges", "pysic_md_run", "pysic_mlp_predict_charges",
+              "pysic_mlp_train", "pysic_mlp_gradients"):
         assert s in syms
 
 
