@@ -17,6 +17,7 @@
 
 #include "../../include/pysic_hip.h"
 #include "../../include/pysic_learned.h"
+#include "../../include/pysic_batch.h"
 
 // ---------------------------------------------------------------------------
 // kernel timing
@@ -133,6 +134,12 @@ struct pysic_ctx {
     EwaldState ew;
     double min_cut = 0.0;         // smallest per-atom list radius of the current lists
     DBuf<double> rates;           // charge rates of relax_charges
+    // captured device evaluations of a batch of cells, one per position
+    // buffer (the sort swaps buffers, so consecutive evaluations alternate)
+    struct Captured {
+        hipGraphExec_t exec = nullptr;
+        std::vector<const void *> sig;   // buffers and settings it was captured with
+    } graph[2];
 };
 
 namespace {
@@ -369,6 +376,132 @@ void tersoff_checked(pysic_ctx *h, int mode)
     }
 }
 
+// Put the atoms back into the caller's order (slot i = atom i).
+void restore_original_order(System &S)
+{
+    const int n = S.n;
+    S.pos2.ensure(n); S.mom2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
+    sys_scatter_original(S, S.pos.p, S.pos2.p);
+    sys_scatter_original(S, S.mom.p, S.mom2.p);
+    k_scatter1<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.q.p, S.q2.p);
+    k_scatter1<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.cut.p, S.cut2.p);
+    std::swap(S.pos, S.pos2); std::swap(S.mom, S.mom2); std::swap(S.q, S.q2);
+    std::swap(S.cut, S.cut2);
+    k_iota<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(S.stream));
+}
+
+void drop_graphs(pysic_ctx *h)
+{
+    for (auto &g : h->graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.sig.clear();
+    }
+}
+
+// The whole device evaluation of a batch: list rebuild (bin, sort, 27-bin
+// scan), Tersoff forces and virials, FP64 site energies, per-cell sums.
+// Only kernel launches (no memset, no host synchronisation), so that it can
+// be captured into a graph.
+void batch_sequence(pysic_ctx *h, bool rebuild)
+{
+    System &S = h->S;
+    nb_clear_flags(S);
+    if (rebuild) {
+        nb_sort_atoms(S);
+        nb_build_lists(S);
+    }
+    ts_site(S, h->variant, SITE_FORCES);
+    ts_gather(S, 0.0, 0);
+    ts_site(S, h->variant, SITE_ENERGY);
+    ts_cell_sums(S);
+}
+
+std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
+{
+    System &S = h->S;
+    return {S.pos.p, S.pos2.p, S.mom.p, S.q.p, S.cut.p, S.id.p, S.key.p, S.wrap.p, S.perm.p,
+            S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.nbr.p, S.nimg.p,
+            S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
+            S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
+            (const void *)(long)S.cap, (const void *)(long)h->variant,
+            (const void *)(long)S.site_grid, (const void *)(long)S.n, (const void *)(long)S.nbins,
+            (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};
+}
+
+void swap_state(System &S)
+{
+    std::swap(S.pos, S.pos2); std::swap(S.mom, S.mom2); std::swap(S.q, S.q2);
+    std::swap(S.cut, S.cut2); std::swap(S.id, S.id2); std::swap(S.key, S.key2);
+    std::swap(S.wrap, S.wrap2);
+}
+
+// Capture batch_sequence for the current buffer state into slot k.  Nothing
+// runs; the host-side buffer swap that the sort performs during capture is
+// undone, so host and device state stay as they were.
+void capture_into(pysic_ctx *h, int k, bool rebuild)
+{
+    System &S = h->S;
+    pysic_ctx::Captured &slot = h->graph[k];
+    if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+    slot.exec = nullptr;
+    slot.sig = graph_signature(h, rebuild);
+    const bool timing = S.timer.on;
+    S.timer.on = false;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        batch_sequence(h, rebuild);
+    } catch (...) {
+        (void)hipStreamEndCapture(S.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        S.timer.on = timing;
+        slot.sig.clear();
+        throw;
+    }
+    S.timer.on = timing;
+    HIP_TRY(hipStreamEndCapture(S.stream, &graph));
+    if (rebuild) swap_state(S);
+    hipError_t e = hipGraphInstantiate(&slot.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        slot.exec = nullptr;
+        slot.sig.clear();
+    }
+    HIP_TRY(e);
+}
+
+// One device evaluation.  With use_graph: replay the graph captured for the
+// current buffer state.  On a miss the evaluation runs eagerly (which also
+// allocates whatever is missing, so no allocation happens under capture),
+// then graphs are captured for both buffer states (the sort alternates
+// them), and later calls only replay.
+void batch_evaluate(pysic_ctx *h, bool use_graph, bool rebuild)
+{
+    System &S = h->S;
+    if (!use_graph) {
+        batch_sequence(h, rebuild);
+        return;
+    }
+    auto sig = graph_signature(h, rebuild);
+    for (auto &g : h->graph)
+        if (g.exec && g.sig == sig) {
+            HIP_TRY(hipGraphLaunch(g.exec, S.stream));
+            if (rebuild) swap_state(S);
+            return;
+        }
+    drop_graphs(h);
+    batch_sequence(h, rebuild);   // eager; buffers now in the next state
+    capture_into(h, 0, rebuild);  // graph for this state
+    if (rebuild) {
+        swap_state(S);            // back to the state before the eager run
+        capture_into(h, 1, rebuild);
+        swap_state(S);            // and forward again: host matches device
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -407,6 +540,7 @@ void pysic_destroy(pysic_ctx *h)
     h->S.timer.reset();
     h->S.release_all();
     ewald_release(h->ew);
+    drop_graphs(h);
     h->rates.release();
     if (h->S.stream) (void)hipStreamDestroy(h->S.stream);
     delete h;
@@ -426,6 +560,8 @@ int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *
         PYSIC_REQUIRE(s >= 0 && s < PYSIC_MAXS, PYSIC_EINVAL, "species must be 0..3");
     h->masses.assign(n, 1.0);
     if (masses) h->masses.assign(masses, masses + n);
+    S.ncells = 0;
+    drop_graphs(h);
     S.pos.ensure(n); S.mom.ensure(n); S.frc.ensure(n); S.q.ensure(n); S.cut.ensure(n); S.id.ensure(n);
     k_iota<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p);
     k_zero4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.frc.p);
@@ -448,6 +584,8 @@ int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse
     std::memcpy(h->S.geom.v, vectors, sizeof(double) * 9);
     std::memcpy(h->S.geom.inv, inverse, sizeof(double) * 9);
     for (int a = 0; a < 3; ++a) h->S.geom.pbc[a] = pbc[a] ? 1 : 0;
+    h->S.ncells = 0;
+    drop_graphs(h);
     h->cell_ready = true;
     h->lists_valid = false;
     h->forces_valid = false;
@@ -768,6 +906,120 @@ int pysic_timing_read(pysic_ctx *h, int max, char *names, int name_len, double *
         calls[k] = h->S.timer.calls[k].second;
     }
     return k;
+}
+
+// ---------------------------------------------------------------------------
+// batches of independent cells (include/pysic_batch.h)
+// ---------------------------------------------------------------------------
+int pysic_set_cells(pysic_ctx *h, int ncells, const int *natoms, const double *vectors,
+                    const double *inverses, const int *pbc)
+{
+    API_BEGIN(h)
+    System &S = h->S;
+    PYSIC_REQUIRE(h->atoms_ready, PYSIC_ESTATE, "set_cells: create the atoms first");
+    PYSIC_REQUIRE(ncells > 0 && natoms && vectors && inverses && pbc, PYSIC_EINVAL,
+                  "set_cells: bad arguments");
+    std::vector<int> a0(ncells + 1, 0);
+    for (int c = 0; c < ncells; ++c) {
+        PYSIC_REQUIRE(natoms[c] > 0, PYSIC_EINVAL, "set_cells: every cell needs atoms");
+        a0[c + 1] = a0[c] + natoms[c];
+    }
+    PYSIC_REQUIRE(a0[ncells] == S.n, PYSIC_EINVAL,
+                  "set_cells: the cells' atom counts must add up to the atoms created");
+    // atoms keep their cell's slot range: start from the input order
+    restore_original_order(S);
+    S.hcells.assign(ncells, CellGeom{});
+    for (int c = 0; c < ncells; ++c) {
+        CellGeom &g = S.hcells[c];
+        std::memcpy(g.v, vectors + 9 * c, sizeof(double) * 9);
+        std::memcpy(g.inv, inverses + 9 * c, sizeof(double) * 9);
+        for (int a = 0; a < 3; ++a) g.pbc[a] = pbc[3 * c + a] ? 1 : 0;
+    }
+    S.hatom0 = a0;
+    std::vector<int> cell_of(S.n);
+    for (int c = 0; c < ncells; ++c)
+        for (int i = a0[c]; i < a0[c + 1]; ++i) cell_of[i] = c;
+    S.cell_of.ensure(S.n);
+    S.atom0.ensure(ncells + 1);
+    HIP_TRY(hipMemcpyAsync(S.cell_of.p, cell_of.data(), sizeof(int) * S.n, hipMemcpyHostToDevice,
+                           S.stream));
+    HIP_TRY(hipMemcpyAsync(S.atom0.p, a0.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice,
+                           S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    S.ncells = ncells;
+    drop_graphs(h);
+    h->cell_ready = true;
+    h->lists_valid = false;
+    h->forces_valid = false;
+    API_END(h)
+}
+
+int pysic_get_number_of_cells(pysic_ctx *h, int *ncells)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(ncells, PYSIC_EINVAL, "get_number_of_cells: null");
+    *ncells = h->S.ncells;
+    API_END(h)
+}
+
+int pysic_set_site_grid(pysic_ctx *h, int blocks)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(blocks >= 0, PYSIC_EINVAL, "set_site_grid: blocks must be >= 0");
+    h->S.site_grid = blocks;
+    API_END(h)
+}
+
+int pysic_evaluate_cells(pysic_ctx *h, int use_graph, int rebuild)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    System &S = h->S;
+    PYSIC_REQUIRE(S.ncells > 0, PYSIC_ESTATE, "evaluate_cells: no batch of cells set");
+    PYSIC_REQUIRE(S.model.has_pairs && !h->ewald.on, PYSIC_EINVAL,
+                  "evaluate_cells: a batch evaluates Tersoff / exponential terms only");
+    S.atom_ev.ensure((size_t)7 * S.n);
+    S.cell_ev.ensure((size_t)8 * S.ncells);
+    batch_evaluate(h, use_graph != 0, rebuild != 0);
+    h->forces_valid = true;
+    API_END(h)
+}
+
+int pysic_check_cells(pysic_ctx *h, int *flags4)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(flags4, PYSIC_EINVAL, "check_cells: null");
+    nb_read_flags(h->S, flags4);
+    API_END(h)
+}
+
+int pysic_get_cell_results(pysic_ctx *h, int ncells, double *energies, double *virials, int n,
+                           double *forces)
+{
+    API_BEGIN(h)
+    System &S = h->S;
+    PYSIC_REQUIRE(S.ncells > 0 && ncells == S.ncells && n == S.n, PYSIC_EINVAL,
+                  "get_cell_results: bad arguments");
+    PYSIC_REQUIRE(h->forces_valid, PYSIC_ESTATE, "get_cell_results: nothing evaluated");
+    int f[4];
+    nb_read_flags(S, f);
+    PYSIC_REQUIRE(f[0] <= S.cap, PYSIC_ECAP,
+                  "neighbor list overflow (" + std::to_string(f[0]) + " > capacity " +
+                      std::to_string(S.cap) + "); raise the capacity and evaluate again");
+    PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
+                  "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
+    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from its box");
+    std::vector<double> ev((size_t)8 * ncells);
+    HIP_TRY(hipMemcpyAsync(ev.data(), S.cell_ev.p, sizeof(double) * ev.size(),
+                           hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    for (int c = 0; c < ncells; ++c) {
+        if (energies) energies[c] = ev[8 * c];
+        if (virials)
+            for (int q = 0; q < 6; ++q) virials[6 * c + q] = ev[8 * c + 1 + q];
+    }
+    if (forces) download_vec3(S, S.frc.p, forces);
+    API_END(h)
 }
 
 int pysic_synchronize(pysic_ctx *h)

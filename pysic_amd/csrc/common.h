@@ -106,6 +106,23 @@ struct CellGeom {
     int ncell;       // dim[0]*dim[1]*dim[2]
 };
 
+// Where a kernel finds the cell of an atom slot.  GeoOne: one periodic cell
+// for every atom (passed by value).  GeoBatch: many independent cells, each
+// owning a fixed range of atom slots (the sort reorders atoms only within
+// their cell, because every cell's bins form their own key range).
+struct GeoOne {
+    CellGeom g;
+    __device__ __forceinline__ const CellGeom &of(int) const { return g; }
+    __device__ __forceinline__ int bin0(int) const { return 0; }
+};
+struct GeoBatch {
+    const CellGeom *cells;   // per cell
+    const int *cell_of;      // cell of each atom slot
+    const int *first_bin;    // first global bin of each cell
+    __device__ __forceinline__ const CellGeom &of(int s) const { return cells[cell_of[s]]; }
+    __device__ __forceinline__ int bin0(int s) const { return first_bin[cell_of[s]]; }
+};
+
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------

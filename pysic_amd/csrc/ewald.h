@@ -24,6 +24,12 @@ struct EwaldState {
     DBuf<double> red;       // reduced scalars
     DBuf<double2> skpart;   // structure-factor partials over atom chunks
     DBuf<double4> rpart;    // per-atom reciprocal partials over k chunks
+    // k-vector segments: runs of at most 16 consecutive k3 at fixed (k1, k2),
+    // {k1, k2, k3_start | len << 16, index of the first k}; natural order and
+    // sorted by length (longest first)
+    DBuf<int4> seg, seg_sorted;
+    int nseg = 0;
+    double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // reciprocal vectors 2 pi M^-1 (rows)
     double volume = 0;
     int skchunks = 0;
 };

@@ -9,11 +9,16 @@
 //    is evaluated from both ends, no scatter); unscaled charges as in the
 //    reference's main loop;
 //  * reciprocal space: the half-space k-vectors with |k| < k_c are enumerated
-//    once on the host in the reference's order; structure factors
-//    S(k) = sum_i q_i s_i exp(i k.r_i) are accumulated per k-tile over atom
-//    chunks staged in LDS (2-D grid, deterministic partial sums), and the
-//    per-atom consumers (forces, electronegativities) loop over k-chunks
-//    staged in LDS;
+//    once on the host in the reference's order and grouped into segments of
+//    up to 16 consecutive k3 at fixed (k1, k2).  With theta_a = b_a . r
+//    (b_a the reciprocal vectors), exp(i k.r) = exp(i (k1 th1 + k2 th2 +
+//    k3 th3)): one sincos per (segment, atom) gives the first phase, and
+//    the rest of the segment follows by the recurrence v <- v exp(i th3)
+//    (a complex multiply instead of a sincos per k and atom).  Structure
+//    factors S(k) = sum_i q_i s_i exp(i k.r_i): thread per segment over atom
+//    chunks staged in LDS (2-D grid, deterministic partial sums); per-atom
+//    consumers (forces, electronegativities): thread per atom over segment
+//    tiles whose k-vectors and S(k) are staged in LDS;
 //  * self and charged-background terms are per-atom closed forms.
 // The reciprocal "stress" uses absolute positions (Potentials.f90:1940-1945)
 // and the background electronegativity uses a zero-initialised charge sum
@@ -113,42 +118,77 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
     if (threadIdx.x < 7) part[(size_t)blockIdx.x * 8 + threadIdx.x] = red[0][threadIdx.x];
 }
 
-// S(k) partial sums: block (ktile, chunk) covers 256 k-vectors and one atom
-// chunk staged through LDS.
-__global__ __launch_bounds__(EB) void k_structure(int nk, int n, int chunk,
-                                                  const double4 *__restrict__ kvec,
+constexpr int LSEG = 16;   // k-vectors per segment
+constexpr int STILE = 32;  // segments per LDS tile of the per-atom kernel
+
+struct Recip {
+    double b[9];   // reciprocal vectors in rows: k = k1 b1 + k2 b2 + k3 b3
+};
+
+__device__ __forceinline__ void phases(const Recip &R, const double4 &p, double &t1, double &t2,
+                                       double &t3)
+{
+    t1 = R.b[0] * p.x + R.b[1] * p.y + R.b[2] * p.z;
+    t2 = R.b[3] * p.x + R.b[4] * p.y + R.b[5] * p.z;
+    t3 = R.b[6] * p.x + R.b[7] * p.y + R.b[8] * p.z;
+}
+
+// S(k) partial sums: block (segment tile, atom chunk); thread = segment.
+__global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk, Recip R,
+                                                  const int4 *__restrict__ segs,
                                                   const double4 *__restrict__ pos,
                                                   const double *__restrict__ q,
                                                   const double *__restrict__ sc,
                                                   double2 *__restrict__ skpart)
 {
-    __shared__ double4 at[EB];
-    int k = blockIdx.x * EB + threadIdx.x;
-    int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
-    double4 kv = k < nk ? kvec[k] : make_double4(0, 0, 0, 0);
-    double cs = 0.0, sn = 0.0;
+    __shared__ double4 at[EB];   // theta1, theta2, theta3, q s
+    __shared__ double2 w3[EB];   // exp(i theta3)
+    const int sidx = blockIdx.x * EB + threadIdx.x;
+    const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
+    int4 sg = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);
+    const double k1 = sg.x, k2 = sg.y, l0 = sg.z & 0xffff;
+    const int len = sg.z >> 16;  // 0 for lanes past the last segment
+    double cr[LSEG], ci[LSEG];
+#pragma unroll
+    for (int l = 0; l < LSEG; ++l) cr[l] = ci[l] = 0.0;
     for (int base = a0; base < a1; base += EB) {
-        int a = base + threadIdx.x;
+        const int a = base + threadIdx.x;
         if (a < a1) {
-            double4 p = pos[a];
-            at[threadIdx.x] = make_double4(p.x, p.y, p.z, q[a] * sc[a]);
+            double t1, t2, t3;
+            phases(R, pos[a], t1, t2, t3);
+            at[threadIdx.x] = make_double4(t1, t2, t3, q[a] * sc[a]);
+            double s3, c3;
+            sincos(t3, &s3, &c3);
+            w3[threadIdx.x] = make_double2(c3, s3);
         } else {
             at[threadIdx.x] = make_double4(0, 0, 0, 0);
+            w3[threadIdx.x] = make_double2(1, 0);
         }
         __syncthreads();
-        int m = min(EB, a1 - base);
+        const int m = min(EB, a1 - base);
         for (int t = 0; t < m; ++t) {
-            double4 p = at[t];
-            if (p.w == 0.0) continue;
-            double dot = kv.x * p.x + kv.y * p.y + kv.z * p.z;
-            double s, c;
-            sincos(dot, &s, &c);
-            cs += p.w * c;
-            sn += p.w * s;
+            const double4 th = at[t];
+            if (th.w == 0.0) continue;
+            const double2 w = w3[t];
+            double vs, vc;
+            sincos(k1 * th.x + k2 * th.y + l0 * th.z, &vs, &vc);
+#pragma unroll
+            for (int l = 0; l < LSEG; ++l) {
+                if (l < len) {
+                    cr[l] += th.w * vc;
+                    ci[l] += th.w * vs;
+                }
+                const double nc = vc * w.x - vs * w.y;
+                vs = vc * w.y + vs * w.x;
+                vc = nc;
+            }
         }
         __syncthreads();
     }
-    if (k < nk) skpart[(size_t)blockIdx.y * nk + k] = make_double2(cs, sn);
+    double2 *out = skpart + (size_t)blockIdx.y * nk + sg.w;
+#pragma unroll
+    for (int l = 0; l < LSEG; ++l)
+        if (l < len) out[l] = make_double2(cr[l], ci[l]);
 }
 
 __global__ void k_structure_sum(int nk, int nchunk, const double2 *__restrict__ skpart,
@@ -186,8 +226,16 @@ __global__ void k_recip_energy(int nk, const double4 *__restrict__ kvec,
 }
 
 // Per-atom reciprocal forces (mode 2) or electronegativities (mode 3) over
-// one k-chunk; block (atom tile, k chunk).
-__global__ __launch_bounds__(EB) void k_recip_atom(int n, int nk, int kchunk, int mode,
+// one chunk of segments (natural order, so each tile of STILE segments covers
+// one contiguous k range); block (atom tile, segment chunk).  Each thread
+// carries RAT atoms (i, i + EB, ...) so that every k-vector and S(k) read
+// from LDS serves RAT atoms: a wave-wide LDS read moves 1 KB into registers,
+// and with one atom per thread that traffic, not the FP64 arithmetic, was
+// the limit.
+constexpr int RAT = 4;
+
+__global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, int mode, Recip R,
+                                                   const int4 *__restrict__ segs,
                                                    const double4 *__restrict__ kvec,
                                                    const double2 *__restrict__ sk,
                                                    const double4 *__restrict__ pos,
@@ -195,41 +243,70 @@ __global__ __launch_bounds__(EB) void k_recip_atom(int n, int nk, int kchunk, in
                                                    const double *__restrict__ sc,
                                                    double4 *__restrict__ part)
 {
-    __shared__ double4 kl[EB];
-    __shared__ double2 sl[EB];
-    int i = blockIdx.x * EB + threadIdx.x;
-    int k0 = blockIdx.y * kchunk, k1 = min(nk, k0 + kchunk);
-    double4 p = i < n ? pos[i] : make_double4(0, 0, 0, 0);
-    double qs = i < n ? q[i] * sc[i] : 0.0;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int base = k0; base < k1; base += EB) {
-        int k = base + threadIdx.x;
-        if (k < k1) {
-            kl[threadIdx.x] = kvec[k];
-            sl[threadIdx.x] = sk[k];
+    __shared__ int4 sl[STILE];
+    __shared__ double4 kl[STILE * LSEG];
+    __shared__ double2 sv[STILE * LSEG];
+    const int i0 = blockIdx.x * (EB * RAT) + threadIdx.x;
+    const int s0 = blockIdx.y * schunk, s1 = min(nseg, s0 + schunk);
+    double t1[RAT], t2[RAT], t3[RAT], wc[RAT], ws[RAT], qs[RAT];
+    double ax[RAT], ay[RAT], az[RAT];
+#pragma unroll
+    for (int a = 0; a < RAT; ++a) {
+        const int i = i0 + a * EB;
+        const double4 p = i < n ? pos[i] : make_double4(0, 0, 0, 0);
+        qs[a] = i < n ? q[i] * sc[i] : 0.0;
+        phases(R, p, t1[a], t2[a], t3[a]);
+        sincos(t3[a], &ws[a], &wc[a]);
+        ax[a] = ay[a] = az[a] = 0.0;
+    }
+    for (int base = s0; base < s1; base += STILE) {
+        const int m = min(STILE, s1 - base);
+        const int kb = segs[base].w;
+        const int4 last = segs[base + m - 1];
+        const int nkt = last.w + (last.z >> 16) - kb;   // <= STILE * LSEG
+        if (threadIdx.x < m) sl[threadIdx.x] = segs[base + threadIdx.x];
+        for (int k = threadIdx.x; k < nkt; k += EB) {
+            kl[k] = kvec[kb + k];
+            sv[k] = sk[kb + k];
         }
         __syncthreads();
-        int m = min(EB, k1 - base);
         for (int t = 0; t < m; ++t) {
-            double4 kv = kl[t];
-            double2 s = sl[t];
-            double dot = kv.x * p.x + kv.y * p.y + kv.z * p.z;
-            double sd, cd;
-            sincos(dot, &sd, &cd);
-            if (mode == 3) {
-                // chi -= 2 c_k (C cos + S sin)   (Potentials.f90:2196-2229)
-                ax -= 2.0 * kv.w * (s.x * cd + s.y * sd);
-            } else {
-                // f -= 2 c_k q s (S cos - C sin) k   (Potentials.f90:1895-1931)
-                double pre = 2.0 * kv.w * qs * (s.y * cd - s.x * sd);
-                ax -= pre * kv.x;
-                ay -= pre * kv.y;
-                az -= pre * kv.z;
+            const int4 sg = sl[t];
+            const int len = sg.z >> 16;
+            const int o = sg.w - kb;
+            const double k1 = sg.x, k2 = sg.y, l0 = sg.z & 0xffff;
+            double vs[RAT], vc[RAT];
+#pragma unroll
+            for (int a = 0; a < RAT; ++a) sincos(k1 * t1[a] + k2 * t2[a] + l0 * t3[a], &vs[a], &vc[a]);
+            for (int l = 0; l < len; ++l) {
+                const double4 kv = kl[o + l];
+                const double2 S = sv[o + l];
+                const double c2 = 2.0 * kv.w;
+#pragma unroll
+                for (int a = 0; a < RAT; ++a) {
+                    if (mode == 3) {
+                        // chi -= 2 c_k (C cos + S sin)   (Potentials.f90:2196-2229)
+                        ax[a] -= c2 * (S.x * vc[a] + S.y * vs[a]);
+                    } else {
+                        // f -= 2 c_k q s (S cos - C sin) k   (Potentials.f90:1895-1931)
+                        const double pre = c2 * qs[a] * (S.y * vc[a] - S.x * vs[a]);
+                        ax[a] -= pre * kv.x;
+                        ay[a] -= pre * kv.y;
+                        az[a] -= pre * kv.z;
+                    }
+                    const double nc = vc[a] * wc[a] - vs[a] * ws[a];
+                    vs[a] = vc[a] * ws[a] + vs[a] * wc[a];
+                    vc[a] = nc;
+                }
             }
         }
         __syncthreads();
     }
-    if (i < n) part[(size_t)blockIdx.y * n + i] = make_double4(ax, ay, az, 0.0);
+#pragma unroll
+    for (int a = 0; a < RAT; ++a) {
+        const int i = i0 + a * EB;
+        if (i < n) part[(size_t)blockIdx.y * n + i] = make_double4(ax[a], ay[a], az[a], 0.0);
+    }
 }
 
 // Sum the k-chunk partials per atom; mode 2 adds forces and forms the
@@ -430,6 +507,7 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
         Consts c = consts_of(P, E.volume);
         double kc2 = P.kc * P.kc;
         std::vector<double4> kv;
+        std::vector<int4> segs;   // {k1, k2, k3 start | len << 16, first k index}
         for (int k1 = -P.K[0]; k1 <= P.K[0]; ++k1)
             for (int k2 = -P.K[1]; k2 <= P.K[1]; ++k2)
                 for (int k3 = 0; k3 <= P.K[2]; ++k3) {
@@ -441,8 +519,31 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
                     if (!(k2n < kc2)) continue;
                     double w = (k3 == 0) ? 1.0 : 2.0;
                     double coef = c.inv_eps_2v * w * std::exp(-P.sigma * P.sigma * k2n * 0.5) / k2n;
+                    int idx = (int)kv.size();
                     kv.push_back(make_double4(x, y, z, coef));
+                    bool extend = false;
+                    if (!segs.empty()) {
+                        int4 &g = segs.back();
+                        int gl = g.z >> 16, g0 = g.z & 0xffff;
+                        extend = g.x == k1 && g.y == k2 && g0 + gl == k3 && gl < LSEG;
+                        if (extend) g.z = g0 | ((gl + 1) << 16);
+                    }
+                    if (!extend) segs.push_back(make_int4(k1, k2, k3 | (1 << 16), idx));
                 }
+        PYSIC_REQUIRE(P.K[2] < 0xffff, PYSIC_EINVAL, "too many reciprocal cells along k3");
+        for (int q = 0; q < 9; ++q) E.b[q] = rcl[q];
+        E.nseg = (int)segs.size();
+        std::vector<int4> sorted = segs;
+        std::stable_sort(sorted.begin(), sorted.end(),
+                         [](const int4 &a, const int4 &b) { return (a.z >> 16) > (b.z >> 16); });
+        E.seg.ensure(segs.size());
+        E.seg_sorted.ensure(segs.size());
+        if (!segs.empty()) {
+            HIP_TRY(hipMemcpyAsync(E.seg.p, segs.data(), sizeof(int4) * segs.size(),
+                                   hipMemcpyHostToDevice, S.stream));
+            HIP_TRY(hipMemcpyAsync(E.seg_sorted.p, sorted.data(), sizeof(int4) * sorted.size(),
+                                   hipMemcpyHostToDevice, S.stream));
+        }
         E.nk = (int)kv.size();
         E.kvec.ensure(kv.size());
         E.sk.ensure(kv.size());
@@ -469,18 +570,26 @@ void ensure_scaler_identity(System &S, EwaldState &E)
     HIP_TRY(hipStreamSynchronize(S.stream));
 }
 
+Recip recip_of(const EwaldState &E)
+{
+    Recip R;
+    for (int q = 0; q < 9; ++q) R.b[q] = E.b[q];
+    return R;
+}
+
 void structure_factors(System &S, EwaldState &E)
 {
     const int n = S.n, nk = E.nk;
     if (nk == 0) return;
-    int ktiles = div_up(nk, EB);
-    int nchunk = std::max(1, std::min(div_up(n, EB), div_up(2048, ktiles)));
+    int stiles = div_up(E.nseg, EB);
+    int nchunk = std::max(1, std::min(div_up(n, EB), div_up(2048, stiles)));
     int chunk = div_up(div_up(n, nchunk), EB) * EB;
     nchunk = div_up(n, chunk);
     E.skpart.ensure((size_t)nchunk * nk);
     hipEvent_t ev;
     S.timer.begin("ewald_sk", S.stream, &ev);
-    k_structure<<<dim3(ktiles, nchunk), EB, 0, S.stream>>>(nk, n, chunk, E.kvec.p, S.pos.p, S.q.p,
+    k_structure<<<dim3(stiles, nchunk), EB, 0, S.stream>>>(E.nseg, nk, n, chunk, recip_of(E),
+                                                          E.seg_sorted.p, S.pos.p, S.q.p,
                                                           E.scaler.p, E.skpart.p);
     k_structure_sum<<<div_up(nk, 256), 256, 0, S.stream>>>(nk, nchunk, E.skpart.p, E.sk.p);
     HIP_TRY(hipGetLastError());
@@ -491,27 +600,29 @@ void structure_factors(System &S, EwaldState &E)
 void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
 {
     const int n = S.n, nk = E.nk;
-    int atiles = div_up(n, EB);
-    int nchunk = std::max(1, std::min(div_up(nk, EB), div_up(2048, atiles)));
-    int kchunk = div_up(div_up(nk, nchunk), EB) * EB;
-    nchunk = nk > 0 ? div_up(nk, kchunk) : 0;
+    int atiles = div_up(n, EB * RAT);
+    int nchunk = std::max(1, std::min(div_up(E.nseg, STILE), div_up(1024, atiles)));
+    int schunk = div_up(div_up(E.nseg, nchunk), STILE) * STILE;
+    nchunk = nk > 0 ? div_up(E.nseg, schunk) : 0;
     if (nchunk == 0) {
         if (stress6) for (int c = 0; c < 6; ++c) stress6[c] = 0.0;
         return;
     }
+    const int ftiles = div_up(n, EB);   // k_recip_finish: one atom per thread
     DBuf<double4> &part = E.rpart;
     part.ensure((size_t)nchunk * n);
-    E.part.ensure((size_t)atiles * 8);
+    E.part.ensure((size_t)ftiles * 8);
     hipEvent_t ev;
     S.timer.begin(mode == 3 ? "ewald_chi" : "ewald_force", S.stream, &ev);
-    k_recip_atom<<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, nk, kchunk, mode, E.kvec.p, E.sk.p,
-                                                           S.pos.p, S.q.p, E.scaler.p, part.p);
-    k_recip_finish<<<atiles, EB, 0, S.stream>>>(n, nchunk, mode, part.p, S.pos.p, S.frc.p, E.chi.p,
+    k_recip_atom<<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, mode, recip_of(E),
+                                                           E.seg.p, E.kvec.p, E.sk.p, S.pos.p,
+                                                           S.q.p, E.scaler.p, part.p);
+    k_recip_finish<<<ftiles, EB, 0, S.stream>>>(n, nchunk, mode, part.p, S.pos.p, S.frc.p, E.chi.p,
                                                 E.part.p);
     HIP_TRY(hipGetLastError());
     S.timer.end(mode == 3 ? "ewald_chi" : "ewald_force", S.stream, ev);
     if (mode == 2) {
-        k_sum_partials<<<1, EB, 0, S.stream>>>(atiles, 8, 6, E.part.p, E.red.p);
+        k_sum_partials<<<1, EB, 0, S.stream>>>(ftiles, 8, 6, E.part.p, E.red.p);
         double h[6];
         HIP_TRY(hipMemcpyAsync(h, E.red.p, sizeof(double) * 6, hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));
@@ -589,6 +700,8 @@ void ewald_release(EwaldState &E)
     E.red.release();
     E.skpart.release();
     E.rpart.release();
+    E.seg.release();
+    E.seg_sorted.release();
 }
 
 double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)

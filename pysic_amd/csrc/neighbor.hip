@@ -65,14 +65,16 @@ __device__ __forceinline__ void bin_atom(const CellGeom &g, const double4 &p, in
     *bad_out = bad;
 }
 
-__global__ void k_bin(int n, CellGeom g, const double4 *__restrict__ pos, int *__restrict__ key,
+template <class Geo>
+__global__ void k_bin(int n, Geo geo, const double4 *__restrict__ pos, int *__restrict__ key,
                       int *__restrict__ wrap, int *__restrict__ cellcount, int *__restrict__ flags)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int k, w;
     bool bad;
-    bin_atom(g, pos[i], &k, &w, &bad);
+    bin_atom(geo.of(i), pos[i], &k, &w, &bad);
+    k += geo.bin0(i);
     if (bad) atomicOr(flags + 2, 1);
     key[i] = k;
     wrap[i] = w;
@@ -80,7 +82,8 @@ __global__ void k_bin(int n, CellGeom g, const double4 *__restrict__ pos, int *_
 }
 
 // velocity-Verlet first half (p += dt/2 F; r += dt p/m) fused with binning
-__global__ void k_drift_bin(int n, CellGeom g, double dt, double half_dt,
+template <class Geo>
+__global__ void k_drift_bin(int n, Geo geo, double dt, double half_dt,
                             const double4 *__restrict__ frc, double4 *__restrict__ mom,
                             double4 *__restrict__ pos, int *__restrict__ key,
                             int *__restrict__ wrap, int *__restrict__ cellcount,
@@ -102,7 +105,8 @@ __global__ void k_drift_bin(int n, CellGeom g, double dt, double half_dt,
     pos[i] = r;
     int k, w;
     bool bad;
-    bin_atom(g, r, &k, &w, &bad);
+    bin_atom(geo.of(i), r, &k, &w, &bad);
+    k += geo.bin0(i);
     if (bad) atomicOr(flags + 2, 1);
     key[i] = k;
     wrap[i] = w;
@@ -174,6 +178,19 @@ __global__ void k_scan_add(int m, int *__restrict__ out, const int *__restrict__
         if (base + q < m) out[base + q] += add;
 }
 
+// zero two bin arrays of length m (a kernel, so that graphs hold no memset)
+__global__ void k_fill_int(int m, int *__restrict__ a, int *__restrict__ b)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) a[t] = 0;
+    else if (t < 2 * m) b[t - m] = 0;
+}
+
+__global__ void k_clear_flags(int *__restrict__ flags)
+{
+    if (threadIdx.x < 4 && blockIdx.x == 0) flags[threadIdx.x] = 0;
+}
+
 __global__ void k_scatter(int n, const int *__restrict__ key, const int *__restrict__ start,
                           int *__restrict__ fill, int *__restrict__ perm)
 {
@@ -205,6 +222,7 @@ __global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__rest
 }
 
 // Forces are not carried: after the drift they are dead until recomputed.
+template <class Geo>
 __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__restrict__ pos,
                           const double4 *__restrict__ mom, const double *__restrict__ q,
                           const double *__restrict__ cut, const int *__restrict__ id,
@@ -212,10 +230,11 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
                           double4 *__restrict__ pos2, double4 *__restrict__ mom2,
                           double *__restrict__ q2, double *__restrict__ cut2,
                           int *__restrict__ id2, int *__restrict__ key2, int *__restrict__ wrap2,
-                          CellGeom g, float4 *__restrict__ wpos)
+                          Geo geo, float4 *__restrict__ wpos)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const CellGeom &g = geo.of(s);   // slot s stays in the cell of slot perm[s]
     int o = perm[s];
     double4 p = pos[o];
     int w = wrap[o];
@@ -239,7 +258,8 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
 // the survivors take the exact FP64 test in the reference's operation order,
 // so the accepted set is the exact one.  The x-adjacent bins of a column are
 // consecutive in memory, so each column costs four bin-start loads.
-__global__ __launch_bounds__(256) void k_neighbors_pf(int n, CellGeom g,
+template <class Geo>
+__global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                                                       const double4 *__restrict__ pos,
                                                       const float4 *__restrict__ wpos,
                                                       const int *__restrict__ key,
@@ -253,9 +273,12 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, CellGeom g,
 {
     int s = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const CellGeom g = geo.of(s);
+    const int b0 = geo.bin0(s);
+    start += b0;
     const double4 ps = pos[s];
     const float4 qs = wpos[s];
-    const int k = key[s];
+    const int k = key[s] - b0;
     const int c0 = k % g.dim[0];
     const int c1 = (k / g.dim[0]) % g.dim[1];
     const int c2 = k / (g.dim[0] * g.dim[1]);
@@ -321,7 +344,8 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, CellGeom g,
 }
 
 // Per-atom scan of the 27 adjacent bins (Core.f90:8182-8252 as a full list).
-__global__ __launch_bounds__(256) void k_neighbors(int n, CellGeom g, const double4 *__restrict__ pos,
+template <class Geo>
+__global__ __launch_bounds__(256) void k_neighbors(int n, Geo geo, const double4 *__restrict__ pos,
                                                    const int *__restrict__ key,
                                                    const int *__restrict__ wrap,
                                                    const int *__restrict__ start,
@@ -331,8 +355,11 @@ __global__ __launch_bounds__(256) void k_neighbors(int n, CellGeom g, const doub
 {
     int s = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const CellGeom g = geo.of(s);
+    const int b0 = geo.bin0(s);
+    start += b0;
     double4 ps = pos[s];
-    int k = key[s];
+    int k = key[s] - b0;
     int c0 = k % g.dim[0];
     int c1 = (k / g.dim[0]) % g.dim[1];
     int c2 = k / (g.dim[0] * g.dim[1]);
@@ -698,27 +725,49 @@ __global__ __launch_bounds__(256) void k_neighbors_group(int n, CellGeom g,
 // 27-bin stencil stays complete).
 void nb_configure_grid(System &S, double max_cut)
 {
-    CellGeom &g = S.geom;
     PYSIC_REQUIRE(max_cut > 0.0, PYSIC_EINVAL, "neighbor cutoff must be positive");
-    double w[3];
-    for (int i = 0; i < 3; ++i) {
-        const double *a = g.v + 3 * ((i + 1) % 3), *b = g.v + 3 * ((i + 2) % 3), *v = g.v + 3 * i;
-        double nx = a[1] * b[2] - a[2] * b[1], ny = a[2] * b[0] - a[0] * b[2],
-               nz = a[0] * b[1] - a[1] * b[0];
-        w[i] = std::fabs(v[0] * nx + v[1] * ny + v[2] * nz) / std::sqrt(nx * nx + ny * ny + nz * nz);
-        g.nc[i] = (int)std::floor(w[i] / max_cut);
-        PYSIC_REQUIRE(g.nc[i] >= 1, PYSIC_ERANGE,
-                      "cell narrower than the neighbor cutoff; the cell list needs width >= cutoff");
+    auto configure = [&](CellGeom &g, long natoms) {
+        double w[3];
+        for (int i = 0; i < 3; ++i) {
+            const double *a = g.v + 3 * ((i + 1) % 3), *b = g.v + 3 * ((i + 2) % 3),
+                         *v = g.v + 3 * i;
+            double nx = a[1] * b[2] - a[2] * b[1], ny = a[2] * b[0] - a[0] * b[2],
+                   nz = a[0] * b[1] - a[1] * b[0];
+            w[i] = std::fabs(v[0] * nx + v[1] * ny + v[2] * nz) /
+                   std::sqrt(nx * nx + ny * ny + nz * nz);
+            g.nc[i] = (int)std::floor(w[i] / max_cut);
+            PYSIC_REQUIRE(g.nc[i] >= 1, PYSIC_ERANGE,
+                          "cell narrower than the neighbor cutoff; the cell list needs width >= cutoff");
+        }
+        long limit = std::max<long>(64, 2L * natoms);
+        while ((long)g.nc[0] * g.nc[1] * g.nc[2] > limit) {
+            int a = 0;
+            for (int i = 1; i < 3; ++i)
+                if (g.nc[i] > g.nc[a]) a = i;
+            g.nc[a] = std::max(1, (int)(g.nc[a] * 0.8));
+        }
+        for (int i = 0; i < 3; ++i) g.dim[i] = g.pbc[i] ? g.nc[i] : g.nc[i] + 2;
+        g.ncell = g.dim[0] * g.dim[1] * g.dim[2];
+    };
+    if (S.ncells == 0) {
+        configure(S.geom, S.n);
+        S.nbins = S.geom.ncell;
+    } else {
+        std::vector<int> fb(S.ncells);
+        long total = 0;
+        for (int c = 0; c < S.ncells; ++c) {
+            configure(S.hcells[c], S.hatom0[c + 1] - S.hatom0[c]);
+            fb[c] = (int)total;
+            total += S.hcells[c].ncell;
+        }
+        PYSIC_REQUIRE(total < (1L << 30), PYSIC_ERANGE, "too many cell-list bins in the batch");
+        S.nbins = (int)total;
+        S.cells.ensure(S.ncells);
+        S.first_bin.ensure(S.ncells);
+        HIP_TRY(hipMemcpy(S.cells.p, S.hcells.data(), sizeof(CellGeom) * S.ncells,
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(S.first_bin.p, fb.data(), sizeof(int) * S.ncells, hipMemcpyHostToDevice));
     }
-    long limit = std::max<long>(64, 2L * S.n);
-    while ((long)g.nc[0] * g.nc[1] * g.nc[2] > limit) {
-        int a = 0;
-        for (int i = 1; i < 3; ++i)
-            if (g.nc[i] > g.nc[a]) a = i;
-        g.nc[a] = std::max(1, (int)(g.nc[a] * 0.8));
-    }
-    for (int i = 0; i < 3; ++i) g.dim[i] = g.pbc[i] ? g.nc[i] : g.nc[i] + 2;
-    g.ncell = g.dim[0] * g.dim[1] * g.dim[2];
     S.max_cut = max_cut;
 }
 
@@ -736,15 +785,15 @@ static void swap_bufs(System &S)
 void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
 {
     const int n = S.n;
-    CellGeom &g = S.geom;
+    const int nbins = S.nbins;
     hipStream_t st = S.stream;
     S.key.ensure(n); S.key2.ensure(n);
     S.wrap.ensure(n); S.wrap2.ensure(n);
     S.perm.ensure(n);
-    S.cellcount.ensure(g.ncell);
-    S.cellfill.ensure(g.ncell);
-    S.cellstart.ensure((size_t)g.ncell + 1);
-    int nb = div_up(g.ncell, SCAN_BLK);
+    S.cellcount.ensure(nbins);
+    S.cellfill.ensure(nbins);
+    S.cellstart.ensure((size_t)nbins + 1);
+    int nb = div_up(nbins, SCAN_BLK);
     S.scanblk.ensure(nb);
     S.pos2.ensure(n); S.mom2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
     S.id2.ensure(n);
@@ -753,25 +802,41 @@ void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
 
     hipEvent_t ev;
     S.timer.begin(drift ? "drift_bin_sort" : "bin_sort", st, &ev);
-    HIP_TRY(hipMemsetAsync(S.cellcount.p, 0, sizeof(int) * g.ncell, st));
-    HIP_TRY(hipMemsetAsync(S.cellfill.p, 0, sizeof(int) * g.ncell, st));
-    if (drift)
-        k_drift_bin<<<div_up(n, 256), 256, 0, st>>>(n, g, dt, half_dt, S.frc.p, S.mom.p, S.pos.p,
-                                                    S.key.p, S.wrap.p, S.cellcount.p, S.flags.p);
-    else
-        k_bin<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.key.p, S.wrap.p, S.cellcount.p,
-                                              S.flags.p);
-    k_scan_local<<<nb, SCAN_T, 0, st>>>(g.ncell, S.cellcount.p, S.cellstart.p, S.scanblk.p);
-    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, g.ncell);
-    k_scan_add<<<nb, SCAN_T, 0, st>>>(g.ncell, S.cellstart.p, S.scanblk.p);
+    // fill kernels rather than memset nodes: this sequence may be captured
+    // into a graph
+    k_fill_int<<<div_up(2 * nbins, 256), 256, 0, st>>>(nbins, S.cellcount.p, S.cellfill.p);
+    with_geo(S, [&](auto geo) {
+        using Geo = decltype(geo);
+        if (drift)
+            k_drift_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(n, geo, dt, half_dt, S.frc.p, S.mom.p,
+                                                             S.pos.p, S.key.p, S.wrap.p,
+                                                             S.cellcount.p, S.flags.p);
+        else
+            k_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(n, geo, S.pos.p, S.key.p, S.wrap.p,
+                                                       S.cellcount.p, S.flags.p);
+    });
+    k_scan_local<<<nb, SCAN_T, 0, st>>>(nbins, S.cellcount.p, S.cellstart.p, S.scanblk.p);
+    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, nbins);
+    k_scan_add<<<nb, SCAN_T, 0, st>>>(nbins, S.cellstart.p, S.scanblk.p);
     k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.cellstart.p, S.cellfill.p, S.perm.p);
-    k_cellsort<<<div_up(g.ncell, 256), 256, 0, st>>>(g.ncell, S.cellstart.p, S.perm.p, S.id.p);
-    k_permute<<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p, S.cut.p, S.id.p,
-                                              S.key.p, S.wrap.p, S.pos2.p, S.mom2.p, S.q2.p,
-                                              S.cut2.p, S.id2.p, S.key2.p, S.wrap2.p, g, S.wpos.p);
+    k_cellsort<<<div_up(nbins, 256), 256, 0, st>>>(nbins, S.cellstart.p, S.perm.p, S.id.p);
+    with_geo(S, [&](auto geo) {
+        using Geo = decltype(geo);
+        k_permute<Geo><<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p,
+                                                       S.cut.p, S.id.p, S.key.p, S.wrap.p,
+                                                       S.pos2.p, S.mom2.p, S.q2.p, S.cut2.p,
+                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p);
+    });
     HIP_TRY(hipGetLastError());
     S.timer.end(drift ? "drift_bin_sort" : "bin_sort", st, ev);
     swap_bufs(S);
+}
+
+void nb_clear_flags(System &S)
+{
+    S.flags.ensure(4);
+    k_clear_flags<<<1, 64, 0, S.stream>>>(S.flags.p);
+    HIP_TRY(hipGetLastError());
 }
 
 void nb_build_lists(System &S)
@@ -785,24 +850,32 @@ void nb_build_lists(System &S)
     double interior = (double)g.nc[0] * g.nc[1] * g.nc[2];
     double apb = n / interior;  // atoms per bin
     bool brick = S.nb_method == 2 || S.nb_method == 3;
+    PYSIC_REQUIRE(S.ncells == 0 || S.nb_method == 0 || S.nb_method == 1 || S.nb_method == 5,
+                  PYSIC_EINVAL, "a batch of cells takes neighbor method 0, 1 or 5");
     hipEvent_t ev;
     S.timer.begin("neighbors", st, &ev);
     if (S.nb_method == 0 || S.nb_method == 5) {
         // FP32 prefilter margin: bound on the rounding of the FP32 squared
         // distance of wrapped positions (|error| per component <= ~1.2e-6 L,
-        // taken 2.5x larger); open axes leave positions unbounded: no prefilter
-        float margin2 = 3.0e38f;
-        if (g.pbc[0] && g.pbc[1] && g.pbc[2]) {
+        // taken 2.5x larger); open axes leave positions unbounded: no
+        // prefilter.  A batch takes the largest cell's bound.
+        float margin2 = 0.0f;
+        auto margin_of = [&](const CellGeom &c) {
+            if (!(c.pbc[0] && c.pbc[1] && c.pbc[2])) return 3.0e38f;
             double L = 0.0;
             for (int a = 0; a < 3; ++a)
-                L += std::sqrt(g.v[3 * a] * g.v[3 * a] + g.v[3 * a + 1] * g.v[3 * a + 1] +
-                               g.v[3 * a + 2] * g.v[3 * a + 2]);
+                L += std::sqrt(c.v[3 * a] * c.v[3 * a] + c.v[3 * a + 1] * c.v[3 * a + 1] +
+                               c.v[3 * a + 2] * c.v[3 * a + 2]);
             double delta = 3.0e-6 * L;
-            margin2 = (float)(4.0 * std::sqrt(3.0) * S.max_cut * delta + 12.0 * delta * delta);
-        }
-        k_neighbors_pf<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.wpos.p, S.key.p, S.wrap.p,
-                                                       S.cellstart.p, S.cut.p, S.cap, margin2,
-                                                       S.nbr.p, S.nimg.p, S.ncnt.p, S.flags.p);
+            return (float)(4.0 * std::sqrt(3.0) * S.max_cut * delta + 12.0 * delta * delta);
+        };
+        if (S.ncells == 0) margin2 = margin_of(g);
+        for (int c = 0; c < S.ncells; ++c) margin2 = std::max(margin2, margin_of(S.hcells[c]));
+        with_geo(S, [&](auto geo) {
+            k_neighbors_pf<decltype(geo)><<<div_up(n, 256), 256, 0, st>>>(
+                n, geo, S.pos.p, S.wpos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap,
+                margin2, S.nbr.p, S.nimg.p, S.ncnt.p, S.flags.p);
+        });
     } else if (S.nb_method == 4) {
         // lane-parallel scan; wider groups when the bins are dense
         if (apb * 3.0 <= 10.0)
@@ -814,9 +887,11 @@ void nb_build_lists(System &S)
                 n, g, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap, S.nbr.p, S.nimg.p,
                 S.ncnt.p, S.flags.p);
     } else if (!brick) {
-        k_neighbors<<<div_up(n, 256), 256, 0, st>>>(n, g, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p,
-                                                     S.cut.p, S.cap, S.nbr.p, S.nimg.p, S.ncnt.p,
-                                                     S.flags.p);
+        with_geo(S, [&](auto geo) {
+            k_neighbors<decltype(geo)><<<div_up(n, 256), 256, 0, st>>>(
+                n, geo, S.pos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap, S.nbr.p,
+                S.nimg.p, S.ncnt.p, S.flags.p);
+        });
     } else {
         // grow the brick one bin-edge at a time (smallest edge first) while
         // its expected atom count stays within one pass of 256 threads

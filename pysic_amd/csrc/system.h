@@ -77,6 +77,7 @@ struct System {
     DBuf<float4> dU;            // dU_i / dr_ij per short-list entry
     DBuf<double> partial;       // per-block energy + virial partials
     int partial_blocks = 0;
+    int site_grid = 0;          // > 0: persistent site launch with this many blocks
     DBuf<double> totals;        // reduced energy + virial (7 doubles)
 
     DBuf<double> host_stage;    // scratch for original-order transfers
@@ -86,10 +87,32 @@ struct System {
     DBuf<Model> dmodel;
     bool model_dirty = true;
 
+    // Batch of independent periodic cells (ncells > 0).  The atoms of cell c
+    // occupy slots [hatom0[c], hatom0[c+1]) in every order: the bins of each
+    // cell form their own key range, so the sort moves atoms within cells.
+    int ncells = 0;
+    std::vector<CellGeom> hcells;
+    std::vector<int> hatom0;
+    DBuf<CellGeom> cells;
+    DBuf<int> cell_of, first_bin, atom0;
+    DBuf<double> atom_ev;       // [7][n]: per-atom site energy and virial
+    DBuf<double> cell_ev;       // [ncells][8]: energy, virial per cell
+    int nbins = 0;              // bins of the sort over all cells
+
     KernelTimer timer;
 
     void release_all();
 };
+
+// Calls f with the geometry policy of S (one cell or a batch of cells).
+template <class F>
+void with_geo(System &S, F &&f)
+{
+    if (S.ncells > 0)
+        f(GeoBatch{S.cells.p, S.cell_of.p, S.first_bin.p});
+    else
+        f(GeoOne{S.geom});
+}
 
 // neighbor.hip
 void nb_configure_grid(System &S, double max_cut);
@@ -97,6 +120,7 @@ void nb_sort_atoms(System &S, bool drift = false, double dt = 0.0,
                    double half_dt = 0.0);  // [drift +] bin + counting sort + permute
 void nb_build_lists(System &S);               // cell-list search, reference semantics
 int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
+void nb_clear_flags(System &S);               // zero the flags (kernel; graph-safe)
 
 // tersoff.hip
 // SITE_ENERGY: energy only (FP64); SITE_FORCES: dU/dr_ij + virial (the
@@ -105,6 +129,7 @@ enum { SITE_ENERGY = 1, SITE_FORCES = 2 };
 void ts_site(System &S, int variant, int mode);
 void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
 void ts_reduce(System &S);                    // totals[0..6] = E, W(6)
+void ts_cell_sums(System &S);                 // batch: cell_ev[c] = E, W(6) of cell c
 int ts_variant_count();
 const char *ts_variant_name(int v);
 int ts_variant_capacity(int v);

@@ -107,26 +107,33 @@ struct SiteShared {
     double red[APB][8];
 };
 
-template <int G, int SCAP>
-__global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4 *__restrict__ pos,
-                                                const int *__restrict__ id,
-                                                const int *__restrict__ nbr,
-                                                const int *__restrict__ nimg,
-                                                const int *__restrict__ ncnt,
-                                                const Model *__restrict__ M,
-                                                int *__restrict__ sj, int *__restrict__ simg,
-                                                int *__restrict__ scnt, float4 *__restrict__ dU,
-                                                double *__restrict__ partial,
-                                                int *__restrict__ flags)
+template <int G, int SCAP, class Geo>
+__device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
+                                          const double4 *__restrict__ pos,
+                                          const int *__restrict__ id,
+                                          const int *__restrict__ nbr,
+                                          const int *__restrict__ nimg,
+                                          const int *__restrict__ ncnt,
+                                          const Model *__restrict__ M,
+                                          int *__restrict__ sj, int *__restrict__ simg,
+                                          int *__restrict__ scnt, float4 *__restrict__ dU,
+                                          double *__restrict__ partial,
+                                          double *__restrict__ atom_ev,
+                                          int *__restrict__ flags)
 {
     using SH = SiteShared<G, SCAP>;
     constexpr int APB = SH::APB;
     __shared__ SH sh;
     const int a = threadIdx.x / G;
     const int l = threadIdx.x % G;
-    const int blk = xcd_block(gridDim.x);
     const int i = blk * APB + a;
     const bool valid = i < n;
+    double cv[9];  // vectors of this atom's cell
+    {
+        const CellGeom &cg = geo.of(valid ? i : 0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
+    }
     const int wl = threadIdx.x & 63;
     const int gbase = (wl / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
@@ -148,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
             j = nbr[(size_t)e * n + i];
             im = nimg[(size_t)e * n + i];
             double4 pj = pos[j];
-            double3 sp3 = separation64(pi, pj, im, g.v);
+            double3 sp3 = separation64(pi, pj, im, cv);
             double r = sqrt(dot64(sp3));
             d = make_double4(sp3.x, sp3.y, sp3.z, r);
             sp = (int)pj.w;
@@ -323,6 +330,8 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
     }
     if (l == 0)
         for (int q = 0; q < 7; ++q) sh.red[a][q] = valid ? vals[q] : 0.0;
+    if (atom_ev && valid && l == 0)
+        for (int q = 0; q < 7; ++q) atom_ev[(size_t)q * n + i] = vals[q];
     __syncthreads();
     if (threadIdx.x < 7) {
         double s = 0.0;
@@ -343,14 +352,16 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, CellGeom g, const double4
 //             virial in FP64.  A velocity-Verlet step needs exactly this, as
 //             calculate_forces returns forces and stress (PyInterface.f90:668).
 // ---------------------------------------------------------------------------
-template <int G, int SCAP>
-__global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const double4 *__restrict__ pos,
-                                                  const int *__restrict__ nbr,
-                                                  const int *__restrict__ nimg,
-                                                  const int *__restrict__ ncnt,
-                                                  const Model *__restrict__ M,
-                                                  double *__restrict__ partial,
-                                                  int *__restrict__ flags)
+template <int G, int SCAP, class Geo>
+__device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
+                                            const double4 *__restrict__ pos,
+                                            const int *__restrict__ nbr,
+                                            const int *__restrict__ nimg,
+                                            const int *__restrict__ ncnt,
+                                            const Model *__restrict__ M,
+                                            double *__restrict__ partial,
+                                            double *__restrict__ atom_ev,
+                                            int *__restrict__ flags)
 {
     constexpr int APB = BLOCK / G;
     constexpr int PER = (SCAP + G - 1) / G;
@@ -361,9 +372,14 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const doubl
 
     const int a = threadIdx.x / G;
     const int l = threadIdx.x % G;
-    const int blk = xcd_block(gridDim.x);
     const int i = blk * APB + a;
     const bool valid = i < n;
+    double cv[9];  // vectors of this atom's cell
+    {
+        const CellGeom &cg = geo.of(valid ? i : 0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
+    }
     const int gbase = ((threadIdx.x & 63) / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
@@ -383,7 +399,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const doubl
         if (e < nf) {
             int j = nbr[(size_t)e * n + i];
             int im = nimg[(size_t)e * n + i];
-            double3 sp3 = separation64(pi, pos[j], im, g.v);
+            double3 sp3 = separation64(pi, pos[j], im, cv);
             double r = sqrt(dot64(sp3));
             d = make_double4(sp3.x, sp3.y, sp3.z, r);
             keep = r < rshort;
@@ -438,6 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const doubl
     }
     for (int off = G / 2; off > 0; off >>= 1) e_lane += __shfl_xor(e_lane, off, G);
     if (l == 0) s_red[a] = valid ? e_lane : 0.0;
+    if (atom_ev && valid && l == 0) atom_ev[i] = e_lane;
     __syncthreads();
     if (threadIdx.x == 0) {
         double sacc = 0.0;
@@ -447,16 +464,18 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, CellGeom g, const doubl
     }
 }
 
-template <int G, int SCAP>
-__global__ __launch_bounds__(BLOCK) void k_site1f(int n, CellGeom g, const double4 *__restrict__ pos,
-                                                  const int *__restrict__ nbr,
-                                                  const int *__restrict__ nimg,
-                                                  const int *__restrict__ ncnt,
-                                                  const Model *__restrict__ M,
-                                                  int *__restrict__ sj, int *__restrict__ simg,
-                                                  int *__restrict__ scnt, float4 *__restrict__ dU,
-                                                  double *__restrict__ partial,
-                                                  int *__restrict__ flags)
+template <int G, int SCAP, class Geo>
+__device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
+                                            const double4 *__restrict__ pos,
+                                            const int *__restrict__ nbr,
+                                            const int *__restrict__ nimg,
+                                            const int *__restrict__ ncnt,
+                                            const Model *__restrict__ M,
+                                            int *__restrict__ sj, int *__restrict__ simg,
+                                            int *__restrict__ scnt, float4 *__restrict__ dU,
+                                            double *__restrict__ partial,
+                                            double *__restrict__ atom_ev,
+                                            int *__restrict__ flags)
 {
     constexpr int APB = BLOCK / G;
     constexpr int PER = (SCAP + G - 1) / G;
@@ -468,9 +487,14 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, CellGeom g, const doubl
 
     const int a = threadIdx.x / G;
     const int l = threadIdx.x % G;
-    const int blk = xcd_block(gridDim.x);
     const int i = blk * APB + a;
     const bool valid = i < n;
+    double cv[9];  // vectors of this atom's cell
+    {
+        const CellGeom &cg = geo.of(valid ? i : 0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
+    }
     const int gbase = ((threadIdx.x & 63) / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
@@ -496,7 +520,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, CellGeom g, const doubl
         if (e < nf) {
             j = nbr[(size_t)e * n + i];
             im = nimg[(size_t)e * n + i];
-            sp3 = separation64(pi, pos[j], im, g.v);
+            sp3 = separation64(pi, pos[j], im, cv);
             r = sqrt(dot64(sp3));
             keep = r < rshort;
         }
@@ -636,12 +660,113 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, CellGeom g, const doubl
         for (int off = G / 2; off > 0; off >>= 1) w6[q] += __shfl_xor(w6[q], off, G);
     if (l == 0)
         for (int q = 0; q < 6; ++q) s_red[a][1 + q] = valid ? w6[q] : 0.0;
+    if (atom_ev && valid && l == 0)
+        for (int q = 0; q < 6; ++q) atom_ev[(size_t)(1 + q) * n + i] = w6[q];
     __syncthreads();
     if (threadIdx.x < 7) {
         double sacc = 0.0;
         if (threadIdx.x > 0)
             for (int t = 0; t < APB; ++t) sacc += s_red[t][threadIdx.x];
         partial[(size_t)blk * 8 + threadIdx.x] = sacc;  // energy slot 0 left at 0
+    }
+}
+
+// Kernels over tiles of BLOCK / G atoms.  With one block per tile each block
+// takes its XCD-remapped tile; with fewer blocks (a persistent launch) block
+// b walks the contiguous tile range [b T / grid, (b + 1) T / grid), so one
+// launch covers any number of cells and each XCD still sees a contiguous run
+// of atoms.
+__device__ __forceinline__ void tile_range(int ntiles, int &t0, int &t1)
+{
+    const long pb = xcd_block(gridDim.x);
+    t0 = (int)(pb * ntiles / gridDim.x);
+    t1 = (int)((pb + 1) * ntiles / gridDim.x);
+}
+
+template <int G, int SCAP, class Geo>
+__global__ __launch_bounds__(BLOCK) void k_site(int n, int ntiles, Geo geo,
+                                                const double4 *__restrict__ pos,
+                                                const int *__restrict__ id,
+                                                const int *__restrict__ nbr,
+                                                const int *__restrict__ nimg,
+                                                const int *__restrict__ ncnt,
+                                                const Model *__restrict__ M,
+                                                int *__restrict__ sj, int *__restrict__ simg,
+                                                int *__restrict__ scnt, float4 *__restrict__ dU,
+                                                double *__restrict__ partial,
+                                                double *__restrict__ atom_ev,
+                                                int *__restrict__ flags)
+{
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        site_tile<G, SCAP>(t, n, geo, pos, id, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
+                           atom_ev, flags);
+        __syncthreads();
+    }
+}
+
+template <int G, int SCAP, class Geo>
+__global__ __launch_bounds__(BLOCK) void k_site1e(int n, int ntiles, Geo geo,
+                                                  const double4 *__restrict__ pos,
+                                                  const int *__restrict__ nbr,
+                                                  const int *__restrict__ nimg,
+                                                  const int *__restrict__ ncnt,
+                                                  const Model *__restrict__ M,
+                                                  double *__restrict__ partial,
+                                                  double *__restrict__ atom_ev,
+                                                  int *__restrict__ flags)
+{
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        site1e_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, partial, atom_ev, flags);
+        __syncthreads();
+    }
+}
+
+template <int G, int SCAP, class Geo>
+__global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
+                                                  const double4 *__restrict__ pos,
+                                                  const int *__restrict__ nbr,
+                                                  const int *__restrict__ nimg,
+                                                  const int *__restrict__ ncnt,
+                                                  const Model *__restrict__ M,
+                                                  int *__restrict__ sj, int *__restrict__ simg,
+                                                  int *__restrict__ scnt, float4 *__restrict__ dU,
+                                                  double *__restrict__ partial,
+                                                  double *__restrict__ atom_ev,
+                                                  int *__restrict__ flags)
+{
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        site1f_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
+                             atom_ev, flags);
+        __syncthreads();
+    }
+}
+
+// Per-cell sums of the per-atom energies and virials (one block per cell,
+// fixed order).
+__global__ __launch_bounds__(256) void k_cell_sums(int n, const int *__restrict__ atom0,
+                                                   const double *__restrict__ atom_ev, int ncomp0,
+                                                   int ncomp1, double *__restrict__ cell_ev)
+{
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    const int a0 = atom0[c], a1 = atom0[c + 1];
+    for (int q = ncomp0; q < ncomp1; ++q) {
+        double s = 0.0;
+        for (int i = a0 + threadIdx.x; i < a1; i += 256) s += atom_ev[(size_t)q * n + i];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) cell_ev[(size_t)c * 8 + q] = red[0];
+        __syncthreads();
     }
 }
 
@@ -720,19 +845,30 @@ template <int G, int SCAP>
 void launch_site(System &S, int mode)
 {
     constexpr int APB = BLOCK / G;
-    int nb = div_up(S.n, APB);
-    S.partial.ensure((size_t)nb * 8);
-    if (S.model.nspecies == 1 && mode == SITE_ENERGY)
-        k_site1e<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
-                                                     S.ncnt.p, S.dmodel.p, S.partial.p, S.flags.p);
-    else if (S.model.nspecies == 1 && mode == SITE_FORCES)
-        k_site1f<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
-                                                     S.ncnt.p, S.dmodel.p, S.sj.p, S.simg.p,
-                                                     S.scnt.p, S.dU.p, S.partial.p, S.flags.p);
-    else
-        k_site<G, SCAP><<<nb, BLOCK, 0, S.stream>>>(S.n, S.geom, S.pos.p, S.id.p, S.nbr.p, S.nimg.p,
-                                                   S.ncnt.p, S.dmodel.p, S.sj.p, S.simg.p, S.scnt.p,
-                                                   S.dU.p, S.partial.p, S.flags.p);
+    const int ntiles = div_up(S.n, APB);
+    // persistent launch: at most site_grid blocks walk all tiles
+    const int grid = (S.site_grid > 0 && S.site_grid < ntiles) ? S.site_grid : ntiles;
+    S.partial.ensure((size_t)ntiles * 8);
+    double *aev = nullptr;
+    if (S.ncells > 0) {
+        S.atom_ev.ensure((size_t)7 * S.n);
+        aev = S.atom_ev.p;
+    }
+    with_geo(S, [&](auto geo) {
+        using Geo = decltype(geo);
+        if (S.model.nspecies == 1 && mode == SITE_ENERGY)
+            k_site1e<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.partial.p,
+                aev, S.flags.p);
+        else if (S.model.nspecies == 1 && mode == SITE_FORCES)
+            k_site1f<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
+                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+        else
+            k_site<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
+                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+    });
 }
 
 }  // namespace
@@ -782,6 +918,16 @@ void ts_gather(System &S, double half_dt, int kick)
                                                      S.frc.p, S.mom.p, half_dt, kick, S.flags.p);
     HIP_TRY(hipGetLastError());
     S.timer.end("tersoff_gather", S.stream, ev);
+}
+
+void ts_cell_sums(System &S)
+{
+    PYSIC_REQUIRE(S.ncells > 0, PYSIC_ESTATE, "not a batch of cells");
+    S.cell_ev.ensure((size_t)8 * S.ncells);
+    // the energy kernel writes component 0, the force kernels 1..6; the
+    // generic kernel writes all seven
+    k_cell_sums<<<S.ncells, 256, 0, S.stream>>>(S.n, S.atom0.p, S.atom_ev.p, 0, 7, S.cell_ev.p);
+    HIP_TRY(hipGetLastError());
 }
 
 void ts_reduce(System &S)

@@ -56,6 +56,13 @@ SIGNATURES = {
     "pysic_timing_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                          _lp]),
     "pysic_synchronize": (ctypes.c_int, [_vp]),
+    # batches of cells (include/pysic_batch.h)
+    "pysic_set_cells": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _ip]),
+    "pysic_get_number_of_cells": (ctypes.c_int, [_vp, _ip]),
+    "pysic_set_site_grid": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pysic_evaluate_cells": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "pysic_check_cells": (ctypes.c_int, [_vp, _ip]),
+    "pysic_get_cell_results": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
     # learned charge warm start (include/pysic_learned.h)
     "pysic_mlp_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     "pysic_mlp_destroy": (None, [_vp]),
@@ -253,6 +260,38 @@ class Engine:
         mom = np.zeros((self.n, 3))
         self._check(self.L.pysic_get_state(self.h, self.n, _d(pos), _d(mom)), "get_state")
         return pos, mom
+
+    # --- batches of cells -------------------------------------------------
+    def set_cells(self, natoms, cells, pbc=None):
+        """Cell c owns the next natoms[c] atoms (in create_atoms order)."""
+        na = np.ascontiguousarray(natoms, dtype=np.int32)
+        cv = _f64(cells, (-1, 3, 3))
+        inv = np.ascontiguousarray(np.transpose(np.linalg.inv(cv), (0, 2, 1)))
+        pb = np.ones((len(na), 3), dtype=np.int32) if pbc is None else \
+            np.ascontiguousarray(pbc, dtype=np.int32).reshape(-1, 3)
+        self._check(self.L.pysic_set_cells(self.h, len(na), _i(na), _d(cv), _d(inv), _i(pb)),
+                    "set_cells")
+        self.ncells = len(na)
+
+    def set_site_grid(self, blocks):
+        self._check(self.L.pysic_set_site_grid(self.h, int(blocks)), "set_site_grid")
+
+    def evaluate_cells(self, use_graph=False, rebuild=True):
+        self._check(self.L.pysic_evaluate_cells(self.h, 1 if use_graph else 0, 1 if rebuild else 0),
+                    "evaluate_cells")
+
+    def check_cells(self):
+        f = np.zeros(4, dtype=np.int32)
+        self._check(self.L.pysic_check_cells(self.h, _i(f)), "check_cells")
+        return f
+
+    def cell_results(self, forces=True):
+        e = np.zeros(self.ncells)
+        w = np.zeros((self.ncells, 6))
+        f = np.zeros((self.n, 3)) if forces else None
+        self._check(self.L.pysic_get_cell_results(self.h, self.ncells, _d(e), _d(w), self.n, _d(f)),
+                    "get_cell_results")
+        return e, w, f
 
     # --- tuning / measurement -------------------------------------------
     def set_variant(self, v):

@@ -180,16 +180,6 @@ def amorphous_si(n, density_gcc, seed, dmin=2.1):
     return Structure(["Si"] * n, pos, np.diag([box] * 3))
 
 
-def batched_amorphous_si(ncells=512, nmin=64, nmax=2048, dmin=2.1, dens=(2.25, 2.35), seed=0):
-    """The batched workload: independent a-Si cells of uneven size."""
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(nmin, nmax + 1, size=ncells)
-    densities = rng.uniform(dens[0], dens[1], size=ncells)
-    seeds = rng.integers(0, 2 ** 31 - 1, size=ncells)
-    return [amorphous_si(int(s), float(d), int(sd), dmin) for s, d, sd in
-            zip(sizes, densities, seeds)]
-
-
 def amorphous_sio2(n_si, density_gcc, seed, dmin=1.5):
     """Random-addition SiO2 (1:2) cell at the given density."""
     rng = np.random.default_rng(seed)

@@ -9,7 +9,8 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("pysic_hip.h", "pysic_learned.h")]
+HEADERS = [os.path.join(ROOT, "include", h)
+           for h in ("pysic_hip.h", "pysic_learned.h", "pysic_batch.h")]
 LIB = os.path.join(ROOT, "pysic_amd", "libpysic_hip.so")
 
 
@@ -26,7 +27,8 @@ def test_header_declares_the_boundary():
               "pysic_generate_neighbor_lists", "pysic_calculate_energy",
               "pysic_calculate_forces", "pysic_calculate_electronegativities",
               "pysic_relax_charges", "pysic_md_run", "pysic_mlp_predict_charges",
-              "pysic_mlp_train", "pysic_mlp_gradients"):
+              "pysic_mlp_train", "pysic_mlp_gradients", "pysic_set_cells",
+              "pysic_evaluate_cells", "pysic_get_cell_results"):
         assert s in syms
 
 
