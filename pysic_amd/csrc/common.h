@@ -111,15 +111,19 @@ struct CellGeom {
 // owning a fixed range of atom slots (the sort reorders atoms only within
 // their cell, because every cell's bins form their own key range).
 struct GeoOne {
+    static constexpr bool kPersistent = false;  // one workgroup per tile
     CellGeom g;
     __device__ __forceinline__ const CellGeom &of(int) const { return g; }
+    __device__ __forceinline__ const double *vec(int) const { return g.v; }
     __device__ __forceinline__ int bin0(int) const { return 0; }
 };
 struct GeoBatch {
+    static constexpr bool kPersistent = true;   // site kernels may walk many tiles
     const CellGeom *cells;   // per cell
     const int *cell_of;      // cell of each atom slot
     const int *first_bin;    // first global bin of each cell
     __device__ __forceinline__ const CellGeom &of(int s) const { return cells[cell_of[s]]; }
+    __device__ __forceinline__ const double *vec(int s) const { return cells[cell_of[s]].v; }
     __device__ __forceinline__ int bin0(int s) const { return first_bin[cell_of[s]]; }
 };
 

@@ -229,10 +229,10 @@ __global__ void k_recip_energy(int nk, const double4 *__restrict__ kvec,
 // one chunk of segments (natural order, so each tile of STILE segments covers
 // one contiguous k range); block (atom tile, segment chunk).  Each thread
 // carries RAT atoms (i, i + EB, ...) so that every k-vector and S(k) read
-// from LDS serves RAT atoms: a wave-wide LDS read moves 1 KB into registers,
-// and with one atom per thread that traffic, not the FP64 arithmetic, was
-// the limit.
-constexpr int RAT = 4;
+// from LDS serves RAT atoms.  Measured on the 24,576-atom SiO2 cell: RAT = 1
+// 1.20 ms, 2: 0.89 ms, 4: 1.32 ms (occupancy 3 waves/SIMD); scalar-cache
+// reads instead of LDS: 0.96 ms.
+constexpr int RAT = 2;
 
 __global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, int mode, Recip R,
                                                    const int4 *__restrict__ segs,
@@ -601,7 +601,7 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
 {
     const int n = S.n, nk = E.nk;
     int atiles = div_up(n, EB * RAT);
-    int nchunk = std::max(1, std::min(div_up(E.nseg, STILE), div_up(1024, atiles)));
+    int nchunk = std::max(1, std::min(div_up(E.nseg, STILE), div_up(2048, atiles)));
     int schunk = div_up(div_up(E.nseg, nchunk), STILE) * STILE;
     nchunk = nk > 0 ? div_up(E.nseg, schunk) : 0;
     if (nchunk == 0) {

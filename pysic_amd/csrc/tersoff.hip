@@ -128,12 +128,7 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
     const int l = threadIdx.x % G;
     const int i = blk * APB + a;
     const bool valid = i < n;
-    double cv[9];  // vectors of this atom's cell
-    {
-        const CellGeom &cg = geo.of(valid ? i : 0);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
-    }
+    const double *cv = geo.vec(valid ? i : 0);  // vectors of this atom's cell
     const int wl = threadIdx.x & 63;
     const int gbase = (wl / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
@@ -374,12 +369,7 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
     const int l = threadIdx.x % G;
     const int i = blk * APB + a;
     const bool valid = i < n;
-    double cv[9];  // vectors of this atom's cell
-    {
-        const CellGeom &cg = geo.of(valid ? i : 0);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
-    }
+    const double *cv = geo.vec(valid ? i : 0);  // vectors of this atom's cell
     const int gbase = ((threadIdx.x & 63) / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
@@ -489,12 +479,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
     const int l = threadIdx.x % G;
     const int i = blk * APB + a;
     const bool valid = i < n;
-    double cv[9];  // vectors of this atom's cell
-    {
-        const CellGeom &cg = geo.of(valid ? i : 0);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) cv[q] = cg.v[q];
-    }
+    const double *cv = geo.vec(valid ? i : 0);  // vectors of this atom's cell
     const int gbase = ((threadIdx.x & 63) / G) * G;
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
 
@@ -676,11 +661,17 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
 // b walks the contiguous tile range [b T / grid, (b + 1) T / grid), so one
 // launch covers any number of cells and each XCD still sees a contiguous run
 // of atoms.
+template <class Geo>
 __device__ __forceinline__ void tile_range(int ntiles, int &t0, int &t1)
 {
     const long pb = xcd_block(gridDim.x);
-    t0 = (int)(pb * ntiles / gridDim.x);
-    t1 = (int)((pb + 1) * ntiles / gridDim.x);
+    if constexpr (Geo::kPersistent) {
+        t0 = (int)(pb * ntiles / gridDim.x);
+        t1 = (int)((pb + 1) * ntiles / gridDim.x);
+    } else {
+        t0 = (int)pb;   // the launch has exactly one block per tile
+        t1 = t0 + 1;
+    }
 }
 
 template <int G, int SCAP, class Geo>
@@ -698,7 +689,7 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, int ntiles, Geo geo,
                                                 int *__restrict__ flags)
 {
     int t0, t1;
-    tile_range(ntiles, t0, t1);
+    tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
         site_tile<G, SCAP>(t, n, geo, pos, id, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
                            atom_ev, flags);
@@ -718,7 +709,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, int ntiles, Geo geo,
                                                   int *__restrict__ flags)
 {
     int t0, t1;
-    tile_range(ntiles, t0, t1);
+    tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
         site1e_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, partial, atom_ev, flags);
         __syncthreads();
@@ -739,7 +730,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
                                                   int *__restrict__ flags)
 {
     int t0, t1;
-    tile_range(ntiles, t0, t1);
+    tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
         site1f_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
                              atom_ev, flags);
@@ -846,8 +837,10 @@ void launch_site(System &S, int mode)
 {
     constexpr int APB = BLOCK / G;
     const int ntiles = div_up(S.n, APB);
-    // persistent launch: at most site_grid blocks walk all tiles
-    const int grid = (S.site_grid > 0 && S.site_grid < ntiles) ? S.site_grid : ntiles;
+    // persistent launch (batches of cells): at most site_grid blocks walk
+    // all tiles; one cell: one block per tile
+    const int grid =
+        (S.ncells > 0 && S.site_grid > 0 && S.site_grid < ntiles) ? S.site_grid : ntiles;
     S.partial.ensure((size_t)ntiles * 8);
     double *aev = nullptr;
     if (S.ncells > 0) {
