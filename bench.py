@@ -223,6 +223,101 @@ def batched_benchmark(cells, t_build, reps=20, seq_cells=32):
             "energy_per_atom_mean_eV": float(E.sum() / n)}
 
 
+def ewald_kcells(cell, kc):
+    rec = 2.0 * np.pi * np.linalg.inv(cell).T
+    vol = rec[0] @ np.cross(rec[1], rec[2])
+    return [int(kc * np.linalg.norm(np.cross(rec[(a + 1) % 3], rec[(a + 2) % 3])) / vol + 0.5)
+            for a in range(3)]
+
+
+EWALD_RC, EWALD_SIGMA, EWALD_KC, EPS0 = 10.0, 2.0443, 2.5713, 0.00552635
+
+
+def ewald_benchmark(reps=3, ref_reps=(4, 4, 4)):
+    """Ewald Coulomb on rocksalt: charges +-1 e, a = 5.64 A, 32^3 conventional
+    cells (262,144 ions), sigma = 0.08 A seeded displacements, real-space
+    cutoff 10 A, Gaussian width 2.0443 A and k-cutoff 2.5713 1/A (1e-6
+    relative error each), FP64 throughout.  One evaluation = energy +
+    forces + stress (two structure-factor passes, as calculate_energy and
+    calculate_forces are separate calls).  The reference core is timed on a
+    4^3 replica (512 ions) and scaled by N * N_k, the reciprocal work."""
+    from pysic_amd import structures as st
+    from pysic_amd.engine import Engine
+    s = st.rocksalt((32, 32, 32), a=5.64, sigma=0.08, seed=0)
+    n = len(s)
+    K = ewald_kcells(s.cell, EWALD_KC)
+    eng = Engine(0)
+    eng.create_atoms((s.charges < 0).astype(np.int32), s.positions, charges=s.charges)
+    eng.create_cell(s.cell, (True, True, True))
+    eng.set_potentials(np.zeros((0, 10)))
+    eng.set_bond_order_factors(np.zeros((0, 14)))
+    eng.set_ewald(EWALD_RC, EWALD_KC, K, EWALD_SIGMA, EPS0)
+    eng.set_neighbor_capacity(256)
+    eng.generate_neighbor_lists(np.full(n, EWALD_RC + 0.5))
+    e = eng.energy()
+    f, w = eng.forces()
+    eng.synchronize()
+    eng.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e = eng.energy()
+        f, w = eng.forces()
+    eng.synchronize()
+    t_eval = (time.perf_counter() - t0) / reps
+    tim = eng.timing_read()
+    eng.timing(False)
+    eng.close()
+    # the number of k-vectors in the half space |k| < kc
+    rec = 2.0 * np.pi * np.linalg.inv(s.cell).T
+    g = np.array(np.meshgrid(np.arange(-K[0], K[0] + 1), np.arange(-K[1], K[1] + 1),
+                             np.arange(0, K[2] + 1), indexing="ij")).reshape(3, -1).T
+    kv = g @ rec
+    k2 = (kv ** 2).sum(axis=1)
+    nk = int(((k2 < EWALD_KC ** 2) & (k2 > 0)).sum())
+    sk_us = tim.get("ewald_sk", (0.0, 1))
+    sk_avg = sk_us[0] / max(sk_us[1], 1) / 1e3
+    # algorithmic work of one structure-factor pass: per (k, ion) one complex
+    # multiply-accumulate of q exp(i k.r): 10 FP64 flops in the recurrence form
+    flops = 10.0 * nk * n
+    out = {"workload": "rocksalt 32^3 cells, 262,144 ions, sigma 0.08 A, Ewald rc 10 A "
+                       "sigma 2.0443 A kc 2.5713 1/A (1e-6), FP64",
+           "ions": n, "k_vectors_half_space": nk, "evaluation": "energy + forces + stress",
+           "ms_per_evaluation": round(t_eval * 1e3, 2),
+           "energy_eV_per_ion": e / n,
+           "kernels_us": {k: round(v[0] / max(v[1], 1) * 1e3, 1) for k, v in tim.items()},
+           "structure_factor_roofline": {"bound": "fp64", "achieved": round(flops / sk_avg / 1e12, 2),
+                                         "peak": 78.6, "unit": "TFLOP/s",
+                                         "frac": round(flops / sk_avg / 1e12 / 78.6, 4)}}
+    # the reference core on a small replica
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if O.ref_available():
+            r = st.rocksalt(ref_reps, a=5.64, sigma=0.08, seed=0)
+            Kr = ewald_kcells(r.cell, EWALD_KC)
+            R = O.RefCore()
+            R.setup(r.symbols, r.positions, r.cell, [True] * 3, charges=r.charges,
+                    cutoffs=np.full(len(r), EWALD_RC + 0.5),
+                    ewald=(EWALD_RC, EWALD_KC, Kr, EWALD_SIGMA, EPS0))
+            R.energy()
+            t0 = time.perf_counter()
+            R.energy()
+            R.forces()
+            t_ref = time.perf_counter() - t0
+            recr = 2.0 * np.pi * np.linalg.inv(r.cell).T
+            gr = np.array(np.meshgrid(np.arange(-Kr[0], Kr[0] + 1), np.arange(-Kr[1], Kr[1] + 1),
+                                      np.arange(0, Kr[2] + 1), indexing="ij")).reshape(3, -1).T
+            k2r = ((gr @ recr) ** 2).sum(axis=1)
+            nkr = int(((k2r < EWALD_KC ** 2) & (k2r > 0)).sum())
+            out["reference_sample"] = {
+                "ions": len(r), "k_vectors": nkr, "seconds": round(t_ref, 4), "cores": 1,
+                "kind": "reference",
+                "extrapolated_seconds_by_N_Nk": round(t_ref * (n * nk) / (len(r) * nkr), 1)}
+    except Exception as exc:  # the checker is optional on the GPU host
+        out["reference_sample"] = {"error": str(exc)}
+    return out
+
+
 def cpu_baseline(seconds_budget=20.0):
     """Reference Fortran core on the host: VV steps (rebuild + forces) on a
     16x16x16-cell sample (32,768 atoms) of the same workload."""
@@ -276,6 +371,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-qeq", action="store_true", help="skip the QEq warm/cold measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cells measurement")
+    ap.add_argument("--no-ewald", action="store_true", help="skip the Ewald rocksalt measurement")
+    ap.add_argument("--only", choices=["qeq", "batch", "ewald"],
+                    help="run one secondary measurement alone and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (PMC traffic calibration)")
     args = ap.parse_args()
@@ -283,6 +381,16 @@ def main():
         print("bench.py: runs on one GPU only (--gpus 1); the multi-GPU decomposition of this "
               "path is designed but not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.only:
+        if args.only == "batch":
+            res = batched_benchmark(*build_batched_cells())
+        elif args.only == "qeq":
+            res = qeq_benchmark(os.path.join(ROOT, "pysic_amd", "data", "sio2_eem_mlp.npz"))
+        else:
+            res = ewald_benchmark()
+        print(json.dumps({args.only: res}))
+        return
 
     # CPU-side structures first: the batch is built in worker processes,
     # which must start before this process initialises the GPU
@@ -347,6 +455,7 @@ def main():
     model = os.path.join(ROOT, "pysic_amd", "data", "sio2_eem_mlp.npz")
     qeq = None if args.no_qeq else qeq_benchmark(model)
     batched = None if batch_cells is None else batched_benchmark(*batch_cells)
+    ewald = None if args.no_ewald else ewald_benchmark()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {"metric": "atom-steps/s Tersoff 1M-atom Si; QEq time-to-1e-6 learned vs cold start",
            "value": round(value, 1),
@@ -360,7 +469,7 @@ def main():
                       "kernel_variant": variant_name,
                       "total_energy_drift_eV_per_atom": drift},
            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "qeq": qeq,
-           "batched": batched}
+           "batched": batched, "ewald": ewald}
     print(json.dumps(out))
 
 

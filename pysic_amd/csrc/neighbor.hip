@@ -252,12 +252,18 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
                           (float)(p.z + wx * g.v[2] + wy * g.v[5] + wz * g.v[8]), 0.f);
 }
 
-// Per-atom scan with an FP32 prefilter: candidates are first tested on
-// FP32 wrapped positions (16 B per candidate) against cutoff^2 + margin2,
-// where margin2 bounds the FP32 rounding error of the squared distance; only
-// the survivors take the exact FP64 test in the reference's operation order,
-// so the accepted set is the exact one.  The x-adjacent bins of a column are
-// consecutive in memory, so each column costs four bin-start loads.
+// Per-atom scan with an FP32 prefilter, in two passes.  Pass 1 tests every
+// candidate of the 27 bins on FP32 wrapped positions (16 B per candidate)
+// against cutoff^2 + margin2, where margin2 bounds the FP32 rounding error
+// of the squared distance, and queues the survivors (slot, bin shift) in
+// this lane's LDS column.  Pass 2 takes the exact FP64 test, in the
+// reference's operation order, on the queued survivors only, so the accepted
+// set is the exact one.  The split matters on a SIMD: tested inline, the
+// FP64 branch is entered whenever any lane of the wave has a survivor, which
+// is almost every candidate; queued, every lane works through its ~4-5
+// survivors together.  A full queue is drained in place.
+constexpr int NB_QUEUE = 12;
+
 template <class Geo>
 __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                                                       const double4 *__restrict__ pos,
@@ -271,8 +277,11 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                                                       int *__restrict__ ncnt,
                                                       int *__restrict__ flags)
 {
+    __shared__ int q_slot[NB_QUEUE][256];
+    __shared__ int q_shift[NB_QUEUE][256];
     int s = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    if (s >= n) return;   // no barrier below: lanes only touch their own LDS column
+    const int tid = threadIdx.x;
     const CellGeom g = geo.of(s);
     const int b0 = geo.bin0(s);
     start += b0;
@@ -290,7 +299,29 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
     const float v0x = (float)g.v[0], v0y = (float)g.v[1], v0z = (float)g.v[2];
     const float v1x = (float)g.v[3], v1y = (float)g.v[4], v1z = (float)g.v[5];
     const float v2x = (float)g.v[6], v2y = (float)g.v[7], v2z = (float)g.v[8];
-    int count = 0;
+    int count = 0, nq = 0;
+    // exact test of the queued survivors, in queue order
+    auto drain = [&]() {
+        for (int u = 0; u < nq; ++u) {
+            const int t = q_slot[u][tid];
+            const int sh = q_shift[u][tid];
+            const int wt = wrap[t];
+            const int ox = img_x(sh) + img_x(wt) - wsx;
+            const int oy = img_y(sh) + img_y(wt) - wsy;
+            const int oz = img_z(sh) + img_z(wt) - wsz;
+            if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
+            const int im = pack_img(ox, oy, oz);
+            const double d2 = dot64(separation64(ps, pos[t], im, g.v));
+            if (d2 < c2lim) {
+                if (count < cap) {
+                    nbr[(size_t)count * n + s] = t;
+                    nimg[(size_t)count * n + s] = im;
+                }
+                ++count;
+            }
+        }
+        nq = 0;
+    };
     for (int dz = -1; dz <= 1; ++dz) {
         int bz = c2 + dz, sz = 0;
         if (g.pbc[2]) {
@@ -316,29 +347,22 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                 const float shx = sxf * v0x + syf * v1x + szf * v2x - qs.x;
                 const float shy = sxf * v0y + syf * v1y + szf * v2y - qs.y;
                 const float shz = sxf * v0z + syf * v1z + szf * v2z - qs.z;
+                const int shift = pack_img(sx, sy, sz);
                 const int t1 = start[row + bx + 1];
                 for (int t = start[row + bx]; t < t1; ++t) {
-                    float4 qt = wpos[t];
-                    float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
-                    if (!(fx * fx + fy * fy + fz * fz < c2f)) continue;
-                    int wt = wrap[t];
-                    int ox = sx + img_x(wt) - wsx;
-                    int oy = sy + img_y(wt) - wsy;
-                    int oz = sz + img_z(wt) - wsz;
-                    if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
-                    int im = pack_img(ox, oy, oz);
-                    double d2 = dot64(separation64(ps, pos[t], im, g.v));
-                    if (d2 < c2lim) {
-                        if (count < cap) {
-                            nbr[(size_t)count * n + s] = t;
-                            nimg[(size_t)count * n + s] = im;
-                        }
-                        ++count;
+                    const float4 qt = wpos[t];
+                    const float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
+                    if (fx * fx + fy * fy + fz * fz < c2f) {
+                        if (nq == NB_QUEUE) drain();
+                        q_slot[nq][tid] = t;
+                        q_shift[nq][tid] = shift;
+                        ++nq;
                     }
                 }
             }
         }
     }
+    drain();
     ncnt[s] = count < cap ? count : cap;
     if (count > cap) atomicMax(flags, count);
 }
