@@ -460,8 +460,11 @@ def main():
     model = os.path.join(ROOT, "pysic_amd", "data", "sio2_eem_mlp.npz")
     qeq = None if args.no_qeq else qeq_benchmark(model)
     batched = None if batch_cells is None else batched_benchmark(*batch_cells)
-    ewald = None if args.no_ewald else ewald_benchmark()
+    # the reference core is one per process and its Ewald switch, once set,
+    # stays on (Core.f90:87, 8106): time the Tersoff baseline before the
+    # Ewald sample
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    ewald = None if args.no_ewald else ewald_benchmark()
     out = {"metric": "atom-steps/s Tersoff 1M-atom Si; QEq time-to-1e-6 learned vs cold start",
            "value": round(value, 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

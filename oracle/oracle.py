@@ -192,6 +192,8 @@ class RefCore:
     """The reference core, driven in the order of calculator.py:1595-1637.
 
     The reference core is a process-wide singleton; one RefCore at a time.
+    Once Ewald parameters are set they stay in force for the process: the
+    reference only ever switches evaluate_ewald on (Core.f90:87, 8106).
     """
 
     def __init__(self):

@@ -489,7 +489,7 @@ void md_step(pysic_ctx *h, double dt, double half)
     System &S = h->S;
     nb_sort_atoms(S, true, dt, half);
     nb_build_lists(S);
-    ts_site(S, h->variant, SITE_FORCES);
+    ts_site(S, h->variant, SITE_FORCES_ONLY);
     ts_gather(S, half, 1);
 }
 
@@ -883,7 +883,7 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
                 md_step(h, dt, half);
             } else {
                 md_drift(S, dt, half);
-                ts_site(S, h->variant, SITE_FORCES);
+                ts_site(S, h->variant, SITE_FORCES_ONLY);
                 ts_gather(S, half, 1);
             }
             if (energies) {
@@ -892,8 +892,6 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
             }
         }
     }
-    // the virial of the last step (the step itself does not need it)
-    if (nsteps > 0 && !energies) ts_reduce(S);
     HIP_TRY(hipGetLastError());
     if (energies && nsteps > 0)
         HIP_TRY(hipMemcpyAsync(energies, ebuf.p, sizeof(double) * nsteps, hipMemcpyDeviceToHost, S.stream));

@@ -125,7 +125,8 @@ void nb_clear_flags(System &S);               // zero the flags (kernel; graph-s
 // tersoff.hip
 // SITE_ENERGY: energy only (FP64); SITE_FORCES: dU/dr_ij + virial (the
 // multi-species kernel computes all three in either mode)
-enum { SITE_ENERGY = 1, SITE_FORCES = 2 };
+// SITE_FORCES_ONLY: dU/dr_ij without the virial (a velocity-Verlet step)
+enum { SITE_ENERGY = 1, SITE_FORCES = 2, SITE_FORCES_ONLY = 3 };
 void ts_site(System &S, int variant, int mode);
 void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
 void ts_reduce(System &S);                    // totals[0..6] = E, W(6)

@@ -454,7 +454,7 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
     }
 }
 
-template <int G, int SCAP, class Geo>
+template <int G, int SCAP, class Geo, bool VIR>
 __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                                             const double4 *__restrict__ pos,
                                             const int *__restrict__ nbr,
@@ -631,28 +631,32 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
         double Dy = cp * ud.y + (double)(cu * uy + dcy);
         double Dz = cp * ud.z + (double)(cu * uz + dcz);
         dU[(size_t)jj * n + i] = make_float4((float)Dx, (float)Dy, (float)Dz, 0.f);
-        w6[0] -= ud.x * Dx;
-        w6[1] -= ud.y * Dy;
-        w6[2] -= ud.z * Dz;
-        w6[3] -= ud.y * Dz;
-        w6[4] -= ud.x * Dz;
-        w6[5] -= ud.x * Dy;
+        if constexpr (VIR) {
+            w6[0] -= ud.x * Dx;
+            w6[1] -= ud.y * Dy;
+            w6[2] -= ud.z * Dz;
+            w6[3] -= ud.y * Dz;
+            w6[4] -= ud.x * Dz;
+            w6[5] -= ud.x * Dy;
+        }
     }
     if (valid && l == 0) scnt[i] = ns;
 
+    if constexpr (VIR) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-        for (int off = G / 2; off > 0; off >>= 1) w6[q] += __shfl_xor(w6[q], off, G);
-    if (l == 0)
-        for (int q = 0; q < 6; ++q) s_red[a][1 + q] = valid ? w6[q] : 0.0;
-    if (atom_ev && valid && l == 0)
-        for (int q = 0; q < 6; ++q) atom_ev[(size_t)(1 + q) * n + i] = w6[q];
-    __syncthreads();
-    if (threadIdx.x < 7) {
-        double sacc = 0.0;
-        if (threadIdx.x > 0)
-            for (int t = 0; t < APB; ++t) sacc += s_red[t][threadIdx.x];
-        partial[(size_t)blk * 8 + threadIdx.x] = sacc;  // energy slot 0 left at 0
+        for (int q = 0; q < 6; ++q)
+            for (int off = G / 2; off > 0; off >>= 1) w6[q] += __shfl_xor(w6[q], off, G);
+        if (l == 0)
+            for (int q = 0; q < 6; ++q) s_red[a][1 + q] = valid ? w6[q] : 0.0;
+        if (atom_ev && valid && l == 0)
+            for (int q = 0; q < 6; ++q) atom_ev[(size_t)(1 + q) * n + i] = w6[q];
+        __syncthreads();
+        if (threadIdx.x < 7) {
+            double sacc = 0.0;
+            if (threadIdx.x > 0)
+                for (int t = 0; t < APB; ++t) sacc += s_red[t][threadIdx.x];
+            partial[(size_t)blk * 8 + threadIdx.x] = sacc;  // energy slot 0 left at 0
+        }
     }
 }
 
@@ -716,7 +720,7 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, int ntiles, Geo geo,
     }
 }
 
-template <int G, int SCAP, class Geo>
+template <int G, int SCAP, class Geo, bool VIR>
 __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
                                                   const double4 *__restrict__ pos,
                                                   const int *__restrict__ nbr,
@@ -732,8 +736,8 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
     int t0, t1;
     tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
-        site1f_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
-                             atom_ev, flags);
+        site1f_tile<G, SCAP, Geo, VIR>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU,
+                                       partial, atom_ev, flags);
         __syncthreads();
     }
 }
@@ -854,7 +858,11 @@ void launch_site(System &S, int mode)
                 S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.partial.p,
                 aev, S.flags.p);
         else if (S.model.nspecies == 1 && mode == SITE_FORCES)
-            k_site1f<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
+            k_site1f<G, SCAP, Geo, true><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
+                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+        else if (S.model.nspecies == 1 && mode == SITE_FORCES_ONLY)
+            k_site1f<G, SCAP, Geo, false><<<grid, BLOCK, 0, S.stream>>>(
                 S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
                 S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
         else
