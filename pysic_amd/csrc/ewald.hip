@@ -134,6 +134,12 @@ __device__ __forceinline__ void phases(const Recip &R, const double4 &p, double 
 }
 
 // S(k) partial sums: block (segment tile, atom chunk); thread = segment.
+// Atoms are staged SK_TA at a time with the table q s exp(i l theta3),
+// l < LSEG, so each k of a segment costs one complex multiply-add with a
+// wave-wide (broadcast) LDS read; one sincos per (segment, atom) gives the
+// segment's first phase.
+constexpr int SK_TA = 64;
+
 __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk, Recip R,
                                                   const int4 *__restrict__ segs,
                                                   const double4 *__restrict__ pos,
@@ -141,8 +147,8 @@ __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int c
                                                   const double *__restrict__ sc,
                                                   double2 *__restrict__ skpart)
 {
-    __shared__ double4 at[EB];   // theta1, theta2, theta3, q s
-    __shared__ double2 w3[EB];   // exp(i theta3)
+    __shared__ double4 at[SK_TA];          // theta1, theta2, theta3, q s
+    __shared__ double2 pw[SK_TA][LSEG];    // q s exp(i l theta3)
     const int sidx = blockIdx.x * EB + threadIdx.x;
     const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
     int4 sg = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);
@@ -151,36 +157,40 @@ __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int c
     double cr[LSEG], ci[LSEG];
 #pragma unroll
     for (int l = 0; l < LSEG; ++l) cr[l] = ci[l] = 0.0;
-    for (int base = a0; base < a1; base += EB) {
-        const int a = base + threadIdx.x;
-        if (a < a1) {
-            double t1, t2, t3;
-            phases(R, pos[a], t1, t2, t3);
-            at[threadIdx.x] = make_double4(t1, t2, t3, q[a] * sc[a]);
+    for (int base = a0; base < a1; base += SK_TA) {
+        if (threadIdx.x < SK_TA) {
+            const int a = base + threadIdx.x;
+            double t1 = 0.0, t2 = 0.0, t3 = 0.0, qs = 0.0;
+            if (a < a1) {
+                phases(R, pos[a], t1, t2, t3);
+                qs = q[a] * sc[a];
+            }
+            at[threadIdx.x] = make_double4(t1, t2, t3, qs);
             double s3, c3;
             sincos(t3, &s3, &c3);
-            w3[threadIdx.x] = make_double2(c3, s3);
-        } else {
-            at[threadIdx.x] = make_double4(0, 0, 0, 0);
-            w3[threadIdx.x] = make_double2(1, 0);
+            double vr = qs, vi = 0.0;
+#pragma unroll
+            for (int l = 0; l < LSEG; ++l) {
+                pw[threadIdx.x][l] = make_double2(vr, vi);
+                const double nr = vr * c3 - vi * s3;
+                vi = vr * s3 + vi * c3;
+                vr = nr;
+            }
         }
         __syncthreads();
-        const int m = min(EB, a1 - base);
+        const int m = min(SK_TA, a1 - base);
         for (int t = 0; t < m; ++t) {
             const double4 th = at[t];
             if (th.w == 0.0) continue;
-            const double2 w = w3[t];
             double vs, vc;
             sincos(k1 * th.x + k2 * th.y + l0 * th.z, &vs, &vc);
 #pragma unroll
             for (int l = 0; l < LSEG; ++l) {
                 if (l < len) {
-                    cr[l] += th.w * vc;
-                    ci[l] += th.w * vs;
+                    const double2 p = pw[t][l];
+                    cr[l] += vc * p.x - vs * p.y;
+                    ci[l] += vc * p.y + vs * p.x;
                 }
-                const double nc = vc * w.x - vs * w.y;
-                vs = vc * w.y + vs * w.x;
-                vc = nc;
             }
         }
         __syncthreads();
@@ -250,6 +260,7 @@ __global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, 
     const int s0 = blockIdx.y * schunk, s1 = min(nseg, s0 + schunk);
     double t1[RAT], t2[RAT], t3[RAT], wc[RAT], ws[RAT], qs[RAT];
     double ax[RAT], ay[RAT], az[RAT];
+    double vs[RAT], vc[RAT];   // the running phase exp(i k.r) of each atom
 #pragma unroll
     for (int a = 0; a < RAT; ++a) {
         const int i = i0 + a * EB;
@@ -258,7 +269,12 @@ __global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, 
         phases(R, p, t1[a], t2[a], t3[a]);
         sincos(t3[a], &ws[a], &wc[a]);
         ax[a] = ay[a] = az[a] = 0.0;
+        vs[a] = 0.0;
+        vc[a] = 1.0;
     }
+    // (k1, k2, next k3) that the running phase stands at; a segment that
+    // continues the previous one's row starts there without a sincos
+    int rk1 = 0x7fffffff, rk2 = 0, rl = -1;
     for (int base = s0; base < s1; base += STILE) {
         const int m = min(STILE, s1 - base);
         const int kb = segs[base].w;
@@ -274,10 +290,16 @@ __global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, 
             const int4 sg = sl[t];
             const int len = sg.z >> 16;
             const int o = sg.w - kb;
-            const double k1 = sg.x, k2 = sg.y, l0 = sg.z & 0xffff;
-            double vs[RAT], vc[RAT];
+            const int il0 = sg.z & 0xffff;
+            if (!(sg.x == rk1 && sg.y == rk2 && il0 == rl)) {
+                const double k1 = sg.x, k2 = sg.y, l0 = il0;
 #pragma unroll
-            for (int a = 0; a < RAT; ++a) sincos(k1 * t1[a] + k2 * t2[a] + l0 * t3[a], &vs[a], &vc[a]);
+                for (int a = 0; a < RAT; ++a)
+                    sincos(k1 * t1[a] + k2 * t2[a] + l0 * t3[a], &vs[a], &vc[a]);
+            }
+            rk1 = sg.x;
+            rk2 = sg.y;
+            rl = il0 + len;
             for (int l = 0; l < len; ++l) {
                 const double4 kv = kl[o + l];
                 const double2 S = sv[o + l];
