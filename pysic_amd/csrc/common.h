@@ -69,6 +69,9 @@ __device__ __forceinline__ double3 separation64(const double4 &ri, const double4
                                                 const double *v)
 {
 #pragma clang fp contract(off)
+    // same image (the common case): adding 0 * v would change no bit of a
+    // nonzero component, so skip the nine multiply-adds
+    if (img == 0) return make_double3(rj.x - ri.x, rj.y - ri.y, rj.z - ri.z);
     double ox = (double)img_x(img), oy = (double)img_y(img), oz = (double)img_z(img);
     double x = rj.x - ri.x, y = rj.y - ri.y, z = rj.z - ri.z;
     x = x + ox * v[0]; y = y + ox * v[1]; z = z + ox * v[2];

@@ -472,7 +472,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
     // [slot][atom]: the atoms of a wave sit on consecutive LDS banks
     __shared__ double4 s_sep[SCAP][APB];      // x, y, z, r (FP64)
     __shared__ float4 s_aux[SCAP][APB];       // 1/r, f_c(r), f_c'(r), A (then P)
-    __shared__ double2 s_pair[SCAP][APB];     // 1/2 d(f v_R)/dr, 1/2 d(f v_A)/dr
+    __shared__ double2 s_pair[SCAP][APB];     // 1/2 d(f v_R)/dr / r, 1/2 d(f v_A)/dr / r
     __shared__ double s_red[APB][8];
 
     const int a = threadIdx.x / G;
@@ -527,10 +527,11 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                     else dR += dv;
                 }
             }
+            const double ir = 1.0 / r;
             s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
-            s_aux[slot][a] = make_float4((float)(1.0 / r), inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
+            s_aux[slot][a] = make_float4((float)ir, inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
                                          inb ? (float)dfc64(r, B.cut, B.soft) : 0.f, (float)A);
-            s_pair[slot][a] = make_double2(dR, dA);
+            s_pair[slot][a] = make_double2(dR * ir, dA * ir);  // divided by r once
             size_t o = (size_t)slot * n + i;
             sj[o] = j;
             simg[o] = im;
@@ -576,7 +577,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                 dbt = inv_eta * (bt / (1.0f + pw)) * etaf * (pw / be) * betaf;
         }
         double2 pp = s_pair[jj][a];
-        cpair[q] = pp.x + pp.y * (double)bt;  // d/dr of 1/2 f (v_R + v_A b~) at fixed b~
+        cpair[q] = pp.x + pp.y * (double)bt;  // (d/dr of 1/2 f (v_R + v_A b~) at fixed b~) / r
         s_aux[jj][a].w = au.w * dbt;           // P = 1/2 f v_A db~/dzeta
     }
     __syncthreads();
@@ -626,7 +627,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                 cu += coef_u * ir1;
             }
         }
-        double cp = cpair[q] / ud.w;
+        double cp = cpair[q];
         double Dx = cp * ud.x + (double)(cu * ux + dcx);
         double Dy = cp * ud.y + (double)(cu * uy + dcy);
         double Dz = cp * ud.z + (double)(cu * uz + dcz);
