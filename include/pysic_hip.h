@@ -9,10 +9,19 @@
  * arrays of 3-vectors are row-major (N,3), cells are given as three vectors
  * in rows, and species are small integers 0..3 chosen by the caller.
  *
- * Potential descriptors (10 doubles per entry, one entry per target
+ * Potential descriptors (12 doubles per entry, one entry per target
  * permutation as calculator.py:1166-1192 adds them):
- *   type (1 'exponential', 2 'charge_self'), n_targets, e1, e2, p1, p2,
- *   cutoff, soft_cutoff (= cutoff - margin), group (-1 none), 0
+ *   type, n_targets, e1, e2, p1, p2, cutoff, soft_cutoff (= cutoff - margin),
+ *   group (-1 none), p3, p4, link
+ * type: 1 'exponential', 2 'charge_self', 3 'LJ', 4 'spring', 5 'Buckingham',
+ *   6 'power', 7 'tabulated', 8 'shift_power', 9 'step', 10 'constant',
+ *   11 'force'; p1..p4 are the parameters in the reference's order
+ *   (Potentials.f90:2909-4990).  Only a lone 'exponential' may carry a
+ *   bond-order group.
+ * link: 0 starts a potential; 1 makes the entry a multiplier of the
+ *   potential before it (a ProductPotential factor: add_potential with
+ *   is_multiplier, calculator.py:1197-1369); a multiplier's targets, cutoffs
+ *   and group are ignored (the product uses its first potential's).
  * Bond-order descriptors ('tersoff', 14 doubles per entry, no permutation as
  * calculator.py:1372):
  *   e1 e2 e3 (center, j, k), beta, eta, a, c, d, h, mu, cutoff, soft_cutoff,
@@ -60,6 +69,10 @@ int pysic_get_number_of_atoms(pysic_ctx *h, int *n);
 
 /* allocate_potentials + add_potential (PyInterface.f90:370, 433) */
 int pysic_set_potentials(pysic_ctx *h, int npot, const double *desc);
+/* The table of 'tabulated' potentials with this id: nrows rows of (value,
+ * derivative), the file table_<id>.txt that create_potential reads
+ * (Potentials.f90:2462-2465, read_table Utility.f90:384-433). */
+int pysic_set_potential_table(pysic_ctx *h, int id, int nrows, const double *rows);
 /* allocate_bond_order_factors + add_bond_order_factor (PyInterface.f90:384, 482)
  * + allocate_bond_order_storage (402) */
 int pysic_set_bond_order_factors(pysic_ctx *h, int nbond, const double *desc);

@@ -7,14 +7,18 @@ Fortran core compiled from its own sources by build_ref.sh).  Only tests/,
 the engine in pysic_amd never imports it.
 
 Descriptor formats shared with the engine's C-ABI:
-  potential rows (10 doubles): type(1 exp, 2 charge_self), n_targets, e1, e2,
-      p1, p2, cutoff, soft_cutoff, group, 0
+  potential rows (12 doubles): type, n_targets, e1, e2, p1, p2, cutoff,
+      soft_cutoff, group, p3, p4, link; type 1 exponential, 2 charge_self,
+      3 LJ, 4 spring, 5 Buckingham, 6 power, 7 tabulated, 8 shift_power,
+      9 step, 10 constant, 11 force; link 1 = multiplier of the preceding
+      potential.  Rows of 10 (p3 = p4 = link = 0) are accepted and padded.
   bond rows (14 doubles, 'tersoff'): e1 e2 e3 (center, j, k), beta, eta,
       a, c, d, h, mu, cutoff, soft_cutoff, group, 0
   ewald (8 doubles): on, real_cutoff, k_cutoff, K1, K2, K3, sigma, epsilon
 """
 import ctypes
 import os
+import tempfile
 
 import numpy as np
 
@@ -56,10 +60,33 @@ def lib():
                                         ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, _dp, _dp]
+        L.orc_set_table.restype = ctypes.c_int
+        L.orc_set_table.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_pick.restype = ctypes.c_int
         L.orc_pick.argtypes = [ctypes.c_int, ctypes.c_int, _ip]
         _lib = L
     return _lib
+
+
+def pot_rows(potdesc):
+    """Potential descriptor rows as a flat array of 12-double rows."""
+    pd = np.asarray(potdesc, dtype=np.float64)
+    if pd.size == 0:
+        return np.zeros(0)
+    if pd.ndim != 2:
+        pd = pd.reshape(-1, 12)
+    if pd.shape[1] == 10:
+        pd = np.hstack([pd, np.zeros((len(pd), 2))])
+    assert pd.shape[1] == 12
+    return np.ascontiguousarray(pd.reshape(-1))
+
+
+def set_table(table_id, values):
+    """Registers a 'tabulated' potential table (rows of value, derivative)."""
+    v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1, 2)
+    rc = lib().orc_set_table(int(table_id), len(v), _d(v))
+    if rc != 0:
+        raise RuntimeError("orc_set_table failed: %d" % rc)
 
 
 def ref_available():
@@ -126,8 +153,8 @@ def evaluate(calc, species, pos, charges, cell, nl, potdesc, bonddesc=None, ewal
     q = np.ascontiguousarray(charges if charges is not None else np.zeros(n), dtype=np.float64)
     cell = np.ascontiguousarray(cell, dtype=np.float64)
     inv = inverse_of(cell)
-    pd = np.ascontiguousarray(potdesc, dtype=np.float64).reshape(-1)
-    npot = len(pd) // 10
+    pd = pot_rows(potdesc)
+    npot = len(pd) // 12
     bd = None if bonddesc is None else np.ascontiguousarray(bonddesc, dtype=np.float64).reshape(-1)
     nbond = 0 if bd is None else len(bd) // 14
     ew = None if ewald is None else np.ascontiguousarray(ewald, dtype=np.float64)
@@ -159,12 +186,12 @@ def relax_dynamic(species, pos, charges, cell, nl, potdesc, ewald, n_steps, dt, 
     r = np.zeros(n) if rates is None else np.array(rates, dtype=np.float64)
     cell = np.ascontiguousarray(cell, dtype=np.float64)
     inv = inverse_of(cell)
-    pd = np.ascontiguousarray(potdesc, dtype=np.float64).reshape(-1)
+    pd = pot_rows(potdesc)
     ew = None if ewald is None else np.ascontiguousarray(ewald, dtype=np.float64)
     sc = None if scaler is None else np.ascontiguousarray(scaler, dtype=np.float64)
     err = np.zeros(1)
     steps = L.orc_relax_dynamic(n, _i(sp), _d(pos), _d(q), _d(cell), _d(inv), _i(nl.start),
-                                _i(nl.idx), _i(nl.off), len(pd) // 10, _d(pd), _d(ew), _d(sc),
+                                _i(nl.idx), _i(nl.off), len(pd) // 12, _d(pd), _d(ew), _d(sc),
                                 n_steps, dt, inertia, friction, tolerance, _d(r), _d(err))
     if steps < 0:
         raise RuntimeError("orc_relax_dynamic failed: %d" % steps)
@@ -204,7 +231,8 @@ class RefCore:
             R = ctypes.CDLL(REF_PATH)
             R.ref_add_potential.restype = ctypes.c_int
             R.ref_add_potential.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
-                                            ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int]
+                                            ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int,
+                                            ctypes.c_int]
             R.ref_add_tersoff.restype = ctypes.c_int
             R.ref_add_tersoff.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _ip,
                                           ctypes.c_int]
@@ -220,6 +248,7 @@ class RefCore:
             R.ref_neighbor_list.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
             R.ref_set_ewald.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
                                         ctypes.c_double, ctypes.c_double, _dp]
+            R.ref_clear_multipliers.argtypes = []
             R.ref_energy.argtypes = [_dp]
             R.ref_forces.argtypes = [ctypes.c_int, _dp, _dp]
             R.ref_electronegativities.argtypes = [ctypes.c_int, _dp]
@@ -228,15 +257,19 @@ class RefCore:
         self.R = _ref
 
     def setup(self, symbols, pos, cell, pbc, charges=None, masses=None, potentials=(),
-              tersoff=(), ngroups=0, cutoffs=None, ewald=None, scaler=None):
-        """potentials: list of (kind, params, cutoff, soft, symbols_tuple, group) with
-        symbols in target order (all permutations already expanded);
-        tersoff: list of (params7, cutoff, soft, (si, sj, sk), group)."""
+              tersoff=(), ngroups=0, cutoffs=None, ewald=None, scaler=None, tables=None,
+              momenta=None):
+        """potentials: list of (kind, params, cutoff, soft, symbols_tuple, group
+        [, multipliers]) with symbols in target order (all permutations already
+        expanded) and multipliers a list of (kind, params) in the order the
+        reference stores them; tersoff: list of (params7, cutoff, soft,
+        (si, sj, sk), group); tables: {id: rows of (value, derivative)}."""
         R = self.R
         n = len(pos)
         self.n = n
         pos_f = np.asfortranarray(np.asarray(pos, dtype=np.float64).T)
-        mom = np.zeros((3, n), order="F")
+        mom = np.zeros((3, n), order="F") if momenta is None else \
+            np.asfortranarray(np.asarray(momenta, dtype=np.float64).T)
         q = np.zeros(n) if charges is None else np.asarray(charges, dtype=np.float64).copy()
         m = np.ones(n) if masses is None else np.asarray(masses, dtype=np.float64).copy()
         tags = np.zeros(n, dtype=np.int32)
@@ -247,12 +280,33 @@ class RefCore:
         R.ref_create_system(n, _d(m), _d(q), _d(pos_f), _d(mom), _i(tags), _i(el), _d(cellv),
                             _d(inv_f), _i(pbc_i))
         R.ref_allocate_potentials(len(potentials))
-        for kind, params, cut, soft, syms, group in potentials:
-            p = np.asarray(params, dtype=np.float64)
-            e = np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
-            ok = R.ref_add_potential(kind, len(syms), len(p), _d(p), cut, soft, _i(e), group)
-            if not ok:
-                raise RuntimeError("reference add_potential failed")
+        here = os.getcwd()
+        if tables:
+            # read_table opens table_<id>.txt in the working directory
+            tdir = tempfile.mkdtemp(prefix="pysic_tables_")
+            for tid, vals in tables.items():
+                np.savetxt(os.path.join(tdir, "table_%04d.txt" % tid),
+                           np.asarray(vals, dtype=np.float64).reshape(-1, 2), fmt="%.17g")
+            os.chdir(tdir)
+        try:
+            for entry in potentials:
+                kind, params, cut, soft, syms, group = entry[:6]
+                mults = entry[6] if len(entry) > 6 else ()
+                e = np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
+                # multipliers first (is_multiplier), then the potential itself
+                for mkind, mparams in mults:
+                    mp = np.asarray(mparams, dtype=np.float64)
+                    if not R.ref_add_potential(mkind, len(syms), len(mp), _d(mp), cut, soft,
+                                               _i(e), group, 1):
+                        raise RuntimeError("reference add_potential (multiplier) failed")
+                p = np.asarray(params, dtype=np.float64)
+                ok = R.ref_add_potential(kind, len(syms), len(p), _d(p), cut, soft, _i(e), group,
+                                         0)
+                R.ref_clear_multipliers()
+                if not ok:
+                    raise RuntimeError("reference add_potential failed")
+        finally:
+            os.chdir(here)
         R.ref_allocate_bond_order_factors(len(tersoff))
         for params, cut, soft, syms, group in tersoff:
             p = np.asarray(params, dtype=np.float64)

@@ -263,19 +263,37 @@ long orc_neighbor_list(int n, const double *pos, const double *cell, const doubl
 /* interaction model                                                         */
 /* ------------------------------------------------------------------------ */
 
-/* Potential descriptor, 10 doubles: type (1 exponential, 2 charge_self),
- * n_targets, e1, e2 (species ids in target order, like apply_elements),
- * p1, p2 (parameters in descriptor order), cutoff, soft cutoff
- * (cutoff - margin), group (-1: no bond-order coordinator), unused. */
-#define ORC_POT_STRIDE 10
+/* Potential descriptor, 12 doubles:
+ *   type, n_targets, e1, e2 (species ids in target order, like
+ *   apply_elements), p1, p2, cutoff, soft cutoff (cutoff - margin),
+ *   group (-1: no bond-order coordinator), p3, p4, link.
+ * type: 1 exponential, 2 charge_self, 3 LJ, 4 spring, 5 Buckingham, 6 power,
+ *   7 tabulated, 8 shift_power, 9 step, 10 constant, 11 force; parameters
+ *   p1..p4 in the order of the reference's characterizers
+ *   (Potentials.f90:2909-4990).
+ * link 0 starts a potential; link 1 makes the row a multiplier of the
+ *   nearest preceding potential (a ProductPotential factor, which the
+ *   reference adds with is_multiplier = .true. before its first factor,
+ *   calculator.py:1197-1369, Potentials.f90:2353-2472): only the row's type
+ *   and parameters count; targets, cutoffs and group are the potential's. */
+#define ORC_POT_STRIDE 12
+#define ORC_MAXF 4
 /* Bond-order descriptor ('tersoff' only), 14 doubles: e1 e2 e3 (center i,
  * j, k as original_elements), beta, eta, a, c, d, h, mu, cutoff, soft
  * cutoff, group, unused. */
 #define ORC_BOND_STRIDE 14
 
+/* one factor of a (product) potential */
 typedef struct {
-    int type, nt, e[2], group, smooth;
-    double p[2], cut, soft;
+    int type;
+    double p[4];
+    const double *tab; /* tabulated: n rows of (value, derivative) */
+    int tlen;
+} fac_t;
+typedef struct {
+    int nt, e[2], group, smooth, nf;
+    double cut, soft;
+    fac_t f[ORC_MAXF]; /* f[0] the potential, then its multipliers */
 } pot_t;
 typedef struct {
     int e[3], group;
@@ -294,6 +312,192 @@ static double orc_dfc(double r, double hard, double soft)
     if (r > hard) return 0.0;
     if (r < soft) return 0.0;
     return -0.5 * orc_pi() / (hard - soft) * sin(orc_pi() * (r - soft) / (hard - soft));
+}
+
+/* Tables of the 'tabulated' potential, by id.  The reference reads them from
+ * table_<id>.txt when the potential is created (Potentials.f90:2462-2465,
+ * read_table Utility.f90:384-433): column 1 the value, column 2 the
+ * derivative with respect to r / range. */
+#define ORC_MAXTAB 64
+static struct {
+    int id, n;
+    double *v;
+} orc_tables[ORC_MAXTAB];
+static int orc_ntables;
+
+int orc_set_table(int id, int n, const double *values)
+{
+    if (n < 2 || !values) return ORC_EINVAL;
+    double *v = malloc(2 * (size_t)n * sizeof(double));
+    if (!v) return ORC_ENOMEM;
+    memcpy(v, values, 2 * (size_t)n * sizeof(double));
+    for (int k = 0; k < orc_ntables; ++k)
+        if (orc_tables[k].id == id) {
+            free(orc_tables[k].v);
+            orc_tables[k].v = v;
+            orc_tables[k].n = n;
+            return ORC_OK;
+        }
+    if (orc_ntables == ORC_MAXTAB) { free(v); return ORC_ENOMEM; }
+    orc_tables[orc_ntables].id = id;
+    orc_tables[orc_ntables].n = n;
+    orc_tables[orc_ntables].v = v;
+    orc_ntables++;
+    return ORC_OK;
+}
+
+/* Interval of the tabulation for r < range: the reference's 1-based
+ * lower_index = floor(r/dr) + 1 (Potentials.f90:4245, 4286), returned 0-based
+ * and kept inside the table (the reference would read past its end if r/dr
+ * rounded up to n - 1). */
+static int orc_table_interval(const fac_t *f, double r, double *dr, double *t)
+{
+    *dr = f->p[1] / (f->tlen - 1);
+    int lower = (int)floor(r / *dr) + 1;
+    if (lower > f->tlen - 1) lower = f->tlen - 1;
+    *t = r / *dr - (lower - 1);
+    return lower - 1;
+}
+
+/* evaluate_energy_component for one factor of a pair potential at distance r
+ * (Potentials.f90:2838-2890); cut is the potential's cutoff (the spring's
+ * energy shift uses it). */
+static double orc_pair_energy(const fac_t *f, double r, double cut)
+{
+    const double *p = f->p;
+    switch (f->type) {
+    case 1: /* evaluate_energy_exp, 3666-3677 */
+        return p[0] * exp(-p[1] * r);
+    case 3: { /* evaluate_energy_LJ, 2993-3005 */
+        double ratio = p[1] / r;
+        double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+        return p[0] * (r6 * r6 - r6);
+    }
+    case 4: { /* evaluate_energy_spring, 3069-3085 */
+        double r2 = r - p[1], r6 = cut - p[1];
+        if (r2 * r2 < r6 * r6) return p[0] * 0.5 * (r2 * r2 - r6 * r6);
+        return 0.0;
+    }
+    case 5: { /* evaluate_energy_buckingham, 4033-4045 */
+        double ratio = r / p[2];
+        double r6 = 1.0 / (ratio * ratio * ratio * ratio * ratio * ratio);
+        return p[0] * exp(-ratio) - p[1] * r6;
+    }
+    case 6: /* evaluate_energy_power, 4141-4152 */
+        return p[0] * pow(p[1] / r, p[2]);
+    case 7: { /* evaluate_energy_table, 4267-4298; constant (unscaled) past the range */
+        if (r >= p[1]) return f->tab[2 * (f->tlen - 1)];
+        double dr, t;
+        int lo = orc_table_interval(f, r, &dr, &t);
+        double t2 = t * t, t3 = t2 * t, mr = p[1];
+        const double *a = f->tab + 2 * lo, *b = a + 2;
+        return p[2] * (a[0] * (2 * t3 - 3 * t2 + 1) + a[1] * dr / mr * (t3 - 2 * t2 + t) +
+                       b[0] * (3 * t2 - 2 * t3) + b[1] * dr / mr * (t3 - t2));
+    }
+    case 8: /* evaluate_energy_shift, 4827-4842 */
+        return p[0] * pow((p[1] - r) / (p[1] - p[2]), p[3]);
+    case 9: /* evaluate_energy_step, 4946-4967 */
+        if (r < p[2]) return p[0];
+        if (r > p[3]) return p[1];
+        return (p[0] + p[1]) * 0.5 * (1.0 + 0.5 * cos(orc_pi() * (r - p[2]) / (p[3] - p[2])));
+    }
+    return 0.0;
+}
+
+/* evaluate_force_component for one factor: the force on the first atom,
+ * f1 (the second gets -f1); zero at r == 0 (Potentials.f90:2717-2730). */
+static void orc_pair_force(const fac_t *f, const double *sep, double r, double cut, double *f1)
+{
+    const double *p = f->p;
+    f1[0] = f1[1] = f1[2] = 0.0;
+    if (r == 0.0) return;
+    switch (f->type) {
+    case 1: /* evaluate_force_exp, 3643-3655 */
+        for (int a = 0; a < 3; ++a) f1[a] = -p[1] * p[0] * exp(-p[1] * r) * sep[a] / r;
+        return;
+    case 3: { /* evaluate_force_LJ, 2971-2984 */
+        double ratio = p[1] / r;
+        double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+        for (int a = 0; a < 3; ++a) f1[a] = p[0] * (6.0 * r6 - 12.0 * r6 * r6) * sep[a] / (r * r);
+        return;
+    }
+    case 4: { /* evaluate_force_spring, 3042-3060 */
+        double r2 = r - p[1], r6 = cut - p[1];
+        if (r2 * r2 < r6 * r6)
+            for (int a = 0; a < 3; ++a) f1[a] = p[0] * r2 * sep[a] / r;
+        return;
+    }
+    case 5: { /* evaluate_force_buckingham, 4009-4023 */
+        double ratio = r / p[2];
+        double r6 = 1.0 / (ratio * ratio * ratio * ratio * ratio * ratio);
+        for (int a = 0; a < 3; ++a)
+            f1[a] = (-p[0] * ratio * exp(-ratio) + p[1] * 6.0 * r6) * sep[a] / (r * r);
+        return;
+    }
+    case 6: { /* evaluate_force_power, 4119-4132 */
+        double n = p[2], ratio = p[1] / r;
+        for (int a = 0; a < 3; ++a) f1[a] = p[0] * (-n * pow(ratio, n)) * sep[a] / (r * r);
+        return;
+    }
+    case 7: { /* evaluate_force_table, 4227-4258; zero past the range */
+        if (r >= p[1]) return;
+        double dr, t;
+        int lo = orc_table_interval(f, r, &dr, &t);
+        double t2 = t * t, mr = p[1];
+        const double *a = f->tab + 2 * lo, *b = a + 2;
+        double s = a[0] / dr * 6 * (t2 - t) + a[1] / mr * (3 * t2 - 4 * t + 1) +
+                   b[0] / dr * 6 * (t - t2) + b[1] / mr * (3 * t2 - 2 * t);
+        for (int c = 0; c < 3; ++c) f1[c] = p[2] * s / r * sep[c];
+        return;
+    }
+    case 8: { /* evaluate_force_shift, 4803-4818 */
+        double r1 = p[1], r2 = p[2], n = p[3];
+        for (int a = 0; a < 3; ++a)
+            f1[a] = -p[0] * n / (r1 - r2) * pow((r1 - r) / (r1 - r2), n - 1) * sep[a] / r;
+        return;
+    }
+    case 9: /* evaluate_force_step, 4917-4937: the same value in every
+             * component (the reference does not project onto r) */
+        if (r < p[2] || r > p[3]) return;
+        for (int a = 0; a < 3; ++a)
+            f1[a] = (p[0] + p[1]) * 0.25 * orc_pi() / (p[3] - p[2]) *
+                    sin(orc_pi() * (r - p[2]) / (p[3] - p[2]));
+        return;
+    }
+}
+
+/* evaluate_energy / evaluate_forces over the factors of a product
+ * (Potentials.f90:2640-2693, 2791-2814): E = prod E_i and
+ * F = sum_i (E / E_i) F_i, or prod_{j != i} E_j F_i where E_i = 0. */
+static double orc_product_energy(const pot_t *P, double r)
+{
+    double e = 1.0;
+    for (int i = 0; i < P->nf; ++i) e = e * orc_pair_energy(P->f + i, r, P->cut);
+    return e;
+}
+static void orc_product_force(const pot_t *P, const double *sep, double r, double *f1)
+{
+    if (P->nf == 1) {
+        orc_pair_force(P->f, sep, r, P->cut, f1);
+        return;
+    }
+    double me[ORC_MAXF], mf[ORC_MAXF][3], e = 1.0;
+    for (int i = 0; i < P->nf; ++i) {
+        me[i] = orc_pair_energy(P->f + i, r, P->cut);
+        orc_pair_force(P->f + i, sep, r, P->cut, mf[i]);
+    }
+    for (int i = 0; i < P->nf; ++i) e = e * me[i];
+    f1[0] = f1[1] = f1[2] = 0.0;
+    for (int i = 0; i < P->nf; ++i) {
+        if (me[i] != 0.0) {
+            for (int a = 0; a < 3; ++a) f1[a] = f1[a] + e / me[i] * mf[i][a];
+        } else {
+            double t = 1.0;
+            for (int j = 0; j < P->nf; ++j)
+                if (j != i) t = t * me[j];
+            for (int a = 0; a < 3; ++a) f1[a] = f1[a] + t * mf[i][a];
+        }
+    }
 }
 
 /* post_process_bond_order_factor_tersoff (Potentials.f90:5910-5921) */
@@ -389,22 +593,46 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     S->pot = calloc(npot > 0 ? npot : 1, sizeof(pot_t));
     S->bond = calloc(nbond > 0 ? nbond : 1, sizeof(bond_t));
     if (!S->pot || !S->bond) return ORC_ENOMEM;
+    int m = 0;
     for (int k = 0; k < npot; ++k) {
         const double *d = pd + ORC_POT_STRIDE * k;
-        pot_t *p = S->pot + k;
-        p->type = (int)d[0];
+        int type = (int)d[0];
+        if (type < 1 || type > 11) return ORC_EINVAL;
+        fac_t fac;
+        memset(&fac, 0, sizeof(fac));
+        fac.type = type;
+        fac.p[0] = d[4]; fac.p[1] = d[5]; fac.p[2] = d[9]; fac.p[3] = d[10];
+        if (type == 7) {
+            int id = (int)d[4];
+            for (int t = 0; t < orc_ntables; ++t)
+                if (orc_tables[t].id == id) { fac.tab = orc_tables[t].v; fac.tlen = orc_tables[t].n; }
+            if (!fac.tab) return ORC_EINVAL;
+        }
+        if (d[11] != 0.0) { /* a multiplier of the previous potential */
+            if (m == 0) return ORC_EINVAL;
+            pot_t *q = S->pot + m - 1;
+            if (q->nf == ORC_MAXF || q->nt != 2 || type == 2 || type >= 10) return ORC_EINVAL;
+            q->f[q->nf++] = fac;
+            continue;
+        }
+        pot_t *p = S->pot + m++;
+        p->nf = 1;
+        p->f[0] = fac;
         p->nt = (int)d[1];
         p->e[0] = (int)d[2];
         p->e[1] = (int)d[3];
-        p->p[0] = d[4];
-        p->p[1] = d[5];
         p->cut = d[6];
         /* create_potential soft-cutoff rule (Potentials.f90:2382-2388) */
         if (d[7] > 0.0 && d[7] < d[6]) { p->soft = d[7]; p->smooth = 1; }
         else { p->soft = d[6]; p->smooth = 0; }
         p->group = (int)d[8];
-        if (p->type != 1 && p->type != 2) return ORC_EINVAL;
+        int single = type == 2 || type >= 10;
+        if (p->nt != (single ? 1 : 2)) return ORC_EINVAL;
     }
+    S->npot = m;
+    for (int k = 0; k < m; ++k) /* bond-order scaling: plain exponentials only */
+        if (S->pot[k].group > -1 && (S->pot[k].f[0].type != 1 || S->pot[k].nf != 1))
+            return ORC_EINVAL;
     for (int k = 0; k < nbond; ++k) {
         const double *d = bd + ORC_BOND_STRIDE * k;
         bond_t *b = S->bond + k;
@@ -418,7 +646,7 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     if (!S->maxpot || !S->maxbond) return ORC_ENOMEM;
     for (int i = 0; i < S->n; ++i) {
         double mp = 0.0, mb = 0.0;
-        for (int k = 0; k < npot; ++k)
+        for (int k = 0; k < S->npot; ++k)
             if (S->pot[k].e[0] == S->sp[i] && S->pot[k].cut > mp) mp = S->pot[k].cut;
         for (int k = 0; k < nbond; ++k)
             if (S->bond[k].e[0] == S->sp[i] && S->bond[k].cut > mb) mb = S->bond[k].cut;
@@ -708,15 +936,28 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
     if (calc == 3) { X = enegs; memset(X, 0, (size_t)n * sizeof(double)); }
 
     for (int i1 = 0; i1 < n; ++i1) {
-        /* core_evaluate_local_singlet (Core.f90:4095-4239): charge_self */
+        /* core_evaluate_local_singlet (Core.f90:4095-4239) */
         for (int k = 0; k < S.npot; ++k) {
             const pot_t *p = S.pot + k;
-            if (p->nt != 1 || p->e[0] != species[i1] || p->type != 2) continue;
-            int nn = (int)p->p[1];
-            double q = charge[i1];
-            /* Potentials.f90:4341-4372: eps*q^n, -n*eps*q^(n-1) (integer powers) */
-            if (calc == 1) en += p->p[0] * pow(q, nn);
-            if (calc == 3) X[i1] += -p->p[0] * nn * pow(q, nn - 1);
+            if (p->nt != 1 || p->e[0] != species[i1]) continue;
+            const double *pp = p->f[0].p;
+            if (p->f[0].type == 2) {
+                int nn = (int)pp[1];
+                double q = charge[i1];
+                /* Potentials.f90:4341-4372: eps*q^n, -n*eps*q^(n-1) (integer powers) */
+                if (calc == 1) en += pp[0] * pow(q, nn);
+                if (calc == 3) X[i1] += -pp[0] * nn * pow(q, nn - 1);
+            } else if (p->f[0].type == 10) {
+                /* constant potential (Potentials.f90:3942-3966): E = V, no force */
+                if (calc == 1) en += pp[0];
+            } else if (p->f[0].type == 11) {
+                /* constant force (Potentials.f90:3122-3147): F, E = -F . r with
+                 * the dot product of Quaternions.f90:475-483 */
+                const double *r = pos + 3 * i1;
+                if (calc == 1) en += (-pp[0] * r[0] + -pp[1] * r[1]) + -pp[2] * r[2];
+                if (calc == 2)
+                    for (int a = 0; a < 3; ++a) F[3 * i1 + a] += pp[a];
+            }
         }
         for (int t = nbstart[i1]; t < nbstart[i1 + 1]; ++t) {
             int i2 = nbidx[t];
@@ -754,14 +995,12 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                 }
             }
             if (!(dist < S.maxpot[i1])) continue;
-            /* doublets (Core.f90:4257-4385, 5371-5606): exponential pairs */
+            /* doublets (Core.f90:4257-4385, 5371-5606) */
             for (int k = 0; k < S.npot; ++k) {
                 const pot_t *p = S.pot + k;
                 if (p->nt != 2 || p->e[0] != species[i1]) continue;
                 if (!(p->cut > dist)) continue;
                 if (p->e[1] != species[i2]) continue;
-                if (p->type != 1) continue;
-                double eps = p->p[0], zeta = p->p[1];
                 double bo[2] = {0, 0}, sums[2] = {0, 0};
                 if (p->group > -1) {
                     orc_pair_bond(&S, i1, i2, sep, dist, p->group, sums, 0, NULL, NULL);
@@ -772,11 +1011,11 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                 }
                 double cutf = p->smooth ? orc_fc(dist, p->cut, p->soft) : 1.0;
                 if (calc == 1) {
-                    double v = eps * exp(-zeta * dist); /* Potentials.f90:3670-3677 */
+                    double v = orc_product_energy(p, dist);
                     if (p->group > -1) en = en + v * cutf * (bo[0] + bo[1]) * 0.5;
                     else en = en + v * cutf;
                 } else if (calc == 2) {
-                    double v = (p->group > -1 || p->smooth) ? eps * exp(-zeta * dist) : 0.0;
+                    double v = (p->group > -1 || p->smooth) ? orc_product_energy(p, dist) : 0.0;
                     double cg[3] = {0, 0, 0};
                     if (p->smooth) {
                         double d = orc_dfc(dist, p->cut, p->soft);
@@ -784,10 +1023,8 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     }
                     if (p->group > -1)
                         orc_pair_bond(&S, i1, i2, sep, dist, p->group, sums, 0.5 * v * cutf, F, st);
-                    /* evaluate_force_exp (Potentials.f90:3643-3655) */
                     double f1[3];
-                    for (int a = 0; a < 3; ++a)
-                        f1[a] = -zeta * eps * exp(-zeta * dist) * sep[a] / dist;
+                    orc_product_force(p, sep, dist, f1);
                     double pf1[3], pf2[3];
                     double bsum = (bo[0] + bo[1]) * 0.5;
                     for (int a = 0; a < 3; ++a) {
@@ -804,7 +1041,7 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     st[4] += sep[0] * pf2[2];
                     st[5] += sep[0] * pf2[1];
                 }
-                /* calc == 3: exponential pairs carry no charge dependence */
+                /* calc == 3: these pair potentials carry no charge dependence */
             }
         }
     }

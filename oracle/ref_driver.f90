@@ -59,31 +59,39 @@ contains
     call allocate_potentials(npot)
   end subroutine ref_allocate_potentials
 
-  ! kind: 1 = 'exponential' (2 targets), 2 = 'charge_self' (1 target)
+  ! kind: the potential type codes of the engine's descriptors
+  ! (1 exponential, 2 charge_self, 3 LJ, 4 spring, 5 Buckingham, 6 power,
+  ! 7 tabulated, 8 shift_power, 9 step, 10 constant, 11 force).
+  ! is_mult /= 0 adds a ProductPotential multiplier, kept until
+  ! ref_clear_multipliers (calculator.py:1262-1277, 1368-1369).
   function ref_add_potential(kind, n_targets, n_params, params, cutoff, soft, elements, &
-       group) result(ok) bind(C, name="ref_add_potential")
-    integer(c_int), value :: kind, n_targets, n_params, group
+       group, is_mult) result(ok) bind(C, name="ref_add_potential")
+    integer(c_int), value :: kind, n_targets, n_params, group, is_mult
     real(c_double), intent(in) :: params(n_params)
     real(c_double), value :: cutoff, soft
     integer(c_int), intent(in) :: elements(2, n_targets)
     integer(c_int) :: ok
     integer :: notags(n_targets), noinds(n_targets)
-    logical :: success
+    logical :: success, mult
+    character(len=11) :: names(11)
+    names = (/ 'exponential', 'charge_self', 'LJ         ', 'spring     ', 'Buckingham ', &
+         'power      ', 'tabulated  ', 'shift_power', 'step       ', 'constant   ', &
+         'force      ' /)
     notags = -9
     noinds = -9
-    if (kind == 1) then
-       call add_potential(n_targets, n_params, 'exponential', params, cutoff, soft, &
-            elements, notags, noinds, elements, notags, noinds, group, .false., success)
-    else if (kind == 2) then
-       call add_potential(n_targets, n_params, 'charge_self', params, cutoff, soft, &
-            elements, notags, noinds, elements, notags, noinds, group, .false., success)
-    else
-       success = .false.
+    mult = is_mult /= 0
+    success = .false.
+    if (kind >= 1 .and. kind <= 11) then
+       call add_potential(n_targets, n_params, trim(names(kind)), params, cutoff, soft, &
+            elements, notags, noinds, elements, notags, noinds, group, mult, success)
     end if
-    call clear_potential_multipliers()
     ok = 0
     if (success) ok = 1
   end function ref_add_potential
+
+  subroutine ref_clear_multipliers() bind(C, name="ref_clear_multipliers")
+    call clear_potential_multipliers()
+  end subroutine ref_clear_multipliers
 
   subroutine ref_allocate_bond_order_factors(nb) bind(C, name="ref_allocate_bond_order_factors")
     integer(c_int), value :: nb

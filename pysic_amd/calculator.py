@@ -12,7 +12,7 @@ import numpy as np
 
 from .coulomb import CoulombSummation
 from .engine import Engine
-from .interactions import build_descriptors
+from .interactions import build_descriptors, read_table, table_ids
 
 
 class MissingAtomsError(Exception):
@@ -246,6 +246,8 @@ class Pysic:
                          momenta=s.get_momenta())
         eng.create_cell(s.get_cell(), s.get_pbc())
         pd, bd, _ = build_descriptors(self.potentials or [], spmap)
+        for tid in table_ids(self.potentials):
+            eng.set_potential_table(tid, read_table(tid))
         eng.set_potentials(pd)
         eng.set_bond_order_factors(bd)
         if self.coulomb is not None:

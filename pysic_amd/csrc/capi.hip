@@ -71,14 +71,18 @@ void System::release_all()
 {
     DBuf<double4> *b4[] = {&pos, &pos2, &mom, &mom2, &frc, &frc2, &out4};
     for (auto *b : b4) b->release();
-    DBuf<double> *bd[] = {&q, &q2, &cut, &cut2, &partial, &totals, &host_stage};
+    DBuf<double> *bd[] = {&q, &q2, &cut, &cut2, &partial, &totals, &host_stage, &gpartial,
+                          &gtotals, &atom_ev, &cell_ev};
     for (auto *b : bd) b->release();
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
-                       &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt};
+                       &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0};
     for (auto *b : bi) b->release();
     dU.release();
     wpos.release();
     dmodel.release();
+    dpmodel.release();
+    tables.release();
+    cells.release();
 }
 
 // ---------------------------------------------------------------------------
@@ -113,9 +117,10 @@ __global__ void k_zero4(int n, double4 *__restrict__ a)
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) a[s] = make_double4(0, 0, 0, 0);
 }
-__global__ void k_copy_total(const double *__restrict__ totals, double *__restrict__ dst)
+__global__ void k_sum_energy(const double *__restrict__ t1, const double *__restrict__ t2,
+                             double *__restrict__ dst)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = totals[0];
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = (t1 ? t1[0] : 0.0) + (t2 ? t2[0] : 0.0);
 }
 }  // namespace
 
@@ -126,6 +131,7 @@ struct pysic_ctx {
     std::vector<int> species;     // original order
     std::vector<double> masses;   // original order
     std::vector<double> potdesc, bonddesc;
+    std::map<int, std::vector<double>> tables;  // 'tabulated' tables by id: rows of (value, derivative)
     int variant = 0;
     bool atoms_ready = false, cell_ready = false;
     bool lists_valid = false;
@@ -160,6 +166,10 @@ int fail(pysic_ctx *h, const PysicError &e)
     catch (const std::exception &e) { return fail(h, PysicError{PYSIC_ENOMEM, e.what()}); }  \
     return PYSIC_OK;
 
+constexpr int POT_STRIDE = 12;  // doubles per potential descriptor row
+
+void drop_graphs(pysic_ctx *h);
+
 // Compile the descriptor rows into species tables (Core.f90:1034-1125 builds
 // the per-atom lists the same way: position-1 element match).
 void compile_model(pysic_ctx *h)
@@ -177,15 +187,31 @@ void compile_model(pysic_ctx *h)
                       "one bond-order group (coordinator) per system is supported");
         M.group = g;
     };
-    size_t np = h->potdesc.size() / 10;
+    PairModel P;
+    std::memset(&P, 0, sizeof(P));
+    std::vector<double2> htab;
+    std::map<int, int> tab_off;
+    const size_t np = h->potdesc.size() / POT_STRIDE;
+    auto row = [&](size_t k) { return h->potdesc.data() + POT_STRIDE * k; };
     for (size_t k = 0; k < np; ++k) {
-        const double *d = h->potdesc.data() + 10 * k;
+        const double *d = row(k);
         int type = (int)d[0], nt = (int)d[1], e1 = (int)d[2], e2 = (int)d[3];
+        PYSIC_REQUIRE(d[11] == 0.0, PYSIC_EINVAL,
+                      "a multiplier row must follow the potential it multiplies");
+        size_t nmul = 0;  // the ProductPotential multipliers that follow
+        while (k + 1 + nmul < np && row(k + 1 + nmul)[11] != 0.0) ++nmul;
         PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && (nt < 2 || (e2 >= 0 && e2 < PYSIC_MAXS)),
                       PYSIC_EINVAL, "potential species out of range");
+        PYSIC_REQUIRE(type >= 1 && type <= 11, PYSIC_EINVAL, "unsupported potential type");
+        const bool single = type == 2 || type >= 10;
+        PYSIC_REQUIRE(nt == (single ? 1 : 2), PYSIC_EINVAL,
+                      single ? "one-body potential needs 1 target" : "pair potential needs 2 targets");
+        PYSIC_REQUIRE(!(single && nmul), PYSIC_EINVAL,
+                      "products of one-body potentials are not supported");
         if (e1 < ns) M.maxpot[e1] = std::max(M.maxpot[e1], d[6]);
-        if (type == 1) {
-            PYSIC_REQUIRE(nt == 2, PYSIC_EINVAL, "'exponential' takes 2 targets");
+        const int group = (int)d[8];
+        if (type == 1 && nmul == 0) {
+            // the Tersoff kernels' pair term (plain or bond-order scaled)
             int &c = M.npair[e1][e2];
             PYSIC_REQUIRE(c < PYSIC_MAXP, PYSIC_EINVAL, "too many exponential terms for one pair");
             PairTerm &t = M.pair[e1][e2][c++];
@@ -195,17 +221,60 @@ void compile_model(pysic_ctx *h)
             // soft-cutoff rule of create_potential (Potentials.f90:2382-2388)
             if (d[7] > 0.0 && d[7] < d[6]) { t.soft = d[7]; t.smooth = 1; }
             else { t.soft = d[6]; t.smooth = 0; }
-            t.group = (int)d[8];
+            t.group = group;
             use_group(t.group);
             M.has_pairs = 1;
         } else if (type == 2) {
-            PYSIC_REQUIRE(nt == 1, PYSIC_EINVAL, "'charge_self' takes 1 target");
             int nexp = (int)d[5];
             PYSIC_REQUIRE(nexp >= 0 && nexp <= 4, PYSIC_EINVAL, "charge_self exponent must be 0..4");
             M.self_eps[e1][nexp] += d[4];
             M.has_self = 1;
+        } else if (type == 10 || type == 11) {
+            PYSIC_REQUIRE(group < 0, PYSIC_EINVAL,
+                          "bond-order scaling of one-body potentials is not supported");
+            if (type == 10) P.konst[e1] += d[4];
+            else { P.kforce[e1][0] += d[4]; P.kforce[e1][1] += d[5]; P.kforce[e1][2] += d[9]; }
+            P.has_single = 1;
         } else {
-            throw PysicError{PYSIC_EINVAL, "unsupported potential type"};
+            PYSIC_REQUIRE(group < 0, PYSIC_EINVAL,
+                          "bond-order scaling applies to a lone 'exponential' potential only");
+            PYSIC_REQUIRE(nmul + 1 <= PYSIC_MAXF, PYSIC_EINVAL, "at most 4 factors in a product");
+            int &c = P.npair[e1][e2];
+            PYSIC_REQUIRE(c < PYSIC_MAXG, PYSIC_EINVAL, "too many pair potentials for one pair");
+            GenPair &T = P.pair[e1][e2][c++];
+            T.cut = d[6];
+            if (d[7] > 0.0 && d[7] < d[6]) { T.soft = d[7]; T.smooth = 1; }
+            else { T.soft = d[6]; T.smooth = 0; }
+            T.nf = (int)nmul + 1;
+            for (int f = 0; f < T.nf; ++f) {
+                const double *r = row(k + f);
+                PairFactor &F = T.f[f];
+                F.type = (int)r[0];
+                PYSIC_REQUIRE(F.type == 1 || (F.type >= 3 && F.type <= 9), PYSIC_EINVAL,
+                              "a product takes pair potentials only");
+                F.p[0] = r[4]; F.p[1] = r[5]; F.p[2] = r[9]; F.p[3] = r[10];
+                if (F.type == 7) {
+                    const int id = (int)r[4];
+                    auto it = h->tables.find(id);
+                    PYSIC_REQUIRE(it != h->tables.end(), PYSIC_EINVAL,
+                                  "no table " + std::to_string(id) +
+                                      " for a 'tabulated' potential (pysic_set_potential_table)");
+                    auto off = tab_off.find(id);
+                    if (off == tab_off.end()) {
+                        off = tab_off.emplace(id, (int)htab.size()).first;
+                        const std::vector<double> &v = it->second;
+                        for (size_t q = 0; q + 1 < v.size(); q += 2)
+                            htab.push_back(make_double2(v[q], v[q + 1]));
+                    }
+                    F.toff = off->second;
+                    F.tlen = (int)(it->second.size() / 2);
+                }
+                if (F.type == 9) P.needs_pick = 1;
+            }
+            const double c2 = T.cut * T.cut * (1.0 + 1e-12);
+            P.cut2_hi[e1][e2] = std::max(P.cut2_hi[e1][e2], c2);
+            P.has_pairs = 1;
+            k += nmul;
         }
     }
     size_t nb = h->bonddesc.size() / 14;
@@ -238,6 +307,15 @@ void compile_model(pysic_ctx *h)
     h->S.dmodel.ensure(1);
     HIP_TRY(hipMemcpyAsync(h->S.dmodel.p, &h->S.model, sizeof(Model), hipMemcpyHostToDevice,
                            h->S.stream));
+    h->S.pmodel = P;
+    h->S.dpmodel.ensure(1);
+    HIP_TRY(hipMemcpyAsync(h->S.dpmodel.p, &h->S.pmodel, sizeof(PairModel),
+                           hipMemcpyHostToDevice, h->S.stream));
+    drop_graphs(h);  // captured sequences depend on which kernel families the model needs
+    h->S.tables.ensure(htab.empty() ? 1 : htab.size());
+    if (!htab.empty())
+        HIP_TRY(hipMemcpyAsync(h->S.tables.p, htab.data(), sizeof(double2) * htab.size(),
+                               hipMemcpyHostToDevice, h->S.stream));
     HIP_TRY(hipStreamSynchronize(h->S.stream));
     h->S.model_dirty = false;
 }
@@ -340,6 +418,9 @@ void build_lists(pysic_ctx *h)
     throw PysicError{PYSIC_ECAP, "neighbor list capacity could not be established"};
 }
 
+// the model has terms for pair.hip (pair potentials or one-body terms)
+bool has_gen(const System &S) { return S.pmodel.has_pairs || S.pmodel.has_single; }
+
 // forces (+ optional momentum kick) and the virial into totals[1..6]
 void tersoff_forces(pysic_ctx *h, bool kick, double half_dt)
 {
@@ -374,6 +455,41 @@ void tersoff_checked(pysic_ctx *h, int mode)
                       "more short-range neighbors (" + std::to_string(f[1]) + ") than any kernel holds");
         h->variant = v;
     }
+}
+
+// Forces of every term except Ewald into S.frc; virials into totals[1..6]
+// (Tersoff kernels) and gtotals[1..6] (pair.hip).
+void local_forces(pysic_ctx *h)
+{
+    System &S = h->S;
+    const bool gen = has_gen(S);
+    if (S.model.has_pairs) tersoff_checked(h, SITE_FORCES);
+    else if (!gen) k_zero4<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.frc.p);
+    if (gen) pr_eval(S, PAIR_FORCES, S.model.has_pairs != 0);
+}
+
+// The forces of a velocity-Verlet step (no virial), with the second
+// half-kick fused into the last kernel.  Kernel launches only (capturable).
+void md_forces(pysic_ctx *h, double half)
+{
+    System &S = h->S;
+    const bool gen = has_gen(S);
+    if (S.model.has_pairs) {
+        ts_site(S, h->variant, SITE_FORCES_ONLY);
+        ts_gather(S, half, gen ? 0 : 1);
+    }
+    if (gen) pr_eval(S, PAIR_FORCES_ONLY, S.model.has_pairs != 0, half, 1);
+}
+
+// Potential energy (all terms but Ewald) into *dst on the device.
+void md_energy(pysic_ctx *h, double *dst)
+{
+    System &S = h->S;
+    const bool gen = has_gen(S);
+    if (S.model.has_pairs) tersoff_energy(h);
+    if (gen) pr_eval(S, PAIR_ENERGY, false);
+    k_sum_energy<<<1, 64, 0, S.stream>>>(S.model.has_pairs ? S.totals.p : nullptr,
+                                         gen ? S.gtotals.p : nullptr, dst);
 }
 
 // Put the atoms back into the caller's order (slot i = atom i).
@@ -427,6 +543,7 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
             S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.nbr.p, S.nimg.p,
             S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
+            S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p,
             (const void *)(long)S.cap, (const void *)(long)h->variant,
             (const void *)(long)S.site_grid, (const void *)(long)S.n, (const void *)(long)S.nbins,
             (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};
@@ -489,8 +606,7 @@ void md_step(pysic_ctx *h, double dt, double half)
     System &S = h->S;
     nb_sort_atoms(S, true, dt, half);
     nb_build_lists(S);
-    ts_site(S, h->variant, SITE_FORCES_ONLY);
-    ts_gather(S, half, 1);
+    md_forces(h, half);
 }
 
 std::vector<const void *> md_signature(pysic_ctx *h, double dt)
@@ -682,7 +798,19 @@ int pysic_set_potentials(pysic_ctx *h, int npot, const double *desc)
 {
     API_BEGIN(h)
     PYSIC_REQUIRE(npot >= 0 && (npot == 0 || desc), PYSIC_EINVAL, "set_potentials: bad arguments");
-    h->potdesc.assign(desc, desc + 10 * (size_t)npot);
+    h->potdesc.assign(desc, desc + POT_STRIDE * (size_t)npot);
+    h->S.model_dirty = true;
+    h->forces_valid = false;
+    if (h->atoms_ready) compile_model(h);
+    API_END(h)
+}
+
+int pysic_set_potential_table(pysic_ctx *h, int id, int nrows, const double *rows)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(nrows >= 2 && rows, PYSIC_EINVAL,
+                  "set_potential_table: a table needs at least 2 rows");
+    h->tables[id].assign(rows, rows + 2 * (size_t)nrows);
     h->S.model_dirty = true;
     h->forces_valid = false;
     if (h->atoms_ready) compile_model(h);
@@ -798,6 +926,13 @@ int pysic_calculate_energy(pysic_ctx *h, double *energy)
         check_flags(h, false);
         e += t[0];
     }
+    if (has_gen(S)) {
+        pr_eval(S, PAIR_ENERGY, false);
+        double t = 0.0;
+        HIP_TRY(hipMemcpyAsync(&t, S.gtotals.p, sizeof(double), hipMemcpyDeviceToHost, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));
+        e += t;
+    }
     if (S.model.has_self || h->ewald.on) e += ewald_eem_energy(S, h->ewald, h->ew);
     *energy = e;
     API_END(h)
@@ -809,16 +944,15 @@ int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress)
     require_ready(h);
     System &S = h->S;
     PYSIC_REQUIRE(n == S.n && forces && stress, PYSIC_EINVAL, "calculate_forces: bad arguments");
-    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (S.model.has_pairs) {
-        tersoff_checked(h, SITE_FORCES);
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0}, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    local_forces(h);
+    if (S.model.has_pairs)
         HIP_TRY(hipMemcpyAsync(t, S.totals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
-        HIP_TRY(hipStreamSynchronize(S.stream));
-        check_flags(h, false);
-    } else {
-        k_zero4<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.frc.p);
-    }
-    for (int q = 0; q < 6; ++q) stress[q] = t[1 + q];
+    if (has_gen(S))
+        HIP_TRY(hipMemcpyAsync(g, S.gtotals.p, sizeof(double) * 7, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    if (S.model.has_pairs) check_flags(h, false);
+    for (int q = 0; q < 6; ++q) stress[q] = t[1 + q] + g[1 + q];
     if (h->ewald.on) {
         double st[6];
         ewald_add_forces(S, h->ewald, h->ew, st);  // adds into S.frc
@@ -865,12 +999,12 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
     require_ready(h);
     System &S = h->S;
     PYSIC_REQUIRE(nsteps >= 0 && rebuild >= 0, PYSIC_EINVAL, "md_run: bad arguments");
-    PYSIC_REQUIRE(!h->ewald.on && S.model.has_pairs, PYSIC_EINVAL,
-                  "md_run drives the Tersoff/pair engine only");
+    PYSIC_REQUIRE(!h->ewald.on && (S.model.has_pairs || has_gen(S)), PYSIC_EINVAL,
+                  "md_run drives the Tersoff and pair potentials (no Ewald)");
     HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     DBuf<double> ebuf;
     if (energies) ebuf.ensure(nsteps > 0 ? nsteps : 1);
-    if (!h->forces_valid) tersoff_checked(h, SITE_FORCES);
+    if (!h->forces_valid) local_forces(h);
     HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));
     const double half = 0.5 * dt;
     if (rebuild == 1 && !energies && !S.timer.on && nsteps >= 3) {
@@ -883,13 +1017,9 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
                 md_step(h, dt, half);
             } else {
                 md_drift(S, dt, half);
-                ts_site(S, h->variant, SITE_FORCES_ONLY);
-                ts_gather(S, half, 1);
+                md_forces(h, half);
             }
-            if (energies) {
-                tersoff_energy(h);  // an observer: not part of the step itself
-                k_copy_total<<<1, 64, 0, S.stream>>>(S.totals.p, ebuf.p + step);
-            }
+            if (energies) md_energy(h, ebuf.p + step);  // an observer, not part of the step
         }
     }
     HIP_TRY(hipGetLastError());
@@ -1041,7 +1171,7 @@ int pysic_evaluate_cells(pysic_ctx *h, int use_graph, int rebuild)
     require_ready(h);
     System &S = h->S;
     PYSIC_REQUIRE(S.ncells > 0, PYSIC_ESTATE, "evaluate_cells: no batch of cells set");
-    PYSIC_REQUIRE(S.model.has_pairs && !h->ewald.on, PYSIC_EINVAL,
+    PYSIC_REQUIRE(S.model.has_pairs && !h->ewald.on && !has_gen(S), PYSIC_EINVAL,
                   "evaluate_cells: a batch evaluates Tersoff / exponential terms only");
     S.atom_ev.ensure((size_t)7 * S.n);
     S.cell_ev.ensure((size_t)8 * S.ncells);

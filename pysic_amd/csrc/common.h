@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <cmath>
 #include <string>
 
 #define PYSIC_MAXS 4      // species per system
@@ -47,6 +49,38 @@ struct Model {
     int has_self;
 };
 
+// Pair potentials outside the Tersoff kernels (pair.hip): every 2-body type
+// of the reference except a bond-order-coordinated exponential, and products
+// of them (ProductPotential).  Type codes are the descriptor's:
+// 1 exponential, 3 LJ, 4 spring, 5 Buckingham, 6 power, 7 tabulated,
+// 8 shift_power, 9 step; parameters in the reference's order.
+#define PYSIC_MAXG 4      // such potentials per ordered species pair
+#define PYSIC_MAXF 4      // factors of one product
+struct PairFactor {
+    int type;
+    int toff, tlen;       // tabulated: first row in the table buffer, rows
+    double p[4];
+};
+struct GenPair {
+    PairFactor f[PYSIC_MAXF];  // f[0] the potential, then its multipliers
+    int nf;
+    int smooth;           // 1 when 0 < soft < cut
+    double cut, soft;
+};
+struct PairModel {
+    int npair[PYSIC_MAXS][PYSIC_MAXS];
+    GenPair pair[PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXG];
+    // squared largest cutoff of the ordered species pair, widened by 1e-12
+    // relative: r^2 at or beyond it means r >= every cutoff, so the square
+    // root and the exact test are skipped
+    double cut2_hi[PYSIC_MAXS][PYSIC_MAXS];
+    // 'constant' (E = V) and 'force' (F; E = -F . r) one-body terms per species
+    double konst[PYSIC_MAXS];
+    double kforce[PYSIC_MAXS][3];
+    int has_pairs, has_single;
+    int needs_pick;       // a 'step' factor: its force depends on which atom the reference takes first
+};
+
 // ---------------------------------------------------------------------------
 // packed periodic image offsets: three signed 10-bit fields
 // ---------------------------------------------------------------------------
@@ -79,6 +113,29 @@ __device__ __forceinline__ double3 separation64(const double4 &ri, const double4
     x = x + oz * v[6]; y = y + oz * v[7]; z = z + oz * v[8];
     return make_double3(x, y, z);
 }
+// pick (Geometry.f90:648-676) on original indices: whether the reference
+// evaluates the pair from atom i1's side
+__device__ __forceinline__ bool pick_dev(int i1, int i2, int img)
+{
+    if (i2 > i1) return true;
+    if (i2 < i1) return false;
+    int ox = img_x(img), oy = img_y(img), oz = img_z(img);
+    return ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz > 0)));
+}
+
+// smoothening_factor and smoothening_derivative (Potentials.f90:442-509)
+__device__ __forceinline__ double fc64(double r, double hard, double soft)
+{
+    if (r > hard) return 0.0;
+    if (r < soft) return 1.0;
+    return 0.5 * (1.0 + cos(M_PI * (r - soft) / (hard - soft)));
+}
+__device__ __forceinline__ double dfc64(double r, double hard, double soft)
+{
+    if (r > hard || r < soft) return 0.0;
+    return -0.5 * M_PI / (hard - soft) * sin(M_PI * (r - soft) / (hard - soft));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Remap so that XCD x runs one contiguous range of logical blocks:
 // spatially adjacent atoms (the cell-sorted order) then share an L2.

@@ -1,0 +1,356 @@
+// pair.hip -- pair potentials and one-body terms outside the Tersoff kernels.
+//
+// Restates the doublet loop of the reference for the plain (uncoordinated)
+// pair potentials: core_evaluate_local_doublet_energy_B and _forces_B
+// (Core.f90:4257-4385, 5371-5606) with evaluate_energy / evaluate_forces and
+// their components (Potentials.f90:2640-2890) for the types LJ, spring,
+// exponential (as a product factor), Buckingham, power, tabulated,
+// shift_power and step, products of them (ProductPotential), and the
+// one-body 'constant' and 'force' terms of core_evaluate_local_singlet
+// (Core.f90:4095-4239).
+//
+// The reference visits every pair once (pick) and scatters the force to both
+// atoms.  Here one thread owns one atom and walks its full neighbor list:
+// each pair is evaluated from both ends, every atom sums its own force, and
+// nothing is scattered (no atomics; the summation order is fixed, so runs
+// are bit-for-bit repeatable).  Per atom and pair the thread adds half the
+// pair energy and half of -r_ij (x) g_i to the virial, g_i being the force
+// on atom i from that pair.  All arithmetic is FP64.
+//
+// The force on the first atom of a pair is written f1 = G s + C (1,1,1),
+// s = r_2 - r_1: every type is central (C = 0) except 'step', whose
+// reference force is the same scalar in all three components.  For a
+// central force both ends agree on g_i = G r_ij; for 'step' the sign of C
+// depends on which atom the reference takes first (pick), so that is
+// evaluated only when the model has a step factor.
+#include "system.h"
+
+namespace {
+
+constexpr int PB = 256;
+
+struct Radial {
+    double v;   // energy of the factor
+    double g;   // f1 = g s + c (1,1,1)
+    double c;
+};
+
+// One factor at distance r > 0 (evaluate_energy_component and
+// evaluate_force_component, Potentials.f90:2717-2890; the per-type
+// routines at 2971-3085, 3643-3677, 4009-4298, 4803-4967).  cut is the
+// potential's cutoff (the spring's energy shift uses it).
+__device__ __forceinline__ Radial factor(const PairFactor &f, double r, double cut,
+                                         const double2 *__restrict__ tab, bool need_force)
+{
+    Radial o{0.0, 0.0, 0.0};
+    const double p0 = f.p[0], p1 = f.p[1], p2 = f.p[2], p3 = f.p[3];
+    switch (f.type) {
+    case 1: {  // exponential: eps exp(-zeta r)
+        const double ex = exp(-p1 * r);
+        o.v = p0 * ex;
+        if (need_force) o.g = -p1 * p0 * ex / r;
+        break;
+    }
+    case 3: {  // LJ: eps ((sigma/r)^12 - (sigma/r)^6)
+        const double ratio = p1 / r;
+        const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+        o.v = p0 * (r6 * r6 - r6);
+        if (need_force) o.g = p0 * (6.0 * r6 - 12.0 * r6 * r6) / (r * r);
+        break;
+    }
+    case 4: {  // spring: k/2 ((r - R0)^2 - (cut - R0)^2) inside |r - R0| < |cut - R0|
+        const double r2 = r - p1, r6 = cut - p1;
+        if (r2 * r2 < r6 * r6) {
+            o.v = p0 * 0.5 * (r2 * r2 - r6 * r6);
+            if (need_force) o.g = p0 * r2 / r;
+        }
+        break;
+    }
+    case 5: {  // Buckingham: A exp(-r/sigma) - C (sigma/r)^6
+        const double ratio = r / p2;
+        const double r6 = 1.0 / (ratio * ratio * ratio * ratio * ratio * ratio);
+        const double ex = exp(-ratio);
+        o.v = p0 * ex - p1 * r6;
+        if (need_force) o.g = (-p0 * ratio * ex + p1 * 6.0 * r6) / (r * r);
+        break;
+    }
+    case 6: {  // power: eps (a/r)^n
+        const double pw = pow(p1 / r, p2);
+        o.v = p0 * pw;
+        if (need_force) o.g = p0 * (-p2 * pw) / (r * r);
+        break;
+    }
+    case 7: {  // tabulated: cubic Hermite on (value, derivative) rows, scaled
+        if (r >= p1) {
+            o.v = tab[f.toff + f.tlen - 1].x;  // the reference does not scale it here
+            break;
+        }
+        const double dr = p1 / (f.tlen - 1);
+        int lo = (int)floor(r / dr) + 1;       // 1-based, as the reference
+        if (lo > f.tlen - 1) lo = f.tlen - 1;  // stay inside the table
+        const double t = r / dr - (lo - 1);
+        const double t2 = t * t, t3 = t2 * t;
+        const double2 a = tab[f.toff + lo - 1], b = tab[f.toff + lo];
+        o.v = p2 * (a.x * (2 * t3 - 3 * t2 + 1) + a.y * dr / p1 * (t3 - 2 * t2 + t) +
+                    b.x * (3 * t2 - 2 * t3) + b.y * dr / p1 * (t3 - t2));
+        if (need_force)
+            o.g = p2 *
+                  (a.x / dr * 6 * (t2 - t) + a.y / p1 * (3 * t2 - 4 * t + 1) +
+                   b.x / dr * 6 * (t - t2) + b.y / p1 * (3 * t2 - 2 * t)) /
+                  r;
+        break;
+    }
+    case 8: {  // shift_power: eps ((r1 - r)/(r1 - r2))^n
+        const double w = p1 - p2, q = (p1 - r) / w;
+        o.v = p0 * pow(q, p3);
+        if (need_force) o.g = -p0 * p3 / w * pow(q, p3 - 1.0) / r;
+        break;
+    }
+    case 9: {  // step: e1 below r1, e2 above r2, a cosine in between
+        const double w = p3 - p2;
+        if (r < p2) {
+            o.v = p0;
+        } else if (r > p3) {
+            o.v = p1;
+        } else {
+            const double ph = M_PI * (r - p2) / w;
+            o.v = (p0 + p1) * 0.5 * (1.0 + 0.5 * cos(ph));
+            if (need_force) o.c = (p0 + p1) * 0.25 * M_PI / w * sin(ph);
+        }
+        break;
+    }
+    default:
+        break;
+    }
+    return o;
+}
+
+// A potential: the product of its factors (evaluate_energy / evaluate_forces,
+// Potentials.f90:2640-2693, 2791-2814): E = prod E_i,
+// f = sum_i (E / E_i) f_i, or prod_{j != i} E_j f_i where E_i = 0.
+// Two passes keep every factor in registers: the first forms E, the product
+// of the nonzero energies and the number of zero ones; the second
+// re-evaluates each factor with its force.
+__device__ __forceinline__ Radial potential(const GenPair &T, double r,
+                                            const double2 *__restrict__ tab, bool need_force)
+{
+    const int nf = T.nf;
+    if (nf == 1) return factor(T.f[0], r, T.cut, tab, need_force);
+    double e = 1.0, enz = 1.0;
+    int zeros = 0;
+    for (int k = 0; k < nf; ++k) {
+        const double v = factor(T.f[k], r, T.cut, tab, false).v;
+        e = e * v;
+        if (v != 0.0) enz = enz * v;
+        else ++zeros;
+    }
+    Radial out{e, 0.0, 0.0};
+    if (!need_force) return out;
+    for (int k = 0; k < nf; ++k) {
+        const Radial q = factor(T.f[k], r, T.cut, tab, true);
+        const double w = q.v != 0.0 ? e / q.v : (zeros == 1 ? enz : 0.0);
+        out.g += w * q.g;
+        out.c += w * q.c;
+    }
+    return out;
+}
+
+// Lennard-Jones alone, the common molecular-dynamics model: the same
+// arithmetic as factor() case 3 with the square root and divisions shared.
+__device__ __forceinline__ Radial lj(const GenPair &T, double r, double inv_r, bool need_force)
+{
+    const double p0 = T.f[0].p[0];
+    const double ratio = T.f[0].p[1] * inv_r;
+    const double r6 = ratio * ratio * ratio * ratio * ratio * ratio;
+    Radial o{p0 * (r6 * r6 - r6), 0.0, 0.0};
+    if (need_force) o.g = p0 * (6.0 * r6 - 12.0 * r6 * r6) * (inv_r * inv_r);
+    return o;
+}
+
+// KIND 0: any pair potentials; KIND 1: every pair potential is a lone LJ.
+template <class Geo, int MODE, int KIND>
+__global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__restrict__ pos,
+                                              const int *__restrict__ id,
+                                              const int *__restrict__ nbr,
+                                              const int *__restrict__ nimg,
+                                              const int *__restrict__ ncnt,
+                                              const PairModel *__restrict__ PM,
+                                              const double2 *__restrict__ tab,
+                                              double4 *__restrict__ frc, int accumulate,
+                                              double4 *__restrict__ mom, double half_dt, int kick,
+                                              double *__restrict__ partial)
+{
+    constexpr bool FORCE = MODE != PAIR_ENERGY;
+    constexpr bool VIR = MODE == PAIR_FORCES;
+    __shared__ double red[7][PB];
+    const int blk = xcd_block(gridDim.x);
+    const int i = blk * PB + threadIdx.x;
+    double e = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int si = (int)pi.w;
+        if (PM->has_single) {
+            const double kfx = PM->kforce[si][0], kfy = PM->kforce[si][1],
+                         kfz = PM->kforce[si][2];
+            if (FORCE) {
+                fx = kfx; fy = kfy; fz = kfz;
+            } else {
+                // E = V and -F . r (the dot product of Quaternions.f90:475-483)
+                e = PM->konst[si] + ((-kfx * pi.x + -kfy * pi.y) + -kfz * pi.z);
+            }
+        }
+        const double *cv = geo.vec(i);
+        const int nf = ncnt[i];
+        const bool pick = FORCE && PM->needs_pick;
+        const int idi = pick ? id[i] : 0;
+        for (int t = 0; t < nf; ++t) {
+            const size_t o = (size_t)t * n + i;
+            const int j = nbr[o];
+            const int im = nimg[o];
+            const double4 pj = pos[j];
+            const int sj = (int)pj.w;
+            const int np = PM->npair[si][sj];
+            if (np == 0) continue;
+            const double3 d = separation64(pi, pj, im, cv);
+            const double r2 = dot64(d);
+            if (!(r2 < PM->cut2_hi[si][sj])) continue;
+            const double r = sqrt(r2);
+            const double inv_r = KIND == 1 ? 1.0 / r : 0.0;
+            double vsum = 0.0, G = 0.0, C = 0.0;
+            for (int c = 0; c < np; ++c) {
+                const GenPair &T = PM->pair[si][sj][c];
+                if (!(T.cut > r)) continue;
+                if (!FORCE) {
+                    const Radial q = KIND == 1 ? lj(T, r, inv_r, false) : potential(T, r, tab, false);
+                    vsum += T.smooth ? q.v * fc64(r, T.cut, T.soft) : q.v;
+                } else if (r > 0.0) {  // the reference's forces vanish at r == 0
+                    const Radial q = KIND == 1 ? lj(T, r, inv_r, true) : potential(T, r, tab, true);
+                    if (T.smooth) {
+                        // f1 fc + v grad(fc), grad(fc) = fc' s / r
+                        const double f = fc64(r, T.cut, T.soft), df = dfc64(r, T.cut, T.soft);
+                        G += q.g * f + q.v * df / r;
+                        C += q.c * f;
+                    } else {
+                        G += q.g;
+                        C += q.c;
+                    }
+                }
+            }
+            if (!FORCE) {
+                e += 0.5 * vsum;
+            } else {
+                // atom i is the first atom when pick(i, j) holds: g_i = f1
+                // with s = r_ij; otherwise g_i = -f1 with s = -r_ij
+                double cs = C;
+                if (pick && C != 0.0 && !pick_dev(idi, id[j], im)) cs = -C;
+                const double gx = G * d.x + cs, gy = G * d.y + cs, gz = G * d.z + cs;
+                fx += gx; fy += gy; fz += gz;
+                if (VIR) {
+                    w0 -= 0.5 * d.x * gx;
+                    w1 -= 0.5 * d.y * gy;
+                    w2 -= 0.5 * d.z * gz;
+                    w3 -= 0.5 * d.y * gz;
+                    w4 -= 0.5 * d.x * gz;
+                    w5 -= 0.5 * d.x * gy;
+                }
+            }
+        }
+        if (FORCE) {
+            double4 f = make_double4(fx, fy, fz, 0.0);
+            if (accumulate) {
+                const double4 a = frc[i];
+                f = make_double4(a.x + fx, a.y + fy, a.z + fz, 0.0);
+            }
+            frc[i] = f;
+            if (kick) {
+                double4 p = mom[i];
+                p.x += half_dt * f.x;
+                p.y += half_dt * f.y;
+                p.z += half_dt * f.z;
+                mom[i] = p;
+            }
+        }
+    }
+    if (MODE == PAIR_FORCES_ONLY) return;
+    // block partials (fixed tree): energy, or the six virial components
+    const int tid = threadIdx.x;
+    red[0][tid] = e;
+    red[1][tid] = w0; red[2][tid] = w1; red[3][tid] = w2;
+    red[4][tid] = w3; red[5][tid] = w4; red[6][tid] = w5;
+    __syncthreads();
+    for (int s = PB / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int c = 0; c < 7; ++c) red[c][tid] += red[c][tid + s];
+        __syncthreads();
+    }
+    if (tid < 7) partial[(size_t)blk * 8 + tid] = red[tid][0];
+}
+
+// gtotals[c] = sum over blocks of partial[b*8 + c], fixed order (block c)
+__global__ __launch_bounds__(PB) void k_pair_totals(int nblocks, const double *__restrict__ partial,
+                                                    double *__restrict__ totals)
+{
+    __shared__ double red[PB];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    double s = 0.0;
+    for (int b = tid; b < nblocks; b += PB) s += partial[(size_t)b * 8 + c];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = PB / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) totals[c] = red[0];
+}
+
+template <class Geo, int KIND>
+void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt, int kick)
+{
+    const int nb = div_up(S.n, PB);
+    const int acc = accumulate ? 1 : 0;
+    auto args = [&](auto kern) {
+        kern<<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p,
+                                      S.dpmodel.p, S.tables.p, S.frc.p, acc, S.mom.p, half_dt,
+                                      kick, S.gpartial.p);
+    };
+    if (mode == PAIR_ENERGY) args(k_pairs<Geo, PAIR_ENERGY, KIND>);
+    else if (mode == PAIR_FORCES) args(k_pairs<Geo, PAIR_FORCES, KIND>);
+    else args(k_pairs<Geo, PAIR_FORCES_ONLY, KIND>);
+    HIP_TRY(hipGetLastError());
+}
+
+// every pair potential of the model is a lone LJ
+bool lj_only(const PairModel &P)
+{
+    for (int a = 0; a < PYSIC_MAXS; ++a)
+        for (int b = 0; b < PYSIC_MAXS; ++b)
+            for (int c = 0; c < P.npair[a][b]; ++c)
+                if (P.pair[a][b][c].nf != 1 || P.pair[a][b][c].f[0].type != 3) return false;
+    return true;
+}
+
+}  // namespace
+
+void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
+{
+    PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
+    PYSIC_REQUIRE(S.n > 0, PYSIC_ESTATE, "no atoms");
+    const int nb = div_up(S.n, PB);
+    S.gpartial.ensure((size_t)nb * 8);
+    S.gtotals.ensure(8);
+    S.gpartial_blocks = nb;
+    hipEvent_t ev{};
+    const char *name = mode == PAIR_ENERGY ? "pair_energy" : "pair_forces";
+    S.timer.begin(name, S.stream, &ev);
+    const bool lj = lj_only(S.pmodel);
+    with_geo(S, [&](auto geo) {
+        if (lj) launch<decltype(geo), 1>(S, geo, mode, accumulate, half_dt, kick);
+        else launch<decltype(geo), 0>(S, geo, mode, accumulate, half_dt, kick);
+    });
+    if (mode != PAIR_FORCES_ONLY) {
+        k_pair_totals<<<7, PB, 0, S.stream>>>(nb, S.gpartial.p, S.gtotals.p);
+        HIP_TRY(hipGetLastError());
+    }
+    S.timer.end(name, S.stream, ev);
+}

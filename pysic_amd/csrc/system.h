@@ -87,6 +87,14 @@ struct System {
     DBuf<Model> dmodel;
     bool model_dirty = true;
 
+    // pair potentials of pair.hip and their tables
+    PairModel pmodel{};
+    DBuf<PairModel> dpmodel;
+    DBuf<double2> tables;       // 'tabulated' rows (value, derivative), all tables
+    DBuf<double> gpartial;      // per-block energy + virial partials of pair.hip
+    int gpartial_blocks = 0;
+    DBuf<double> gtotals;       // reduced energy + virial (7 doubles)
+
     // Batch of independent periodic cells (ncells > 0).  The atoms of cell c
     // occupy slots [hatom0[c], hatom0[c+1]) in every order: the bins of each
     // cell form their own key range, so the sort moves atoms within cells.
@@ -135,6 +143,14 @@ int ts_variant_count();
 const char *ts_variant_name(int v);
 int ts_variant_capacity(int v);
 int ts_variant_for_capacity(int need);
+
+// pair.hip
+// PAIR_ENERGY: gtotals[0] = E; PAIR_FORCES: forces into frc, gtotals[1..6] =
+// virial; PAIR_FORCES_ONLY: forces into frc, then p += half_dt F if kick
+// (the second half-kick of a velocity-Verlet step, with the total force).
+// accumulate: add to frc instead of overwriting it.
+enum { PAIR_ENERGY = 0, PAIR_FORCES = 1, PAIR_FORCES_ONLY = 2 };
+void pr_eval(System &S, int mode, bool accumulate, double half_dt = 0.0, int kick = 0);
 
 // md.hip
 void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m (no rebuild)

@@ -32,19 +32,6 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-__device__ __forceinline__ double fc64(double r, double hard, double soft)
-{
-    // smoothening_factor (Potentials.f90:442-458)
-    if (r > hard) return 0.0;
-    if (r < soft) return 1.0;
-    return 0.5 * (1.0 + cos(M_PI * (r - soft) / (hard - soft)));
-}
-__device__ __forceinline__ double dfc64(double r, double hard, double soft)
-{
-    // smoothening_derivative (Potentials.f90:485-501)
-    if (r > hard || r < soft) return 0.0;
-    return -0.5 * M_PI / (hard - soft) * sin(M_PI * (r - soft) / (hard - soft));
-}
 __device__ __forceinline__ float fc32(float r, float hard, float soft)
 {
     if (r > hard) return 0.0f;
@@ -83,15 +70,6 @@ __device__ __forceinline__ void bt64(double zeta, const BondTerm &p, double *bt,
         *dbt = 0.0;
     else
         *dbt = inv_eta * pow(1.0 + pw, inv_eta - 1.0) * p.eta * pow(be, p.eta - 1.0) * p.beta;
-}
-
-// pick (Geometry.f90:648-676) on original indices
-__device__ __forceinline__ bool pick_dev(int i1, int i2, int img)
-{
-    if (i2 > i1) return true;
-    if (i2 < i1) return false;
-    int ox = img_x(img), oy = img_y(img), oz = img_z(img);
-    return ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz > 0)));
 }
 
 template <int G, int SCAP>

@@ -33,6 +33,7 @@ SIGNATURES = {
     "pysic_update_atom_charges": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pysic_get_number_of_atoms": (ctypes.c_int, [_vp, _ip]),
     "pysic_set_potentials": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_set_potential_table": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "pysic_set_bond_order_factors": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pysic_set_ewald": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
                                        ctypes.c_double, ctypes.c_double, _dp]),
@@ -179,8 +180,26 @@ class Engine:
         self._check(self.L.pysic_update_atom_charges(self.h, len(q), _d(q)), "update_atom_charges")
 
     def set_potentials(self, potdesc):
-        pd = _f64(potdesc).reshape(-1)
-        self._check(self.L.pysic_set_potentials(self.h, len(pd) // 10, _d(pd)), "set_potentials")
+        """Descriptor rows of 12 doubles (include/pysic_hip.h); rows of 10
+        (no p3, p4, link) are padded with zeros."""
+        pd = np.asarray(potdesc, dtype=np.float64)
+        if pd.size == 0:
+            pd = np.zeros((0, 12))
+        elif pd.ndim != 2:
+            pd = pd.reshape(-1, 12)
+        if pd.shape[1] == 10:
+            pd = np.hstack([pd, np.zeros((len(pd), 2))])
+        if pd.shape[1] != 12:
+            raise ValueError("potential descriptor rows hold 12 doubles")
+        pd = np.ascontiguousarray(pd.reshape(-1))
+        self._check(self.L.pysic_set_potentials(self.h, len(pd) // 12, _d(pd)), "set_potentials")
+
+    def set_potential_table(self, table_id, rows):
+        """Table of the 'tabulated' potentials with this id: rows of
+        (value, derivative)."""
+        t = _f64(rows, (-1, 2))
+        self._check(self.L.pysic_set_potential_table(self.h, int(table_id), len(t), _d(t)),
+                    "set_potential_table")
 
     def set_bond_order_factors(self, bonddesc):
         bd = _f64(bonddesc).reshape(-1)

@@ -1,13 +1,16 @@
-"""Interaction objects of the pysic API: Potential, BondOrderParameters,
-Coordinator, and their translation into engine descriptors.
+"""Interaction objects of the pysic API: Potential, ProductPotential,
+BondOrderParameters, Coordinator, and their translation into engine
+descriptors.
 
-Mirrors pysic/interactions/local.py (Potential, local.py:284-776) and
-pysic/interactions/bondorder.py (BondOrderParameters :7-375, Coordinator
-:379-532) for the potential and bond-order types on the Tersoff / EEM path:
-'exponential' and 'charge_self' potentials and the 'tersoff' bond-order
-factor.  Parameter names and orders follow the reference's descriptor
-registry (Potentials.f90:3612-3677, 4312-4372, 5685-5736).
+Mirrors pysic/interactions/local.py (ProductPotential local.py:7-282,
+Potential :284-776) and pysic/interactions/bondorder.py (BondOrderParameters
+:7-375, Coordinator :379-532) for the pair and one-body potential types
+(exponential, LJ, spring, Buckingham, power, tabulated, shift_power, step,
+charge_self, constant, force) and the 'tersoff' bond-order factor.
+Parameter names and orders follow the reference's descriptor registry
+(Potentials.f90:2909-4990, 5685-5736).
 """
+import os
 from itertools import permutations
 
 import numpy as np
@@ -21,12 +24,30 @@ class InvalidParametersError(Exception):
     """Raised for bad parameter lists (utility/error.py)."""
 
 
-# name -> (n_targets, parameter names)  (Potentials.f90 characterizers)
+# name -> (n_targets, parameter names)  (Potentials.f90 characterizers:
+# LJ 2916-2961, spring 3016-3033, force 3096-3115, exponential 3612-3630,
+# constant 3917-3932, Buckingham 3980-3999, power 4060-4109, tabulated
+# 4169-4217, charge_self 4312-4330, shift_power 4744-4793, step 4858-4907).
+# The reference's 'step' registry writes 'e1' and then 'e2' into the first
+# name slot and leaves the second blank; the names below are the ones its
+# description gives (V = e1 below r1, e2 above r2).
 POTENTIAL_TYPES = {
     "exponential": (2, ["epsilon", "zeta"]),
     "charge_self": (1, ["epsilon", "n"]),
+    "LJ": (2, ["epsilon", "sigma"]),
+    "spring": (2, ["k", "R_0"]),
+    "Buckingham": (2, ["A", "C", "sigma"]),
+    "power": (2, ["epsilon", "a", "n"]),
+    "tabulated": (2, ["id", "range", "scale"]),
+    "shift_power": (2, ["epsilon", "r1", "r2", "n"]),
+    "step": (2, ["e1", "e2", "r1", "r2"]),
+    "constant": (1, ["V"]),
+    "force": (1, ["Fx", "Fy", "Fz"]),
 }
-POTENTIAL_CODES = {"exponential": 1, "charge_self": 2}
+# descriptor type codes (include/pysic_hip.h)
+POTENTIAL_CODES = {"exponential": 1, "charge_self": 2, "LJ": 3, "spring": 4, "Buckingham": 5,
+                   "power": 6, "tabulated": 7, "shift_power": 8, "step": 9, "constant": 10,
+                   "force": 11}
 
 # name -> (n_targets, level, [[scaling params], [local params]])
 BOND_ORDER_TYPES = {
@@ -259,6 +280,14 @@ class Potential:
     def get_potential_type(self):
         return self.potential_type
 
+    def get_potentials(self):
+        """[self], as ProductPotential.get_potentials (local.py:751-757)."""
+        return [self]
+
+    def is_multiplier(self):
+        """[False], as ProductPotential.is_multiplier (local.py:759-765)."""
+        return [False]
+
     def __eq__(self, other):
         return (isinstance(other, Potential) and self.potential_type == other.potential_type
                 and self.symbols == other.symbols and self.parameters == other.parameters
@@ -268,6 +297,104 @@ class Potential:
     def __repr__(self):
         return "Potential('%s',symbols=%s,parameters=%s,cutoff=%s,cutoff_margin=%s)" % (
             self.potential_type, self.symbols, self.parameters, self.cutoff, self.cutoff_margin)
+
+
+class ProductPotential:
+    """The product of potentials with the same number of targets
+    (local.py:7-282).  Targets, cutoffs and coordinator are the first
+    potential's; the others only multiply its value."""
+
+    def __init__(self, potentials):
+        self.potentials = []
+        self.n_targets = None
+        for pot in potentials:
+            self.add_potential(pot)
+
+    def set_potentials(self, potentials):
+        self.potentials = []
+        self.n_targets = None
+        for pot in potentials:
+            self.add_potential(pot)
+
+    def add_potential(self, potential):
+        """local.py:53-84: every factor must have the same number of targets."""
+        if self.n_targets is None:
+            self.n_targets = potential.get_number_of_targets()
+        elif self.n_targets != potential.get_number_of_targets():
+            raise InvalidPotentialError(
+                "You may only multiply potentials with the same number of targets.")
+        self.potentials.append(potential)
+
+    def get_potentials(self):
+        return self.potentials
+
+    def is_multiplier(self):
+        """[False, True, ..., True] (local.py:256-263)."""
+        return [False] + [True] * (len(self.potentials) - 1)
+
+    def get_symbols(self):
+        return self.potentials[0].get_symbols()
+
+    def set_symbols(self, symbols):
+        self.potentials[0].set_symbols(symbols)
+
+    def get_different_symbols(self):
+        return self.potentials[0].get_different_symbols()
+
+    def get_cutoff(self):
+        return self.potentials[0].get_cutoff()
+
+    def set_cutoff(self, cutoff):
+        self.potentials[0].set_cutoff(cutoff)
+
+    def get_cutoff_margin(self):
+        return self.potentials[0].get_cutoff_margin()
+
+    def set_cutoff_margin(self, margin):
+        self.potentials[0].set_cutoff_margin(margin)
+
+    def get_soft_cutoff(self):
+        return self.potentials[0].get_soft_cutoff()
+
+    def get_number_of_targets(self):
+        return self.n_targets
+
+    def get_coordinator(self):
+        return self.potentials[0].get_coordinator()
+
+    def set_coordinator(self, coordinator):
+        self.potentials[0].set_coordinator(coordinator)
+
+    def __eq__(self, other):
+        return (isinstance(other, ProductPotential) and self.n_targets == other.n_targets
+                and self.potentials == other.potentials)
+
+    def __repr__(self):
+        return "ProductPotential(%s)" % self.potentials
+
+
+def table_ids(potentials):
+    """Ids of the tables that the 'tabulated' potentials (and product
+    factors) use."""
+    ids = set()
+    for pots in potentials or []:
+        for pot in pots.get_potentials():
+            if pot.get_potential_type() == "tabulated":
+                ids.add(int(pot.get_parameter_values()[0]))
+    return sorted(ids)
+
+
+def read_table(table_id, directory="."):
+    """The file table_<id>.txt that the reference reads when it creates a
+    'tabulated' potential (Potentials.f90:2462-2465, read_table
+    Utility.f90:384-433): one row per point, value and derivative."""
+    name = os.path.join(directory, "table_%04d.txt" % int(table_id))
+    if not os.path.exists(name):
+        raise InvalidPotentialError("table file %s not found" % name)
+    rows = np.loadtxt(name, dtype=np.float64, ndmin=2)
+    if rows.shape[1] < 2 or len(rows) < 2:
+        raise InvalidPotentialError("%s needs at least 2 rows of value and derivative" % name)
+    return np.ascontiguousarray(rows[:, :2])
 
 
 def build_descriptors(potentials, species_of):
@@ -281,9 +408,13 @@ def build_descriptors(potentials, species_of):
     * bond-order factors are added per target tuple without permutation
       (permutate = False, 1372).
 
+    * a ProductPotential adds its first potential per permutation, each
+      followed by the other factors as multiplier rows, in the order the
+      reference stores them (reversed, calculator.py:1209-1213, 1262-1277).
+
     species_of maps element symbols to engine species ids; potentials naming
     elements absent from the system are dropped (they can act on no atom).
-    Returns (potdesc (k,10), bonddesc (m,14), n_groups).
+    Returns (potdesc (k,12), bonddesc (m,14), n_groups).
     """
     pot_rows, bond_rows = [], []
     coord_list = []  # (coordinator, group index)
@@ -303,15 +434,23 @@ def build_descriptors(potentials, species_of):
                     group = g
                     break
             coord.set_group_index(group)
-        code = POTENTIAL_CODES[pot.get_potential_type()]
-        p = pot.get_parameter_values()
+        factors = pot.get_potentials()
+        master, mults = factors[0], list(reversed(factors[1:]))
+        nt = pot.get_number_of_targets()
+
+        def row(f, e, link):
+            p = list(f.get_parameter_values()) + [0.0] * (4 - len(f.get_parameter_values()))
+            return [POTENTIAL_CODES[f.get_potential_type()], nt, e[0], e[1], p[0], p[1],
+                    pot.get_cutoff(), pot.get_soft_cutoff(), group, p[2], p[3], link]
+
         for targets in pot.get_symbols() or []:
             for perm in sorted(set(permutations(targets))):
                 if any(s not in species_of for s in perm):
                     continue
                 e = [species_of[s] for s in perm] + [0] * (2 - len(perm))
-                pot_rows.append([code, pot.get_number_of_targets(), e[0], e[1], p[0], p[1],
-                                 pot.get_cutoff(), pot.get_soft_cutoff(), group, 0.0])
+                pot_rows.append(row(master, e, 0))
+                for m in mults:
+                    pot_rows.append(row(m, e, 1))
     for coord, g in coord_list:
         for bond in coord.get_bond_order_parameters():
             if bond.get_bond_order_type() != "tersoff":
@@ -322,7 +461,7 @@ def build_descriptors(potentials, species_of):
                 e = [species_of[s] for s in targets]
                 bond_rows.append(e + bond.get_parameters_as_list() +
                                  [bond.get_cutoff(), bond.get_soft_cutoff(), g, 0.0])
-    pd = np.array(pot_rows, dtype=np.float64).reshape(-1, 10)
+    pd = np.array(pot_rows, dtype=np.float64).reshape(-1, 12)
     bd = np.array(bond_rows, dtype=np.float64).reshape(-1, 14)
     return pd, bd, len(coord_list)
 
@@ -350,3 +489,16 @@ def tersoff_si(params=None):
                     cutoff=P["cutoff"], cutoff_margin=P["margin"],
                     coordinator=Coordinator([bond]))
     return [rep, att]
+
+
+# Argon, Lennard-Jones: epsilon/k_B = 119.8 K, sigma = 3.405 A.  pysic's LJ is
+# epsilon ((sigma/r)^12 - (sigma/r)^6) (Potentials.f90:2993-3005), so its
+# epsilon is 4 times the well depth.
+ARGON_LJ = dict(epsilon=4 * 119.8 * 8.617333262e-5, sigma=3.405)
+
+
+def lj_argon(cutoff=2.5 * 3.405, margin=0.0, symbol="Ar"):
+    """Argon as one pysic 'LJ' potential (cutoff 2.5 sigma by default)."""
+    return [Potential("LJ", symbols=[[symbol, symbol]],
+                      parameters=[ARGON_LJ["epsilon"], ARGON_LJ["sigma"]],
+                      cutoff=cutoff, cutoff_margin=margin)]

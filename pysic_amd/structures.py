@@ -1,4 +1,5 @@
-"""Seeded synthetic structures for the Tersoff, Ewald and EEM workloads.
+"""Seeded synthetic structures for the Tersoff, pair-potential, Ewald and EEM
+workloads.
 
 The reference obtains geometries from ASE (absent here), so the engine owns
 its own generators.  All randomness comes from numpy's PCG64 generator with
@@ -12,8 +13,11 @@ AMU_SI = 28.0855
 AMU_O = 15.999
 AMU_NA = 22.98976928
 AMU_CL = 35.453
+AMU_AR = 39.948
+AMU_KR = 83.798
 AVOGADRO = 6.02214076e23
 
+FCC_BASIS = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
 DIAMOND_BASIS = np.array([
     [0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],
     [0.25, 0.25, 0.25], [0.25, 0.75, 0.75], [0.75, 0.25, 0.75], [0.75, 0.75, 0.25],
@@ -39,7 +43,8 @@ class Structure:
         self.pbc = np.array([bool(p) for p in pbc])
         n = len(self.positions)
         if masses is None:
-            table = {"Si": AMU_SI, "O": AMU_O, "Na": AMU_NA, "Cl": AMU_CL}
+            table = {"Si": AMU_SI, "O": AMU_O, "Na": AMU_NA, "Cl": AMU_CL, "Ar": AMU_AR,
+                     "Kr": AMU_KR}
             masses = [table.get(s, 1.0) for s in self.symbols]
         self.masses = np.array(masses, dtype=np.float64)
         self.charges = np.zeros(n) if charges is None else np.array(charges, dtype=np.float64)
@@ -115,6 +120,26 @@ def diamond_si(reps=(8, 8, 8), a=5.431, sigma=0.0, seed=0):
         pos = pos + rng.normal(0.0, sigma, size=pos.shape)
     cell = np.diag([a * reps[0], a * reps[1], a * reps[2]])
     return Structure(["Si"] * len(pos), pos, cell)
+
+
+def fcc(reps=(4, 4, 4), a=5.26, sigma=0.0, seed=0, symbol="Ar"):
+    """Face-centred cubic crystal (argon: a = 5.26 A near its triple point),
+    a x reps conventional cells, Gaussian displacements."""
+    pos = _lattice(FCC_BASIS, a, reps)
+    if sigma > 0:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.normal(0.0, sigma, size=pos.shape)
+    cell = np.diag([a * reps[0], a * reps[1], a * reps[2]])
+    return Structure([symbol] * len(pos), pos, cell)
+
+
+def random_mixture(n, box, dmin, symbols, seed):
+    """Random-addition cubic cell of n atoms; atom k takes
+    symbols[k % len(symbols)]."""
+    rng = np.random.default_rng(seed)
+    pos = random_sequential_addition(n, box, dmin, rng)
+    sym = [symbols[k % len(symbols)] for k in range(n)]
+    return Structure(sym, pos, np.diag([box] * 3))
 
 
 def rocksalt(reps=(4, 4, 4), a=5.64, sigma=0.0, seed=0, cation="Na", anion="Cl"):

@@ -3,7 +3,8 @@
 Runs only in a container that has the reference sources: build the checker
 first with oracle/build_ref.sh, then
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py pairs    # the pair-potential fixtures
 
 Every fixture is data only (inputs and the reference's outputs).  The
 system set-up replays pysic/calculator.py:1595-1637 through the reference's
@@ -20,8 +21,11 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
 import oracle as O  # noqa: E402
 from pysic_amd import structures as st  # noqa: E402
+from pysic_amd.interactions import build_descriptors  # noqa: E402
 
 T = dict(A=1830.8, l1=2.4799, B=471.18, l2=1.7322)
 TERS = [1.1e-6, 0.78734, 0.0, 100390.0, 16.217, -0.59825, 3.0]
@@ -85,9 +89,48 @@ def run_ewald(name, struct, rc, accuracy_scale=3.0, charge_self=None, scaler=Non
     print(name, len(struct), "E = %.12f" % e, "K", K)
 
 
+def run_pairs(name, struct, model, tables=None, skin=0.5):
+    """A pair-potential case: descriptor rows (engine format) in, the
+    reference's energy, forces and stress out."""
+    from helpers import ref_entries
+    names = sorted(set(struct.symbols))
+    pd, bd, _ = build_descriptors(model, {s: k for k, s in enumerate(names)})
+    sp = np.array([names.index(x) for x in struct.symbols], dtype=np.int32)
+    rc = max(p.get_cutoff() for p in model)
+    cuts = np.full(len(struct), rc + skin)
+    R = O.RefCore()
+    R.setup(struct.symbols, struct.positions, struct.cell, struct.pbc,
+            potentials=ref_entries(pd, names), cutoffs=cuts, tables=tables)
+    e = R.energy()
+    f, s = R.forces()
+    tid, tab = (next(iter(tables.items())) if tables else (-1, np.zeros((0, 2))))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), positions=struct.positions,
+                        cell=struct.cell, pbc=struct.pbc.astype(np.int32), species=sp,
+                        symbols=np.array(struct.symbols), potdesc=pd, cutoffs=cuts,
+                        table_id=np.array(tid), table=np.asarray(tab), energy=e, forces=f,
+                        stress=s)
+    print(name, len(struct), "E = %.12f" % e)
+
+
+def pair_cases():
+    from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table
+    from pysic_amd.interactions import lj_argon
+    # every pair type, a product, one-body terms: Ar/Kr random mixture
+    mix = st.random_mixture(120, 18.0, 2.9, ["Kr", "Ar", "Ar"], seed=0)
+    run_pairs("pairs_arkr_mixture_120", mix, mixture_model(),
+              tables={MIXTURE_TABLE_ID: mixture_table()})
+    # argon fcc, LJ at 2.5 sigma (sharp and smoothed cutoff)
+    ar = st.fcc((3, 3, 3), a=5.26, sigma=0.08, seed=2)
+    run_pairs("lj_argon_fcc_108", ar, lj_argon())
+    run_pairs("lj_argon_fcc_108_smooth", ar, lj_argon(margin=1.0))
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference checker first: oracle/build_ref.sh")
+    if sys.argv[1:] == ["pairs"]:
+        pair_cases()
+        return
     # Tersoff Si, the parity configuration at reduced and full size
     run_tersoff("si_diamond_2x2x2_s005", st.diamond_si((2, 2, 2), sigma=0.05, seed=1))
     run_tersoff("si_diamond_4x4x4_s005", st.diamond_si((4, 4, 4), sigma=0.05, seed=0))
@@ -116,6 +159,7 @@ def main():
     sio2.set_initial_charges(q)
     run_ewald("sio2_48_eem_rc5", sio2, rc=5.0,
               charge_self={"Si": (4.77, 8.0), "O": (8.74, 13.4)})
+    pair_cases()
 
 
 if __name__ == "__main__":

@@ -1,7 +1,8 @@
 """Shared set-up for the parity tests: descriptors for the golden cases."""
 import numpy as np
 
-from pysic_amd.interactions import TERSOFF_SI, build_descriptors, tersoff_si
+from pysic_amd.interactions import (POTENTIAL_CODES, POTENTIAL_TYPES, TERSOFF_SI, Potential,
+                                    ProductPotential, build_descriptors, tersoff_si)
 
 
 def tersoff_descriptors(ters=None):
@@ -35,3 +36,56 @@ def ewald_descriptors(g):
 def rel_maxnorm(a, b, floor=1e-300):
     """max|a-b| / max(max|b|, floor)"""
     return float(np.abs(a - b).max() / max(np.abs(b).max(), floor))
+
+
+# A two-species model with every pair type of the pair kernel, a product and
+# the one-body 'constant' and 'force' terms.  Table 7 is an exponential
+# tabulated on 60 points over 7.5 A (column 2: derivative times the range).
+MIXTURE_TABLE_ID = 7
+
+
+def mixture_table():
+    x = np.linspace(0.0, 7.5, 60)
+    return np.column_stack([0.3 * np.exp(-x / 2.0), -0.3 / 2.0 * np.exp(-x / 2.0) * 7.5])
+
+
+def mixture_model():
+    P = Potential
+    return [
+        P("LJ", symbols=[["Ar", "Ar"]], parameters=[0.04, 3.4], cutoff=8.0, cutoff_margin=1.0),
+        P("Buckingham", symbols=[["Ar", "Kr"]], parameters=[500.0, 5e3, 0.35], cutoff=7.0),
+        P("spring", symbols=[["Kr", "Kr"]], parameters=[0.5, 4.0], cutoff=6.0,
+          cutoff_margin=0.5),
+        P("power", symbols=[["Ar", "Kr"]], parameters=[0.01, 3.5, 7.5], cutoff=7.0,
+          cutoff_margin=1.0),
+        P("shift_power", symbols=[["Ar", "Ar"]], parameters=[0.02, 7.0, 3.0, 2.0], cutoff=6.5),
+        P("step", symbols=[["Kr", "Kr"]], parameters=[0.1, 0.0, 3.5, 5.0], cutoff=6.0),
+        P("tabulated", symbols=[["Ar", "Kr"]], parameters=[MIXTURE_TABLE_ID, 7.5, 1.3],
+          cutoff=7.4, cutoff_margin=0.5),
+        ProductPotential([
+            P("LJ", symbols=[["Kr", "Kr"]], parameters=[0.05, 3.6], cutoff=7.0,
+              cutoff_margin=1.0),
+            P("power", symbols=[["Kr", "Kr"]], parameters=[1.0, 4.0, 1.0]),
+            P("exponential", symbols=[["Kr", "Kr"]], parameters=[2.0, 0.3]),
+        ]),
+        P("constant", symbols=[["Ar"]], parameters=[-0.3]),
+        P("force", symbols=[["Kr"]], parameters=[0.01, -0.02, 0.03]),
+    ]
+
+
+def ref_entries(pd, names):
+    """RefCore potential entries (kind, params, cutoff, soft, symbols, group,
+    multipliers) from engine descriptor rows (12 doubles each)."""
+    inv = {v: k for k, v in POTENTIAL_CODES.items()}
+    out = []
+    for r in np.asarray(pd, dtype=np.float64).reshape(-1, 12):
+        kind = int(r[0])
+        npar = len(POTENTIAL_TYPES[inv[kind]][1])
+        params = [r[4], r[5], r[9], r[10]][:npar]
+        if r[11] != 0:
+            out[-1][6].append((kind, params))
+            continue
+        nt = int(r[1])
+        syms = tuple(names[int(e)] for e in r[2:2 + nt])
+        out.append((kind, params, float(r[6]), float(r[7]), syms, int(r[8]), []))
+    return out
