@@ -16,7 +16,9 @@ Prints one JSON line.  The cpu_baseline leg times the reference Fortran core
 (oracle/_ref, built from the reference sources) on a bounded sample on the
 host, single core.  The "qeq" object holds the second half of the metric:
 QEq time-to-tolerance on 24,576-atom amorphous SiO2, learned warm start
-against cold start (see qeq_benchmark).
+against cold start (see qeq_benchmark).  Secondary objects: "batched"
+(many a-Si cells), "ewald" (rocksalt Coulomb), "lj" (argon Lennard-Jones
+molecular dynamics through the pair kernel).
 """
 import argparse
 import json
@@ -318,6 +320,97 @@ def ewald_benchmark(reps=3, ref_reps=(4, 4, 4)):
     return out
 
 
+LJ_CELLS, LJ_A, LJ_T, LJ_SKIN, LJ_REBUILD = 64, 5.26, 80.0, 0.5, 10
+
+
+def lj_benchmark(steps=100, warmup=10, ref_cells=6, reference=True):
+    """Lennard-Jones argon molecular dynamics (the pair kernel, pair.hip):
+    fcc argon, 64^3 conventional cells = 1,048,576 atoms, a = 5.26 A,
+    sigma = 0.05 A seeded displacements, Maxwell-Boltzmann momenta at 80 K,
+    LJ epsilon/k_B = 119.8 K, sigma = 3.405 A cut at 2.5 sigma (8.51 A),
+    lists with a 0.5 A skin rebuilt every 10 steps, velocity Verlet with
+    dt = 2 fs, FP64.  A step: drift, [list rebuild], forces with the second
+    half-kick.  The reference core is timed on a 6^3-cell sample (864
+    atoms): list build + forces per step, its own loop order."""
+    from pysic_amd import structures as st
+    from pysic_amd.engine import Engine
+    from pysic_amd.interactions import build_descriptors, lj_argon
+    s = st.fcc((LJ_CELLS,) * 3, a=LJ_A, sigma=0.05, seed=0)
+    st.maxwell_boltzmann(s, LJ_T, seed=1)
+    n = len(s)
+    rc = 2.5 * 3.405
+    pd, _, _ = build_descriptors(lj_argon(cutoff=rc), {"Ar": 0})
+    eng = Engine(0)
+    eng.create_atoms(np.zeros(n, np.int32), s.positions, masses=s.masses, momenta=s.momenta)
+    eng.create_cell(s.cell, s.pbc)
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(np.zeros((0, 14)))
+    eng.ewald_off()
+    eng.set_neighbor_capacity(112)
+    eng.generate_neighbor_lists(np.full(n, rc + LJ_SKIN))
+    counts, _, _ = eng.neighbor_lists()
+    n_list = float(counts.mean())
+    dt = 2.0 * st.ASE_FS
+    eng.md_run(warmup, dt, rebuild=LJ_REBUILD)
+    m = s.masses[:, None]
+    e0 = eng.energy() + float((eng.state()[1] ** 2 / (2 * m)).sum())
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.md_run(steps, dt, rebuild=LJ_REBUILD)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    e1 = eng.energy() + float((eng.state()[1] ** 2 / (2 * m)).sum())
+    eng.timing(True)
+    eng.md_run(2 * LJ_REBUILD, dt, rebuild=LJ_REBUILD)
+    eng.synchronize()
+    tim = eng.timing_read()
+    eng.timing(False)
+    eng.close()
+    pf_ms, pf_calls = tim.get("pair_forces", (0.0, 1))
+    pf_s = pf_ms / max(pf_calls, 1) / 1e3
+    # algorithmic bytes of one forces launch per atom: position 32, list
+    # entries 8 each (j + image), momentum read + write 64, force write 32
+    bytes_launch = (128.0 + 8.0 * n_list) * n
+    out = {"workload": "LJ argon fcc 64^3 cells, 1,048,576 atoms, rc 2.5 sigma = 8.51 A, "
+                       "skin 0.5 A, lists rebuilt every 10 steps, VV dt = 2 fs, 80 K, FP64",
+           "atoms": n, "list_entries_per_atom": round(n_list, 2), "steps": steps,
+           "value": round(n * steps / wall, 1), "unit": "atom-steps/s",
+           "ms_per_step": round(wall / steps * 1e3, 4),
+           "total_energy_drift_eV_per_atom": (e1 - e0) / n,
+           "kernels_us": {k: round(v[0] / max(v[1], 1) * 1e3, 1) for k, v in tim.items()},
+           "pair_forces_roofline": {
+               "bound": "hbm", "achieved": round(bytes_launch / pf_s / 1e9, 1) if pf_s else 0.0,
+               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(bytes_launch / pf_s / 1e9 / HBM_PEAK_GBS, 4) if pf_s else 0.0,
+               "algorithmic_bytes_per_launch": int(bytes_launch)}}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if reference and O.ref_available():
+            r = st.fcc((ref_cells,) * 3, a=LJ_A, sigma=0.05, seed=0)
+            cut = np.full(len(r), rc + LJ_SKIN)
+            R = O.RefCore()
+            R.setup(r.symbols, r.positions, r.cell, r.pbc,
+                    potentials=[(3, list(pd[0, [4, 5]]), rc, rc, ("Ar", "Ar"), -1)],
+                    cutoffs=cut)
+            R.forces()
+            k = 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < 5.0 and k < 20:
+                R.rebuild(r.positions, cut)
+                R.forces()
+                k += 1
+            t_ref = (time.perf_counter() - t0) / k
+            out["reference_sample"] = {
+                "atoms": len(r), "seconds_per_step": round(t_ref, 4), "cores": 1,
+                "kind": "reference", "atom_steps_per_s": round(len(r) / t_ref, 1),
+                "note": "list rebuild + forces every step (the reference has no "
+                        "rebuild interval); serial amdflang -O2 build"}
+    except Exception as exc:  # the checker is optional on the GPU host
+        out["reference_sample"] = {"error": str(exc)}
+    return out
+
+
 def cpu_baseline(seconds_budget=20.0):
     """Reference Fortran core on the host: VV steps (rebuild + forces) on a
     16x16x16-cell sample (32,768 atoms) of the same workload."""
@@ -372,7 +465,8 @@ def main():
     ap.add_argument("--no-qeq", action="store_true", help="skip the QEq warm/cold measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cells measurement")
     ap.add_argument("--no-ewald", action="store_true", help="skip the Ewald rocksalt measurement")
-    ap.add_argument("--only", choices=["qeq", "batch", "ewald"],
+    ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon measurement")
+    ap.add_argument("--only", choices=["qeq", "batch", "ewald", "lj"],
                     help="run one secondary measurement alone and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (PMC traffic calibration)")
@@ -387,6 +481,8 @@ def main():
             res = batched_benchmark(*build_batched_cells())
         elif args.only == "qeq":
             res = qeq_benchmark(os.path.join(ROOT, "pysic_amd", "data", "sio2_eem_mlp.npz"))
+        elif args.only == "lj":
+            res = lj_benchmark()
         else:
             res = ewald_benchmark()
         print(json.dumps({args.only: res}))
@@ -464,6 +560,7 @@ def main():
     # stays on (Core.f90:87, 8106): time the Tersoff baseline before the
     # Ewald sample
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    lj = None if args.no_lj else lj_benchmark()
     ewald = None if args.no_ewald else ewald_benchmark()
     out = {"metric": "atom-steps/s Tersoff 1M-atom Si; QEq time-to-1e-6 learned vs cold start",
            "value": round(value, 1),
@@ -477,7 +574,7 @@ def main():
                       "kernel_variant": variant_name,
                       "total_energy_drift_eV_per_atom": drift},
            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "qeq": qeq,
-           "batched": batched, "ewald": ewald}
+           "batched": batched, "ewald": ewald, "lj": lj}
     print(json.dumps(out))
 
 

@@ -307,6 +307,26 @@ void compile_model(pysic_ctx *h)
     h->S.dmodel.ensure(1);
     HIP_TRY(hipMemcpyAsync(h->S.dmodel.p, &h->S.model, sizeof(Model), hipMemcpyHostToDevice,
                            h->S.stream));
+    // the LJ table of a model whose pair potentials are lone LJs
+    P.lj_only = 1;
+    for (int a = 0; a < PYSIC_MAXS; ++a)
+        for (int b = 0; b < PYSIC_MAXS; ++b) {
+            const int np = P.npair[a][b];
+            if (np > 1 || (np == 1 && (P.pair[a][b][0].nf != 1 || P.pair[a][b][0].f[0].type != 3)))
+                P.lj_only = 0;
+            LJPair &q = P.lj[a][b];
+            q.on = np > 0;
+            if (!q.on) continue;
+            const GenPair &T = P.pair[a][b][0];
+            q.cut = T.cut;
+            q.soft = T.soft;
+            q.smooth = T.smooth;
+            q.lo2 = T.cut * T.cut * (1.0 - 1e-12);
+            q.hi2 = P.cut2_hi[a][b];
+            q.soft_lo2 = T.soft * T.soft * (1.0 - 1e-12);
+            q.eps = T.f[0].p[0];
+            q.sig2 = T.f[0].p[1] * T.f[0].p[1];
+        }
     h->S.pmodel = P;
     h->S.dpmodel.ensure(1);
     HIP_TRY(hipMemcpyAsync(h->S.dpmodel.p, &h->S.pmodel, sizeof(PairModel),

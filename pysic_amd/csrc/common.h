@@ -67,6 +67,15 @@ struct GenPair {
     int smooth;           // 1 when 0 < soft < cut
     double cut, soft;
 };
+// A lone LJ on an ordered species pair, with the bounds that let its force
+// (which needs r^2 only) skip the square root: r^2 < lo2 means r < cut,
+// r^2 >= hi2 means r >= cut, r^2 < soft_lo2 means r < soft (f_c = 1); only
+// between them is r itself compared.
+struct LJPair {
+    double lo2, hi2, soft_lo2;
+    double cut, soft, eps, sig2;
+    int on, smooth;
+};
 struct PairModel {
     int npair[PYSIC_MAXS][PYSIC_MAXS];
     GenPair pair[PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXG];
@@ -79,6 +88,10 @@ struct PairModel {
     double kforce[PYSIC_MAXS][3];
     int has_pairs, has_single;
     int needs_pick;       // a 'step' factor: its force depends on which atom the reference takes first
+    // every ordered species pair holds at most one potential, a lone LJ
+    // (lj_only), and its table
+    int lj_only;
+    LJPair lj[PYSIC_MAXS][PYSIC_MAXS];
 };
 
 // ---------------------------------------------------------------------------

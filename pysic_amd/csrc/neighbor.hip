@@ -352,8 +352,12 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                 for (int t = start[row + bx]; t < t1; ++t) {
                     const float4 qt = wpos[t];
                     const float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
-                    if (fx * fx + fy * fy + fz * fz < c2f) {
-                        if (nq == NB_QUEUE) drain();
+                    const bool hit = fx * fx + fy * fy + fz * fz < c2f;
+                    // a full queue drains the queues of every lane in this
+                    // iteration at once: on dense lists lanes fill at similar
+                    // rates, and lanes draining one by one serialise the wave
+                    if (__ballot(hit && nq == NB_QUEUE)) drain();
+                    if (hit) {
                         q_slot[nq][tid] = t;
                         q_shift[nq][tid] = shift;
                         ++nq;

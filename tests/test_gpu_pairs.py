@@ -151,6 +151,45 @@ def test_lj_md_graph_matches_eager():
         b.close()
 
 
+def test_lj_md_matches_oracle_velocity_verlet(oracle):
+    """ASE VelocityVerlet semantics with lists rebuilt every 10 steps (the
+    reference rebuilds when ASE's skin test says so): 30 steps of 2 fs."""
+    s = st.fcc((5, 5, 5), a=5.26, sigma=0.08, seed=8)
+    st.maxwell_boltzmann(s, 120.0, seed=9)
+    n = len(s)
+    print("case: LJ argon %d atoms, 30 VV steps vs the oracle" % n, flush=True)
+    rc = 2.5 * 3.405
+    cut = np.full(n, rc + 0.5)
+    pd, _, _ = build_descriptors(lj_argon(cutoff=rc), {"Ar": 0})
+    e = _lj_md_engine(s)
+    try:
+        dt = 2.0 * st.ASE_FS
+        e_gpu = e.md_run(30, dt, rebuild=10, energies=True)
+        pos_g, mom_g = e.state()
+    finally:
+        e.close()
+    sp = np.zeros(n, np.int32)
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd)
+    e_or = []
+    for step in range(30):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        if (step + 1) % 10 == 0:
+            nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd)
+        p = p + 0.5 * dt * f
+        e_or.append(oracle.evaluate("energy", sp, r, None, s.cell, nl, pd))
+    print("  max|dr| %.2e  max|dp| %.2e  max|dE| %.2e" % (
+        np.abs(pos_g - r).max(), np.abs(mom_g - p).max(),
+        np.abs(np.array(e_gpu) - np.array(e_or)).max()), flush=True)
+    assert np.abs(pos_g - r).max() < 1e-10
+    assert np.abs(mom_g - p).max() < 1e-9 * np.abs(p).max()
+    assert np.abs(np.array(e_gpu) - np.array(e_or)).max() <= 1e-10 * abs(e_or[-1])
+
+
 def test_lj_md_conserves_energy():
     """Velocity Verlet, 2 fs, smoothed LJ cutoff, lists rebuilt every 10
     steps (0.5 A skin): total energy drifts by less than 1e-5 eV/atom over
