@@ -16,6 +16,8 @@
 // the partition, only the cost does.
 #include "system.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 
@@ -247,24 +249,31 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
     wrap2[s] = w;
     // wrapped position in FP32 for the neighbor prefilter
     double wx = img_x(w), wy = img_y(w), wz = img_z(w);
+    // w carries the packed wrap offsets (bits), so the prefilter forms the
+    // list image without loading wrap[]
     wpos[s] = make_float4((float)(p.x + wx * g.v[0] + wy * g.v[3] + wz * g.v[6]),
                           (float)(p.y + wx * g.v[1] + wy * g.v[4] + wz * g.v[7]),
-                          (float)(p.z + wx * g.v[2] + wy * g.v[5] + wz * g.v[8]), 0.f);
+                          (float)(p.z + wx * g.v[2] + wy * g.v[5] + wz * g.v[8]),
+                          __int_as_float(w));
 }
 
-// Per-atom scan with an FP32 prefilter, in two passes.  Pass 1 tests every
-// candidate of the 27 bins on FP32 wrapped positions (16 B per candidate)
-// against cutoff^2 + margin2, where margin2 bounds the FP32 rounding error
-// of the squared distance, and queues the survivors (slot, bin shift) in
-// this lane's LDS column.  Pass 2 takes the exact FP64 test, in the
-// reference's operation order, on the queued survivors only, so the accepted
-// set is the exact one.  The split matters on a SIMD: tested inline, the
-// FP64 branch is entered whenever any lane of the wave has a survivor, which
-// is almost every candidate; queued, every lane works through its ~4-5
-// survivors together.  A full queue is drained in place.
+// Per-atom scan with an FP32 prefilter.  Every candidate of the 27 bins is
+// tested on FP32 wrapped positions (16 B per candidate, the wrap offsets in
+// the fourth word); margin2 bounds the FP32 error of the squared distance.
+// A candidate with d2 < cutoff^2 - 2 margin2 is a neighbor for sure, one
+// with d2 >= cutoff^2 + margin2 is not; the thin shell between takes the
+// exact FP64 test in the reference's operation order, so the accepted set is
+// the exact one.  Accepted and shell candidates are queued (slot, image,
+// shell flag) in this lane's LDS column and written out by drain(): on a
+// SIMD, writing (or testing) them inline would enter that branch whenever
+// any lane of the wave has one, which is almost every candidate; queued,
+// every lane works through its entries together.
 constexpr int NB_QUEUE = 12;
+#ifndef PYSIC_NB_UNIFORM_SPAN
+#define PYSIC_NB_UNIFORM_SPAN 4
+#endif
 
-template <class Geo>
+template <class Geo, bool DENSE>
 __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
                                                       const double4 *__restrict__ pos,
                                                       const float4 *__restrict__ wpos,
@@ -299,69 +308,127 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
     const float v0x = (float)g.v[0], v0y = (float)g.v[1], v0z = (float)g.v[2];
     const float v1x = (float)g.v[3], v1y = (float)g.v[4], v1z = (float)g.v[5];
     const float v2x = (float)g.v[6], v2y = (float)g.v[7], v2z = (float)g.v[8];
+    const float c2sure = (float)c2lim - 2.0f * margin2;
     int count = 0, nq = 0;
-    // exact test of the queued survivors, in queue order
+    constexpr int SHELL = 1 << 30;  // queue flag: exact test needed (images use bits 0-29)
+    // write out the queued entries, in queue order; shell entries first pass
+    // the exact test
     auto drain = [&]() {
         for (int u = 0; u < nq; ++u) {
             const int t = q_slot[u][tid];
-            const int sh = q_shift[u][tid];
-            const int wt = wrap[t];
-            const int ox = img_x(sh) + img_x(wt) - wsx;
-            const int oy = img_y(sh) + img_y(wt) - wsy;
-            const int oz = img_z(sh) + img_z(wt) - wsz;
-            if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
-            const int im = pack_img(ox, oy, oz);
-            const double d2 = dot64(separation64(ps, pos[t], im, g.v));
-            if (d2 < c2lim) {
-                if (count < cap) {
-                    nbr[(size_t)count * n + s] = t;
-                    nimg[(size_t)count * n + s] = im;
-                }
-                ++count;
+            const int q = q_shift[u][tid];
+            const int im = q & ~SHELL;
+            if ((q & SHELL) && !(dot64(separation64(ps, pos[t], im, g.v)) < c2lim)) continue;
+            if (count < cap) {
+                nbr[(size_t)count * n + s] = t;
+                nimg[(size_t)count * n + s] = im;
             }
+            ++count;
         }
         nq = 0;
     };
-    for (int dz = -1; dz <= 1; ++dz) {
-        int bz = c2 + dz, sz = 0;
-        if (g.pbc[2]) {
+    // FP32 test of every candidate of one bin (row + bx) under the bin's
+    // periodic shift (sx, sy, sz).  UNIFORM: every lane of the wave scans the
+    // same candidates, so a full queue drains every lane's queue at once.
+    auto scan_bin = [&](auto uniform_tag, int row, int bx, int sx, int sy, int sz) {
+        constexpr bool UNIFORM = decltype(uniform_tag)::value;
+        const float sxf = (float)sx, syf = (float)sy, szf = (float)sz;
+        const float shx = sxf * v0x + syf * v1x + szf * v2x - qs.x;
+        const float shy = sxf * v0y + syf * v1y + szf * v2y - qs.y;
+        const float shz = sxf * v0z + syf * v1z + szf * v2z - qs.z;
+        const int ox0 = sx - wsx, oy0 = sy - wsy, oz0 = sz - wsz;
+        const int t1 = start[row + bx + 1];
+        for (int t = start[row + bx]; t < t1; ++t) {
+            const float4 qt = wpos[t];
+            const float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
+            const float d2f = fx * fx + fy * fy + fz * fz;
+            const bool hit = d2f < c2f;
+            if (UNIFORM) {
+                if (__ballot(hit && nq == NB_QUEUE)) drain();
+            }
+            if (hit) {
+                if (!UNIFORM && nq == NB_QUEUE) drain();
+                // image of the pair: bin shift + wrap(t) - wrap(s)
+                const int wt = __float_as_int(qt.w);
+                const int ox = ox0 + img_x(wt), oy = oy0 + img_y(wt), oz = oz0 + img_z(wt);
+                if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
+                    q_slot[nq][tid] = t;
+                    q_shift[nq][tid] = pack_img(ox, oy, oz) | (d2f < c2sure ? 0 : SHELL);
+                    ++nq;
+                }
+            }
+        }
+    };
+    // Wave-uniform scan: when the wave's atoms share one bin row of a
+    // periodic cell at least 3 bins wide on every axis, and span at most
+    // PYSIC_NB_UNIFORM_SPAN + 1 bins of it (dense bins: the union of their
+    // stencils is then at most (SPAN + 3) / 3 times one stencil), every lane
+    // scans that union -- the 9 neighboring rows over x from one bin before
+    // the wave's first bin to one after its last.  Loop bounds
+    // and candidates are then the same for all lanes (scalar loads, no
+    // divergence); the extra candidates lie beyond every lane's cutoff or
+    // are genuine neighbors, and with 3 or more bins per axis no atom image
+    // is reached twice, so the accepted set is the 27-bin stencil's.
+    bool uniform = false;
+    int ux0 = 0, ux1 = 0, uc1 = 0, uc2 = 0;
+    if constexpr (DENSE) {  // one periodic cell, >= 3 bins per axis (checked by the host)
+        {
+            const unsigned long long act = __ballot(1);
+            const int lo = __ffsll((unsigned long long)act) - 1;
+            const int hi = 63 - __clzll((long long)act);
+            // atoms are sorted by bin: the first active lane holds the
+            // wave's lowest bin, the last its highest
+            const int kmin = __builtin_amdgcn_readfirstlane(__shfl(k, lo));
+            const int kmax = __builtin_amdgcn_readfirstlane(__shfl(k, hi));
+            if (kmin / g.dim[0] == kmax / g.dim[0] &&
+                kmax - kmin <= PYSIC_NB_UNIFORM_SPAN) {
+                uniform = true;
+                ux0 = kmin % g.dim[0];
+                ux1 = kmax % g.dim[0];
+                uc1 = (kmin / g.dim[0]) % g.dim[1];
+                uc2 = kmin / (g.dim[0] * g.dim[1]);
+            }
+        }
+    }
+    if (uniform) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            int bz = uc2 + dz, sz = 0;
             if (bz < 0) { bz += g.nc[2]; sz = -1; }
             else if (bz >= g.nc[2]) { bz -= g.nc[2]; sz = 1; }
-        } else if (bz < 0 || bz >= g.dim[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int by = c1 + dy, sy = 0;
-            if (g.pbc[1]) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                int by = uc1 + dy, sy = 0;
                 if (by < 0) { by += g.nc[1]; sy = -1; }
                 else if (by >= g.nc[1]) { by -= g.nc[1]; sy = 1; }
-            } else if (by < 0 || by >= g.dim[1]) continue;
-            const int row = g.dim[0] * (by + g.dim[1] * bz);
-            const float syf = (float)sy, szf = (float)sz;
-            for (int dx = -1; dx <= 1; ++dx) {
-                int bx = c0 + dx, sx = 0;
-                if (g.pbc[0]) {
+                const int row = g.dim[0] * (by + g.dim[1] * bz);
+                for (int x = ux0 - 1; x <= ux1 + 1; ++x) {
+                    int bx = x, sx = 0;
                     if (bx < 0) { bx += g.nc[0]; sx = -1; }
                     else if (bx >= g.nc[0]) { bx -= g.nc[0]; sx = 1; }
-                } else if (bx < 0 || bx >= g.dim[0]) continue;
-                const float sxf = (float)sx;
-                // image shift of this bin, FP32
-                const float shx = sxf * v0x + syf * v1x + szf * v2x - qs.x;
-                const float shy = sxf * v0y + syf * v1y + szf * v2y - qs.y;
-                const float shz = sxf * v0z + syf * v1z + szf * v2z - qs.z;
-                const int shift = pack_img(sx, sy, sz);
-                const int t1 = start[row + bx + 1];
-                for (int t = start[row + bx]; t < t1; ++t) {
-                    const float4 qt = wpos[t];
-                    const float fx = qt.x + shx, fy = qt.y + shy, fz = qt.z + shz;
-                    const bool hit = fx * fx + fy * fy + fz * fz < c2f;
-                    // a full queue drains the queues of every lane in this
-                    // iteration at once: on dense lists lanes fill at similar
-                    // rates, and lanes draining one by one serialise the wave
-                    if (__ballot(hit && nq == NB_QUEUE)) drain();
-                    if (hit) {
-                        q_slot[nq][tid] = t;
-                        q_shift[nq][tid] = shift;
-                        ++nq;
-                    }
+                    scan_bin(std::true_type{}, row, bx, sx, sy, sz);
+                }
+            }
+        }
+    } else {
+        for (int dz = -1; dz <= 1; ++dz) {
+            int bz = c2 + dz, sz = 0;
+            if (g.pbc[2]) {
+                if (bz < 0) { bz += g.nc[2]; sz = -1; }
+                else if (bz >= g.nc[2]) { bz -= g.nc[2]; sz = 1; }
+            } else if (bz < 0 || bz >= g.dim[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int by = c1 + dy, sy = 0;
+                if (g.pbc[1]) {
+                    if (by < 0) { by += g.nc[1]; sy = -1; }
+                    else if (by >= g.nc[1]) { by -= g.nc[1]; sy = 1; }
+                } else if (by < 0 || by >= g.dim[1]) continue;
+                const int row = g.dim[0] * (by + g.dim[1] * bz);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int bx = c0 + dx, sx = 0;
+                    if (g.pbc[0]) {
+                        if (bx < 0) { bx += g.nc[0]; sx = -1; }
+                        else if (bx >= g.nc[0]) { bx -= g.nc[0]; sx = 1; }
+                    } else if (bx < 0 || bx >= g.dim[0]) continue;
+                    scan_bin(std::false_type{}, row, bx, sx, sy, sz);
                 }
             }
         }
@@ -899,10 +966,23 @@ void nb_build_lists(System &S)
         };
         if (S.ncells == 0) margin2 = margin_of(g);
         for (int c = 0; c < S.ncells; ++c) margin2 = std::max(margin2, margin_of(S.hcells[c]));
+        // dense bins of one periodic cell: the kernel with the wave-uniform
+        // scan (a separate instance: its second scan path costs registers)
+        const bool dense = S.ncells == 0 && g.pbc[0] && g.pbc[1] && g.pbc[2] && g.nc[0] >= 3 &&
+                           g.nc[1] >= 3 && g.nc[2] >= 3 && apb >= 6.0;
         with_geo(S, [&](auto geo) {
-            k_neighbors_pf<decltype(geo)><<<div_up(n, 256), 256, 0, st>>>(
-                n, geo, S.pos.p, S.wpos.p, S.key.p, S.wrap.p, S.cellstart.p, S.cut.p, S.cap,
-                margin2, S.nbr.p, S.nimg.p, S.ncnt.p, S.flags.p);
+            using Geo = decltype(geo);
+            auto go = [&](auto kern) {
+                kern<<<div_up(n, 256), 256, 0, st>>>(n, geo, S.pos.p, S.wpos.p, S.key.p, S.wrap.p,
+                                                     S.cellstart.p, S.cut.p, S.cap, margin2,
+                                                     S.nbr.p, S.nimg.p, S.ncnt.p, S.flags.p);
+            };
+            if constexpr (std::is_same<Geo, GeoOne>::value) {
+                if (dense) go(k_neighbors_pf<Geo, true>);
+                else go(k_neighbors_pf<Geo, false>);
+            } else {
+                go(k_neighbors_pf<Geo, false>);
+            }
         });
     } else if (S.nb_method == 4) {
         // lane-parallel scan; wider groups when the bins are dense
