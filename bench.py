@@ -48,19 +48,25 @@ def tersoff_bytes_per_atom(n_nb):
     return 24.0 + 4.0 * n_nb + 24.0 + 8.0
 
 
-def measured_traffic(cells, variant_name):
-    """HBM bytes per tersoff_site launch, measured with rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE, scaled by what the same counters read for a
-    plain device copy of known size) and recorded in profiles/.  Returns None
-    unless the recorded workload and kernel variant are the ones run here."""
+# one wave64 VALU instruction per SIMD every 4 cycles: 256 CUs x 4 SIMDs x
+# 2.4 GHz / 4
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def measured_counters(cells, variant_name, field):
+    """A per-launch counter of the tersoff_site kernel recorded in profiles/
+    from rocprofv3 --pmc: "traffic_bytes_per_launch" (HBM bytes: FETCH_SIZE
+    and WRITE_SIZE, scaled by what the same counters read for a plain device
+    copy of known size) or "valu_wave_instructions_per_launch"
+    (SQ_INSTS_VALU).  None unless the recorded workload and kernel variant
+    are the ones run here."""
     path = os.path.join(ROOT, "profiles", "tersoff_site_si1m_pmc.txt")
     if not os.path.exists(path):
         return None
     rec = {}
     with open(path) as fh:
         for line in fh:
-            if ":" in line and line.split(":", 1)[0].strip() in ("cells", "kernel_variant",
-                                                                  "traffic_bytes_per_launch"):
+            if ":" in line and line.split(":", 1)[0].strip() in ("cells", "kernel_variant", field):
                 k, v = line.split(":", 1)
                 rec[k.strip()] = v.strip()
     try:
@@ -68,7 +74,7 @@ def measured_traffic(cells, variant_name):
             return None
         if rec["kernel_variant"] != variant_name:
             return None
-        return int(rec["traffic_bytes_per_launch"])
+        return int(rec[field])
     except (KeyError, ValueError):
         return None
 
@@ -545,9 +551,18 @@ def main():
     variant_name = eng.variants()[variant]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.cells, variant_name),
+                "traffic": measured_counters(args.cells, variant_name, "traffic_bytes_per_launch"),
                 "kernel": "tersoff_site", "avg_launch_us": round(site_avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": int(bytes_site)}
+    # the bound that holds this kernel: VALU issue (instruction count per
+    # launch from the recorded SQ_INSTS_VALU, over the live launch time)
+    valu = measured_counters(args.cells, variant_name, "valu_wave_instructions_per_launch")
+    if valu and site_avg_s > 0:
+        rate = valu / site_avg_s
+        roofline["valu_issue"] = {"achieved": round(rate / 1e9, 1), "peak": VALU_ISSUE_PEAK / 1e9,
+                                  "unit": "G wave-instructions/s",
+                                  "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                                  "wave_instructions_per_launch": valu}
     kernels = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "calls": v[1]}
                for k, v in timers.items()}
     if args.copy_calibration:

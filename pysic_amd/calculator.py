@@ -7,6 +7,7 @@ engine per calculator (the reference keeps one process-global Fortran core,
 Core.f90:19-97; here every calculator owns its own handle).
 """
 import math
+import warnings
 
 import numpy as np
 
@@ -25,6 +26,22 @@ class LockedCoreError(Exception):
 
 def _symbols(atoms):
     return list(atoms.get_chemical_symbols())
+
+
+def _warn_charge(q):
+    """calculator.py:1545-1547"""
+    total = float(np.sum(q))
+    if total > 0.001 or total < -0.001:
+        warnings.warn("There is non-zero total charge in the system: {tc}".format(tc=total))
+
+
+def _warn_potentials(potentials):
+    """calculator.py:1201-1207"""
+    for pot in potentials or []:
+        if pot.get_number_of_targets() > 1 and pot.get_cutoff() < 0.01:
+            warnings.warn("Potential with zero cutoff present: \n\n" + str(pot))
+        if pot.get_symbols() is None:
+            warnings.warn("Potential with no targets present: \n\n" + str(pot))
 
 
 def _charges(atoms):
@@ -126,6 +143,32 @@ class Pysic:
         self.set_charge_relaxation(charge_relaxation)
         self.set_coulomb_summation(coulomb)
 
+    def __eq__(self, other):
+        """Same structure (and charges), neighbor list, potentials and extra
+        calculators (calculator.py:260-287)."""
+        try:
+            return (isinstance(other, Pysic)
+                    and (self.structure is other.structure or
+                         (self.structure is not None and other.structure is not None and
+                          self._same_structure(other.structure)))
+                    and self.potentials == other.potentials
+                    and self.extra_calculators == other.extra_calculators)
+        except Exception:
+            return False
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+    def __repr__(self):
+        return "Pysic(atoms={atoms},potentials={pots},full_initialization={init})".format(
+            atoms=str(self.structure), pots=str(self.potentials),
+            init=str(self.force_core_initialization))
+
+    def core_initialization_is_forced(self):
+        """True if the engine is fully re-initialised before every
+        calculation (calculator.py:297-300)."""
+        return self.force_core_initialization
+
     # --- configuration (calculator.py:537-820) -------------------------------
     def _invalidate(self):
         self.forces = self.stress = self.energy = self.electronegativities = None
@@ -188,11 +231,43 @@ class Pysic:
     def add_calculator(self, calc):
         self.extra_calculators.append(calc)
 
+    def remove_calculator(self, calc):
+        """calculator.py:703-711"""
+        self.extra_calculators.remove(calc)
+
+    def remove_potential(self, potential):
+        """calculator.py:714-735: the potential list is replaced, so cutoffs
+        and the engine's model are recomputed."""
+        new_pots = list(self.get_potentials() or [])
+        new_pots.remove(potential)
+        self.set_potentials(new_pots)
+
     def get_calculators(self):
         return self.extra_calculators
 
     def get_neighbor_lists(self):
         return self.neighbor_list
+
+    def get_neighbor_list(self):
+        """calculator.py:854-857"""
+        return self.neighbor_list
+
+    def get_charges(self, system=None):
+        """The initial charges of system, or of the calculator's structure
+        (calculator.py:1515-1529)."""
+        return _charges(system if system is not None else self.structure)
+
+    def set_cutoffs(self, cutoffs):
+        """calculator.py:1681-1689"""
+        self.saved_cutoffs = None if cutoffs is None else list(cutoffs)
+
+    def neighbor_lists_expanded(self, cutoffs):
+        """True if any atom's cutoff grew since set_cutoffs (or nothing was
+        saved): the lists must then be rebuilt (calculator.py:1692-1718)."""
+        saved = getattr(self, "saved_cutoffs", None)
+        if saved is None or cutoffs is None or len(saved) != len(cutoffs):
+            return True
+        return any(old < new for old, new in zip(saved, cutoffs))
 
     def get_individual_cutoffs(self, scaler=1.0):
         """Per-atom maximum cutoff over the potentials, bond factors and the
@@ -245,6 +320,7 @@ class Pysic:
         eng.create_atoms(species, s.get_positions(), masses=s.get_masses(), charges=_charges(s),
                          momenta=s.get_momenta())
         eng.create_cell(s.get_cell(), s.get_pbc())
+        _warn_potentials(self.potentials)
         pd, bd, _ = build_descriptors(self.potentials or [], spmap)
         for tid in table_ids(self.potentials):
             eng.set_potential_table(tid, read_table(tid))
@@ -283,8 +359,82 @@ class Pysic:
             self.neighbor_list.update(s)
         if not np.array_equal(c["charges"], _charges(s)):
             q = _charges(s)
+            _warn_charge(q)
             self.engine.update_charges(q)
             c["charges"] = q.copy()
+
+    # The reference pushes each part of the calculator's state into its core
+    # with one call per part (calculator.py:998-1482); set_core() does what
+    # is needed before every calculation, and these entry points do one part
+    # each on demand.
+    def initialize_fortran_core(self):
+        """Everything, from scratch (calculator.py:1595-1637)."""
+        self._full_init()
+
+    def _core_ready(self):
+        if self._core is None:
+            self._full_init()
+            return False
+        return True
+
+    def update_core_coordinates(self):
+        """Positions and momenta (calculator.py:1485-1513)."""
+        if self._core_ready():
+            s = self.structure
+            self.engine.update_coordinates(s.get_positions(), s.get_momenta())
+            self._core["positions"] = s.get_positions().copy()
+            self._core["momenta"] = s.get_momenta().copy()
+            self.neighbor_list.update(s)
+
+    def update_core_charges(self):
+        """Charges (calculator.py:1532-1553), with its warning for a charged
+        system."""
+        if self._core_ready():
+            q = _charges(self.structure)
+            _warn_charge(q)
+            self.engine.update_charges(q)
+            self._core["charges"] = q.copy()
+            self._invalidate()
+
+    def update_core_supercell(self):
+        """The cell (calculator.py:1555-1565): the cell lists depend on it,
+        so the engine is set up again."""
+        self._full_init()
+
+    def update_core_potentials(self):
+        """Potentials, tables and bond-order factors (calculator.py:1063-1446)."""
+        if self._core_ready():
+            spmap = self._species_map()
+            _warn_potentials(self.potentials)
+            pd, bd, _ = build_descriptors(self.potentials or [], spmap)
+            for tid in table_ids(self.potentials):
+                self.engine.set_potential_table(tid, read_table(tid))
+            self.engine.set_potentials(pd)
+            self.engine.set_bond_order_factors(bd)
+            self._invalidate()
+
+    def update_core_potential_lists(self):
+        """The per-atom potential lists (calculator.py:1048-1061): the engine
+        compiles its species tables with the potentials, so nothing is left
+        to do once update_core_potentials has run."""
+        self._core_ready()
+
+    def update_core_coulomb(self):
+        """Ewald parameters and scaling factors (calculator.py:1450-1483)."""
+        if self._core_ready():
+            if self.coulomb is not None:
+                p = self.coulomb.parameters
+                sc = self.coulomb.get_scaling_factors()
+                self.engine.set_ewald(p["real_cutoff"], p["k_cutoff"], self._kcells(),
+                                      p["sigma"], p["epsilon"], sc)
+            else:
+                self.engine.ewald_off()
+            self._invalidate()
+
+    def update_core_neighbor_lists(self):
+        """Rebuild the neighbor lists (calculator.py:1567-1593)."""
+        if self._core_ready():
+            self.neighbor_list.build(self.structure)
 
     # --- calculations (calculator.py:315-534, 918-990) ---------------------
     def calculation_required(self, atoms=None, quantities=("forces", "energy", "stress",

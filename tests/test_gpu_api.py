@@ -144,3 +144,78 @@ def test_device_md_through_the_calculator(oracle):
     k1 = (t.get_momenta() ** 2 / (2 * t.get_masses()[:, None])).sum()
     # NVE at 0.5 fs conserves the total energy well
     assert abs((e1 + k1) - (e0 + k0)) / len(s) < 2e-4
+
+
+def test_calculator_housekeeping_and_core_updates(oracle):
+    """The reference's calculator entry points beyond get_* (calculator.py:
+    260-305, 703-735, 1048-1637): equality, removal, cutoff bookkeeping,
+    warnings, and the update_core_* calls that push one part of the state."""
+    import warnings as W
+    from pysic_amd import Pysic
+    from pysic_amd.interactions import build_descriptors, lj_argon
+    s = st.fcc((4, 4, 4), a=5.26, sigma=0.05, seed=2)
+    lj = lj_argon()
+    extra = Potential("LJ", symbols=[["Ar", "Ar"]], parameters=[0.001, 3.0], cutoff=6.0)
+    calc = Pysic(s, lj + [extra])
+    try:
+        assert calc == calc and not (calc != calc)
+        assert repr(calc).startswith("Pysic(atoms=")
+        assert calc.core_initialization_is_forced() is False
+        assert calc.get_neighbor_list() is None
+        e_two = calc.get_potential_energy()
+        calc.remove_potential(extra)
+        e_one = calc.get_potential_energy()
+        pd, _, _ = build_descriptors(lj, {"Ar": 0})
+        cut = np.full(len(s), 2.5 * 3.405)
+        nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], cut)
+        e_ref = oracle.evaluate("energy", np.zeros(len(s), np.int32), s.positions, None, s.cell,
+                                nl, pd)
+        assert abs(e_one - e_ref) <= 1e-10 * abs(e_ref) and e_two != e_one
+        # cutoff bookkeeping (calculator.py:1681-1718)
+        calc.set_cutoffs([5.0] * len(s))
+        assert not calc.neighbor_lists_expanded([5.0] * len(s))
+        assert calc.neighbor_lists_expanded([5.0] * (len(s) - 1) + [5.1])
+        # push new coordinates and charges explicitly
+        t = calc.get_atoms()
+        p = t.get_positions()
+        p[0, 0] += 0.02
+        t.set_positions(p)
+        calc.update_core_coordinates()
+        nl = oracle.neighbor_list(p, s.cell, [1, 1, 1], cut)
+        f_ref, _ = oracle.evaluate("forces", np.zeros(len(s), np.int32), p, None, s.cell, nl, pd)
+        calc.forces = None
+        f = calc.get_forces()
+        assert np.abs(f - f_ref).max() <= 1e-10 * max(1.0, np.abs(f_ref).max())
+        q = np.zeros(len(s))
+        q[0] = 0.5
+        t.set_initial_charges(q)
+        with W.catch_warnings(record=True) as rec:
+            W.simplefilter("always")
+            calc.update_core_charges()
+        assert any("non-zero total charge" in str(r.message) for r in rec)
+        assert np.array_equal(calc.get_charges(), q)
+        # a pair potential without a cutoff warns (calculator.py:1201-1207)
+        with W.catch_warnings(record=True) as rec:
+            W.simplefilter("always")
+            calc.set_potentials(lj + [Potential("LJ", symbols=[["Ar", "Ar"]],
+                                                parameters=[0.0, 1.0])])
+            calc.initialize_fortran_core()
+        assert any("zero cutoff" in str(r.message) for r in rec)
+        calc.update_core_potentials()
+        calc.update_core_potential_lists()
+        calc.update_core_neighbor_lists()
+        calc.update_core_supercell()
+        calc.update_core_coulomb()
+        assert calc.get_potential_energy() == pytest.approx(
+            oracle.evaluate("energy", np.zeros(len(s), np.int32), p, None, s.cell, nl, pd),
+            rel=1e-10)
+        other = Pysic(s, lj)
+        try:
+            calc.add_calculator(other)
+            assert calc.get_calculators() == [other]
+            calc.remove_calculator(other)
+            assert calc.get_calculators() == []
+        finally:
+            other.engine.close()
+    finally:
+        calc.engine.close()
