@@ -136,6 +136,18 @@ __device__ __forceinline__ bool pick_dev(int i1, int i2, int img)
     return ox > 0 || (ox == 0 && (oy > 0 || (oy == 0 && oz > 0)));
 }
 
+// 1/x for a normal x: the hardware reciprocal estimate and two Newton
+// steps (e = 1 - x y, y += y e), within an ulp or two of the correctly
+// rounded quotient (x = 0 gives NaN; callers handle r = 0 first).
+__device__ __forceinline__ double recip64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
 // smoothening_factor and smoothening_derivative (Potentials.f90:442-509)
 __device__ __forceinline__ double fc64(double r, double hard, double soft)
 {

@@ -166,17 +166,6 @@ template <int KIND> struct PairLds { LJPair lj[PYSIC_MAXS][PYSIC_MAXS]; };
 template <> struct PairLds<0> { PairModel pm; };
 template <> struct PairLds<2> { int unused; };
 
-// 1/x for a normal x: the hardware reciprocal estimate and two Newton
-// steps (e = 1 - x y, y += y e), within an ulp or two of the correctly
-// rounded quotient (x = 0 gives NaN; the callers handle r = 0 first).
-__device__ __forceinline__ double recip64(double x)
-{
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
-}
 
 // The model into LDS (a fixed copy per block; the caller synchronises).
 __device__ __forceinline__ void stage(PairLds<0> &L, const PairModel *__restrict__ PM)

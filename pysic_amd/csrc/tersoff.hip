@@ -505,7 +505,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                     else dR += dv;
                 }
             }
-            const double ir = 1.0 / r;
+            const double ir = recip64(r);  // r > 0: the list holds no atom at its own position
             s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
             s_aux[slot][a] = make_float4((float)ir, inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
                                          inb ? (float)dfc64(r, B.cut, B.soft) : 0.f, (float)A);
@@ -542,7 +542,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                 float t2 = t * t;
                 float xi = ak.y;
                 if (!B.a_zero) xi *= __expf(powf((float)B.a * (r1 - (float)vd.w), (float)B.mu));
-                zeta += xi * (1.0f + cd2f * t2 * __frcp_rn(ddf + t2));
+                zeta += xi * (1.0f + cd2f * t2 * __builtin_amdgcn_rcpf(ddf + t2));
             }
         }
         float bt = 1.f, dbt = 0.f;
@@ -582,7 +582,7 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                 float ir12 = ir1 * ak.x;
                 float cs = (ux * vx + uy * vy + uz * vz) * ir12;
                 float t = hf - cs;
-                float iden = __frcp_rn(ddf + t * t);
+                float iden = __builtin_amdgcn_rcpf(ddf + t * t);
                 float gee = 1.0f + cd2f * t * t * iden;
                 float gp = -2.0f * ccf * t * iden * iden;
                 float coef_u, coef_c;
