@@ -239,3 +239,41 @@ def test_calculator_reads_table_file(oracle, tmp_path, monkeypatch):
     f_ref, _ = evaluate("forces", sp, s.positions, None, s.cell, nl, pd)
     assert abs(e - e_ref) <= TOL * max(1.0, abs(e_ref))
     assert np.abs(f - f_ref).max() <= TOL * max(1.0, np.abs(f_ref).max())
+
+
+def _sheared(s, shear):
+    """The structure under a shear of the cell (fractional coordinates kept)."""
+    frac = np.linalg.solve(s.cell.T, s.positions.T).T
+    cell = s.cell + np.array(shear) * s.cell[0, 0]
+    t = s.copy()
+    t.cell = cell
+    t.set_positions(frac @ cell)
+    return t
+
+
+def test_triclinic_cell(eng, oracle):
+    """Pairs and Tersoff together in a triclinic cell: every image shift
+    mixes all three cell vectors."""
+    print("case: Ar/Kr mixture and Si Tersoff, triclinic cells", flush=True)
+    s = _sheared(st.random_mixture(400, 26.0, 2.9, ["Kr", "Ar", "Ar"], seed=21),
+                 [[0, 0, 0], [0.31, 0, 0], [0.17, -0.23, 0]])
+    _vs_oracle(eng, oracle, s, mixture_model(), 8.5, {MIXTURE_TABLE_ID: mixture_table()})
+    si = _sheared(st.diamond_si((4, 4, 4), sigma=0.05, seed=6),
+                  [[0, 0, 0], [0.2, 0, 0], [-0.1, 0.15, 0]])
+    model = tersoff_si() + [Potential("Buckingham", symbols=[["Si", "Si"]],
+                                      parameters=[300.0, 10.0, 0.4], cutoff=4.5)]
+    e, f, w = _vs_oracle(eng, oracle, si, model, 5.0, tol=1e-5)
+    assert np.isfinite(e)
+
+
+def test_open_boundaries(eng, oracle):
+    """An LJ cluster with no periodic axis and a slab periodic in x only."""
+    print("case: LJ argon cluster and slab, open boundaries", flush=True)
+    c = st.fcc((4, 4, 4), a=5.26, sigma=0.05, seed=12)
+    c.pbc[:] = False
+    c.cell = np.diag([40.0, 40.0, 40.0])
+    _vs_oracle(eng, oracle, c, lj_argon(margin=1.0), 9.0)
+    slab = st.fcc((4, 4, 4), a=5.26, sigma=0.05, seed=13)
+    slab.pbc[:] = [True, False, False]
+    slab.cell = np.diag([21.04, 40.0, 40.0])
+    _vs_oracle(eng, oracle, slab, lj_argon(), 9.0)
