@@ -143,7 +143,59 @@ def qeq_benchmark(model_path, reps=2):
                                                     "val_maxerr", "epochs")}}
     pred.close()
     sysm.close()
+    out["reference_sample"] = qeq_reference_sample(n, c[1])
     return out
+
+
+def _n_kvectors(cell, kc):
+    K = ewald_kcells(cell, kc)
+    rec = 2.0 * np.pi * np.linalg.inv(cell).T
+    g = np.array(np.meshgrid(np.arange(-K[0], K[0] + 1), np.arange(-K[1], K[1] + 1),
+                             np.arange(0, K[2] + 1), indexing="ij")).reshape(3, -1).T
+    k2 = ((g @ rec) ** 2).sum(axis=1)
+    return K, int(((k2 < kc ** 2) & (k2 > 0)).sum())
+
+
+def qeq_reference_sample(n_full, steps_full, n_si=128):
+    """The reference core's cost of one relaxation step (one electronegativity
+    evaluation, relaxation.py:417-450 calls it once per step) on a 384-atom
+    sample of the same SiO2 model, scaled to the full system by N * N_k (the
+    reciprocal sum, which dominates) and by the cold start's step count."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if not O.ref_available():
+            return None
+        from pysic_amd import structures as st
+        from pysic_amd.learned import SIO2_EEM, SIO2_EWALD
+        r = st.amorphous_sio2(n_si, 2.2, seed=5)
+        ew = SIO2_EWALD
+        K, nk = _n_kvectors(r.cell, ew["k_cutoff"])
+        pots = []
+        for sym, (chi0, hard) in sorted(SIO2_EEM.items()):
+            pots.append((2, [chi0, 1.0], 0.0, 0.0, (sym,), -1))
+            pots.append((2, [0.5 * hard, 2.0], 0.0, 0.0, (sym,), -1))
+        q = np.random.default_rng(1).normal(0.0, 0.2, len(r))
+        q -= q.mean()
+        R = O.RefCore()
+        R.setup(r.symbols, r.positions, r.cell, r.pbc, charges=q, potentials=pots,
+                cutoffs=np.full(len(r), ew["real_cutoff"] + 0.5),
+                ewald=(ew["real_cutoff"], ew["k_cutoff"], K, ew["sigma"], ew["epsilon"]))
+        R.electronegativities()
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 3.0 and k < 20:
+            R.electronegativities()
+            k += 1
+        t_step = (time.perf_counter() - t0) / k
+        big = st.amorphous_sio2(n_full // 3, 2.2, seed=777).cell
+        _, nk_full = _n_kvectors(big, ew["k_cutoff"])
+        t_full = t_step * (n_full * nk_full) / (len(r) * nk)
+        return {"atoms": len(r), "k_vectors": nk, "seconds_per_step": round(t_step, 4),
+                "cores": 1, "kind": "reference",
+                "extrapolated_seconds_per_step_by_N_Nk": round(t_full, 2),
+                "extrapolated_seconds_to_tolerance": round(t_full * steps_full, 1)}
+    except Exception as exc:  # the checker is optional on the GPU host
+        return {"error": str(exc)}
 
 
 def build_batched_cells(ncells=512):
