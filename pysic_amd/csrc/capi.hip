@@ -629,6 +629,22 @@ void md_step(pysic_ctx *h, double dt, double half)
     md_forces(h, half);
 }
 
+// A model whose forces all come from the Tersoff kernels: its MD steps can
+// fuse the force gather into the next step's drift (md_step_fused).
+bool fused_md(pysic_ctx *h) { return h->S.model.has_pairs && !has_gen(h->S); }
+
+// The step boundary and the next step up to its per-bond gradients: gather
+// of the previous step's forces with its second half-kick, first half-kick,
+// drift, bin and sort (one pass, forces never stored), neighbor scan,
+// Tersoff site kernel.  Kernel launches only (capturable).
+void md_step_fused(pysic_ctx *h, double dt, double half)
+{
+    System &S = h->S;
+    nb_gather_drift_sort(S, dt, half);
+    nb_build_lists(S);
+    ts_site(S, h->variant, SITE_FORCES_ONLY);
+}
+
 std::vector<const void *> md_signature(pysic_ctx *h, double dt)
 {
     auto sig = graph_signature(h, true);
@@ -638,6 +654,7 @@ std::vector<const void *> md_signature(pysic_ctx *h, double dt)
     std::memcpy(&d2, &half, sizeof(double));
     sig.push_back(d1);
     sig.push_back(d2);
+    sig.push_back((const void *)(long)(fused_md(h) ? 1 : 0));
     return sig;
 }
 
@@ -648,24 +665,40 @@ void md_run_graph(pysic_ctx *h, int nsteps, double dt)
 {
     System &S = h->S;
     const double half = 0.5 * dt;
-    md_step(h, dt, half);  // eager
-    if (nsteps <= 1) return;
-    for (int k = 0; k < 2; ++k) {
-        auto sig = md_signature(h, dt);
-        pysic_ctx::Captured &slot = h->mdgraph[k];
-        if (!(slot.exec && slot.sig == sig))
-            capture_seq(h, slot, sig, true, [&]() { md_step(h, dt, half); });
-        swap_state(S);   // the state of the other parity
+    const bool fused = fused_md(h);
+    // the first step runs eagerly, which allocates whatever is missing; with
+    // the fused loop it stops at its per-bond gradients, whose gather is the
+    // first thing the next step does
+    if (fused) {
+        nb_sort_atoms(S, true, dt, half);
+        nb_build_lists(S);
+        ts_site(S, h->variant, SITE_FORCES_ONLY);
+    } else {
+        md_step(h, dt, half);
     }
-    for (int step = 1; step < nsteps; ++step) {
-        auto sig = md_signature(h, dt);
-        pysic_ctx::Captured *g = nullptr;
-        for (auto &c : h->mdgraph)
-            if (c.exec && c.sig == sig) g = &c;
-        PYSIC_REQUIRE(g, PYSIC_ESTATE, "md graph lost its buffers");
-        HIP_TRY(hipGraphLaunch(g->exec, S.stream));
-        swap_state(S);
+    if (nsteps > 1) {
+        for (int k = 0; k < 2; ++k) {
+            auto sig = md_signature(h, dt);
+            pysic_ctx::Captured &slot = h->mdgraph[k];
+            if (!(slot.exec && slot.sig == sig))
+                capture_seq(h, slot, sig, true, [&]() {
+                    if (fused) md_step_fused(h, dt, half);
+                    else md_step(h, dt, half);
+                });
+            swap_state(S);   // the state of the other parity
+        }
+        for (int step = 1; step < nsteps; ++step) {
+            auto sig = md_signature(h, dt);
+            pysic_ctx::Captured *g = nullptr;
+            for (auto &c : h->mdgraph)
+                if (c.exec && c.sig == sig) g = &c;
+            PYSIC_REQUIRE(g, PYSIC_ESTATE, "md graph lost its buffers");
+            HIP_TRY(hipGraphLaunch(g->exec, S.stream));
+            swap_state(S);
+        }
     }
+    // the last step's forces and its second half-kick
+    if (fused) ts_gather(S, half, 1);
 }
 
 // One device evaluation.  With use_graph: replay the graph captured for the

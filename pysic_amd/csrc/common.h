@@ -161,6 +161,41 @@ __device__ __forceinline__ double dfc64(double r, double hard, double soft)
     return -0.5 * M_PI / (hard - soft) * sin(M_PI * (r - soft) / (hard - soft));
 }
 
+// F_i = sum_j (D_ij - D_ji) over atom i's short list: D_ij = dU_i/dr_ij is
+// the site kernel's output for entry (i, j, img) and D_ji the entry (j, i,
+// -img) of j's list, found by scanning it (column-major: entry t of atom j
+// at t n + j).  miss reports a bond with no reverse entry.
+__device__ __forceinline__ double3 gather_bond_force(int i, int n, const int *__restrict__ sj,
+                                                     const int *__restrict__ simg,
+                                                     const int *__restrict__ scnt,
+                                                     const float4 *__restrict__ dU, bool &miss)
+{
+    const int ns = scnt[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        const size_t o = (size_t)s * n + i;
+        const int j = sj[o];
+        const int rim = neg_img(simg[o]);
+        const float4 d = dU[o];
+        const int nj = scnt[j];
+        float4 dr = make_float4(0, 0, 0, 0);
+        bool found = false;
+        for (int t = 0; t < nj; ++t) {
+            const size_t q = (size_t)t * n + j;
+            if (sj[q] == i && simg[q] == rim) {
+                dr = dU[q];
+                found = true;
+                break;
+            }
+        }
+        miss |= !found;
+        fx += (double)d.x - (double)dr.x;
+        fy += (double)d.y - (double)dr.y;
+        fz += (double)d.z - (double)dr.z;
+    }
+    return make_double3(fx, fy, fz);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Remap so that XCD x runs one contiguous range of logical blocks:
 // spatially adjacent atoms (the cell-sorted order) then share an L2.

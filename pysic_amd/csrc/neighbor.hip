@@ -84,6 +84,50 @@ __global__ void k_bin(int n, Geo geo, const double4 *__restrict__ pos, int *__re
 }
 
 // velocity-Verlet first half (p += dt/2 F; r += dt p/m) fused with binning
+// One velocity-Verlet step boundary in one pass, for the MD loop of a model
+// whose forces are all Tersoff-kernel forces: gather the forces of the step
+// that ended from the per-bond gradients (gather_bond_force), its second
+// half-kick, the next step's first half-kick, drift, and binning.  The same
+// arithmetic in the same order as k_gather (kick as an fma) followed by
+// k_drift_bin, so the trajectory is bit-identical; the forces are never
+// stored, and momenta and positions are read and written once.
+template <class Geo>
+__global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
+                                   const int *__restrict__ sj, const int *__restrict__ simg,
+                                   const int *__restrict__ scnt, const float4 *__restrict__ dU,
+                                   double4 *__restrict__ mom, double4 *__restrict__ pos,
+                                   int *__restrict__ key, int *__restrict__ wrap,
+                                   int *__restrict__ cellcount, int *__restrict__ flags)
+{
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool miss = false;
+    const double3 f = gather_bond_force(i, n, sj, simg, scnt, dU, miss);
+    if (miss) atomicOr(flags + 3, 1);
+    double4 p = mom[i];
+    p.x = fma(half_dt, f.x, p.x);
+    p.y = fma(half_dt, f.y, p.y);
+    p.z = fma(half_dt, f.z, p.z);
+    p.x += half_dt * f.x;
+    p.y += half_dt * f.y;
+    p.z += half_dt * f.z;
+    mom[i] = p;
+    double4 r = pos[i];
+    double s = dt / p.w;  // p.w carries the mass
+    r.x += s * p.x;
+    r.y += s * p.y;
+    r.z += s * p.z;
+    pos[i] = r;
+    int k, w;
+    bool bad;
+    bin_atom(geo.of(i), r, &k, &w, &bad);
+    k += geo.bin0(i);
+    if (bad) atomicOr(flags + 2, 1);
+    key[i] = k;
+    wrap[i] = w;
+    atomicAdd(cellcount + k, 1);
+}
+
 template <class Geo>
 __global__ void k_drift_bin(int n, Geo geo, double dt, double half_dt,
                             const double4 *__restrict__ frc, double4 *__restrict__ mom,
@@ -877,23 +921,52 @@ static void swap_bufs(System &S)
     std::swap(S.wrap, S.wrap2);
 }
 
-void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
+// The counting sort after binning: scan of the bin counts, scatter, per-bin
+// sort by original index, permutation of every per-atom array.
+static void sort_binned(System &S)
 {
     const int n = S.n;
     const int nbins = S.nbins;
     hipStream_t st = S.stream;
+    const int nb = div_up(nbins, SCAN_BLK);
+    k_scan_local<<<nb, SCAN_T, 0, st>>>(nbins, S.cellcount.p, S.cellstart.p, S.scanblk.p);
+    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, nbins);
+    k_scan_add<<<nb, SCAN_T, 0, st>>>(nbins, S.cellstart.p, S.scanblk.p);
+    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.cellstart.p, S.cellfill.p, S.perm.p);
+    k_cellsort<<<div_up(nbins, 256), 256, 0, st>>>(nbins, S.cellstart.p, S.perm.p, S.id.p);
+    with_geo(S, [&](auto geo) {
+        using Geo = decltype(geo);
+        k_permute<Geo><<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p,
+                                                       S.cut.p, S.id.p, S.key.p, S.wrap.p,
+                                                       S.pos2.p, S.mom2.p, S.q2.p, S.cut2.p,
+                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p);
+    });
+    HIP_TRY(hipGetLastError());
+}
+
+static void ensure_sort_buffers(System &S)
+{
+    const int n = S.n;
+    const int nbins = S.nbins;
     S.key.ensure(n); S.key2.ensure(n);
     S.wrap.ensure(n); S.wrap2.ensure(n);
     S.perm.ensure(n);
     S.cellcount.ensure(nbins);
     S.cellfill.ensure(nbins);
     S.cellstart.ensure((size_t)nbins + 1);
-    int nb = div_up(nbins, SCAN_BLK);
-    S.scanblk.ensure(nb);
+    S.scanblk.ensure(div_up(nbins, SCAN_BLK));
     S.pos2.ensure(n); S.mom2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
     S.id2.ensure(n);
     S.wpos.ensure(n);
     S.flags.ensure(4);
+}
+
+void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
+{
+    const int n = S.n;
+    const int nbins = S.nbins;
+    hipStream_t st = S.stream;
+    ensure_sort_buffers(S);
 
     hipEvent_t ev;
     S.timer.begin(drift ? "drift_bin_sort" : "bin_sort", st, &ev);
@@ -910,20 +983,30 @@ void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
             k_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(n, geo, S.pos.p, S.key.p, S.wrap.p,
                                                        S.cellcount.p, S.flags.p);
     });
-    k_scan_local<<<nb, SCAN_T, 0, st>>>(nbins, S.cellcount.p, S.cellstart.p, S.scanblk.p);
-    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, nbins);
-    k_scan_add<<<nb, SCAN_T, 0, st>>>(nbins, S.cellstart.p, S.scanblk.p);
-    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.cellstart.p, S.cellfill.p, S.perm.p);
-    k_cellsort<<<div_up(nbins, 256), 256, 0, st>>>(nbins, S.cellstart.p, S.perm.p, S.id.p);
+    sort_binned(S);
+    S.timer.end(drift ? "drift_bin_sort" : "bin_sort", st, ev);
+    swap_bufs(S);
+}
+
+void nb_gather_drift_sort(System &S, double dt, double half_dt)
+{
+    const int n = S.n;
+    const int nbins = S.nbins;
+    hipStream_t st = S.stream;
+    PYSIC_REQUIRE(S.sj.p && S.simg.p && S.scnt.p && S.dU.p, PYSIC_ESTATE,
+                  "no per-bond gradients to gather");
+    ensure_sort_buffers(S);
+    hipEvent_t ev;
+    S.timer.begin("gather_drift_bin_sort", st, &ev);
+    k_fill_int<<<div_up(2 * nbins, 256), 256, 0, st>>>(nbins, S.cellcount.p, S.cellfill.p);
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
-        k_permute<Geo><<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p,
-                                                       S.cut.p, S.id.p, S.key.p, S.wrap.p,
-                                                       S.pos2.p, S.mom2.p, S.q2.p, S.cut2.p,
-                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p);
+        k_gather_drift_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(
+            n, geo, dt, half_dt, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.mom.p, S.pos.p, S.key.p,
+            S.wrap.p, S.cellcount.p, S.flags.p);
     });
-    HIP_TRY(hipGetLastError());
-    S.timer.end(drift ? "drift_bin_sort" : "bin_sort", st, ev);
+    sort_binned(S);
+    S.timer.end("gather_drift_bin_sort", st, ev);
     swap_bufs(S);
 }
 

@@ -126,6 +126,10 @@ void with_geo(System &S, F &&f)
 void nb_configure_grid(System &S, double max_cut);
 void nb_sort_atoms(System &S, bool drift = false, double dt = 0.0,
                    double half_dt = 0.0);  // [drift +] bin + counting sort + permute
+// gather of the last step's Tersoff forces from dU + its second half-kick +
+// the next first half-kick + drift, then bin + counting sort + permute
+// (the forces are not stored)
+void nb_gather_drift_sort(System &S, double dt, double half_dt);
 void nb_build_lists(System &S);               // cell-list search, reference semantics
 int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
 void nb_clear_flags(System &S);               // zero the flags (kernel; graph-safe)

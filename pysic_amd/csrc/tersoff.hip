@@ -755,37 +755,17 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ s
 {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int ns = scnt[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     bool miss = false;
-    for (int s = 0; s < ns; ++s) {
-        size_t o = (size_t)s * n + i;
-        int j = sj[o];
-        int rim = neg_img(simg[o]);
-        float4 d = dU[o];
-        int nj = scnt[j];
-        float4 dr = make_float4(0, 0, 0, 0);
-        bool found = false;
-        for (int t = 0; t < nj; ++t) {
-            size_t q = (size_t)t * n + j;
-            if (sj[q] == i && simg[q] == rim) {
-                dr = dU[q];
-                found = true;
-                break;
-            }
-        }
-        miss |= !found;
-        fx += (double)d.x - (double)dr.x;
-        fy += (double)d.y - (double)dr.y;
-        fz += (double)d.z - (double)dr.z;
-    }
+    const double3 f = gather_bond_force(i, n, sj, simg, scnt, dU, miss);
     if (miss) atomicOr(flags + 3, 1);
-    frc[i] = make_double4(fx, fy, fz, 0.0);
+    frc[i] = make_double4(f.x, f.y, f.z, 0.0);
     if (kick) {
+        // the second half-kick of velocity Verlet (fma: the fused
+        // gather-drift kernel of the MD loop rounds it the same way)
         double4 p = mom[i];
-        p.x += half_dt * fx;
-        p.y += half_dt * fy;
-        p.z += half_dt * fz;
+        p.x = fma(half_dt, f.x, p.x);
+        p.y = fma(half_dt, f.y, p.y);
+        p.z = fma(half_dt, f.z, p.z);
         mom[i] = p;
     }
 }
