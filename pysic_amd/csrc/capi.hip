@@ -814,6 +814,7 @@ int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse
     for (int a = 0; a < 3; ++a) h->S.geom.pbc[a] = pbc[a] ? 1 : 0;
     h->S.ncells = 0;
     drop_graphs(h);
+    h->ew.dirty = true;   // the k-vectors follow the cell
     h->cell_ready = true;
     h->lists_valid = false;
     h->forces_valid = false;

@@ -24,12 +24,21 @@ struct EwaldState {
     DBuf<double> red;       // reduced scalars
     DBuf<double2> skpart;   // structure-factor partials over atom chunks
     DBuf<double4> rpart;    // per-atom reciprocal partials over k chunks
+    DBuf<double4> th;       // per-atom fractional coordinates and q s
+    DBuf<double2> pw;       // per-atom q s exp(2 pi i l u3), 16 per atom
+    DBuf<double4> zw;       // per-k coefficients of the per-atom sums
+    DBuf<unsigned long long> hpart;   // state fingerprint partials
+    // sk and th hold the structure factors of the state whose fingerprint
+    // is sk_hash
+    bool sk_valid = false;
+    unsigned long long sk_hash = 0;
     // k-vector segments: runs of at most 16 consecutive k3 at fixed (k1, k2),
     // {k1, k2, k3_start | len << 16, index of the first k}; natural order and
     // sorted by length (longest first)
     DBuf<int4> seg, seg_sorted;
     int nseg = 0;
     double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // reciprocal vectors 2 pi M^-1 (rows)
+    double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // M^-1 (rows): fractional coordinates
     double volume = 0;
     int skchunks = 0;
 };

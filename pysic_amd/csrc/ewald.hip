@@ -119,86 +119,142 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
 }
 
 constexpr int LSEG = 16;   // k-vectors per segment
-constexpr int STILE = 32;  // segments per LDS tile of the per-atom kernel
+constexpr int SCHUNK = 32; // segment-chunk granularity of the per-atom kernel
 
 struct Recip {
     double b[9];   // reciprocal vectors in rows: k = k1 b1 + k2 b2 + k3 b3
+    double f[9];   // the same over 2 pi: rows of the inverse cell matrix
 };
 
-__device__ __forceinline__ void phases(const Recip &R, const double4 &p, double &t1, double &t2,
-                                       double &t3)
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
-    t1 = R.b[0] * p.x + R.b[1] * p.y + R.b[2] * p.z;
-    t2 = R.b[3] * p.x + R.b[4] * p.y + R.b[5] * p.z;
-    t3 = R.b[6] * p.x + R.b[7] * p.y + R.b[8] * p.z;
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// S(k) partial sums: block (segment tile, atom chunk); thread = segment.
-// Atoms are staged SK_TA at a time with the table q s exp(i l theta3),
-// l < LSEG, so each k of a segment costs one complex multiply-add with a
-// wave-wide (broadcast) LDS read; one sincos per (segment, atom) gives the
-// segment's first phase.
-constexpr int SK_TA = 64;
+// exp(2 pi i u) for a phase u in turns.  The reduction is exact: removing
+// the integer part leaves r in [-1/2, 1/2] (|u| is far below 2^52), and
+// removing the nearest quarter turn q/4 leaves f in [-1/8, 1/8] (Sterbenz).
+// sin and cos of x = 2 pi f, |x| <= pi/4, are their Taylor series through
+// x^17 and x^16 (the first dropped terms are below 1e-17), and the quarter
+// turns rotate the result: exp(2 pi i u) = i^q exp(i x).
+__device__ __forceinline__ double2 turn(double u)
+{
+    const double r = u - rint(u);
+    const double qd = rint(4.0 * r);
+    const double x = 6.283185307179586 * fma(-0.25, qd, r);
+    const double x2 = x * x;
+    // Horner in x^2 with the coefficients (-1)^k / (2k+1)! and (-1)^k / (2k)!
+    double ps = -2.8114572543455206e-15;              // -1/17!
+    ps = fma(ps, x2, 7.647163731819816e-13);          //  1/15!
+    ps = fma(ps, x2, -1.6059043836821613e-10);        // -1/13!
+    ps = fma(ps, x2, 2.505210838544172e-08);          //  1/11!
+    ps = fma(ps, x2, -2.7557319223985893e-06);        // -1/9!
+    ps = fma(ps, x2, 1.984126984126984e-04);          //  1/7!
+    ps = fma(ps, x2, -8.333333333333333e-03);         // -1/5!
+    ps = fma(ps, x2, 1.6666666666666666e-01);         //  1/3!
+    const double sn = fma(-x * x2, ps, x);
+    double pc = 4.779477332387385e-14;                //  1/16!
+    pc = fma(pc, x2, -1.1470745597729725e-11);        // -1/14!
+    pc = fma(pc, x2, 2.08767569878681e-09);           //  1/12!
+    pc = fma(pc, x2, -2.755731922398589e-07);         // -1/10!
+    pc = fma(pc, x2, 2.48015873015873e-05);           //  1/8!
+    pc = fma(pc, x2, -1.388888888888889e-03);         // -1/6!
+    pc = fma(pc, x2, 4.1666666666666664e-02);         //  1/4!
+    pc = fma(pc, x2, -0.5);                           // -1/2!
+    const double cs = fma(pc, x2, 1.0);
+    const int q = (int)qd & 3;   // i^q: 1, i, -1, -i
+    const double c = (q & 1) ? sn : cs;
+    const double s = (q & 1) ? cs : sn;
+    return make_double2((q == 1 || q == 2) ? -c : c, (q >= 2) ? -s : s);
+}
 
-__global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk, Recip R,
+// Per-atom phase table: th[i] = (u1, u2, u3, q s), u_a = f_a . r the
+// fractional coordinates (k . r = 2 pi (k1 u1 + k2 u2 + k3 u3)), and the
+// structure-factor row pw[i][l] = q s exp(2 pi i l u3), l < LSEG.
+__global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double4 *__restrict__ pos,
+                                                    const double *__restrict__ q,
+                                                    const double *__restrict__ sc,
+                                                    double4 *__restrict__ th,
+                                                    double2 *__restrict__ pw)
+{
+    const int i = blockIdx.x * EB + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    const double u1 = R.f[0] * p.x + R.f[1] * p.y + R.f[2] * p.z;
+    const double u2 = R.f[3] * p.x + R.f[4] * p.y + R.f[5] * p.z;
+    const double u3 = R.f[6] * p.x + R.f[7] * p.y + R.f[8] * p.z;
+    const double qs = q[i] * sc[i];
+    th[i] = make_double4(u1, u2, u3, qs);
+    double2 *row = pw + (size_t)i * LSEG;
+    for (int l = 0; l < LSEG; ++l) {
+        const double2 e = turn(l * u3);
+        row[l] = make_double2(qs * e.x, qs * e.y);
+    }
+}
+
+// S(k) partial sums: block (segment tile, atom chunk); each thread carries
+// SPT segments (sidx, sidx + EB, ...).  The atom loop is wave-uniform: each
+// atom's phases and its LSEG-entry structure-factor row come through the
+// scalar cache (one read per wave, not a 64-lane LDS broadcast) and serve
+// SPT segments.  One sincos per (segment, atom) gives the segment's first
+// phase; each k then costs one complex multiply-add.
+#ifndef PYSIC_SK_SPT
+#define PYSIC_SK_SPT 2
+#endif
+constexpr int SPT = PYSIC_SK_SPT;
+
+__global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk,
                                                   const int4 *__restrict__ segs,
-                                                  const double4 *__restrict__ pos,
-                                                  const double *__restrict__ q,
-                                                  const double *__restrict__ sc,
+                                                  const double4 *__restrict__ th,
+                                                  const double2 *__restrict__ pw,
                                                   double2 *__restrict__ skpart)
 {
-    __shared__ double4 at[SK_TA];          // theta1, theta2, theta3, q s
-    __shared__ double2 pw[SK_TA][LSEG];    // q s exp(i l theta3)
-    const int sidx = blockIdx.x * EB + threadIdx.x;
     const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
-    int4 sg = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);
-    const double k1 = sg.x, k2 = sg.y, l0 = sg.z & 0xffff;
-    const int len = sg.z >> 16;  // 0 for lanes past the last segment
-    double cr[LSEG], ci[LSEG];
+    double k1[SPT], k2[SPT], l0[SPT];
+    int4 sg[SPT];
 #pragma unroll
-    for (int l = 0; l < LSEG; ++l) cr[l] = ci[l] = 0.0;
-    for (int base = a0; base < a1; base += SK_TA) {
-        if (threadIdx.x < SK_TA) {
-            const int a = base + threadIdx.x;
-            double t1 = 0.0, t2 = 0.0, t3 = 0.0, qs = 0.0;
-            if (a < a1) {
-                phases(R, pos[a], t1, t2, t3);
-                qs = q[a] * sc[a];
-            }
-            at[threadIdx.x] = make_double4(t1, t2, t3, qs);
-            double s3, c3;
-            sincos(t3, &s3, &c3);
-            double vr = qs, vi = 0.0;
-#pragma unroll
-            for (int l = 0; l < LSEG; ++l) {
-                pw[threadIdx.x][l] = make_double2(vr, vi);
-                const double nr = vr * c3 - vi * s3;
-                vi = vr * s3 + vi * c3;
-                vr = nr;
-            }
-        }
-        __syncthreads();
-        const int m = min(SK_TA, a1 - base);
-        for (int t = 0; t < m; ++t) {
-            const double4 th = at[t];
-            if (th.w == 0.0) continue;
-            double vs, vc;
-            sincos(k1 * th.x + k2 * th.y + l0 * th.z, &vs, &vc);
-#pragma unroll
-            for (int l = 0; l < LSEG; ++l) {
-                if (l < len) {
-                    const double2 p = pw[t][l];
-                    cr[l] += vc * p.x - vs * p.y;
-                    ci[l] += vc * p.y + vs * p.x;
-                }
-            }
-        }
-        __syncthreads();
+    for (int m = 0; m < SPT; ++m) {
+        const int sidx = (blockIdx.x * SPT + m) * EB + threadIdx.x;
+        sg[m] = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);   // length 0: no output
+        k1[m] = sg[m].x;
+        k2[m] = sg[m].y;
+        l0[m] = sg[m].z & 0xffff;
     }
-    double2 *out = skpart + (size_t)blockIdx.y * nk + sg.w;
+    double cr[SPT][LSEG], ci[SPT][LSEG];
 #pragma unroll
-    for (int l = 0; l < LSEG; ++l)
-        if (l < len) out[l] = make_double2(cr[l], ci[l]);
+    for (int m = 0; m < SPT; ++m)
+#pragma unroll
+        for (int l = 0; l < LSEG; ++l) cr[m][l] = ci[m][l] = 0.0;
+    double4 tn = a0 < a1 ? th[a0] : make_double4(0, 0, 0, 0);
+    for (int a = a0; a < a1; ++a) {
+        const double4 t = tn;
+        tn = th[min(a + 1, a1 - 1)];   // the next atom's entry, loaded early
+        if (t.w == 0.0) continue;
+        double2 v[SPT];
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) v[m] = turn(k1[m] * t.x + k2[m] * t.y + l0[m] * t.z);
+        const double2 *row = pw + (size_t)a * LSEG;
+        // every lane runs all LSEG entries (no branch between the scalar
+        // loads, so they issue together); entries past a segment's length
+        // accumulate but are not stored
+#pragma unroll
+        for (int l = 0; l < LSEG; ++l) {
+            const double2 p = row[l];
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                cr[m][l] = fma(v[m].x, p.x, fma(-v[m].y, p.y, cr[m][l]));
+                ci[m][l] = fma(v[m].x, p.y, fma(v[m].y, p.x, ci[m][l]));
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int len = sg[m].z >> 16;
+        double2 *out = skpart + (size_t)blockIdx.y * nk + sg[m].w;
+#pragma unroll
+        for (int l = 0; l < LSEG; ++l)
+            if (l < len) out[l] = make_double2(cr[m][l], ci[m][l]);
+    }
 }
 
 __global__ void k_structure_sum(int nk, int nchunk, const double2 *__restrict__ skpart,
@@ -235,99 +291,128 @@ __global__ void k_recip_energy(int nk, const double4 *__restrict__ kvec,
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Per-atom reciprocal forces (mode 2) or electronegativities (mode 3) over
-// one chunk of segments (natural order, so each tile of STILE segments covers
-// one contiguous k range); block (atom tile, segment chunk).  Each thread
-// carries RAT atoms (i, i + EB, ...) so that every k-vector and S(k) read
-// from LDS serves RAT atoms.  Measured on the 24,576-atom SiO2 cell: RAT = 1
-// 1.20 ms, 2: 0.89 ms, 4: 1.32 ms (occupancy 3 waves/SIMD); scalar-cache
-// reads instead of LDS: 0.96 ms.
-constexpr int RAT = 2;
+// Per-k coefficients of the per-atom sums (natural k order; thread per
+// segment).  Forces (mode 2): z = 2 c_k (S_sin + i S_cos), the force on atom
+// i being -q_i s_i sum_k Re(z exp(i k.r_i)) k (Potentials.f90:1895-1931);
+// w = l z with l the k's place in its segment.  Electronegativities (mode
+// 3): z = 2 c_k (S_cos - i S_sin), chi_i -= sum_k Re(z exp(i k.r_i))
+// (Potentials.f90:2196-2229).
+__global__ void k_recip_coef(int nseg, int mode, const int4 *__restrict__ segs,
+                             const double4 *__restrict__ kvec, const double2 *__restrict__ sk,
+                             double4 *__restrict__ zw)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const int4 sg = segs[s];
+    const int len = sg.z >> 16;
+    for (int l = 0; l < len; ++l) {
+        const int k = sg.w + l;
+        const double c2 = 2.0 * kvec[k].w;
+        const double2 S = sk[k];
+        const double zr = mode == 3 ? c2 * S.x : c2 * S.y;
+        const double zi = mode == 3 ? -(c2 * S.y) : c2 * S.x;
+        zw[k] = make_double4(zr, zi, l * zr, l * zi);
+    }
+}
 
-__global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, int mode, Recip R,
+// Per-atom reciprocal forces (MODE 2) or electronegativities (MODE 3) over
+// one chunk of segments in natural order; block (atom tile, segment chunk),
+// RAT atoms per thread.  The segment loop is wave-uniform and the
+// coefficients come through the scalar cache.  With v the phase of a
+// segment's first k and w = exp(2 pi i u3) the step along k3,
+//   sum_l Re(z_l v w^l) = Re(v P(w)),   P(w) = sum_l z_l w^l   (Horner),
+// and for forces k = k1 b1 + k2 b2 + (l0 + l) b3 splits the sum into
+// Re(v P) (k1 b1 + k2 b2 + l0 b3) + Re(v Q) b3,  Q(w) = sum_l l z_l w^l.
+// A segment that continues its row starts at v w^LSEG, so only row starts
+// take a sincospi.
+#ifndef PYSIC_EW_RAT
+#define PYSIC_EW_RAT 4
+#endif
+constexpr int RAT = PYSIC_EW_RAT;
+
+template <int MODE>
+__global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, Recip R,
                                                    const int4 *__restrict__ segs,
-                                                   const double4 *__restrict__ kvec,
-                                                   const double2 *__restrict__ sk,
-                                                   const double4 *__restrict__ pos,
-                                                   const double *__restrict__ q,
-                                                   const double *__restrict__ sc,
+                                                   const double4 *__restrict__ zw,
+                                                   const double4 *__restrict__ th,
                                                    double4 *__restrict__ part)
 {
-    __shared__ int4 sl[STILE];
-    __shared__ double4 kl[STILE * LSEG];
-    __shared__ double2 sv[STILE * LSEG];
     const int i0 = blockIdx.x * (EB * RAT) + threadIdx.x;
     const int s0 = blockIdx.y * schunk, s1 = min(nseg, s0 + schunk);
-    double t1[RAT], t2[RAT], t3[RAT], wc[RAT], ws[RAT], qs[RAT];
-    double ax[RAT], ay[RAT], az[RAT];
-    double vs[RAT], vc[RAT];   // the running phase exp(i k.r) of each atom
+    double u1[RAT], u2[RAT], u3[RAT], qs[RAT], g1[RAT], g2[RAT], g3[RAT];
+    double2 w[RAT], wl[RAT], v[RAT];
 #pragma unroll
     for (int a = 0; a < RAT; ++a) {
         const int i = i0 + a * EB;
-        const double4 p = i < n ? pos[i] : make_double4(0, 0, 0, 0);
-        qs[a] = i < n ? q[i] * sc[i] : 0.0;
-        phases(R, p, t1[a], t2[a], t3[a]);
-        sincos(t3[a], &ws[a], &wc[a]);
-        ax[a] = ay[a] = az[a] = 0.0;
-        vs[a] = 0.0;
-        vc[a] = 1.0;
+        const double4 t = i < n ? th[i] : make_double4(0, 0, 0, 0);
+        u1[a] = t.x;
+        u2[a] = t.y;
+        u3[a] = t.z;
+        qs[a] = t.w;
+        w[a] = turn(t.z);
+        wl[a] = turn(LSEG * t.z);
+        v[a] = make_double2(1.0, 0.0);
+        g1[a] = g2[a] = g3[a] = 0.0;
     }
-    // (k1, k2, next k3) that the running phase stands at; a segment that
-    // continues the previous one's row starts there without a sincos
+    // (k1, k2, k3) at which the next segment would continue the last one
     int rk1 = 0x7fffffff, rk2 = 0, rl = -1;
-    for (int base = s0; base < s1; base += STILE) {
-        const int m = min(STILE, s1 - base);
-        const int kb = segs[base].w;
-        const int4 last = segs[base + m - 1];
-        const int nkt = last.w + (last.z >> 16) - kb;   // <= STILE * LSEG
-        if (threadIdx.x < m) sl[threadIdx.x] = segs[base + threadIdx.x];
-        for (int k = threadIdx.x; k < nkt; k += EB) {
-            kl[k] = kvec[kb + k];
-            sv[k] = sk[kb + k];
-        }
-        __syncthreads();
-        for (int t = 0; t < m; ++t) {
-            const int4 sg = sl[t];
-            const int len = sg.z >> 16;
-            const int o = sg.w - kb;
-            const int il0 = sg.z & 0xffff;
-            if (!(sg.x == rk1 && sg.y == rk2 && il0 == rl)) {
-                const double k1 = sg.x, k2 = sg.y, l0 = il0;
+    for (int s = s0; s < s1; ++s) {
+        const int4 sg = segs[s];
+        const int len = sg.z >> 16, il0 = sg.z & 0xffff;
+        const double k1 = sg.x, k2 = sg.y, l0 = il0;
+        if (sg.x == rk1 && sg.y == rk2 && il0 == rl) {
 #pragma unroll
-                for (int a = 0; a < RAT; ++a)
-                    sincos(k1 * t1[a] + k2 * t2[a] + l0 * t3[a], &vs[a], &vc[a]);
-            }
-            rk1 = sg.x;
-            rk2 = sg.y;
-            rl = il0 + len;
-            for (int l = 0; l < len; ++l) {
-                const double4 kv = kl[o + l];
-                const double2 S = sv[o + l];
-                const double c2 = 2.0 * kv.w;
+            for (int a = 0; a < RAT; ++a) v[a] = cmul(v[a], wl[a]);
+        } else {
+#pragma unroll
+            for (int a = 0; a < RAT; ++a) v[a] = turn(k1 * u1[a] + k2 * u2[a] + l0 * u3[a]);
+        }
+        rk1 = sg.x;
+        rk2 = sg.y;
+        rl = len == LSEG ? il0 + LSEG : -1;
+        double2 P[RAT], Q[RAT];
+#pragma unroll
+        for (int a = 0; a < RAT; ++a) P[a] = Q[a] = make_double2(0.0, 0.0);
+        const double4 *z = zw + sg.w;
+#pragma unroll
+        for (int l = LSEG - 1; l >= 0; --l) {
+            if (l < len) {
+                const double4 c = z[l];
 #pragma unroll
                 for (int a = 0; a < RAT; ++a) {
-                    if (mode == 3) {
-                        // chi -= 2 c_k (C cos + S sin)   (Potentials.f90:2196-2229)
-                        ax[a] -= c2 * (S.x * vc[a] + S.y * vs[a]);
-                    } else {
-                        // f -= 2 c_k q s (S cos - C sin) k   (Potentials.f90:1895-1931)
-                        const double pre = c2 * qs[a] * (S.y * vc[a] - S.x * vs[a]);
-                        ax[a] -= pre * kv.x;
-                        ay[a] -= pre * kv.y;
-                        az[a] -= pre * kv.z;
-                    }
-                    const double nc = vc[a] * wc[a] - vs[a] * ws[a];
-                    vs[a] = vc[a] * ws[a] + vs[a] * wc[a];
-                    vc[a] = nc;
+                    P[a] = make_double2(fma(P[a].x, w[a].x, fma(-P[a].y, w[a].y, c.x)),
+                                        fma(P[a].x, w[a].y, fma(P[a].y, w[a].x, c.y)));
+                    if (MODE == 2)
+                        Q[a] = make_double2(fma(Q[a].x, w[a].x, fma(-Q[a].y, w[a].y, c.z)),
+                                            fma(Q[a].x, w[a].y, fma(Q[a].y, w[a].x, c.w)));
                 }
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < RAT; ++a) {
+            const double r0 = v[a].x * P[a].x - v[a].y * P[a].y;
+            if (MODE == 2) {
+                const double r1 = v[a].x * Q[a].x - v[a].y * Q[a].y;
+                g1[a] = fma(k1, r0, g1[a]);
+                g2[a] = fma(k2, r0, g2[a]);
+                g3[a] = fma(l0, r0, g3[a]) + r1;
+            } else {
+                g1[a] += r0;
+            }
+        }
     }
 #pragma unroll
     for (int a = 0; a < RAT; ++a) {
         const int i = i0 + a * EB;
-        if (i < n) part[(size_t)blockIdx.y * n + i] = make_double4(ax[a], ay[a], az[a], 0.0);
+        if (i >= n) continue;
+        double4 o = make_double4(-g1[a], 0.0, 0.0, 0.0);
+        if (MODE == 2) {
+            const double m = -qs[a];
+            o.x = m * (g1[a] * R.b[0] + g2[a] * R.b[3] + g3[a] * R.b[6]);
+            o.y = m * (g1[a] * R.b[1] + g2[a] * R.b[4] + g3[a] * R.b[7]);
+            o.z = m * (g1[a] * R.b[2] + g2[a] * R.b[5] + g3[a] * R.b[8]);
+        }
+        part[(size_t)blockIdx.y * n + i] = o;
     }
 }
 
@@ -369,6 +454,57 @@ __global__ __launch_bounds__(EB) void k_recip_finish(int n, int nchunk, int mode
         __syncthreads();
     }
     if (threadIdx.x < 6) bpart[(size_t)blockIdx.x * 8 + threadIdx.x] = red[0][threadIdx.x];
+}
+
+// Fingerprint of what S(k) and the phase table depend on besides the cell
+// and the k-vectors: positions, charges and scalers in the current atom
+// order.  Each atom's words and index go through the splitmix64 finaliser;
+// the atoms' values are summed (block partials, then k_hash_sum).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(EB) void k_state_hash(int n, const double4 *__restrict__ pos,
+                                                   const double *__restrict__ q,
+                                                   const double *__restrict__ sc,
+                                                   unsigned long long *__restrict__ part)
+{
+    __shared__ unsigned long long red[EB];
+    const int i = blockIdx.x * EB + threadIdx.x;
+    unsigned long long h = 0;
+    if (i < n) {
+        const double4 p = pos[i];
+        h = mix64(((unsigned long long)i * 0x9e3779b97f4a7c15ULL) ^ __double_as_longlong(p.x));
+        h = mix64(h ^ __double_as_longlong(p.y));
+        h = mix64(h ^ __double_as_longlong(p.z));
+        h = mix64(h ^ __double_as_longlong(q[i]));
+        h = mix64(h ^ __double_as_longlong(sc[i]));
+    }
+    red[threadIdx.x] = h;
+    __syncthreads();
+    for (int w = EB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(EB) void k_hash_sum(int count, const unsigned long long *__restrict__ part,
+                                                 unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long red[EB];
+    unsigned long long h = 0;
+    for (int b = threadIdx.x; b < count; b += EB) h += part[b];
+    red[threadIdx.x] = h;
+    __syncthreads();
+    for (int w = EB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // sum of q*s and of (q*s)^2 (block partials)
@@ -522,6 +658,7 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
     E.red.ensure(16);
     if (!P.on) return;
     if (E.dirty) {
+        E.sk_valid = false;
         // k-vectors in the reference's loop order (Potentials.f90:1526-1548)
         const double pi = 4.0 * std::atan(1.0);
         double rcl[9];
@@ -553,7 +690,10 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
                     if (!extend) segs.push_back(make_int4(k1, k2, k3 | (1 << 16), idx));
                 }
         PYSIC_REQUIRE(P.K[2] < 0xffff, PYSIC_EINVAL, "too many reciprocal cells along k3");
-        for (int q = 0; q < 9; ++q) E.b[q] = rcl[q];
+        for (int q = 0; q < 9; ++q) {
+            E.b[q] = rcl[q];
+            E.f[q] = S.geom.inv[q];
+        }
         E.nseg = (int)segs.size();
         std::vector<int4> sorted = segs;
         std::stable_sort(sorted.begin(), sorted.end(),
@@ -595,36 +735,45 @@ void ensure_scaler_identity(System &S, EwaldState &E)
 Recip recip_of(const EwaldState &E)
 {
     Recip R;
-    for (int q = 0; q < 9; ++q) R.b[q] = E.b[q];
+    for (int q = 0; q < 9; ++q) {
+        R.b[q] = E.b[q];
+        R.f[q] = E.f[q];
+    }
     return R;
 }
 
+// S(k) into E.sk; also leaves the per-atom phase table E.th that
+// recip_atoms reads.
 void structure_factors(System &S, EwaldState &E)
 {
     const int n = S.n, nk = E.nk;
     if (nk == 0) return;
-    int stiles = div_up(E.nseg, EB);
+    int stiles = div_up(E.nseg, EB * SPT);
     int nchunk = std::max(1, std::min(div_up(n, EB), div_up(2048, stiles)));
-    int chunk = div_up(div_up(n, nchunk), EB) * EB;
+    int chunk = div_up(n, nchunk);
     nchunk = div_up(n, chunk);
     E.skpart.ensure((size_t)nchunk * nk);
+    E.th.ensure(n);
+    E.pw.ensure((size_t)n * LSEG);
     hipEvent_t ev;
     S.timer.begin("ewald_sk", S.stream, &ev);
-    k_structure<<<dim3(stiles, nchunk), EB, 0, S.stream>>>(E.nseg, nk, n, chunk, recip_of(E),
-                                                          E.seg_sorted.p, S.pos.p, S.q.p,
-                                                          E.scaler.p, E.skpart.p);
+    k_phase_table<<<div_up(n, EB), EB, 0, S.stream>>>(n, recip_of(E), S.pos.p, S.q.p, E.scaler.p,
+                                                      E.th.p, E.pw.p);
+    k_structure<<<dim3(stiles, nchunk), EB, 0, S.stream>>>(E.nseg, nk, n, chunk, E.seg_sorted.p,
+                                                          E.th.p, E.pw.p, E.skpart.p);
     k_structure_sum<<<div_up(nk, 256), 256, 0, S.stream>>>(nk, nchunk, E.skpart.p, E.sk.p);
     HIP_TRY(hipGetLastError());
     S.timer.end("ewald_sk", S.stream, ev);
 }
 
-// mode 2: add forces + return stress; mode 3: add into chi
+// mode 2: add forces + return stress; mode 3: add into chi.  Needs the
+// structure factors and phase table of the current positions.
 void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
 {
     const int n = S.n, nk = E.nk;
     int atiles = div_up(n, EB * RAT);
-    int nchunk = std::max(1, std::min(div_up(E.nseg, STILE), div_up(2048, atiles)));
-    int schunk = div_up(div_up(E.nseg, nchunk), STILE) * STILE;
+    int nchunk = std::max(1, std::min(div_up(E.nseg, SCHUNK), div_up(2048, atiles)));
+    int schunk = div_up(div_up(E.nseg, nchunk), SCHUNK) * SCHUNK;
     nchunk = nk > 0 ? div_up(E.nseg, schunk) : 0;
     if (nchunk == 0) {
         if (stress6) for (int c = 0; c < 6; ++c) stress6[c] = 0.0;
@@ -634,11 +783,17 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
     DBuf<double4> &part = E.rpart;
     part.ensure((size_t)nchunk * n);
     E.part.ensure((size_t)ftiles * 8);
+    E.zw.ensure(nk);
     hipEvent_t ev;
     S.timer.begin(mode == 3 ? "ewald_chi" : "ewald_force", S.stream, &ev);
-    k_recip_atom<<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, mode, recip_of(E),
-                                                           E.seg.p, E.kvec.p, E.sk.p, S.pos.p,
-                                                           S.q.p, E.scaler.p, part.p);
+    k_recip_coef<<<div_up(E.nseg, EB), EB, 0, S.stream>>>(E.nseg, mode, E.seg.p, E.kvec.p, E.sk.p,
+                                                          E.zw.p);
+    if (mode == 3)
+        k_recip_atom<3><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, recip_of(E),
+                                                                 E.seg.p, E.zw.p, E.th.p, part.p);
+    else
+        k_recip_atom<2><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, recip_of(E),
+                                                                 E.seg.p, E.zw.p, E.th.p, part.p);
     k_recip_finish<<<ftiles, EB, 0, S.stream>>>(n, nchunk, mode, part.p, S.pos.p, S.frc.p, E.chi.p,
                                                 E.part.p);
     HIP_TRY(hipGetLastError());
@@ -650,6 +805,32 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
         HIP_TRY(hipStreamSynchronize(S.stream));
         for (int c = 0; c < 6; ++c) stress6[c] = h[c];
     }
+}
+
+unsigned long long state_hash(System &S, EwaldState &E)
+{
+    const int nb = div_up(S.n, EB);
+    E.hpart.ensure(nb + 1);
+    k_state_hash<<<nb, EB, 0, S.stream>>>(S.n, S.pos.p, S.q.p, E.scaler.p, E.hpart.p);
+    k_hash_sum<<<1, EB, 0, S.stream>>>(nb, E.hpart.p, E.hpart.p + nb);
+    HIP_TRY(hipGetLastError());
+    unsigned long long h;
+    HIP_TRY(hipMemcpyAsync(&h, E.hpart.p + nb, sizeof(h), hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    return h;
+}
+
+// S(k) and the phase table of the current state.  An energy call followed by
+// a forces call on the same configuration (the usual calculator sequence)
+// computes them once: they are kept while the fingerprint of positions,
+// charges and scalers matches and neither the cell nor the k-vectors changed.
+void current_structure_factors(System &S, EwaldState &E)
+{
+    const unsigned long long h = state_hash(S, E);
+    if (E.sk_valid && E.sk_hash == h) return;
+    structure_factors(S, E);
+    E.sk_hash = h;
+    E.sk_valid = true;
 }
 
 // q sums: out[0] = sum q s, out[1] = sum (q s)^2
@@ -705,7 +886,9 @@ void chi_eval(System &S, const EwaldParams &P, EwaldState &E)
     }
     real_pass(S, P, E, 3, 0.0, nullptr, bg_dev);
     if (P.on && E.nk > 0) {
+        // the relaxation changes the charges every step: no fingerprint
         structure_factors(S, E);
+        E.sk_valid = false;
         recip_atoms(S, E, 3, nullptr);
     }
 }
@@ -724,6 +907,11 @@ void ewald_release(EwaldState &E)
     E.rpart.release();
     E.seg.release();
     E.seg_sorted.release();
+    E.th.release();
+    E.pw.release();
+    E.zw.release();
+    E.hpart.release();
+    E.sk_valid = false;
 }
 
 double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
@@ -740,7 +928,7 @@ double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
         e += -qs[1] * c.inv_eps_4pi * c.inv_sig_sqrt2pi;                         // self
         e += -qs[0] * qs[0] * 0.5 * c.inv_eps_2v * P.sigma * P.sigma;            // background
         if (E.nk > 0) {
-            structure_factors(S, E);
+            current_structure_factors(S, E);
             int nb = div_up(E.nk, EB);
             E.part.ensure((size_t)nb);
             k_recip_energy<<<nb, EB, 0, S.stream>>>(E.nk, E.kvec.p, E.sk.p, E.part.p);
@@ -761,7 +949,7 @@ void ewald_add_forces(System &S, const EwaldParams &P, EwaldState &E, double *st
     real_pass(S, P, E, 2, 0.0, r7);
     for (int c = 0; c < 6; ++c) stress[c] = r7[1 + c];
     if (P.on && E.nk > 0) {
-        structure_factors(S, E);
+        current_structure_factors(S, E);
         double st[6];
         recip_atoms(S, E, 2, st);
         for (int c = 0; c < 6; ++c) stress[c] += st[c];

@@ -85,6 +85,47 @@ def test_rocksalt_8x8x8_rc10(eng, oracle):
     check(eng, oracle, sp, s.positions, s.charges, s.cell, np.zeros((0, 10)), ew)
 
 
+def test_structure_factors_follow_the_state(eng, oracle):
+    """S(k) computed for an energy call is reused by a forces call on the
+    same configuration; every change of positions, charges or cell between
+    calls must reach the result (each step against the oracle)."""
+    s = st.rocksalt((4, 4, 4), sigma=0.08, seed=7)
+    rc, sigma, kc = 10.0, 2.04430, 2.5713
+    K = kcells(s.cell, kc)
+    ew = np.array([1, rc, kc, K[0], K[1], K[2], sigma, 0.00552635])
+    sp = (s.charges < 0).astype(np.int32)
+    pd = np.zeros((0, 10))
+    pos, q, cell = s.positions.copy(), s.charges.copy(), s.cell.copy()
+    n = len(pos)
+    load_ewald(eng, sp, pos, q, cell, pd, ew)
+
+    def agree(what):
+        nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(n, rc + 0.5))
+        e_ref = oracle.evaluate("energy", sp, pos, q, cell, nl, pd, None, ew)
+        f_ref, s_ref = oracle.evaluate("forces", sp, pos, q, cell, nl, pd, None, ew)
+        e = eng.energy()
+        f, w = eng.forces()
+        print("  %s: |dE|/N %.2e rel|dF| %.2e" % (what, abs(e - e_ref) / n, rel_maxnorm(f, f_ref)),
+              flush=True)
+        assert abs(e - e_ref) / n <= 1e-9, what
+        assert rel_maxnorm(f, f_ref) <= 1e-9, what
+        assert np.abs(w - s_ref).max() / n <= 1e-9, what
+
+    agree("initial")
+    pos[3] += [0.01, -0.02, 0.005]
+    eng.update_coordinates(pos)
+    agree("one atom moved")
+    q *= 0.9
+    eng.update_charges(q)
+    agree("charges scaled")
+    strain = np.diag([1.01, 0.995, 1.0])
+    cell, pos = cell @ strain, pos @ strain
+    eng.create_cell(cell, (True, True, True))
+    eng.update_coordinates(pos)
+    eng.generate_neighbor_lists(np.full(n, rc + 0.5))
+    agree("cell strained")
+
+
 def test_madelung(eng, oracle):
     s = st.rocksalt((4, 4, 4))
     rc, sigma = 10.0, 2.04430
