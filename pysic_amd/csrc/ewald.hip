@@ -10,15 +10,15 @@
 //    reference's main loop;
 //  * reciprocal space: the half-space k-vectors with |k| < k_c are enumerated
 //    once on the host in the reference's order and grouped into segments of
-//    up to 16 consecutive k3 at fixed (k1, k2).  With theta_a = b_a . r
-//    (b_a the reciprocal vectors), exp(i k.r) = exp(i (k1 th1 + k2 th2 +
-//    k3 th3)): one sincos per (segment, atom) gives the first phase, and
-//    the rest of the segment follows by the recurrence v <- v exp(i th3)
-//    (a complex multiply instead of a sincos per k and atom).  Structure
-//    factors S(k) = sum_i q_i s_i exp(i k.r_i): thread per segment over atom
-//    chunks staged in LDS (2-D grid, deterministic partial sums); per-atom
-//    consumers (forces, electronegativities): thread per atom over segment
-//    tiles whose k-vectors and S(k) are staged in LDS;
+//    up to 16 consecutive k3 at fixed (k1, k2).  With the fractional
+//    coordinates u_a (k . r = 2 pi (k1 u1 + k2 u2 + k3 u3)), one phase per
+//    (segment, atom) gives a segment's first factor and the per-atom row
+//    exp(2 pi i l u3) the rest.  Structure factors S(k) = sum_i q_i s_i
+//    exp(i k.r_i): a (segment x atom) by (atom x 16) complex product on the
+//    FP64 matrix cores, deterministic partial sums over atom chunks.  Per-atom
+//    consumers (forces, electronegativities): thread per atom, a Horner
+//    polynomial in exp(2 pi i u3) per segment, coefficients through the
+//    scalar cache;
 //  * self and charged-background terms are per-atom closed forms.
 // The reciprocal "stress" uses absolute positions (Potentials.f90:1940-1945)
 // and the background electronegativity uses a zero-initialised charge sum
@@ -192,16 +192,20 @@ __global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double
     }
 }
 
-// S(k) partial sums: block (segment tile, atom chunk); each thread carries
-// SPT segments (sidx, sidx + EB, ...).  The atom loop is wave-uniform: each
-// atom's phases and its LSEG-entry structure-factor row come through the
-// scalar cache (one read per wave, not a 64-lane LDS broadcast) and serve
-// SPT segments.  One sincos per (segment, atom) gives the segment's first
-// phase; each k then costs one complex multiply-add.
-#ifndef PYSIC_SK_SPT
-#define PYSIC_SK_SPT 2
-#endif
-constexpr int SPT = PYSIC_SK_SPT;
+// S(k) partials on the matrix cores.  Over an atom chunk, for a segment
+// (k1, k2, l0) and its entries l < LSEG,
+//   S_seg(l) = sum_a exp(2 pi i (k1 u1 + k2 u2 + l0 u3)_a) pw_a(l),
+// a complex product of the (segment x atom) phase matrix, built on the fly,
+// and the (atom x entry) table pw: FP64 MFMA 16x16x4, four real products
+// per complex one.  A wave owns SK_MT tiles of 16 segments and steps through
+// the atoms 4 at a time; lane L computes the phase of segment L % 16 and
+// atom L / 16 of each tile (the A operand) and loads entry L % 16 of that
+// atom's row (the B operand, coalesced).  Operand layout of
+// v_mfma_f64_16x16x4f64: A (16 x 4) lane i + 16 k holds A[i][k]; B (4 x 16)
+// lane j + 16 k holds B[k][j]; D (16 x 16) lane j + 16 (i % 4), register
+// i / 4, holds D[i][j].
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+constexpr int SK_MT = 4;   // 16-segment tiles per wave
 
 __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk,
                                                   const int4 *__restrict__ segs,
@@ -209,52 +213,69 @@ __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int c
                                                   const double2 *__restrict__ pw,
                                                   double2 *__restrict__ skpart)
 {
+    const int lane = threadIdx.x & 63, row = lane & 15, kq = lane >> 4;
+    const int sbase = (blockIdx.x * (EB / 64) + (threadIdx.x >> 6)) * (16 * SK_MT);
     const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
-    double k1[SPT], k2[SPT], l0[SPT];
-    int4 sg[SPT];
+    double k1[SK_MT], k2[SK_MT], l0[SK_MT];
+    mfma_d4 dre[SK_MT], dim[SK_MT];
 #pragma unroll
-    for (int m = 0; m < SPT; ++m) {
-        const int sidx = (blockIdx.x * SPT + m) * EB + threadIdx.x;
-        sg[m] = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);   // length 0: no output
-        k1[m] = sg[m].x;
-        k2[m] = sg[m].y;
-        l0[m] = sg[m].z & 0xffff;
+    for (int t = 0; t < SK_MT; ++t) {
+        const int sidx = sbase + 16 * t + row;
+        const int4 sg = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);
+        k1[t] = sg.x;
+        k2[t] = sg.y;
+        l0[t] = sg.z & 0xffff;
+        dre[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        dim[t] = dre[t];
     }
-    double cr[SPT][LSEG], ci[SPT][LSEG];
+    // software pipeline: the products of step a are issued before the
+    // phases of step a + 4 are formed from operands loaded one step
+    // earlier; step a + 8 is being loaded.  Atoms past the chunk carry a
+    // zero row and add nothing.
+    const double4 zero4 = make_double4(0, 0, 0, 0);
+    const double2 zero2 = make_double2(0, 0);
+    int at = a0 + kq;
+    double2 pa = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
+    const double4 ta = at < a1 ? th[at] : zero4;
+    at += 4;
+    double4 tb = at < a1 ? th[at] : zero4;
+    double2 pb = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
+    double2 v[SK_MT];
 #pragma unroll
-    for (int m = 0; m < SPT; ++m)
+    for (int t = 0; t < SK_MT; ++t) v[t] = turn(k1[t] * ta.x + k2[t] * ta.y + l0[t] * ta.z);
+    for (int a = a0; a < a1; a += 4) {
+        at = a + 8 + kq;
+        const double4 tc = at < a1 ? th[at] : zero4;
+        const double2 pc = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
+        // the two products into one accumulator are 2 SK_MT products apart,
+        // so neither waits for the other's result
 #pragma unroll
-        for (int l = 0; l < LSEG; ++l) cr[m][l] = ci[m][l] = 0.0;
-    double4 tn = a0 < a1 ? th[a0] : make_double4(0, 0, 0, 0);
-    for (int a = a0; a < a1; ++a) {
-        const double4 t = tn;
-        tn = th[min(a + 1, a1 - 1)];   // the next atom's entry, loaded early
-        if (t.w == 0.0) continue;
-        double2 v[SPT];
-#pragma unroll
-        for (int m = 0; m < SPT; ++m) v[m] = turn(k1[m] * t.x + k2[m] * t.y + l0[m] * t.z);
-        const double2 *row = pw + (size_t)a * LSEG;
-        // every lane runs all LSEG entries (no branch between the scalar
-        // loads, so they issue together); entries past a segment's length
-        // accumulate but are not stored
-#pragma unroll
-        for (int l = 0; l < LSEG; ++l) {
-            const double2 p = row[l];
-#pragma unroll
-            for (int m = 0; m < SPT; ++m) {
-                cr[m][l] = fma(v[m].x, p.x, fma(-v[m].y, p.y, cr[m][l]));
-                ci[m][l] = fma(v[m].x, p.y, fma(v[m].y, p.x, ci[m][l]));
-            }
+        for (int t = 0; t < SK_MT; ++t) {
+            dre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t].x, pa.x, dre[t], 0, 0, 0);
+            dim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t].x, pa.y, dim[t], 0, 0, 0);
         }
-    }
 #pragma unroll
-    for (int m = 0; m < SPT; ++m) {
-        const int len = sg[m].z >> 16;
-        double2 *out = skpart + (size_t)blockIdx.y * nk + sg[m].w;
+        for (int t = 0; t < SK_MT; ++t) {
+            dre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v[t].y, pa.y, dre[t], 0, 0, 0);
+            dim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t].y, pa.x, dim[t], 0, 0, 0);
+        }
 #pragma unroll
-        for (int l = 0; l < LSEG; ++l)
-            if (l < len) out[l] = make_double2(cr[m][l], ci[m][l]);
+        for (int t = 0; t < SK_MT; ++t) v[t] = turn(k1[t] * tb.x + k2[t] * tb.y + l0[t] * tb.z);
+        pa = pb;
+        tb = tc;
+        pb = pc;
     }
+    // lane holds D[i][j] for segment row i = kq + 4 r and entry j = row
+    double2 *out = skpart + (size_t)blockIdx.y * nk;
+#pragma unroll
+    for (int t = 0; t < SK_MT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int sidx = sbase + 16 * t + kq + 4 * r;
+            if (sidx >= nseg) continue;
+            const int4 sg = segs[sidx];
+            if (row < (sg.z >> 16)) out[sg.w + row] = make_double2(dre[t][r], dim[t][r]);
+        }
 }
 
 __global__ void k_structure_sum(int nk, int nchunk, const double2 *__restrict__ skpart,
@@ -748,7 +769,7 @@ void structure_factors(System &S, EwaldState &E)
 {
     const int n = S.n, nk = E.nk;
     if (nk == 0) return;
-    int stiles = div_up(E.nseg, EB * SPT);
+    int stiles = div_up(E.nseg, (EB / 64) * 16 * SK_MT);
     int nchunk = std::max(1, std::min(div_up(n, EB), div_up(2048, stiles)));
     int chunk = div_up(n, nchunk);
     nchunk = div_up(n, chunk);
