@@ -89,6 +89,16 @@ def run_ewald(name, struct, rc, accuracy_scale=3.0, charge_self=None, scaler=Non
     print(name, len(struct), "E = %.12f" % e, "K", K)
 
 
+def sheared_ewald():
+    """Ewald in a triclinic cell: rocksalt 2x2x2 with displacements, cell
+    and positions sheared by the same matrix (same fractional coordinates)."""
+    s = st.rocksalt((2, 2, 2), sigma=0.08, seed=6)
+    shear = np.array([[1.0, 0.0, 0.0], [0.2, 1.0, 0.0], [-0.1, 0.15, 1.0]])
+    s.cell = s.cell @ shear
+    s.positions = s.positions @ shear
+    run_ewald("nacl_2x2x2_sheared_rc5", s, rc=5.0)
+
+
 def run_pairs(name, struct, model, tables=None, skin=0.5):
     """A pair-potential case: descriptor rows (engine format) in, the
     reference's energy, forces and stress out."""
@@ -151,6 +161,7 @@ def main():
     # Ewald: rocksalt with displacements
     run_ewald("nacl_2x2x2_s008_rc5", st.rocksalt((2, 2, 2), sigma=0.08, seed=3), rc=5.0)
     run_ewald("nacl_3x3x3_s008_rc7", st.rocksalt((3, 3, 3), sigma=0.08, seed=4), rc=7.0)
+    sheared_ewald()
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
     rng = np.random.default_rng(8)
@@ -163,4 +174,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if sys.argv[1:] == ["sheared_ewald"]:
+        sheared_ewald()
+    else:
+        main()

@@ -194,8 +194,9 @@ def test_madelung_energy_and_sigma_invariance(oracle):
     assert abs(e1 - e2) < 1e-5
 
 
-def test_ewald_forces_and_chi_are_gradients(oracle):
-    g = load_golden("sio2_48_eem_rc5")
+@pytest.mark.parametrize("name", ["sio2_48_eem_rc5", "nacl_2x2x2_sheared_rc5"])
+def test_ewald_forces_and_chi_are_gradients(oracle, name):
+    g = load_golden(name)
     sp, pd, ew = ewald_descriptors(g)
     pos, q, cell = g["positions"], g["charges"], g["cell"]
     nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(len(sp), 5.5))
