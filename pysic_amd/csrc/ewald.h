@@ -26,6 +26,7 @@ struct EwaldState {
     DBuf<double4> rpart;    // per-atom reciprocal partials over k chunks
     DBuf<double4> th;       // per-atom fractional coordinates and q s
     DBuf<double2> pw;       // per-atom q s exp(2 pi i l u3), 16 per atom
+    DBuf<double2> wl;       // per-atom exp(2 pi i 16 u3)
     DBuf<double4> zw;       // per-k coefficients of the per-atom sums
     DBuf<unsigned long long> hpart;   // state fingerprint partials
     // sk and th hold the structure factors of the state whose fingerprint
@@ -33,10 +34,12 @@ struct EwaldState {
     bool sk_valid = false;
     unsigned long long sk_hash = 0;
     // k-vector segments: runs of at most 16 consecutive k3 at fixed (k1, k2),
-    // {k1, k2, k3_start | len << 16, index of the first k}; natural order and
-    // sorted by length (longest first)
-    DBuf<int4> seg, seg_sorted;
-    int nseg = 0;
+    // {k1, k2, k3_start | len << 16, index of the first k}, natural order;
+    // the structure-factor kernel's work plan over them (16 x 4 segments
+    // per wave) and its continuation flags
+    DBuf<int4> seg;
+    DBuf<int> plan, pflags;
+    int nseg = 0, nwaves = 0;
     double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // reciprocal vectors 2 pi M^-1 (rows)
     double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // M^-1 (rows): fractional coordinates
     double volume = 0;

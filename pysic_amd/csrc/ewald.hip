@@ -169,13 +169,15 @@ __device__ __forceinline__ double2 turn(double u)
 }
 
 // Per-atom phase table: th[i] = (u1, u2, u3, q s), u_a = f_a . r the
-// fractional coordinates (k . r = 2 pi (k1 u1 + k2 u2 + k3 u3)), and the
-// structure-factor row pw[i][l] = q s exp(2 pi i l u3), l < LSEG.
+// fractional coordinates (k . r = 2 pi (k1 u1 + k2 u2 + k3 u3)), the
+// structure-factor row pw[i][l] = q s exp(2 pi i l u3), l < LSEG, and the
+// step between consecutive segments of a row, wl[i] = exp(2 pi i LSEG u3).
 __global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double4 *__restrict__ pos,
                                                     const double *__restrict__ q,
                                                     const double *__restrict__ sc,
                                                     double4 *__restrict__ th,
-                                                    double2 *__restrict__ pw)
+                                                    double2 *__restrict__ pw,
+                                                    double2 *__restrict__ wl)
 {
     const int i = blockIdx.x * EB + threadIdx.x;
     if (i >= n) return;
@@ -185,6 +187,7 @@ __global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double
     const double u3 = R.f[6] * p.x + R.f[7] * p.y + R.f[8] * p.z;
     const double qs = q[i] * sc[i];
     th[i] = make_double4(u1, u2, u3, qs);
+    wl[i] = turn(LSEG * u3);
     double2 *row = pw + (size_t)i * LSEG;
     for (int l = 0; l < LSEG; ++l) {
         const double2 e = turn(l * u3);
@@ -197,34 +200,43 @@ __global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double
 //   S_seg(l) = sum_a exp(2 pi i (k1 u1 + k2 u2 + l0 u3)_a) pw_a(l),
 // a complex product of the (segment x atom) phase matrix, built on the fly,
 // and the (atom x entry) table pw: FP64 MFMA 16x16x4, four real products
-// per complex one.  A wave owns SK_MT tiles of 16 segments and steps through
-// the atoms 4 at a time; lane L computes the phase of segment L % 16 and
-// atom L / 16 of each tile (the A operand) and loads entry L % 16 of that
-// atom's row (the B operand, coalesced).  Operand layout of
+// per complex one.  A wave owns SK_MT tiles of 16 segments (the work plan
+// below) and steps through the atoms 4 at a time; lane L forms the phase of
+// the segment in row L % 16 and atom L / 16 of each tile (the A operand)
+// and loads entry L % 16 of that atom's row (the B operand, coalesced).
+// When tile t holds the segments that follow tile t - 1's in the same
+// (k1, k2) rows, its phases are tile t - 1's times exp(2 pi i LSEG u3): one
+// complex multiply instead of a phase.  Operand layout of
 // v_mfma_f64_16x16x4f64: A (16 x 4) lane i + 16 k holds A[i][k]; B (4 x 16)
 // lane j + 16 k holds B[k][j]; D (16 x 16) lane j + 16 (i % 4), register
 // i / 4, holds D[i][j].
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 constexpr int SK_MT = 4;   // 16-segment tiles per wave
 
-__global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int chunk,
+__global__ __launch_bounds__(EB) void k_structure(int nwaves, int nk, int n, int chunk,
+                                                  const int *__restrict__ plan,
+                                                  const int *__restrict__ pflags,
                                                   const int4 *__restrict__ segs,
                                                   const double4 *__restrict__ th,
+                                                  const double2 *__restrict__ wl,
                                                   const double2 *__restrict__ pw,
                                                   double2 *__restrict__ skpart)
 {
+    const int wave = blockIdx.x * (EB / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wave >= nwaves) return;   // whole waves only; the kernel has no barrier
     const int lane = threadIdx.x & 63, row = lane & 15, kq = lane >> 4;
-    const int sbase = (blockIdx.x * (EB / 64) + (threadIdx.x >> 6)) * (16 * SK_MT);
+    const int *wplan = plan + (size_t)wave * (16 * SK_MT);
+    const int flags = pflags[wave];   // bit t: tile t continues tile t - 1
     const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
-    double k1[SK_MT], k2[SK_MT], l0[SK_MT];
+    // each tile's (k1, k2, l0) packed in one register: 11 + 11 signed bits
+    // and 10 bits (|k1|, |k2|, l0 < 1024, checked on the host)
+    int kp[SK_MT];
     mfma_d4 dre[SK_MT], dim[SK_MT];
 #pragma unroll
     for (int t = 0; t < SK_MT; ++t) {
-        const int sidx = sbase + 16 * t + row;
-        const int4 sg = sidx < nseg ? segs[sidx] : make_int4(0, 0, 0, 0);
-        k1[t] = sg.x;
-        k2[t] = sg.y;
-        l0[t] = sg.z & 0xffff;
+        const int sidx = wplan[t * 16 + row];
+        const int4 sg = sidx >= 0 ? segs[sidx] : make_int4(0, 0, 0, 0);
+        kp[t] = (sg.x & 0x7ff) | ((sg.y & 0x7ff) << 11) | ((sg.z & 0x3ff) << 22);
         dre[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
         dim[t] = dre[t];
     }
@@ -234,18 +246,27 @@ __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int c
     // zero row and add nothing.
     const double4 zero4 = make_double4(0, 0, 0, 0);
     const double2 zero2 = make_double2(0, 0);
+    double2 v[SK_MT];
+    auto phases = [&](const double4 &u, const double2 &w) {
+#pragma unroll
+        for (int t = 0; t < SK_MT; ++t)
+            v[t] = (t > 0 && ((flags >> t) & 1))
+                       ? cmul(v[t > 0 ? t - 1 : 0], w)
+                       : turn((double)((kp[t] << 21) >> 21) * u.x +
+                              (double)((kp[t] << 10) >> 21) * u.y +
+                              (double)((unsigned)kp[t] >> 22) * u.z);
+    };
     int at = a0 + kq;
     double2 pa = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
-    const double4 ta = at < a1 ? th[at] : zero4;
+    phases(at < a1 ? th[at] : zero4, at < a1 ? wl[at] : zero2);
     at += 4;
     double4 tb = at < a1 ? th[at] : zero4;
+    double2 wb = at < a1 ? wl[at] : zero2;
     double2 pb = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
-    double2 v[SK_MT];
-#pragma unroll
-    for (int t = 0; t < SK_MT; ++t) v[t] = turn(k1[t] * ta.x + k2[t] * ta.y + l0[t] * ta.z);
     for (int a = a0; a < a1; a += 4) {
         at = a + 8 + kq;
         const double4 tc = at < a1 ? th[at] : zero4;
+        const double2 wc = at < a1 ? wl[at] : zero2;
         const double2 pc = at < a1 ? pw[(size_t)at * LSEG + row] : zero2;
         // the two products into one accumulator are 2 SK_MT products apart,
         // so neither waits for the other's result
@@ -259,20 +280,20 @@ __global__ __launch_bounds__(EB) void k_structure(int nseg, int nk, int n, int c
             dre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v[t].y, pa.y, dre[t], 0, 0, 0);
             dim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t].y, pa.x, dim[t], 0, 0, 0);
         }
-#pragma unroll
-        for (int t = 0; t < SK_MT; ++t) v[t] = turn(k1[t] * tb.x + k2[t] * tb.y + l0[t] * tb.z);
+        phases(tb, wb);
         pa = pb;
         tb = tc;
+        wb = wc;
         pb = pc;
     }
-    // lane holds D[i][j] for segment row i = kq + 4 r and entry j = row
+    // lane holds D[i][j] for row i = kq + 4 r and entry j = row
     double2 *out = skpart + (size_t)blockIdx.y * nk;
 #pragma unroll
     for (int t = 0; t < SK_MT; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int sidx = sbase + 16 * t + kq + 4 * r;
-            if (sidx >= nseg) continue;
+            const int sidx = wplan[t * 16 + kq + 4 * r];
+            if (sidx < 0) continue;
             const int4 sg = segs[sidx];
             if (row < (sg.z >> 16)) out[sg.w + row] = make_double2(dre[t][r], dim[t][r]);
         }
@@ -651,6 +672,48 @@ __global__ void k_sum_chi(int n, const double *__restrict__ chi, double *__restr
 
 // ---- host side ----------------------------------------------------------------
 
+// Work plan of k_structure.  Chains of up to SK_MT consecutive segments of
+// one (k1, k2) row (the segments of a row are consecutive in the natural
+// order, all but the last 16 long), in groups of 16 chains of one length,
+// placed first-fit (longest first) into waves of SK_MT tiles.
+// plan[w][t][i] is the segment in row i of tile t of wave w (-1: none);
+// flags[w] bit t says tile t continues tile t - 1.
+void build_plan(const std::vector<int4> &segs, std::vector<int> &plan, std::vector<int> &flags)
+{
+    std::vector<int> first[SK_MT + 1];   // first segment of each chain, by length
+    for (size_t s = 0; s < segs.size();) {
+        size_t e = s + 1;
+        while (e < segs.size() && segs[e].x == segs[s].x && segs[e].y == segs[s].y) ++e;
+        for (size_t o = s; o < e; o += SK_MT)
+            first[std::min<size_t>(SK_MT, e - o)].push_back((int)o);
+        s = e;
+    }
+    plan.clear();
+    flags.clear();
+    std::vector<int> used;   // tiles taken in each wave
+    size_t open = 0;         // waves before this one are full
+    for (int len = SK_MT; len >= 1; --len) {
+        const std::vector<int> &c = first[len];
+        for (size_t g = 0; g < c.size(); g += 16) {
+            while (open < used.size() && used[open] == SK_MT) ++open;
+            size_t w = open;
+            while (w < used.size() && used[w] + len > SK_MT) ++w;
+            if (w == used.size()) {
+                used.push_back(0);
+                flags.push_back(0);
+                plan.resize(plan.size() + 16 * SK_MT, -1);
+            }
+            for (int j = 0; j < len; ++j) {
+                const int t = used[w] + j;
+                for (size_t i = 0; i < 16 && g + i < c.size(); ++i)
+                    plan[w * 16 * SK_MT + t * 16 + i] = c[g + i] + j;
+                if (j > 0) flags[w] |= 1 << t;
+            }
+            used[w] += len;
+        }
+    }
+}
+
 struct Consts {
     double inv_eps_4pi, inv_eps_2v, inv_sig_sqrt2, inv_sig_sqrt2pi, inv_sig_sq2;
 };
@@ -710,21 +773,25 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
                     }
                     if (!extend) segs.push_back(make_int4(k1, k2, k3 | (1 << 16), idx));
                 }
-        PYSIC_REQUIRE(P.K[2] < 0xffff, PYSIC_EINVAL, "too many reciprocal cells along k3");
+        PYSIC_REQUIRE(P.K[0] < 1024 && P.K[1] < 1024 && P.K[2] < 1024, PYSIC_EINVAL,
+                      "at most 1023 reciprocal cells along each axis");
         for (int q = 0; q < 9; ++q) {
             E.b[q] = rcl[q];
             E.f[q] = S.geom.inv[q];
         }
         E.nseg = (int)segs.size();
-        std::vector<int4> sorted = segs;
-        std::stable_sort(sorted.begin(), sorted.end(),
-                         [](const int4 &a, const int4 &b) { return (a.z >> 16) > (b.z >> 16); });
+        std::vector<int> plan, flags;
+        build_plan(segs, plan, flags);
+        E.nwaves = (int)flags.size();
         E.seg.ensure(segs.size());
-        E.seg_sorted.ensure(segs.size());
+        E.plan.ensure(plan.size());
+        E.pflags.ensure(flags.size());
         if (!segs.empty()) {
             HIP_TRY(hipMemcpyAsync(E.seg.p, segs.data(), sizeof(int4) * segs.size(),
                                    hipMemcpyHostToDevice, S.stream));
-            HIP_TRY(hipMemcpyAsync(E.seg_sorted.p, sorted.data(), sizeof(int4) * sorted.size(),
+            HIP_TRY(hipMemcpyAsync(E.plan.p, plan.data(), sizeof(int) * plan.size(),
+                                   hipMemcpyHostToDevice, S.stream));
+            HIP_TRY(hipMemcpyAsync(E.pflags.p, flags.data(), sizeof(int) * flags.size(),
                                    hipMemcpyHostToDevice, S.stream));
         }
         E.nk = (int)kv.size();
@@ -733,6 +800,7 @@ void prepare(System &S, const EwaldParams &P, EwaldState &E)
         if (!kv.empty())
             HIP_TRY(hipMemcpyAsync(E.kvec.p, kv.data(), sizeof(double4) * kv.size(),
                                    hipMemcpyHostToDevice, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));   // before the host vectors go
         E.dirty = false;
     }
     // scaler in the current atom order
@@ -769,19 +837,21 @@ void structure_factors(System &S, EwaldState &E)
 {
     const int n = S.n, nk = E.nk;
     if (nk == 0) return;
-    int stiles = div_up(E.nseg, (EB / 64) * 16 * SK_MT);
+    int stiles = div_up(E.nwaves, EB / 64);
     int nchunk = std::max(1, std::min(div_up(n, EB), div_up(2048, stiles)));
     int chunk = div_up(n, nchunk);
     nchunk = div_up(n, chunk);
     E.skpart.ensure((size_t)nchunk * nk);
     E.th.ensure(n);
     E.pw.ensure((size_t)n * LSEG);
+    E.wl.ensure(n);
     hipEvent_t ev;
     S.timer.begin("ewald_sk", S.stream, &ev);
     k_phase_table<<<div_up(n, EB), EB, 0, S.stream>>>(n, recip_of(E), S.pos.p, S.q.p, E.scaler.p,
-                                                      E.th.p, E.pw.p);
-    k_structure<<<dim3(stiles, nchunk), EB, 0, S.stream>>>(E.nseg, nk, n, chunk, E.seg_sorted.p,
-                                                          E.th.p, E.pw.p, E.skpart.p);
+                                                      E.th.p, E.pw.p, E.wl.p);
+    k_structure<<<dim3(stiles, nchunk), EB, 0, S.stream>>>(E.nwaves, nk, n, chunk, E.plan.p,
+                                                          E.pflags.p, E.seg.p, E.th.p, E.wl.p,
+                                                          E.pw.p, E.skpart.p);
     k_structure_sum<<<div_up(nk, 256), 256, 0, S.stream>>>(nk, nchunk, E.skpart.p, E.sk.p);
     HIP_TRY(hipGetLastError());
     S.timer.end("ewald_sk", S.stream, ev);
@@ -927,7 +997,9 @@ void ewald_release(EwaldState &E)
     E.skpart.release();
     E.rpart.release();
     E.seg.release();
-    E.seg_sorted.release();
+    E.plan.release();
+    E.pflags.release();
+    E.wl.release();
     E.th.release();
     E.pw.release();
     E.zw.release();
