@@ -27,7 +27,8 @@ struct EwaldState {
     DBuf<double4> th;       // per-atom fractional coordinates and q s
     DBuf<double2> pw;       // per-atom q s exp(2 pi i l u3), 16 per atom
     DBuf<double2> wl;       // per-atom exp(2 pi i 16 u3)
-    DBuf<double4> zw;       // per-k coefficients of the per-atom sums
+    DBuf<double4> zw;       // coefficients of the per-atom sums (plan order)
+    DBuf<int> kpk;          // packed (k1, k2, l0) per plan slot
     DBuf<unsigned long long> hpart;   // state fingerprint partials
     // sk and th hold the structure factors of the state whose fingerprint
     // is sk_hash

@@ -16,9 +16,9 @@
 //    exp(2 pi i l u3) the rest.  Structure factors S(k) = sum_i q_i s_i
 //    exp(i k.r_i): a (segment x atom) by (atom x 16) complex product on the
 //    FP64 matrix cores, deterministic partial sums over atom chunks.  Per-atom
-//    consumers (forces, electronegativities): thread per atom, a Horner
-//    polynomial in exp(2 pi i u3) per segment, coefficients through the
-//    scalar cache;
+//    consumers (forces, electronegativities): per segment, the polynomial
+//    sum_l z(l) exp(2 pi i l u3) of each atom, again a complex product on
+//    the matrix cores (16 atoms x 16 powers by 16 powers x 16 segments);
 //  * self and charged-background terms are per-atom closed forms.
 // The reciprocal "stress" uses absolute positions (Potentials.f90:1940-1945)
 // and the background electronegativity uses a zero-initialised charge sum
@@ -119,7 +119,6 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
 }
 
 constexpr int LSEG = 16;   // k-vectors per segment
-constexpr int SCHUNK = 32; // segment-chunk granularity of the per-atom kernel
 
 struct Recip {
     double b[9];   // reciprocal vectors in rows: k = k1 b1 + k2 b2 + k3 b3
@@ -333,126 +332,166 @@ __global__ void k_recip_energy(int nk, const double4 *__restrict__ kvec,
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Per-k coefficients of the per-atom sums (natural k order; thread per
-// segment).  Forces (mode 2): z = 2 c_k (S_sin + i S_cos), the force on atom
-// i being -q_i s_i sum_k Re(z exp(i k.r_i)) k (Potentials.f90:1895-1931);
-// w = l z with l the k's place in its segment.  Electronegativities (mode
-// 3): z = 2 c_k (S_cos - i S_sin), chi_i -= sum_k Re(z exp(i k.r_i))
-// (Potentials.f90:2196-2229).
-__global__ void k_recip_coef(int nseg, int mode, const int4 *__restrict__ segs,
-                             const double4 *__restrict__ kvec, const double2 *__restrict__ sk,
-                             double4 *__restrict__ zw)
+// Coefficients of the per-atom sums in the order of the structure-factor
+// work plan, for k_recip_mfma.  Forces (mode 2): z = 2 c_k (S_sin + i S_cos),
+// the force on atom i being -q_i s_i sum_k Re(z exp(i k.r_i)) k
+// (Potentials.f90:1895-1931).  Electronegativities (mode 3):
+// z = 2 c_k (S_cos - i S_sin), chi_i -= sum_k Re(z exp(i k.r_i))
+// (Potentials.f90:2196-2229).  zp[(T * 16 + l) * 16 + c] = (z, l z) for
+// entry l of the segment in column c of plan tile T (zero past its length
+// or for an empty column); kpk[T * 16 + c] is its (k1, k2, l0) packed as in
+// k_structure.  Thread per (tile, entry, column).
+__global__ void k_recip_coef_plan(int ntiles, int mode, const int *__restrict__ plan,
+                                  const int4 *__restrict__ segs, const double4 *__restrict__ kvec,
+                                  const double2 *__restrict__ sk, double4 *__restrict__ zp,
+                                  int *__restrict__ kpk)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    const int4 sg = segs[s];
-    const int len = sg.z >> 16;
-    for (int l = 0; l < len; ++l) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntiles * 256) return;
+    const int T = e >> 8, l = (e >> 4) & 15, c = e & 15;
+    const int sidx = plan[T * 16 + c];
+    const int4 sg = sidx >= 0 ? segs[sidx] : make_int4(0, 0, 0, 0);
+    double4 o = make_double4(0, 0, 0, 0);
+    if (l < (sg.z >> 16)) {
         const int k = sg.w + l;
         const double c2 = 2.0 * kvec[k].w;
         const double2 S = sk[k];
         const double zr = mode == 3 ? c2 * S.x : c2 * S.y;
         const double zi = mode == 3 ? -(c2 * S.y) : c2 * S.x;
-        zw[k] = make_double4(zr, zi, l * zr, l * zi);
+        o = make_double4(zr, zi, l * zr, l * zi);
     }
+    zp[e] = o;
+    if (l == 0) kpk[T * 16 + c] = (sg.x & 0x7ff) | ((sg.y & 0x7ff) << 11) | ((sg.z & 0x3ff) << 22);
 }
 
-// Per-atom reciprocal forces (MODE 2) or electronegativities (MODE 3) over
-// one chunk of segments in natural order; block (atom tile, segment chunk),
-// RAT atoms per thread.  The segment loop is wave-uniform and the
-// coefficients come through the scalar cache.  With v the phase of a
-// segment's first k and w = exp(2 pi i u3) the step along k3,
-//   sum_l Re(z_l v w^l) = Re(v P(w)),   P(w) = sum_l z_l w^l   (Horner),
-// and for forces k = k1 b1 + k2 b2 + (l0 + l) b3 splits the sum into
-// Re(v P) (k1 b1 + k2 b2 + l0 b3) + Re(v Q) b3,  Q(w) = sum_l l z_l w^l.
-// A segment that continues its row starts at v w^LSEG, so only row starts
-// take a sincospi.
-#ifndef PYSIC_EW_RAT
-#define PYSIC_EW_RAT 4
-#endif
-constexpr int RAT = PYSIC_EW_RAT;
-
+// Per-atom reciprocal forces (MODE 2) or electronegativities (MODE 3) on
+// the matrix cores.  For atom i and segment j with coefficients z_j(l),
+//   P[i][j] = sum_l w_i^l z_j(l),   Q[i][j] = sum_l w_i^l l z_j(l),
+// w_i = exp(2 pi i u3_i): a (16 atoms x 16 powers) by (16 x 16 segments)
+// complex product per tile, FP64 MFMA 16x16x4 over 4 steps of l, four real
+// products per complex one.  With v_ij the phase of segment j's first k at
+// atom i,
+//   chi_i -= sum_j Re(v P),
+//   F_i = -q_i s_i sum_j [Re(v P) (k1 b1 + k2 b2 + l0 b3) + Re(v Q) b3].
+// A wave owns 16 atoms and walks a chunk of the structure-factor work plan
+// (waves of 4 tiles of 16 segments); in a continuing tile v is the previous
+// tile's times exp(2 pi i 16 u3).  Lane L: A operand w^(4 ks + L / 16) of
+// atom L % 16; B operand coefficient 4 ks + L / 16 of the tile's segment
+// L % 16; D holds (atom L / 16 + 4 r, segment L % 16), r < 4, and the
+// per-atom sums are reduced over the 16 lanes of a row at the end.
 template <int MODE>
-__global__ __launch_bounds__(EB) void k_recip_atom(int n, int nseg, int schunk, Recip R,
-                                                   const int4 *__restrict__ segs,
-                                                   const double4 *__restrict__ zw,
+__global__ __launch_bounds__(EB) void k_recip_mfma(int n, int nwaves, int wchunk, Recip R,
+                                                   const int *__restrict__ pflags,
+                                                   const double4 *__restrict__ zp,
+                                                   const int *__restrict__ kpk,
                                                    const double4 *__restrict__ th,
+                                                   const double2 *__restrict__ wl,
                                                    double4 *__restrict__ part)
 {
-    const int i0 = blockIdx.x * (EB * RAT) + threadIdx.x;
-    const int s0 = blockIdx.y * schunk, s1 = min(nseg, s0 + schunk);
-    double u1[RAT], u2[RAT], u3[RAT], qs[RAT], g1[RAT], g2[RAT], g3[RAT];
-    double2 w[RAT], wl[RAT], v[RAT];
+    const int lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
+    const int ab = (blockIdx.x * (EB / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * 16;
+    if (ab >= n) return;   // whole waves only; the kernel has no barrier
+    const int T0 = blockIdx.y * wchunk * 4, T1 = min(nwaves, (blockIdx.y + 1) * wchunk) * 4;
+    const double4 zero4 = make_double4(0, 0, 0, 0);
+    // A operands: powers of w of atom ab + col (rows past n are dropped)
+    double2 pa[4];
+    {
+        const int i = ab + col;
+        const double u = i < n ? th[i].z : 0.0;
 #pragma unroll
-    for (int a = 0; a < RAT; ++a) {
-        const int i = i0 + a * EB;
-        const double4 t = i < n ? th[i] : make_double4(0, 0, 0, 0);
-        u1[a] = t.x;
-        u2[a] = t.y;
-        u3[a] = t.z;
-        qs[a] = t.w;
-        w[a] = turn(t.z);
-        wl[a] = turn(LSEG * t.z);
-        v[a] = make_double2(1.0, 0.0);
-        g1[a] = g2[a] = g3[a] = 0.0;
+        for (int ks = 0; ks < 4; ++ks) pa[ks] = turn((4 * ks + kq) * u);
     }
-    // (k1, k2, k3) at which the next segment would continue the last one
-    int rk1 = 0x7fffffff, rk2 = 0, rl = -1;
-    for (int s = s0; s < s1; ++s) {
-        const int4 sg = segs[s];
-        const int len = sg.z >> 16, il0 = sg.z & 0xffff;
-        const double k1 = sg.x, k2 = sg.y, l0 = il0;
-        if (sg.x == rk1 && sg.y == rk2 && il0 == rl) {
+    // the atoms of this lane's D rows
+    double u1[4], u2[4], u3[4], g1[4], g2[4], g3[4];
+    double2 stp[4], v[4];
 #pragma unroll
-            for (int a = 0; a < RAT; ++a) v[a] = cmul(v[a], wl[a]);
-        } else {
+    for (int r = 0; r < 4; ++r) {
+        const int i = ab + kq + 4 * r;
+        const double4 t = i < n ? th[i] : zero4;
+        u1[r] = t.x;
+        u2[r] = t.y;
+        u3[r] = t.z;
+        stp[r] = i < n ? wl[i] : make_double2(1.0, 0.0);
+        v[r] = make_double2(1.0, 0.0);
+        g1[r] = g2[r] = g3[r] = 0.0;
+    }
+    // the next tile's operands are loaded while this tile's products run
+    double4 cn[4];
+    int kpn = 0;
+    if (T0 < T1) {
 #pragma unroll
-            for (int a = 0; a < RAT; ++a) v[a] = turn(k1 * u1[a] + k2 * u2[a] + l0 * u3[a]);
+        for (int ks = 0; ks < 4; ++ks) cn[ks] = zp[((size_t)T0 * 16 + 4 * ks + kq) * 16 + col];
+        kpn = kpk[T0 * 16 + col];
+    }
+#pragma unroll 1
+    for (int T = T0; T < T1; ++T) {
+        double4 c[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) c[ks] = cn[ks];
+        const int kp = kpn;
+        if (T + 1 < T1) {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                cn[ks] = zp[((size_t)(T + 1) * 16 + 4 * ks + kq) * 16 + col];
+            kpn = kpk[(T + 1) * 16 + col];
         }
-        rk1 = sg.x;
-        rk2 = sg.y;
-        rl = len == LSEG ? il0 + LSEG : -1;
-        double2 P[RAT], Q[RAT];
+        mfma_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = pr, qr = pr, qi = pr;
 #pragma unroll
-        for (int a = 0; a < RAT; ++a) P[a] = Q[a] = make_double2(0.0, 0.0);
-        const double4 *z = zw + sg.w;
-#pragma unroll
-        for (int l = LSEG - 1; l >= 0; --l) {
-            if (l < len) {
-                const double4 c = z[l];
-#pragma unroll
-                for (int a = 0; a < RAT; ++a) {
-                    P[a] = make_double2(fma(P[a].x, w[a].x, fma(-P[a].y, w[a].y, c.x)),
-                                        fma(P[a].x, w[a].y, fma(P[a].y, w[a].x, c.y)));
-                    if (MODE == 2)
-                        Q[a] = make_double2(fma(Q[a].x, w[a].x, fma(-Q[a].y, w[a].y, c.z)),
-                                            fma(Q[a].x, w[a].y, fma(Q[a].y, w[a].x, c.w)));
-                }
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < RAT; ++a) {
-            const double r0 = v[a].x * P[a].x - v[a].y * P[a].y;
+        for (int ks = 0; ks < 4; ++ks) {
+            pr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].x, pr, 0, 0, 0);
+            pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].y, pi, 0, 0, 0);
             if (MODE == 2) {
-                const double r1 = v[a].x * Q[a].x - v[a].y * Q[a].y;
-                g1[a] = fma(k1, r0, g1[a]);
-                g2[a] = fma(k2, r0, g2[a]);
-                g3[a] = fma(l0, r0, g3[a]) + r1;
+                qr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].z, qr, 0, 0, 0);
+                qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].w, qi, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].y, pr, 0, 0, 0);
+            pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].x, pi, 0, 0, 0);
+            if (MODE == 2) {
+                qr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].w, qr, 0, 0, 0);
+                qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].z, qi, 0, 0, 0);
+            }
+        }
+        const double k1 = (kp << 21) >> 21, k2 = (kp << 10) >> 21, l0 = (unsigned)kp >> 22;
+        const bool cont = (pflags[T >> 2] >> (T & 3)) & 1;   // never set for tile 0 of a wave
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            v[r] = cont ? cmul(v[r], stp[r]) : turn(k1 * u1[r] + k2 * u2[r] + l0 * u3[r]);
+            const double r0 = v[r].x * pr[r] - v[r].y * pi[r];
+            if (MODE == 2) {
+                const double r1 = v[r].x * qr[r] - v[r].y * qi[r];
+                g1[r] = fma(k1, r0, g1[r]);
+                g2[r] = fma(k2, r0, g2[r]);
+                g3[r] = fma(l0, r0, g3[r]) + r1;
             } else {
-                g1[a] += r0;
+                g1[r] += r0;
             }
         }
     }
+    // sum over the 16 lanes (segment columns) of each atom row
 #pragma unroll
-    for (int a = 0; a < RAT; ++a) {
-        const int i = i0 + a * EB;
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            g1[r] += __shfl_xor(g1[r], m, 16);
+            if (MODE == 2) {
+                g2[r] += __shfl_xor(g2[r], m, 16);
+                g3[r] += __shfl_xor(g3[r], m, 16);
+            }
+        }
+    if (col != 0) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = ab + kq + 4 * r;
         if (i >= n) continue;
-        double4 o = make_double4(-g1[a], 0.0, 0.0, 0.0);
+        double4 o = make_double4(-g1[r], 0.0, 0.0, 0.0);
         if (MODE == 2) {
-            const double m = -qs[a];
-            o.x = m * (g1[a] * R.b[0] + g2[a] * R.b[3] + g3[a] * R.b[6]);
-            o.y = m * (g1[a] * R.b[1] + g2[a] * R.b[4] + g3[a] * R.b[7]);
-            o.z = m * (g1[a] * R.b[2] + g2[a] * R.b[5] + g3[a] * R.b[8]);
+            const double m = -th[i].w;
+            o.x = m * (g1[r] * R.b[0] + g2[r] * R.b[3] + g3[r] * R.b[6]);
+            o.y = m * (g1[r] * R.b[1] + g2[r] * R.b[4] + g3[r] * R.b[7]);
+            o.z = m * (g1[r] * R.b[2] + g2[r] * R.b[5] + g3[r] * R.b[8]);
         }
         part[(size_t)blockIdx.y * n + i] = o;
     }
@@ -862,10 +901,10 @@ void structure_factors(System &S, EwaldState &E)
 void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
 {
     const int n = S.n, nk = E.nk;
-    int atiles = div_up(n, EB * RAT);
-    int nchunk = std::max(1, std::min(div_up(E.nseg, SCHUNK), div_up(2048, atiles)));
-    int schunk = div_up(div_up(E.nseg, nchunk), SCHUNK) * SCHUNK;
-    nchunk = nk > 0 ? div_up(E.nseg, schunk) : 0;
+    const int atiles = div_up(n, 16 * (EB / 64));
+    int nchunk = std::max(1, std::min(E.nwaves, div_up(4096, atiles)));
+    const int wchunk = div_up(E.nwaves, nchunk);
+    nchunk = nk > 0 ? div_up(E.nwaves, wchunk) : 0;
     if (nchunk == 0) {
         if (stress6) for (int c = 0; c < 6; ++c) stress6[c] = 0.0;
         return;
@@ -874,17 +913,19 @@ void recip_atoms(System &S, EwaldState &E, int mode, double *stress6)
     DBuf<double4> &part = E.rpart;
     part.ensure((size_t)nchunk * n);
     E.part.ensure((size_t)ftiles * 8);
-    E.zw.ensure(nk);
     hipEvent_t ev;
     S.timer.begin(mode == 3 ? "ewald_chi" : "ewald_force", S.stream, &ev);
-    k_recip_coef<<<div_up(E.nseg, EB), EB, 0, S.stream>>>(E.nseg, mode, E.seg.p, E.kvec.p, E.sk.p,
-                                                          E.zw.p);
+    const int ntiles = E.nwaves * 4;
+    E.zw.ensure((size_t)ntiles * 256);
+    E.kpk.ensure((size_t)ntiles * 16);
+    k_recip_coef_plan<<<div_up(ntiles * 256, EB), EB, 0, S.stream>>>(
+        ntiles, mode, E.plan.p, E.seg.p, E.kvec.p, E.sk.p, E.zw.p, E.kpk.p);
     if (mode == 3)
-        k_recip_atom<3><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, recip_of(E),
-                                                                 E.seg.p, E.zw.p, E.th.p, part.p);
+        k_recip_mfma<3><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(
+            n, E.nwaves, wchunk, recip_of(E), E.pflags.p, E.zw.p, E.kpk.p, E.th.p, E.wl.p, part.p);
     else
-        k_recip_atom<2><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(n, E.nseg, schunk, recip_of(E),
-                                                                 E.seg.p, E.zw.p, E.th.p, part.p);
+        k_recip_mfma<2><<<dim3(atiles, nchunk), EB, 0, S.stream>>>(
+            n, E.nwaves, wchunk, recip_of(E), E.pflags.p, E.zw.p, E.kpk.p, E.th.p, E.wl.p, part.p);
     k_recip_finish<<<ftiles, EB, 0, S.stream>>>(n, nchunk, mode, part.p, S.pos.p, S.frc.p, E.chi.p,
                                                 E.part.p);
     HIP_TRY(hipGetLastError());
@@ -1003,6 +1044,7 @@ void ewald_release(EwaldState &E)
     E.th.release();
     E.pw.release();
     E.zw.release();
+    E.kpk.release();
     E.hpart.release();
     E.sk_valid = false;
 }
