@@ -212,6 +212,7 @@ __global__ __launch_bounds__(EB) void k_phase_table(int n, Recip R, const double
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 constexpr int SK_MT = 4;   // 16-segment tiles per wave
 
+
 __global__ __launch_bounds__(EB) void k_structure(int nwaves, int nk, int n, int chunk,
                                                   const int *__restrict__ plan,
                                                   const int *__restrict__ pflags,
@@ -401,6 +402,13 @@ __global__ __launch_bounds__(EB) void k_recip_mfma(int n, int nwaves, int wchunk
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) pa[ks] = turn((4 * ks + kq) * u);
     }
+    // electronegativities (P only) take three real products per complex
+    // one: at 2 waves per SIMD either way, it is the faster form there
+    // (forces, with Q as well, would drop to 1 wave per SIMD)
+    constexpr bool GAUSS = MODE == 3;
+    double asum[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) asum[ks] = pa[ks].x + pa[ks].y;
     // the atoms of this lane's D rows
     double u1[4], u2[4], u3[4], g1[4], g2[4], g3[4];
     double2 stp[4], v[4];
@@ -436,22 +444,39 @@ __global__ __launch_bounds__(EB) void k_recip_mfma(int n, int nwaves, int wchunk
             kpn = kpk[(T + 1) * 16 + col];
         }
         mfma_d4 pr = {0.0, 0.0, 0.0, 0.0}, pi = pr, qr = pr, qi = pr;
+        if (GAUSS) {
+            // pr, pi, ps: Re A Re B, Im A Im B, (Re A + Im A)(Re B + Im B)
+            mfma_d4 ps = pr;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            pr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].x, pr, 0, 0, 0);
-            pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].y, pi, 0, 0, 0);
-            if (MODE == 2) {
-                qr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].z, qr, 0, 0, 0);
-                qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].w, qi, 0, 0, 0);
+            for (int ks = 0; ks < 4; ++ks) {
+                pr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].x, pr, 0, 0, 0);
+                pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].y, pi, 0, 0, 0);
+                ps = __builtin_amdgcn_mfma_f64_16x16x4f64(asum[ks], c[ks].x + c[ks].y, ps, 0, 0, 0);
             }
-        }
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].y, pr, 0, 0, 0);
-            pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].x, pi, 0, 0, 0);
-            if (MODE == 2) {
-                qr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].w, qr, 0, 0, 0);
-                qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].z, qi, 0, 0, 0);
+            for (int r = 0; r < 4; ++r) {
+                const double a = pr[r], b = pi[r];
+                pr[r] = a - b;
+                pi[r] = ps[r] - a - b;
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                pr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].x, pr, 0, 0, 0);
+                pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].y, pi, 0, 0, 0);
+                if (MODE == 2) {
+                    qr = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].z, qr, 0, 0, 0);
+                    qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].x, c[ks].w, qi, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].y, pr, 0, 0, 0);
+                pi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].x, pi, 0, 0, 0);
+                if (MODE == 2) {
+                    qr = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[ks].y, c[ks].w, qr, 0, 0, 0);
+                    qi = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks].y, c[ks].z, qi, 0, 0, 0);
+                }
             }
         }
         const double k1 = (kp << 21) >> 21, k2 = (kp << 10) >> 21, l0 = (unsigned)kp >> 22;
