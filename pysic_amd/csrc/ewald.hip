@@ -33,6 +33,10 @@ namespace {
 constexpr int EB = 256;
 
 // Real-space Ewald + charge_self singlets + self/background chi, per atom.
+// G lanes share an atom (G divides 64): lane g takes list entries g, g + G,
+// ...; the per-atom sums are reduced over the G lanes with shuffles, so
+// small systems still fill the GPU.
+template <int G>
 __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *__restrict__ pos,
                                              const double *__restrict__ q,
                                              const double *__restrict__ sc,
@@ -47,7 +51,8 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
                                              double4 *__restrict__ frc, double *__restrict__ part)
 {
     __shared__ double red[EB][7];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / G, sub = gid % G;
     double e = 0.0, chi = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     double w[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
         int si = (int)pi.w;
         double qi = q[i];
         // charge_self singlets: E = eps q^n, chi = -n eps q^(n-1)
-        if (M->has_self) {
+        if (M->has_self && sub == 0) {
             double qp = 1.0;  // q^(nn-1)
             e += M->self_eps[si][0];
             for (int nn = 1; nn <= 4; ++nn) {
@@ -66,13 +71,15 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
             }
         }
         if (ewald_on) {
-            double qs = qi * sc[i];
-            // self (Potentials.f90:2126) and background (2166) electronegativity
-            const double bg = bg_dev ? bg_dev[0] : bg_chi;
-            chi += 2.0 * qs * inv_eps_4pi * inv_sig_sqrt2pi + sc[i] * bg;
+            if (sub == 0) {
+                // self (Potentials.f90:2126) and background (2166) electronegativity
+                const double qs = qi * sc[i];
+                const double bg = bg_dev ? bg_dev[0] : bg_chi;
+                chi += 2.0 * qs * inv_eps_4pi * inv_sig_sqrt2pi + sc[i] * bg;
+            }
             const double inv2perpi = 2.0 * inv_sig_sqrt2pi;
             int nf = ncnt[i];
-            for (int t = 0; t < nf; ++t) {
+            for (int t = sub; t < nf; t += G) {
                 int j = nbr[(size_t)t * n + i];
                 int im = nimg[(size_t)t * n + i];
                 double3 sp3 = separation64(pi, pos[j], im, g.v);
@@ -100,13 +107,29 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
                 }
             }
         }
+    }
+    // per-atom sums over the G lanes of an atom (all lanes of a wave take
+    // part: G divides 64 and the lanes past n carry zeros)
+    if (G > 1) {
+#pragma unroll
+        for (int m = 1; m < G; m <<= 1) {
+            if (mode == 2) {
+                fx += __shfl_xor(fx, m, G);
+                fy += __shfl_xor(fy, m, G);
+                fz += __shfl_xor(fz, m, G);
+            }
+            if (mode == 3) chi += __shfl_xor(chi, m, G);
+        }
+    }
+    if (i < n && sub == 0) {
         if (mode == 2) {
             double4 f = frc[i];
             frc[i] = make_double4(f.x + fx, f.y + fy, f.z + fz, 0.0);
         }
         if (mode == 3) out1[i] = chi;
     }
-    // block partials: energy (mode 1) or the 6 virial components (mode 2)
+    // block partials: energy (mode 1) or the 6 virial components (mode 2),
+    // summed over every lane
     red[threadIdx.x][0] = (i < n) ? e : 0.0;
     for (int c = 0; c < 6; ++c) red[threadIdx.x][1 + c] = (i < n) ? w[c] : 0.0;
     __syncthreads();
@@ -1007,15 +1030,25 @@ void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double 
 {
     const int n = S.n;
     Consts c = consts_of(P, E.volume);
-    int nb = div_up(n, EB);
+    // lanes per atom: enough atoms x lanes for about 2 waves per SIMD
+    const int lanes = n >= 131072 ? 1 : n >= 65536 ? 2 : n >= 32768 ? 4 : n >= 16384 ? 8 : 16;
+    int nb = div_up(n * lanes, EB);
     E.part.ensure((size_t)nb * 8);
     hipEvent_t ev;
     S.timer.begin("ewald_real", S.stream, &ev);
-    k_real<<<nb, EB, 0, S.stream>>>(n, S.geom, S.pos.p, S.q.p, E.scaler.p, S.nbr.p, S.nimg.p, S.ncnt.p,
-                                    S.dmodel.p, P.on, P.rc, c.inv_eps_4pi, c.inv_sig_sqrt2,
-                                    c.inv_sig_sqrt2pi, c.inv_sig_sq2, bg_chi, bg_dev, mode, E.chi.p,
-                                    S.frc.p,
-                                    E.part.p);
+    auto launch = [&](auto kern) {
+        kern<<<nb, EB, 0, S.stream>>>(n, S.geom, S.pos.p, S.q.p, E.scaler.p, S.nbr.p, S.nimg.p,
+                                      S.ncnt.p, S.dmodel.p, P.on, P.rc, c.inv_eps_4pi,
+                                      c.inv_sig_sqrt2, c.inv_sig_sqrt2pi, c.inv_sig_sq2, bg_chi,
+                                      bg_dev, mode, E.chi.p, S.frc.p, E.part.p);
+    };
+    switch (lanes) {
+    case 1: launch(k_real<1>); break;
+    case 2: launch(k_real<2>); break;
+    case 4: launch(k_real<4>); break;
+    case 8: launch(k_real<8>); break;
+    default: launch(k_real<16>); break;
+    }
     HIP_TRY(hipGetLastError());
     S.timer.end("ewald_real", S.stream, ev);
     if (out7) {
