@@ -77,7 +77,8 @@ int pysic_set_potential_table(pysic_ctx *h, int id, int nrows, const double *row
  * + allocate_bond_order_storage (402) */
 int pysic_set_bond_order_factors(pysic_ctx *h, int nbond, const double *desc);
 /* set_ewald_parameters (PyInterface.f90:1039); scaler may be NULL; on = 0
- * switches the Coulomb summation off */
+ * switches the Coulomb summation off.  kcells[a] (the reciprocal cells per
+ * axis, K_a) must lie in 0..1023, else PYSIC_EINVAL. */
 int pysic_set_ewald(pysic_ctx *h, int on, double real_cutoff, double k_cutoff,
                     const int *kcells, double sigma, double epsilon, const double *scaler);
 

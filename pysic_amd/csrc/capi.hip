@@ -887,10 +887,15 @@ int pysic_set_ewald(pysic_ctx *h, int on, double real_cutoff, double k_cutoff, c
 {
     API_BEGIN(h)
     EwaldParams &p = h->ewald;
-    p.on = on ? 1 : 0;
-    if (p.on) {
+    if (on) {   // checked before anything changes
         PYSIC_REQUIRE(kcells && sigma > 0 && epsilon > 0 && real_cutoff > 0 && k_cutoff > 0,
                       PYSIC_EINVAL, "set_ewald: bad parameters");
+        for (int a = 0; a < 3; ++a)
+            PYSIC_REQUIRE(kcells[a] >= 0 && kcells[a] < 1024, PYSIC_EINVAL,
+                          "set_ewald: at most 1023 reciprocal cells per axis");
+    }
+    p.on = on ? 1 : 0;
+    if (p.on) {
         p.rc = real_cutoff;
         p.kc = k_cutoff;
         for (int a = 0; a < 3; ++a) p.K[a] = kcells[a];

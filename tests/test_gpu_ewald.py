@@ -126,6 +126,15 @@ def test_structure_factors_follow_the_state(eng, oracle):
     agree("cell strained")
 
 
+def test_set_ewald_rejects_oversized_kcells(eng):
+    from pysic_amd.engine import PysicEngineError
+    s = st.rocksalt((1, 1, 1))
+    eng.create_atoms((s.charges < 0).astype(np.int32), s.positions, charges=s.charges)
+    eng.create_cell(s.cell, (True, True, True))
+    with pytest.raises(PysicEngineError, match="reciprocal cells"):
+        eng.set_ewald(5.0, 2.0, [4, 1024, 4], 1.5, 0.00552635)
+
+
 def test_madelung(eng, oracle):
     s = st.rocksalt((4, 4, 4))
     rc, sigma = 10.0, 2.04430
