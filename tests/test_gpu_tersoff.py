@@ -46,14 +46,14 @@ def pair_set(counts, idx, off):
     return rows[np.lexsort(rows.T[::-1])]
 
 
-def check_tersoff(eng, oracle, pos, cell, pbc, pd, bd, cutoffs, ref=None):
+def check_tersoff(eng, oracle, pos, cell, pbc, pd, bd, cutoffs, ref=None, species=None):
     n = len(pos)
     print("case: n=%d" % n, flush=True)
-    load_system(eng, pos, cell, pbc, pd, bd, cutoffs)
+    load_system(eng, pos, cell, pbc, pd, bd, cutoffs, species=species)
     nl = oracle.neighbor_list(pos, cell, pbc, cutoffs)
     counts, idx, off = eng.neighbor_lists()
     assert np.array_equal(pair_set(counts, idx, off), nl.pair_set())
-    sp = np.zeros(n, np.int32)
+    sp = np.zeros(n, np.int32) if species is None else species
     if ref is None:
         e_ref = oracle.evaluate("energy", sp, pos, None, cell, nl, pd, bd)
         f_ref, s_ref = oracle.evaluate("forces", sp, pos, None, cell, nl, pd, bd)
@@ -77,6 +77,21 @@ def test_golden_cases(eng, oracle, name):
     pd, bd = tersoff_descriptors(g["tersoff"])
     ref = (float(g["energy"]), g["forces"], g["stress"])
     check_tersoff(eng, oracle, g["positions"], g["cell"], g["pbc"], pd, bd, g["cutoffs"], ref)
+
+
+def test_mixed_species_golden(eng, oracle):
+    """Si/C Tersoff (8 triplet factors, Si-Si, Si-C and C-C pair terms)
+    against the reference core, in every site-kernel shape."""
+    g = load_golden("sic_mixed_192")
+    names = sorted(set(str(x) for x in g["symbols"]))
+    sp = np.array([names.index(str(x)) for x in g["symbols"]], np.int32)
+    ref = (float(g["energy"]), g["forces"], g["stress"])
+    for v in range(len(eng.variants())):
+        eng.set_variant(v)
+        print("  variant %s" % eng.variants()[v], flush=True)
+        check_tersoff(eng, oracle, g["positions"], g["cell"], g["pbc"], g["potdesc"],
+                      g["bonddesc"], g["cutoffs"], ref, species=sp)
+    eng.set_variant(0)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])

@@ -108,6 +108,45 @@ def test_tersoff_stress_is_strain_derivative(oracle):
         assert abs(-dE - s[q]) < 1e-5 * max(1.0, np.abs(s).max())
 
 
+def _sic(O):
+    g = load_golden("sic_mixed_192")
+    names = sorted(set(str(x) for x in g["symbols"]))
+    sp = np.array([names.index(str(x)) for x in g["symbols"]], np.int32)
+    nl = O.neighbor_list(g["positions"], g["cell"], g["pbc"], g["cutoffs"])
+    return g, sp, nl
+
+
+def test_mixed_species_tersoff_matches_reference_core(oracle):
+    """Si/C Tersoff (8 triplet factors, 3 pair types) against the reference
+    Fortran core."""
+    g, sp, nl = _sic(oracle)
+    args = (sp, g["positions"], None, g["cell"], nl, g["potdesc"], g["bonddesc"])
+    e = oracle.evaluate("energy", *args)
+    f, s = oracle.evaluate("forces", *args)
+    assert abs(e - float(g["energy"])) <= 1e-11 * abs(float(g["energy"]))
+    assert rel_maxnorm(f, g["forces"]) <= 1e-11
+    assert np.abs(s - g["stress"]).max() <= 1e-10 * np.abs(g["stress"]).max()
+
+
+def test_mixed_species_tersoff_forces_are_gradient(oracle):
+    g, sp, nl = _sic(oracle)
+    pos, cell = g["positions"], g["cell"]
+    pd, bd = g["potdesc"], g["bonddesc"]
+    f, _ = oracle.evaluate("forces", sp, pos, None, cell, nl, pd, bd)
+    h = 1e-5
+    picks = [int(np.flatnonzero(sp == 0)[0]), int(np.flatnonzero(sp == 0)[3]),
+             int(np.flatnonzero(sp == 1)[0]), int(np.flatnonzero(sp == 1)[5])]
+    for i in picks:
+        for c in range(3):
+            ep = []
+            for sgn in (1, -1):
+                p = pos.copy()
+                p[i, c] += sgn * h
+                ep.append(oracle.evaluate("energy", sp, p, None, cell, nl, pd, bd))
+            fd = -(ep[0] - ep[1]) / (2 * h)
+            assert abs(fd - f[i, c]) < 1e-6 * max(1.0, abs(f[i, c])), (i, c, fd, f[i, c])
+
+
 def _brute_force_pairs(pos, cell, pbc, rc):
     """All (i, j, o) with |r_j + o.V - r_i| < rc by direct image enumeration."""
     rows = []
