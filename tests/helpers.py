@@ -1,7 +1,10 @@
 """Shared set-up for the parity tests: descriptors for the golden cases."""
+import math
+
 import numpy as np
 
-from pysic_amd.interactions import (POTENTIAL_CODES, POTENTIAL_TYPES, TERSOFF_SI, Potential,
+from pysic_amd.interactions import (POTENTIAL_CODES, POTENTIAL_TYPES, TERSOFF_SI,
+                                    BondOrderParameters, Coordinator, Potential,
                                     ProductPotential, build_descriptors, tersoff_si)
 
 
@@ -89,3 +92,45 @@ def ref_entries(pd, names):
         syms = tuple(names[int(e)] for e in r[2:2 + nt])
         out.append((kind, params, float(r[6]), float(r[7]), syms, int(r[8]), []))
     return out
+
+
+# Two-species Tersoff (Si, C; J. Tersoff, PRB 39, 5566 (1989) element
+# parameters).  Pair terms mix as A = sqrt(A_i A_j), lambda = (l_i + l_j)/2,
+# R = sqrt(R_i R_j), S = sqrt(S_i S_j); the bond-order factor of triplet
+# (i, j, k) takes centre i's beta, eta, c, d, h and the (i, k) cutoffs.
+SIC = {"Si": dict(A=1830.8, l1=2.4799, B=471.18, l2=1.7322, beta=1.1e-6, n=0.78734,
+                  c=100390.0, d=16.217, h=-0.59825, R=2.7, S=3.0),
+       "C": dict(A=1393.6, l1=3.4879, B=346.74, l2=2.2119, beta=1.5724e-7, n=0.72751,
+                 c=38049.0, d=4.3484, h=-0.57058, R=1.8, S=2.1)}
+
+
+def sic_model():
+    els = ["C", "Si"]
+
+    def rs(i, j):
+        return (math.sqrt(SIC[i]["R"] * SIC[j]["R"]), math.sqrt(SIC[i]["S"] * SIC[j]["S"]))
+
+    bonds = []
+    for i in els:
+        p = SIC[i]
+        for j in els:
+            for k in els:
+                R, S = rs(i, k)
+                bonds.append(BondOrderParameters(
+                    "tersoff", cutoff=S, cutoff_margin=S - R,
+                    parameters=[[p["beta"], p["n"]], [0.0, p["c"], p["d"], p["h"], 3.0]],
+                    symbols=[[i, j, k]]))
+    coord = Coordinator(bonds)
+    pots = []
+    for a, i in enumerate(els):
+        for j in els[a:]:
+            R, S = rs(i, j)
+            A = math.sqrt(SIC[i]["A"] * SIC[j]["A"])
+            B = math.sqrt(SIC[i]["B"] * SIC[j]["B"])
+            l1 = 0.5 * (SIC[i]["l1"] + SIC[j]["l1"])
+            l2 = 0.5 * (SIC[i]["l2"] + SIC[j]["l2"])
+            pots.append(Potential("exponential", symbols=[[i, j]], parameters=[A, l1],
+                                  cutoff=S, cutoff_margin=S - R))
+            pots.append(Potential("exponential", symbols=[[i, j]], parameters=[-B, l2],
+                                  cutoff=S, cutoff_margin=S - R, coordinator=coord))
+    return pots

@@ -13,6 +13,7 @@ import pytest
 from helpers import rel_maxnorm, tersoff_descriptors
 from pysic_amd import tersoff_si
 from pysic_amd.batch import BatchedCells, batched_amorphous_si
+from pysic_amd.interactions import build_descriptors
 
 pytestmark = pytest.mark.gpu
 
@@ -107,3 +108,40 @@ def test_autotuner_picks_a_working_shape(batch):
     print("  autotune:", info, flush=True)
     batch.evaluate(use_graph=True)
     batch.results()
+
+
+def test_mixed_species_cells(oracle):
+    """Two Si/C Tersoff cells in one batch: the golden cell against the
+    reference core, a second, smaller one against the oracle.  The virial is
+    held to the forces' relative bound, max|dW| / max|W| <= 1e-5: in these
+    random Si/C cells W reaches ~250 eV (2.5 eV/atom, forces up to 32 eV/A),
+    and the FP32 three-body terms leave ~1e-6 of it."""
+    from conftest import load_golden
+    from helpers import sic_model
+    from pysic_amd import structures as st
+    g = load_golden("sic_mixed_192")
+    s0 = st.Structure([str(x) for x in g["symbols"]], g["positions"], g["cell"])
+    s1 = st.random_mixture(100, (100 / 0.05) ** (1.0 / 3.0), 1.7, ["Si", "C", "Si"], seed=22)
+    b = BatchedCells([s0, s1], sic_model())
+    try:
+        b.evaluate()
+        E, W, F = b.results()
+        names = sorted(b.spmap, key=b.spmap.get)
+        pd, bd, _ = build_descriptors(sic_model(), b.spmap)
+        for c, s in enumerate([s0, s1]):
+            sl = b.cell_slice(c)
+            n = len(s)
+            sp = np.array([names.index(x) for x in s.symbols], np.int32)
+            if c == 0:
+                e_ref, f_ref, w_ref = float(g["energy"]), g["forces"], g["stress"]
+            else:
+                nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, b.list_radius))
+                e_ref = oracle.evaluate("energy", sp, s.positions, None, s.cell, nl, pd, bd)
+                f_ref, w_ref = oracle.evaluate("forces", sp, s.positions, None, s.cell, nl, pd, bd)
+            de, df = abs(E[c] - e_ref) / n, rel_maxnorm(F[sl], f_ref, 1e-2)
+            dw = rel_maxnorm(W[c], w_ref)
+            print("  cell %d n=%d |dE|/N %.2e rel|dF| %.2e rel|dW| %.2e" % (c, n, de, df, dw),
+                  flush=True)
+            assert de <= 1e-7 and df <= 1e-5 and dw <= 1e-5
+    finally:
+        b.close()
