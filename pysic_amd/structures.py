@@ -17,6 +17,17 @@ AMU_AR = 39.948
 AMU_KR = 83.798
 AVOGADRO = 6.02214076e23
 
+# standard atomic weights (amu) of the elements a structure may name without
+# explicit masses
+ATOMIC_MASS = {
+    "H": 1.008, "He": 4.002602, "Li": 6.94, "Be": 9.0121831, "B": 10.81, "C": 12.011,
+    "N": 14.007, "O": AMU_O, "F": 18.998403163, "Ne": 20.1797, "Na": AMU_NA, "Mg": 24.305,
+    "Al": 26.9815385, "Si": AMU_SI, "P": 30.973761998, "S": 32.06, "Cl": AMU_CL,
+    "Ar": AMU_AR, "K": 39.0983, "Ca": 40.078, "Ti": 47.867, "Fe": 55.845, "Ni": 58.6934,
+    "Cu": 63.546, "Zn": 65.38, "Ga": 69.723, "Ge": 72.630, "As": 74.921595, "Se": 78.971,
+    "Br": 79.904, "Kr": AMU_KR, "Ag": 107.8682, "Sn": 118.710, "Au": 196.966569,
+}
+
 FCC_BASIS = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
 DIAMOND_BASIS = np.array([
     [0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],
@@ -43,9 +54,10 @@ class Structure:
         self.pbc = np.array([bool(p) for p in pbc])
         n = len(self.positions)
         if masses is None:
-            table = {"Si": AMU_SI, "O": AMU_O, "Na": AMU_NA, "Cl": AMU_CL, "Ar": AMU_AR,
-                     "Kr": AMU_KR}
-            masses = [table.get(s, 1.0) for s in self.symbols]
+            missing = sorted(set(s for s in self.symbols if s not in ATOMIC_MASS))
+            if missing:
+                raise ValueError("no standard mass for %s: pass masses" % ", ".join(missing))
+            masses = [ATOMIC_MASS[s] for s in self.symbols]
         self.masses = np.array(masses, dtype=np.float64)
         self.charges = np.zeros(n) if charges is None else np.array(charges, dtype=np.float64)
         self.momenta = np.zeros((n, 3)) if momenta is None else np.array(momenta, dtype=np.float64)
