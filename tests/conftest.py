@@ -34,4 +34,4 @@ def load_golden(name):
 TERSOFF_CASES = ["si_diamond_2x2x2_s005", "si_diamond_4x4x4_s005", "si_diamond_3x3x3_perfect",
                  "si_amorphous_216", "si_amorphous_128_a13", "si_slab_open_yz"]
 EWALD_CASES = ["nacl_2x2x2_s008_rc5", "nacl_3x3x3_s008_rc7", "sio2_48_eem_rc5",
-               "nacl_2x2x2_sheared_rc5"]
+               "nacl_2x2x2_sheared_rc5", "nacl_4x2x1_scaled_rc5"]

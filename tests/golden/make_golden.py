@@ -99,6 +99,15 @@ def sheared_ewald():
     run_ewald("nacl_2x2x2_sheared_rc5", s, rc=5.0)
 
 
+def scaled_ewald():
+    """Ewald in an anisotropic cell (K = 6, 3, 2) with per-atom charge
+    scalers uniform in [0.5, 1.5]: the scaled charges are not neutral, so the
+    background term enters too."""
+    s = st.rocksalt((4, 2, 1), sigma=0.08, seed=9)
+    sc = np.random.default_rng(10).uniform(0.5, 1.5, len(s))
+    run_ewald("nacl_4x2x1_scaled_rc5", s, rc=5.0, scaler=sc)
+
+
 def sic_mixed():
     """Mixed Si/C Tersoff: 192 atoms (2:1 Si:C) by random addition with a
     1.7 A minimum separation at 0.05 atoms/A^3, so Si-Si, Si-C and C-C pairs
@@ -184,6 +193,7 @@ def main():
     run_ewald("nacl_2x2x2_s008_rc5", st.rocksalt((2, 2, 2), sigma=0.08, seed=3), rc=5.0)
     run_ewald("nacl_3x3x3_s008_rc7", st.rocksalt((3, 3, 3), sigma=0.08, seed=4), rc=7.0)
     sheared_ewald()
+    scaled_ewald()
     sic_mixed()
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
@@ -202,5 +212,7 @@ if __name__ == "__main__":
         sheared_ewald()
     elif sys.argv[1:] == ["sic_mixed"]:
         sic_mixed()
+    elif sys.argv[1:] == ["scaled_ewald"]:
+        scaled_ewald()
     else:
         main()

@@ -32,26 +32,27 @@ def kcells(cell, kc):
             for a in range(3)]
 
 
-def load_ewald(eng, species, pos, q, cell, pd, ew):
+def load_ewald(eng, species, pos, q, cell, pd, ew, scaler=None):
     n = len(pos)
     eng.create_atoms(species, pos, charges=q)
     eng.create_cell(cell, (True, True, True))
     eng.set_potentials(pd)
     eng.set_bond_order_factors(np.zeros((0, 14)))
     _, rc, kc, k1, k2, k3, sigma, eps = ew
-    eng.set_ewald(rc, kc, [int(k1), int(k2), int(k3)], sigma, eps)
+    eng.set_ewald(rc, kc, [int(k1), int(k2), int(k3)], sigma, eps, scaler=scaler)
     eng.generate_neighbor_lists(np.full(n, rc + 0.5))
 
 
-def check(eng, oracle, species, pos, q, cell, pd, ew, ref=None):
+def check(eng, oracle, species, pos, q, cell, pd, ew, ref=None, scaler=None):
     n = len(pos)
     print("case: n=%d" % n, flush=True)
-    load_ewald(eng, species, pos, q, cell, pd, ew)
+    load_ewald(eng, species, pos, q, cell, pd, ew, scaler)
     if ref is None:
         nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(n, ew[1] + 0.5))
-        e_ref = oracle.evaluate("energy", species, pos, q, cell, nl, pd, None, ew)
-        f_ref, s_ref = oracle.evaluate("forces", species, pos, q, cell, nl, pd, None, ew)
-        x_ref = oracle.evaluate("enegs", species, pos, q, cell, nl, pd, None, ew)
+        args = (species, pos, q, cell, nl, pd, None, ew, scaler)
+        e_ref = oracle.evaluate("energy", *args)
+        f_ref, s_ref = oracle.evaluate("forces", *args)
+        x_ref = oracle.evaluate("enegs", *args)
     else:
         e_ref, f_ref, s_ref, x_ref = ref
     e = eng.energy()
@@ -71,7 +72,7 @@ def test_golden_cases(eng, oracle, name):
     g = load_golden(name)
     sp, pd, ew = ewald_descriptors(g)
     ref = (float(g["energy"]), g["forces"], g["stress"], g["enegs"])
-    check(eng, oracle, sp, g["positions"], g["charges"], g["cell"], pd, ew, ref)
+    check(eng, oracle, sp, g["positions"], g["charges"], g["cell"], pd, ew, ref, g["scaler"])
 
 
 def test_rocksalt_8x8x8_rc10(eng, oracle):

@@ -198,9 +198,11 @@ def test_ewald_matches_reference_core(oracle, name):
     sp, pd, ew = ewald_descriptors(g)
     rc = float(g["ewald"][0])
     nl = oracle.neighbor_list(g["positions"], g["cell"], [1, 1, 1], np.full(len(sp), rc + 0.5))
-    e = oracle.evaluate("energy", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
-    f, s = oracle.evaluate("forces", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
-    x = oracle.evaluate("enegs", sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew)
+    sc = g["scaler"]
+    args = (sp, g["positions"], g["charges"], g["cell"], nl, pd, None, ew, sc)
+    e = oracle.evaluate("energy", *args)
+    f, s = oracle.evaluate("forces", *args)
+    x = oracle.evaluate("enegs", *args)
     assert abs(e - float(g["energy"])) <= 1e-11 * max(1.0, abs(float(g["energy"])))
     assert rel_maxnorm(f, g["forces"]) < 1e-11
     assert np.abs(s - g["stress"]).max() < 1e-10 * max(1.0, np.abs(g["stress"]).max())
