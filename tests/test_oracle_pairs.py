@@ -161,3 +161,40 @@ def test_reference_core_agrees_on_a_fresh_mixture(oracle):
     fr, sr = R.forces()
     assert abs(e - R.energy()) <= 1e-12 * abs(e)
     assert rel_maxnorm(f, fr) <= 1e-12 and rel_maxnorm(st6, sr) <= 1e-12
+
+
+def test_lj_rebuild_interval_keeps_every_pair(oracle):
+    """The LJ workload's list policy (0.5 A skin, rebuilt every 10 steps of
+    2 fs, fcc argon at 80 K): velocity Verlet on the oracle, 864 atoms, 30
+    steps.  No atom moves more than skin/2 between rebuilds, and no pair
+    inside the cutoff is ever missing from the current list."""
+    s = st.fcc((6, 6, 6), a=5.26, sigma=0.05, seed=0)
+    st.maxwell_boltzmann(s, 80.0, seed=1)
+    n, rc, skin, every = len(s), 2.5 * ARGON_LJ["sigma"], 0.5, 10
+    pd, _, _ = build_descriptors(lj_argon(cutoff=rc), {"Ar": 0})
+    sp, m, box = np.zeros(n, np.int32), s.masses[:, None], s.cell[0, 0]
+    dt = 2.0 * st.ASE_FS
+
+    def within(r, d):
+        dx = r[:, None, :] - r[None, :, :]
+        dx -= box * np.round(dx / box)
+        d2 = (dx ** 2).sum(-1)
+        np.fill_diagonal(d2, np.inf)
+        return d2 < d * d
+
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], np.full(n, rc + skin))
+    listed, r_ref = within(r, rc + skin), r.copy()
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd)
+    worst = 0.0
+    for step in range(1, 31):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        if step % every == 0:
+            nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], np.full(n, rc + skin))
+            listed, r_ref = within(r, rc + skin), r.copy()
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd)
+        p = p + 0.5 * dt * f
+        worst = max(worst, float(np.sqrt(((r - r_ref) ** 2).sum(1)).max()))
+        assert not (within(r, rc) & ~listed).any(), step
+    assert worst < 0.5 * skin
