@@ -109,7 +109,9 @@ int pysic_relax_charges(pysic_ctx *h, int n, double *charges, double *rates, int
 /* ASE VelocityVerlet (usage at doc/run examples.rst:177-268) run on the
  * device for nsteps with the neighbor list rebuilt every `rebuild` steps;
  * energies (nsteps, may be NULL) receives the potential energy after each
- * step. */
+ * step.  Between rebuilds the lists are not checked: the caller picks an
+ * interval over which no atom moves more than half the skin (the list
+ * radius minus the largest cutoff).  rebuild = 1 is always exact. */
 int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energies);
 /* positions and momenta back in original order (after pysic_md_run) */
 int pysic_get_state(pysic_ctx *h, int n, double *positions, double *momenta);
