@@ -215,6 +215,10 @@ def _label_code(sym):
     return out[:2]
 
 
+def _labels(syms):
+    return np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
+
+
 class RefCore:
     """The reference core, driven in the order of calculator.py:1595-1637.
 
@@ -233,6 +237,10 @@ class RefCore:
             R.ref_add_potential.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                             ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int,
                                             ctypes.c_int]
+            R.ref_add_potential_orig.restype = ctypes.c_int
+            R.ref_add_potential_orig.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                                 ctypes.c_double, ctypes.c_double, _ip, _ip,
+                                                 ctypes.c_int, ctypes.c_int]
             R.ref_add_tersoff.restype = ctypes.c_int
             R.ref_add_tersoff.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _ip,
                                           ctypes.c_int]
@@ -260,9 +268,11 @@ class RefCore:
               tersoff=(), ngroups=0, cutoffs=None, ewald=None, scaler=None, tables=None,
               momenta=None):
         """potentials: list of (kind, params, cutoff, soft, symbols_tuple, group
-        [, multipliers]) with symbols in target order (all permutations already
-        expanded) and multipliers a list of (kind, params) in the order the
-        reference stores them; tersoff: list of (params7, cutoff, soft,
+        [, multipliers[, original_symbols]]) with symbols in target order (all
+        permutations already expanded; original_symbols, default the same, are
+        the unpermuted targets, calculator.py:1247-1277) and multipliers a list
+        of (kind, params[, original_symbols]) in the order the reference stores
+        them; tersoff: list of (params7, cutoff, soft,
         (si, sj, sk), group); tables: {id: rows of (value, derivative)}."""
         R = self.R
         n = len(pos)
@@ -292,16 +302,19 @@ class RefCore:
             for entry in potentials:
                 kind, params, cut, soft, syms, group = entry[:6]
                 mults = entry[6] if len(entry) > 6 else ()
-                e = np.asfortranarray(np.array([_label_code(s) for s in syms], dtype=np.int32).T)
+                orig = entry[7] if len(entry) > 7 else syms
+                e = _labels(syms)
                 # multipliers first (is_multiplier), then the potential itself
-                for mkind, mparams in mults:
-                    mp = np.asarray(mparams, dtype=np.float64)
-                    if not R.ref_add_potential(mkind, len(syms), len(mp), _d(mp), cut, soft,
-                                               _i(e), group, 1):
+                for m in mults:
+                    mp = np.asarray(m[1], dtype=np.float64)
+                    mo = _labels(m[2] if len(m) > 2 else syms)
+                    if not R.ref_add_potential_orig(m[0], len(syms), len(mp), _d(mp), cut, soft,
+                                                    _i(e), _i(mo), group, 1):
                         raise RuntimeError("reference add_potential (multiplier) failed")
                 p = np.asarray(params, dtype=np.float64)
-                ok = R.ref_add_potential(kind, len(syms), len(p), _d(p), cut, soft, _i(e), group,
-                                         0)
+                o = _labels(orig)
+                ok = R.ref_add_potential_orig(kind, len(syms), len(p), _d(p), cut, soft, _i(e),
+                                              _i(o), group, 0)
                 R.ref_clear_multipliers()
                 if not ok:
                     raise RuntimeError("reference add_potential failed")

@@ -275,7 +275,10 @@ long orc_neighbor_list(int n, const double *pos, const double *cell, const doubl
  *   nearest preceding potential (a ProductPotential factor, which the
  *   reference adds with is_multiplier = .true. before its first factor,
  *   calculator.py:1197-1369, Potentials.f90:2353-2472): only the row's type
- *   and parameters count; targets, cutoffs and group are the potential's. */
+ *   and parameters count; targets, cutoffs and group are the potential's.
+ * type 12 charge_exp (11 parameters) and 13 charge_abs (8): the row is
+ *   followed by a continuation row, link 2: type, the species of the
+ *   original first target (original_elements(1)), parameters 5..13. */
 #define ORC_POT_STRIDE 12
 #define ORC_MAXF 4
 /* Bond-order descriptor ('tersoff' only), 14 doubles: e1 e2 e3 (center i,
@@ -286,10 +289,18 @@ long orc_neighbor_list(int n, const double *pos, const double *cell, const doubl
 /* one factor of a (product) potential */
 typedef struct {
     int type;
-    double p[4];
+    double p[13];
     const double *tab; /* tabulated: n rows of (value, derivative) */
     int tlen;
+    int orig;          /* charge_exp / charge_abs: species of the original first target */
+    double eta[2], beta[2]; /* charge_exp derived parameters (Potentials.f90:3747-3766) */
 } fac_t;
+/* what a charge-dependent factor reads of its pair: the first atom's
+ * species and both charges */
+typedef struct {
+    int sp1;
+    double q1, q2;
+} qpair_t;
 typedef struct {
     int nt, e[2], group, smooth, nf;
     double cut, soft;
@@ -362,7 +373,72 @@ static int orc_table_interval(const fac_t *f, double r, double *dr, double *t)
 /* evaluate_energy_component for one factor of a pair potential at distance r
  * (Potentials.f90:2838-2890); cut is the potential's cutoff (the spring's
  * energy shift uses it). */
-static double orc_pair_energy(const fac_t *f, double r, double cut)
+/* The per-atom parameters of a charge-dependent pair term as the reference
+ * picks them: atom 1 takes parameter set 1 when its species is the
+ * potential's original first target, set 2 otherwise (inverse_params,
+ * Potentials.f90:3797-3829, 4580-4626). */
+static int orc_charge_inverse(const fac_t *f, const qpair_t *qp)
+{
+    return qp->sp1 != f->orig;
+}
+
+/* evaluate_energy_charge_exp (Potentials.f90:3847-3903) and
+ * _electronegativity_charge_exp (3776-3836); x1, x2 = -dE/dq1, -dE/dq2 */
+static double orc_charge_exp(const fac_t *f, const qpair_t *qp, double *x1, double *x2)
+{
+    const double *p = f->p;
+    double r2, r3, r4, r5, r6, r7, r8, r9, d3, d4;
+    if (orc_charge_inverse(f, qp)) {
+        r2 = p[5]; r3 = p[1]; r4 = p[7]; r5 = p[3];
+        r6 = f->eta[1]; r7 = f->eta[0]; r8 = f->beta[1]; r9 = f->beta[0];
+        d3 = p[10]; d4 = p[9];
+    } else {
+        r2 = p[1]; r3 = p[5]; r4 = p[3]; r5 = p[7];
+        r6 = f->eta[0]; r7 = f->eta[1]; r8 = f->beta[0]; r9 = f->beta[1];
+        d3 = p[9]; d4 = p[10];
+    }
+    double d5 = r8 * (r4 - qp->q1);
+    double d6 = r9 * (r5 - qp->q2);
+    double d1 = r2 + pow(fabs(d5), r6);
+    double d2 = r3 + pow(fabs(d6), r7);
+    double e = p[0] * exp(0.5 * (d3 * d1 + d4 * d2));
+    if (x1) {
+        *x2 = e * d4 * 0.5 * r7 * pow(fabs(d6), r7 - 1.0) * copysign(1.0, d6) * r9;
+        *x1 = e * d3 * 0.5 * r6 * pow(fabs(d5), r6 - 1.0) * copysign(1.0, d5) * r8;
+    }
+    return e;
+}
+
+/* evaluate_energy_charge_pair_abs (Potentials.f90:4680-4731) and
+ * _electronegativity_charge_pair_abs (4571-4670) */
+static double orc_charge_abs(const fac_t *f, const qpair_t *qp, double *x1, double *x2)
+{
+    const double *p = f->p;
+    int inv = orc_charge_inverse(f, qp);
+    const double *s1 = inv ? p + 4 : p, *s2 = inv ? p : p + 4;
+    double a1 = s1[0], b1 = s1[1], Q1 = s1[2], n1 = s1[3];
+    double a2 = s2[0], b2 = s2[1], Q2 = s2[2], n2 = s2[3];
+    if (x1) {
+        double abs1 = fabs(qp->q1 - Q1), abs2 = fabs(qp->q2 - Q2);
+        double pow1 = pow(abs1, n1 - 1.0), pow2 = pow(abs2, n2 - 1.0);
+        double f1 = a1 + b1 * abs1 * pow1;
+        double f2 = a2 + b2 * abs2 * pow2;
+        double dd1 = -n1 * pow1 * copysign(1.0, qp->q1 - Q1) * b1;
+        double dd2 = -n2 * pow2 * copysign(1.0, qp->q2 - Q2) * b2;
+        double inv_sqrt = 0.5 / sqrt(f1 * f2);
+        if (f1 < 0.0) { f1 = 0.0; inv_sqrt = 0.0; }
+        if (f2 < 0.0) { f2 = 0.0; inv_sqrt = 0.0; }
+        *x1 = inv_sqrt * dd1 * f2;
+        *x2 = inv_sqrt * dd2 * f1;
+    }
+    double f1 = a1 + b1 * pow(fabs(qp->q1 - Q1), n1);
+    double f2 = a2 + b2 * pow(fabs(qp->q2 - Q2), n2);
+    if (f1 < 0.0) f1 = 0.0;
+    if (f2 < 0.0) f2 = 0.0;
+    return sqrt(f1 * f2);
+}
+
+static double orc_pair_energy(const fac_t *f, double r, double cut, const qpair_t *qp)
 {
     const double *p = f->p;
     switch (f->type) {
@@ -396,6 +472,10 @@ static double orc_pair_energy(const fac_t *f, double r, double cut)
     }
     case 8: /* evaluate_energy_shift, 4827-4842 */
         return p[0] * pow((p[1] - r) / (p[1] - p[2]), p[3]);
+    case 12:
+        return orc_charge_exp(f, qp, NULL, NULL);
+    case 13:
+        return orc_charge_abs(f, qp, NULL, NULL);
     case 9: /* evaluate_energy_step, 4946-4967 */
         if (r < p[2]) return p[0];
         if (r > p[3]) return p[1];
@@ -469,13 +549,14 @@ static void orc_pair_force(const fac_t *f, const double *sep, double r, double c
 /* evaluate_energy / evaluate_forces over the factors of a product
  * (Potentials.f90:2640-2693, 2791-2814): E = prod E_i and
  * F = sum_i (E / E_i) F_i, or prod_{j != i} E_j F_i where E_i = 0. */
-static double orc_product_energy(const pot_t *P, double r)
+static double orc_product_energy(const pot_t *P, double r, const qpair_t *qp)
 {
     double e = 1.0;
-    for (int i = 0; i < P->nf; ++i) e = e * orc_pair_energy(P->f + i, r, P->cut);
+    for (int i = 0; i < P->nf; ++i) e = e * orc_pair_energy(P->f + i, r, P->cut, qp);
     return e;
 }
-static void orc_product_force(const pot_t *P, const double *sep, double r, double *f1)
+static void orc_product_force(const pot_t *P, const double *sep, double r, const qpair_t *qp,
+                              double *f1)
 {
     if (P->nf == 1) {
         orc_pair_force(P->f, sep, r, P->cut, f1);
@@ -483,7 +564,7 @@ static void orc_product_force(const pot_t *P, const double *sep, double r, doubl
     }
     double me[ORC_MAXF], mf[ORC_MAXF][3], e = 1.0;
     for (int i = 0; i < P->nf; ++i) {
-        me[i] = orc_pair_energy(P->f + i, r, P->cut);
+        me[i] = orc_pair_energy(P->f + i, r, P->cut, qp);
         orc_pair_force(P->f + i, sep, r, P->cut, mf[i]);
     }
     for (int i = 0; i < P->nf; ++i) e = e * me[i];
@@ -496,6 +577,40 @@ static void orc_product_force(const pot_t *P, const double *sep, double r, doubl
             for (int j = 0; j < P->nf; ++j)
                 if (j != i) t = t * me[j];
             for (int a = 0; a < 3; ++a) f1[a] = f1[a] + t * mf[i][a];
+        }
+    }
+}
+
+/* evaluate_electronegativity over the factors of a product (Potentials.f90:
+ * 2580-2637): factors without charge dependence contribute nothing;
+ * X = sum_i (E / E_i) X_i, or prod_{j != i} E_j X_i where E_i = 0. */
+static void orc_pair_eneg(const fac_t *f, const qpair_t *qp, double *x)
+{
+    x[0] = x[1] = 0.0;
+    if (f->type == 12) orc_charge_exp(f, qp, x, x + 1);
+    if (f->type == 13) orc_charge_abs(f, qp, x, x + 1);
+}
+static void orc_product_eneg(const pot_t *P, double r, const qpair_t *qp, double *x)
+{
+    if (P->nf == 1) {
+        orc_pair_eneg(P->f, qp, x);
+        return;
+    }
+    double me[ORC_MAXF], mx[ORC_MAXF][2], e = 1.0;
+    for (int i = 0; i < P->nf; ++i) {
+        me[i] = orc_pair_energy(P->f + i, r, P->cut, qp);
+        orc_pair_eneg(P->f + i, qp, mx[i]);
+    }
+    for (int i = 0; i < P->nf; ++i) e = e * me[i];
+    x[0] = x[1] = 0.0;
+    for (int i = 0; i < P->nf; ++i) {
+        if (me[i] != 0.0) {
+            for (int a = 0; a < 2; ++a) x[a] = x[a] + e / me[i] * mx[i][a];
+        } else {
+            double t = 1.0;
+            for (int j = 0; j < P->nf; ++j)
+                if (j != i) t = t * me[j];
+            for (int a = 0; a < 2; ++a) x[a] = x[a] + t * mx[i][a];
         }
     }
 }
@@ -594,13 +709,30 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     S->bond = calloc(nbond > 0 ? nbond : 1, sizeof(bond_t));
     if (!S->pot || !S->bond) return ORC_ENOMEM;
     int m = 0;
+    fac_t *last = NULL;   /* the factor a continuation row completes */
     for (int k = 0; k < npot; ++k) {
         const double *d = pd + ORC_POT_STRIDE * k;
         int type = (int)d[0];
-        if (type < 1 || type > 11) return ORC_EINVAL;
+        if (type < 1 || type > 13) return ORC_EINVAL;
+        if (d[11] == 2.0) { /* parameters 5.. and original_elements(1) of the row above */
+            if (!last || last->type != type || (type != 12 && type != 13)) return ORC_EINVAL;
+            for (int q = 0; q < 9; ++q) last->p[4 + q] = d[2 + q];
+            last->orig = (int)d[1];
+            if (type == 12) { /* calculate_derived_parameters_charge_exp, 3747-3766 */
+                double *pp = last->p;
+                last->eta[0] = log(pp[1] / (pp[1] - pp[2])) / log(pp[3] / (pp[3] - pp[4]));
+                last->eta[1] = log(pp[5] / (pp[5] - pp[6])) / log(pp[7] / (pp[7] - pp[8]));
+                last->beta[0] = pow(pp[2] - pp[1], 1.0 / last->eta[0]) / (pp[3] - pp[4]);
+                last->beta[1] = pow(pp[6] - pp[5], 1.0 / last->eta[1]) / (pp[7] - pp[8]);
+            }
+            last = NULL;
+            continue;
+        }
+        if (last) return ORC_EINVAL;   /* a charge term without its continuation row */
         fac_t fac;
         memset(&fac, 0, sizeof(fac));
         fac.type = type;
+        fac.orig = -1;
         fac.p[0] = d[4]; fac.p[1] = d[5]; fac.p[2] = d[9]; fac.p[3] = d[10];
         if (type == 7) {
             int id = (int)d[4];
@@ -608,16 +740,19 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
                 if (orc_tables[t].id == id) { fac.tab = orc_tables[t].v; fac.tlen = orc_tables[t].n; }
             if (!fac.tab) return ORC_EINVAL;
         }
+        int single = type == 2 || type == 10 || type == 11;
         if (d[11] != 0.0) { /* a multiplier of the previous potential */
             if (m == 0) return ORC_EINVAL;
             pot_t *q = S->pot + m - 1;
-            if (q->nf == ORC_MAXF || q->nt != 2 || type == 2 || type >= 10) return ORC_EINVAL;
+            if (q->nf == ORC_MAXF || q->nt != 2 || single) return ORC_EINVAL;
             q->f[q->nf++] = fac;
+            if (type == 12 || type == 13) last = q->f + q->nf - 1;
             continue;
         }
         pot_t *p = S->pot + m++;
         p->nf = 1;
         p->f[0] = fac;
+        if (type == 12 || type == 13) last = p->f;
         p->nt = (int)d[1];
         p->e[0] = (int)d[2];
         p->e[1] = (int)d[3];
@@ -626,9 +761,9 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
         if (d[7] > 0.0 && d[7] < d[6]) { p->soft = d[7]; p->smooth = 1; }
         else { p->soft = d[6]; p->smooth = 0; }
         p->group = (int)d[8];
-        int single = type == 2 || type >= 10;
         if (p->nt != (single ? 1 : 2)) return ORC_EINVAL;
     }
+    if (last) return ORC_EINVAL;
     S->npot = m;
     for (int k = 0; k < m; ++k) /* bond-order scaling: plain exponentials only */
         if (S->pot[k].group > -1 && (S->pot[k].f[0].type != 1 || S->pot[k].nf != 1))
@@ -1010,12 +1145,13 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     bo[1] = b2 ? orc_tersoff_bt(sums[1], b2) : sums[1];
                 }
                 double cutf = p->smooth ? orc_fc(dist, p->cut, p->soft) : 1.0;
+                const qpair_t qp = {species[i1], charge[i1], charge[i2]};
                 if (calc == 1) {
-                    double v = orc_product_energy(p, dist);
+                    double v = orc_product_energy(p, dist, &qp);
                     if (p->group > -1) en = en + v * cutf * (bo[0] + bo[1]) * 0.5;
                     else en = en + v * cutf;
                 } else if (calc == 2) {
-                    double v = (p->group > -1 || p->smooth) ? orc_product_energy(p, dist) : 0.0;
+                    double v = (p->group > -1 || p->smooth) ? orc_product_energy(p, dist, &qp) : 0.0;
                     double cg[3] = {0, 0, 0};
                     if (p->smooth) {
                         double d = orc_dfc(dist, p->cut, p->soft);
@@ -1024,7 +1160,7 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     if (p->group > -1)
                         orc_pair_bond(&S, i1, i2, sep, dist, p->group, sums, 0.5 * v * cutf, F, st);
                     double f1[3];
-                    orc_product_force(p, sep, dist, f1);
+                    orc_product_force(p, sep, dist, &qp, f1);
                     double pf1[3], pf2[3];
                     double bsum = (bo[0] + bo[1]) * 0.5;
                     for (int a = 0; a < 3; ++a) {
@@ -1040,8 +1176,16 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     st[3] += sep[1] * pf2[2];
                     st[4] += sep[0] * pf2[2];
                     st[5] += sep[0] * pf2[1];
+                } else if (calc == 3) {
+                    /* core_evaluate_local_doublet_electronegativities_B
+                     * (Core.f90:5622-5758): charge_exp / charge_abs terms,
+                     * times the smoothing factor (bond-order scaling applies
+                     * to lone exponentials only, which carry no charge) */
+                    double x[2];
+                    orc_product_eneg(p, dist, &qp, x);
+                    X[i1] += x[0] * cutf;
+                    X[i2] += x[1] * cutf;
                 }
-                /* calc == 3: these pair potentials carry no charge dependence */
             }
         }
     }

@@ -73,21 +73,51 @@ contains
     integer(c_int) :: ok
     integer :: notags(n_targets), noinds(n_targets)
     logical :: success, mult
-    character(len=11) :: names(11)
-    names = (/ 'exponential', 'charge_self', 'LJ         ', 'spring     ', 'Buckingham ', &
-         'power      ', 'tabulated  ', 'shift_power', 'step       ', 'constant   ', &
-         'force      ' /)
     notags = -9
     noinds = -9
     mult = is_mult /= 0
     success = .false.
-    if (kind >= 1 .and. kind <= 11) then
-       call add_potential(n_targets, n_params, trim(names(kind)), params, cutoff, soft, &
+    if (kind >= 1 .and. kind <= 13) then
+       call add_potential(n_targets, n_params, trim(pot_name(kind)), params, cutoff, soft, &
             elements, notags, noinds, elements, notags, noinds, group, mult, success)
     end if
     ok = 0
     if (success) ok = 1
   end function ref_add_potential
+
+  ! The same with the original targets given apart from the applied ones (a
+  ! permuted copy keeps its potential's original elements, calculator.py:
+  ! 1241-1277); the charge-dependent pair terms read them.
+  function ref_add_potential_orig(kind, n_targets, n_params, params, cutoff, soft, elements, &
+       orig_elements, group, is_mult) result(ok) bind(C, name="ref_add_potential_orig")
+    integer(c_int), value :: kind, n_targets, n_params, group, is_mult
+    real(c_double), intent(in) :: params(n_params)
+    real(c_double), value :: cutoff, soft
+    integer(c_int), intent(in) :: elements(2, n_targets), orig_elements(2, n_targets)
+    integer(c_int) :: ok
+    integer :: notags(n_targets), noinds(n_targets)
+    logical :: success, mult
+    notags = -9
+    noinds = -9
+    mult = is_mult /= 0
+    success = .false.
+    if (kind >= 1 .and. kind <= 13) then
+       call add_potential(n_targets, n_params, trim(pot_name(kind)), params, cutoff, soft, &
+            elements, notags, noinds, orig_elements, notags, noinds, group, mult, success)
+    end if
+    ok = 0
+    if (success) ok = 1
+  end function ref_add_potential_orig
+
+  function pot_name(kind) result(name)
+    integer(c_int), intent(in) :: kind
+    character(len=11) :: name
+    character(len=11) :: names(13)
+    names = (/ 'exponential', 'charge_self', 'LJ         ', 'spring     ', 'Buckingham ', &
+         'power      ', 'tabulated  ', 'shift_power', 'step       ', 'constant   ', &
+         'force      ', 'charge_exp ', 'charge_abs ' /)
+    name = names(kind)
+  end function pot_name
 
   subroutine ref_clear_multipliers() bind(C, name="ref_clear_multipliers")
     call clear_potential_multipliers()

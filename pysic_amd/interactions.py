@@ -43,11 +43,18 @@ POTENTIAL_TYPES = {
     "step": (2, ["e1", "e2", "r1", "r2"]),
     "constant": (1, ["V"]),
     "force": (1, ["Fx", "Fy", "Fz"]),
+    # charge-dependent pair terms of COMB (Potentials.f90:3691-3903, 4524-4731)
+    "charge_exp": (2, ["epsilon", "Rmax1", "Rmin1", "Qmax1", "Qmin1", "Rmax2", "Rmin2",
+                       "Qmax2", "Qmin2", "xi1", "xi2"]),
+    "charge_abs": (2, ["a1", "b1", "Q1", "n1", "a2", "b2", "Q2", "n2"]),
 }
 # descriptor type codes (include/pysic_hip.h)
 POTENTIAL_CODES = {"exponential": 1, "charge_self": 2, "LJ": 3, "spring": 4, "Buckingham": 5,
                    "power": 6, "tabulated": 7, "shift_power": 8, "step": 9, "constant": 10,
-                   "force": 11}
+                   "force": 11, "charge_exp": 12, "charge_abs": 13}
+# types whose energy depends on the charges and on which atom of the pair
+# the parameters belong to; their rows are followed by a continuation row
+CHARGE_PAIR_TYPES = ("charge_exp", "charge_abs")
 
 # name -> (n_targets, level, [[scaling params], [local params]])
 BOND_ORDER_TYPES = {
@@ -438,19 +445,31 @@ def build_descriptors(potentials, species_of):
         master, mults = factors[0], list(reversed(factors[1:]))
         nt = pot.get_number_of_targets()
 
-        def row(f, e, link):
-            p = list(f.get_parameter_values()) + [0.0] * (4 - len(f.get_parameter_values()))
-            return [POTENTIAL_CODES[f.get_potential_type()], nt, e[0], e[1], p[0], p[1],
-                    pot.get_cutoff(), pot.get_soft_cutoff(), group, p[2], p[3], link]
+        def rows(f, e, link, orig):
+            """The factor's row; a charge-dependent factor adds a
+            continuation row (link 2): type, the species of its original
+            first target (its parameters 1 belong to that atom,
+            Potentials.f90:3797-3829), parameters 5-13."""
+            v = list(f.get_parameter_values())
+            p = v + [0.0] * (13 - len(v))
+            code = POTENTIAL_CODES[f.get_potential_type()]
+            out = [[code, nt, e[0], e[1], p[0], p[1], pot.get_cutoff(), pot.get_soft_cutoff(),
+                    group, p[2], p[3], link]]
+            if f.get_potential_type() in CHARGE_PAIR_TYPES:
+                out.append([code, orig] + p[4:13] + [2])
+            return out
 
-        for targets in pot.get_symbols() or []:
+        tuples = pot.get_symbols() or []
+        for targets in tuples:
             for perm in sorted(set(permutations(targets))):
                 if any(s not in species_of for s in perm):
                     continue
                 e = [species_of[s] for s in perm] + [0] * (2 - len(perm))
-                pot_rows.append(row(master, e, 0))
+                # the reference keeps each copy's original targets; its
+                # multipliers carry the first target tuple (calculator.py:1241-1277)
+                pot_rows += rows(master, e, 0, species_of.get(targets[0], -1))
                 for m in mults:
-                    pot_rows.append(row(m, e, 1))
+                    pot_rows += rows(m, e, 1, species_of.get(tuples[0][0], -1))
     for coord, g in coord_list:
         for bond in coord.get_bond_order_parameters():
             if bond.get_bond_order_type() != "tersoff":

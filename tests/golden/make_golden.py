@@ -153,6 +153,31 @@ def run_pairs(name, struct, model, tables=None, skin=0.5):
     print(name, len(struct), "E = %.12f" % e)
 
 
+def comb_charge_pairs():
+    """COMB's charge-dependent pair terms on amorphous SiO2 (48 atoms, no
+    coordinators, random charges): energy, forces, stress and the
+    electronegativities -dE/dq."""
+    from helpers import comb_charge_model, comb_charges, ref_entries
+    s = st.amorphous_sio2(16, 2.2, seed=21)
+    q = comb_charges(s.symbols, seed=22)
+    names = sorted(set(s.symbols))
+    model = comb_charge_model()
+    pd, _, _ = build_descriptors(model, {x: k for k, x in enumerate(names)})
+    sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
+    cuts = np.full(len(s), 3.05 + 0.5)
+    R = O.RefCore()
+    R.setup(s.symbols, s.positions, s.cell, s.pbc, charges=q,
+            potentials=ref_entries(pd, names), cutoffs=cuts)
+    e = R.energy()
+    f, w = R.forces()
+    x = R.electronegativities()
+    np.savez_compressed(os.path.join(HERE, "sio2_48_comb_charge_pairs.npz"),
+                        positions=s.positions, cell=s.cell, pbc=s.pbc.astype(np.int32),
+                        species=sp, symbols=np.array(s.symbols), charges=q, potdesc=pd,
+                        cutoffs=cuts, energy=e, forces=f, stress=w, enegs=x)
+    print("sio2_48_comb_charge_pairs", len(s), "E = %.12f" % e)
+
+
 def pair_cases():
     from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table
     from pysic_amd.interactions import lj_argon
@@ -195,6 +220,7 @@ def main():
     sheared_ewald()
     scaled_ewald()
     sic_mixed()
+    comb_charge_pairs()
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
     rng = np.random.default_rng(8)
@@ -212,6 +238,8 @@ if __name__ == "__main__":
         sheared_ewald()
     elif sys.argv[1:] == ["sic_mixed"]:
         sic_mixed()
+    elif sys.argv[1:] == ["comb_charge_pairs"]:
+        comb_charge_pairs()
     elif sys.argv[1:] == ["scaled_ewald"]:
         scaled_ewald()
     else:

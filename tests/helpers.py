@@ -76,21 +76,98 @@ def mixture_model():
     ]
 
 
+# COMB charge-dependent pair terms with the defaults of the reference's COMB
+# builder (pysic/interactions/comb.py:126-444, 665-990), without its
+# bond-order coordinators: per-element charge-screening radii (rmax, rmin at
+# qmax, qmin), attractive alpha and charge_abs a, b, Q, n; attractive B and
+# repulsive A, lambda mix as sqrt(B_i B_j), (alpha_i + alpha_j)/2,
+# sqrt(A_i A_j), (lambda_i + lambda_j)/2.
+COMB = {"Si": dict(rmax=-1.658949, rmin=1.651725, qmax=4.0, qmin=-4.0, alpha=1.7322, a=1.0,
+                   b=-9.5367431640625e-07, q=0.0, n=10.0, B=471.18, A=1830.8, lam=2.4799,
+                   cut=3.0, margin=0.2),
+        "O": dict(rmax=-0.00112, rmin=0.00148, qmax=5.5046, qmin=-1.8349, alpha=2.68, a=1.00098,
+                  b=-2.26e-6, q=1.83485, n=10.0, B=260.8931, A=3326.699, lam=5.36, cut=3.0,
+                  margin=0.4)}
+
+
+def comb_charge_model():
+    """Repulsive charge_exp x exponential and attractive charge_exp x
+    charge_abs x exponential products for Si-Si, O-O and Si-O (the order of
+    comb.py's pieces), plus a lone charge_abs O-O term."""
+    C = COMB
+    P = Potential
+
+    def cexp(i, j, eps, xi, cut, margin):
+        return P("charge_exp", symbols=[[i, j]], cutoff=cut, cutoff_margin=margin,
+                 parameters=[eps, C[i]["rmax"], C[i]["rmin"], C[i]["qmax"], C[i]["qmin"],
+                             C[j]["rmax"], C[j]["rmin"], C[j]["qmax"], C[j]["qmin"],
+                             C[i][xi], C[j][xi]])
+
+    def cabs(i, j):
+        return P("charge_abs", symbols=[[i, j]],
+                 parameters=[C[i]["a"], C[i]["b"], C[i]["q"], C[i]["n"],
+                             C[j]["a"], C[j]["b"], C[j]["q"], C[j]["n"]])
+
+    pots = []
+    for i, j in (("Si", "Si"), ("O", "O"), ("Si", "O")):
+        cut, margin = (C[i]["cut"], C[i]["margin"]) if i == j else (3.05, 0.5)
+        B = math.sqrt(C[i]["B"] * C[j]["B"])
+        zeta = 0.5 * (C[i]["alpha"] + C[j]["alpha"])
+        pots.append(ProductPotential([
+            cexp(i, j, B, "alpha", cut, margin), cabs(i, j),
+            P("exponential", symbols=[[i, j]], parameters=[-1.0, zeta])]))
+    for i, j in (("Si", "Si"), ("O", "O"), ("Si", "O")):
+        cut, margin = (C[i]["cut"], C[i]["margin"]) if i == j else (3.05, 0.5)
+        A = math.sqrt(C[i]["A"] * C[j]["A"])
+        lam = 0.5 * (C[i]["lam"] + C[j]["lam"])
+        pots.append(ProductPotential([
+            cexp(i, j, A, "lam", cut, margin),
+            P("exponential", symbols=[[i, j]], parameters=[1.0, lam])]))
+    lone = cabs("O", "O")
+    lone.set_cutoff(3.0)
+    lone.set_cutoff_margin(0.5)
+    pots.append(lone)
+    return pots
+
+
+def comb_charges(symbols, seed):
+    """Random charges in the ranges COMB SiO2 visits: Si in [0.5, 2.2],
+    O in [-1.3, -0.2] (not neutral: no Ewald term here)."""
+    rng = np.random.default_rng(seed)
+    return np.array([rng.uniform(0.5, 2.2) if s == "Si" else rng.uniform(-1.3, -0.2)
+                     for s in symbols])
+
+
 def ref_entries(pd, names):
     """RefCore potential entries (kind, params, cutoff, soft, symbols, group,
-    multipliers) from engine descriptor rows (12 doubles each)."""
+    multipliers, original symbols) from engine descriptor rows (12 doubles
+    each; a link-2 row continues the row before it with the parameters 5-13
+    and the species of the original first target)."""
     inv = {v: k for k, v in POTENTIAL_CODES.items()}
     out = []
+    last = None
     for r in np.asarray(pd, dtype=np.float64).reshape(-1, 12):
         kind = int(r[0])
         npar = len(POTENTIAL_TYPES[inv[kind]][1])
-        params = [r[4], r[5], r[9], r[10]][:npar]
-        if r[11] != 0:
-            out[-1][6].append((kind, params))
+        if r[11] == 2:
+            params = last[1] + list(r[2:11])
+            o1 = "Xx" if r[1] < 0 else names[int(r[1])]
+            syms = last[2]
+            orig = (o1, syms[1] if syms[0] == o1 else syms[0])
+            if last[0] == 0:
+                e = out[-1]
+                out[-1] = e[:1] + (params[:npar],) + e[2:7] + (orig,)
+            else:
+                out[-1][6][-1] = (kind, params[:npar], orig)
             continue
+        params = [r[4], r[5], r[9], r[10]]
         nt = int(r[1])
         syms = tuple(names[int(e)] for e in r[2:2 + nt])
-        out.append((kind, params, float(r[6]), float(r[7]), syms, int(r[8]), []))
+        last = (int(r[11]), params, syms)
+        if r[11] != 0:
+            out[-1][6].append((kind, params[:npar]))
+            continue
+        out.append((kind, params[:npar], float(r[6]), float(r[7]), syms, int(r[8]), [], syms))
     return out
 
 

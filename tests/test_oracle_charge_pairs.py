@@ -1,0 +1,150 @@
+"""CPU tests that pin the oracle's charge-dependent pair terms of COMB
+(oracle/pysic_oracle.c: orc_charge_exp, orc_charge_abs, orc_product_eneg).
+
+Pins, each chosen so that a plausible slip fails one of them:
+  * the golden vectors of the reference Fortran core
+    (tests/golden/sio2_48_comb_charge_pairs.npz: COMB's repulsive and
+    attractive products for Si-Si, O-O, Si-O and a lone charge_abs term on
+    amorphous SiO2 with random charges): energy, forces, stress and
+    electronegativities;
+  * closed forms the screening function fixes: D(Qmax) = Rmax and
+    D(Qmin) = Rmin, so charge_exp at those charges is
+    epsilon exp((xi1 Rmax1 + xi2 Rmin2)/2); charge_abs at q = Q is
+    sqrt(a1 a2) and at q = Q + 1 is sqrt((a1 + b1) a2);
+  * the parameter swap when the pair's first atom is not the element the
+    parameters 1 were given for: the same potential written for (O, Si)
+    with its parameter halves exchanged gives the same energy;
+  * chi = -dE/dq and F = -dE/dr by central differences.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from helpers import COMB, comb_charge_model, comb_charges, ref_entries, rel_maxnorm
+from pysic_amd import structures as st
+from pysic_amd.interactions import Potential, build_descriptors
+
+NAME = "sio2_48_comb_charge_pairs"
+
+
+def _golden(O):
+    g = load_golden(NAME)
+    nl = O.neighbor_list(g["positions"], g["cell"], g["pbc"], g["cutoffs"])
+    return g, nl, (g["species"], g["positions"], g["charges"], g["cell"], nl, g["potdesc"])
+
+
+def test_comb_matches_reference_core(oracle):
+    g, _, args = _golden(oracle)
+    e = oracle.evaluate("energy", *args)
+    f, s = oracle.evaluate("forces", *args)
+    x = oracle.evaluate("enegs", *args)
+    assert abs(e - float(g["energy"])) <= 1e-12 * abs(float(g["energy"]))
+    assert rel_maxnorm(f, g["forces"]) <= 1e-12
+    assert rel_maxnorm(s, g["stress"]) <= 1e-12
+    assert rel_maxnorm(x, g["enegs"]) <= 1e-12
+
+
+def test_comb_enegs_are_charge_gradient(oracle):
+    g, nl, args = _golden(oracle)
+    sp, pos, q, cell, _, pd = args
+    x = oracle.evaluate("enegs", *args)
+    h = 1e-5
+    for i in range(0, len(q), 5):
+        qp, qm = q.copy(), q.copy()
+        qp[i] += h
+        qm[i] -= h
+        d = (oracle.evaluate("energy", sp, pos, qp, cell, nl, pd)
+             - oracle.evaluate("energy", sp, pos, qm, cell, nl, pd)) / (2 * h)
+        assert abs(x[i] + d) <= 1e-6 * max(1.0, abs(d)), i
+
+
+def test_comb_forces_are_energy_gradient(oracle):
+    g, nl, args = _golden(oracle)
+    sp, pos, q, cell, _, pd = args
+    f, _ = oracle.evaluate("forces", *args)
+    h = 1e-5
+    for i in range(0, len(pos), 7):
+        for a in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, a] += h
+            pm[i, a] -= h
+            d = (oracle.evaluate("energy", sp, pp, q, cell, nl, pd)
+                 - oracle.evaluate("energy", sp, pm, q, cell, nl, pd)) / (2 * h)
+            assert abs(f[i, a] + d) <= 1e-6 * max(1.0, abs(d)), (i, a)
+
+
+def _pair(O, pots, first, second, q, r=1.6):
+    """Atoms `first`, `second` r apart in a large open box (inside the
+    smooth-cutoff start, so the cutoff factor is 1)."""
+    names = ["O", "Si"]
+    pd, _, _ = build_descriptors(pots, {s: k for k, s in enumerate(names)})
+    pos = np.array([[5.0, 5.0, 5.0], [5.0, 5.0, 5.0 + r]])
+    cell = np.eye(3) * 30.0
+    sp = np.array([names.index(first), names.index(second)], np.int32)
+    nl = O.neighbor_list(pos, cell, [0, 0, 0], np.full(2, 3.05))
+    return O.evaluate("energy", sp, pos, np.asarray(q, float), cell, nl, pd), \
+        O.evaluate("enegs", sp, pos, np.asarray(q, float), cell, nl, pd)
+
+
+def _cexp(i, j, eps=2.0, xi=(0.7, 1.3)):
+    C = COMB
+    return Potential("charge_exp", symbols=[[i, j]], cutoff=3.05, cutoff_margin=0.5,
+                     parameters=[eps, C[i]["rmax"], C[i]["rmin"], C[i]["qmax"], C[i]["qmin"],
+                                 C[j]["rmax"], C[j]["rmin"], C[j]["qmax"], C[j]["qmin"],
+                                 xi[0], xi[1]])
+
+
+@pytest.mark.parametrize("order", [("Si", "O"), ("O", "Si")])
+def test_charge_exp_closed_form(oracle, order):
+    C = COMB
+    q = {"Si": C["Si"]["qmax"], "O": C["O"]["qmin"]}
+    e, _ = _pair(oracle, [_cexp("Si", "O")], order[0], order[1], [q[order[0]], q[order[1]]])
+    want = 2.0 * math.exp(0.5 * (0.7 * C["Si"]["rmax"] + 1.3 * C["O"]["rmin"]))
+    assert abs(e - want) <= 1e-13 * want
+    # the same term written for (O, Si), parameter halves exchanged, at
+    # generic charges (dD/dq is unbounded at Qmax when eta < 1, as for Si)
+    q = {"Si": 1.3, "O": -0.8}
+    qs = [q[order[0]], q[order[1]]]
+    e1, x1 = _pair(oracle, [_cexp("Si", "O")], order[0], order[1], qs)
+    e2, x2 = _pair(oracle, [_cexp("O", "Si", xi=(1.3, 0.7))], order[0], order[1], qs)
+    assert abs(e2 - e1) <= 1e-13 * abs(e1)
+    assert np.abs(x2 - x1).max() <= 1e-12 * np.abs(x1).max()
+
+
+@pytest.mark.parametrize("order", [("Si", "O"), ("O", "Si")])
+def test_charge_abs_closed_form(oracle, order):
+    p = [2.0, -0.25, 0.4, 3.0, 1.5, -0.1, -0.6, 2.0]  # a1 b1 Q1 n1 a2 b2 Q2 n2 (Si, O)
+    pot = Potential("charge_abs", symbols=[["Si", "O"]], parameters=p, cutoff=3.05,
+                    cutoff_margin=0.5)
+    for dq, want in ((0.0, math.sqrt(2.0 * 1.5)), (1.0, math.sqrt((2.0 - 0.25) * 1.5))):
+        q = {"Si": 0.4 + dq, "O": -0.6}
+        e, _ = _pair(oracle, [pot], order[0], order[1], [q[order[0]], q[order[1]]])
+        assert abs(e - want) <= 1e-13 * want, dq
+
+
+def test_reference_core_agrees_on_fresh_comb(oracle):
+    """The C restatement against the reference Fortran core on a structure
+    and charges none of the fixtures holds (only where the reference build
+    exists)."""
+    if not oracle.ref_available():
+        pytest.skip("reference checker not built (oracle/build_ref.sh)")
+    s = st.amorphous_sio2(24, 2.4, seed=31)
+    q = comb_charges(s.symbols, seed=32)
+    names = sorted(set(s.symbols))
+    pd, _, _ = build_descriptors(comb_charge_model(), {x: k for k, x in enumerate(names)})
+    sp = np.array([names.index(x) for x in s.symbols], np.int32)
+    cut = np.full(len(s), 3.55)
+    R = oracle.RefCore()
+    R.setup(s.symbols, s.positions, s.cell, s.pbc, charges=q, potentials=ref_entries(pd, names),
+            cutoffs=cut)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], cut)
+    args = (sp, s.positions, q, s.cell, nl, pd)
+    e = oracle.evaluate("energy", *args)
+    f, w = oracle.evaluate("forces", *args)
+    x = oracle.evaluate("enegs", *args)
+    fr, wr = R.forces()
+    assert abs(e - R.energy()) <= 1e-12 * abs(e)
+    assert rel_maxnorm(f, fr) <= 1e-12 and rel_maxnorm(w, wr) <= 1e-12
+    assert rel_maxnorm(x, R.electronegativities()) <= 1e-12
