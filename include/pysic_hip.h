@@ -15,13 +15,19 @@
  *   group (-1 none), p3, p4, link
  * type: 1 'exponential', 2 'charge_self', 3 'LJ', 4 'spring', 5 'Buckingham',
  *   6 'power', 7 'tabulated', 8 'shift_power', 9 'step', 10 'constant',
- *   11 'force'; p1..p4 are the parameters in the reference's order
- *   (Potentials.f90:2909-4990).  Only a lone 'exponential' may carry a
- *   bond-order group.
+ *   11 'force', 12 'charge_exp', 13 'charge_abs', 14 'charge_pair'; p1..p4
+ *   are the parameters in the reference's order (Potentials.f90:2909-4990).
+ *   Only a lone 'exponential' may carry a bond-order group.
  * link: 0 starts a potential; 1 makes the entry a multiplier of the
  *   potential before it (a ProductPotential factor: add_potential with
  *   is_multiplier, calculator.py:1197-1369); a multiplier's targets, cutoffs
- *   and group are ignored (the product uses its first potential's).
+ *   and group are ignored (the product uses its first potential's); 2
+ *   continues the entry before it, which must be of type 12-14, as
+ *     type, s, p5, p6, ..., p13, 2
+ *   with p5.. its further parameters and s the species of the potential's
+ *   original first target (the element its parameters 1 belong to; -1 for
+ *   one absent from the system), calculator.py:1241-1277 and
+ *   Potentials.f90:3797-3829.  At most 16 distinct such factors.
  * Bond-order descriptors ('tersoff', 14 doubles per entry, no permutation as
  * calculator.py:1372):
  *   e1 e2 e3 (center, j, k), beta, eta, a, c, d, h, mu, cutoff, soft_cutoff,
@@ -96,7 +102,8 @@ int pysic_calculate_energy(pysic_ctx *h, double *energy);
 /* calculate_forces (PyInterface.f90:668): forces (N,3), stress(6) =
  * sum r (x) f in the order xx yy zz yz xz xy */
 int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress);
-/* calculate_electronegativities (PyInterface.f90:684) */
+/* calculate_electronegativities (PyInterface.f90:684): -dE/dq of the
+ * charge_self, Ewald and charge-dependent pair terms */
 int pysic_calculate_electronegativities(pysic_ctx *h, int n, double *enegs);
 
 /* ChargeRelaxation.charge_relaxation 'dynamic' (charges/relaxation.py:417-450)

@@ -438,6 +438,35 @@ static double orc_charge_abs(const fac_t *f, const qpair_t *qp, double *x1, doub
     return sqrt(f1 * f2);
 }
 
+/* x**n for an integer n, as Fortran's real**integer (n < 0: 1 / x**-n) */
+static double orc_ipow(double x, int n)
+{
+    int m = n < 0 ? -n : n;
+    double y = 1.0;
+    for (int k = 0; k < m; ++k) y = y * x;
+    return n < 0 ? 1.0 / y : y;
+}
+
+/* evaluate_energy_charge_pair (Potentials.f90:4469-4506) and
+ * _electronegativity_charge_pair (4415-4462): E = eps q1^n1 q2^n2 with
+ * integer exponents int(n1), int(n2), swapped as for charge_exp */
+static double orc_charge_pair(const fac_t *f, const qpair_t *qp, double *x1, double *x2)
+{
+    const double *p = f->p;
+    int inv = orc_charge_inverse(f, qp);
+    int n1 = (int)(inv ? p[2] : p[1]), n2 = (int)(inv ? p[1] : p[2]);
+    if (x1) {
+        *x1 = n1 > 0 ? -p[0] * (n1 * orc_ipow(qp->q1, n1 - 1) * orc_ipow(qp->q2, n2)) : 0.0;
+        *x2 = n2 > 0 ? -p[0] * (n2 * orc_ipow(qp->q1, n1) * orc_ipow(qp->q2, n2 - 1)) : 0.0;
+    }
+    return p[0] * (orc_ipow(qp->q1, n1) * orc_ipow(qp->q2, n2));
+}
+
+static int orc_is_charge(int type)
+{
+    return type >= 12 && type <= 14;
+}
+
 static double orc_pair_energy(const fac_t *f, double r, double cut, const qpair_t *qp)
 {
     const double *p = f->p;
@@ -476,6 +505,8 @@ static double orc_pair_energy(const fac_t *f, double r, double cut, const qpair_
         return orc_charge_exp(f, qp, NULL, NULL);
     case 13:
         return orc_charge_abs(f, qp, NULL, NULL);
+    case 14:
+        return orc_charge_pair(f, qp, NULL, NULL);
     case 9: /* evaluate_energy_step, 4946-4967 */
         if (r < p[2]) return p[0];
         if (r > p[3]) return p[1];
@@ -589,6 +620,7 @@ static void orc_pair_eneg(const fac_t *f, const qpair_t *qp, double *x)
     x[0] = x[1] = 0.0;
     if (f->type == 12) orc_charge_exp(f, qp, x, x + 1);
     if (f->type == 13) orc_charge_abs(f, qp, x, x + 1);
+    if (f->type == 14) orc_charge_pair(f, qp, x, x + 1);
 }
 static void orc_product_eneg(const pot_t *P, double r, const qpair_t *qp, double *x)
 {
@@ -713,9 +745,9 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     for (int k = 0; k < npot; ++k) {
         const double *d = pd + ORC_POT_STRIDE * k;
         int type = (int)d[0];
-        if (type < 1 || type > 13) return ORC_EINVAL;
+        if (type < 1 || type > 14) return ORC_EINVAL;
         if (d[11] == 2.0) { /* parameters 5.. and original_elements(1) of the row above */
-            if (!last || last->type != type || (type != 12 && type != 13)) return ORC_EINVAL;
+            if (!last || last->type != type || !orc_is_charge(type)) return ORC_EINVAL;
             for (int q = 0; q < 9; ++q) last->p[4 + q] = d[2 + q];
             last->orig = (int)d[1];
             if (type == 12) { /* calculate_derived_parameters_charge_exp, 3747-3766 */
@@ -746,13 +778,13 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
             pot_t *q = S->pot + m - 1;
             if (q->nf == ORC_MAXF || q->nt != 2 || single) return ORC_EINVAL;
             q->f[q->nf++] = fac;
-            if (type == 12 || type == 13) last = q->f + q->nf - 1;
+            if (orc_is_charge(type)) last = q->f + q->nf - 1;
             continue;
         }
         pot_t *p = S->pot + m++;
         p->nf = 1;
         p->f[0] = fac;
-        if (type == 12 || type == 13) last = p->f;
+        if (orc_is_charge(type)) last = p->f;
         p->nt = (int)d[1];
         p->e[0] = (int)d[2];
         p->e[1] = (int)d[3];

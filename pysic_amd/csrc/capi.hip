@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -198,12 +199,31 @@ void compile_model(pysic_ctx *h)
         int type = (int)d[0], nt = (int)d[1], e1 = (int)d[2], e2 = (int)d[3];
         PYSIC_REQUIRE(d[11] == 0.0, PYSIC_EINVAL,
                       "a multiplier row must follow the potential it multiplies");
-        size_t nmul = 0;  // the ProductPotential multipliers that follow
-        while (k + 1 + nmul < np && row(k + 1 + nmul)[11] != 0.0) ++nmul;
+        // the factor rows of this potential (link 0, then its ProductPotential
+        // multipliers, link 1), each charge factor followed by its
+        // continuation row (link 2)
+        size_t frow[PYSIC_MAXF + 1], crow[PYSIC_MAXF + 1];
+        size_t nfac = 0, next = k;
+        while (next < np && (next == k || row(next)[11] == 1.0)) {
+            PYSIC_REQUIRE(nfac < PYSIC_MAXF, PYSIC_EINVAL, "at most 4 factors in a product");
+            frow[nfac] = next;
+            crow[nfac] = SIZE_MAX;
+            ++next;
+            if (next < np && row(next)[11] == 2.0) crow[nfac] = next++;
+            const int ft = (int)row(frow[nfac])[0];
+            PYSIC_REQUIRE((ft >= 12 && ft <= 14) == (crow[nfac] != SIZE_MAX) &&
+                              (crow[nfac] == SIZE_MAX || (int)row(crow[nfac])[0] == ft),
+                          PYSIC_EINVAL,
+                          "a charge-dependent pair row needs its continuation row, and only it");
+            ++nfac;
+        }
+        PYSIC_REQUIRE(next >= np || row(next)[11] == 0.0, PYSIC_EINVAL,
+                      "a continuation row must follow a charge-dependent pair row");
+        const size_t nmul = nfac - 1;  // the ProductPotential multipliers
         PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && (nt < 2 || (e2 >= 0 && e2 < PYSIC_MAXS)),
                       PYSIC_EINVAL, "potential species out of range");
-        PYSIC_REQUIRE(type >= 1 && type <= 11, PYSIC_EINVAL, "unsupported potential type");
-        const bool single = type == 2 || type >= 10;
+        PYSIC_REQUIRE(type >= 1 && type <= 14, PYSIC_EINVAL, "unsupported potential type");
+        const bool single = type == 2 || type == 10 || type == 11;
         PYSIC_REQUIRE(nt == (single ? 1 : 2), PYSIC_EINVAL,
                       single ? "one-body potential needs 1 target" : "pair potential needs 2 targets");
         PYSIC_REQUIRE(!(single && nmul), PYSIC_EINVAL,
@@ -240,19 +260,49 @@ void compile_model(pysic_ctx *h)
                           "bond-order scaling applies to a lone 'exponential' potential only");
             PYSIC_REQUIRE(nmul + 1 <= PYSIC_MAXF, PYSIC_EINVAL, "at most 4 factors in a product");
             int &c = P.npair[e1][e2];
-            PYSIC_REQUIRE(c < PYSIC_MAXG, PYSIC_EINVAL, "too many pair potentials for one pair");
+            PYSIC_REQUIRE(c < PYSIC_MAXG, PYSIC_EINVAL, "too many pair potentials for one species pair (at most 8)");
             GenPair &T = P.pair[e1][e2][c++];
             T.cut = d[6];
             if (d[7] > 0.0 && d[7] < d[6]) { T.soft = d[7]; T.smooth = 1; }
             else { T.soft = d[6]; T.smooth = 0; }
-            T.nf = (int)nmul + 1;
+            T.nf = (int)nfac;
             for (int f = 0; f < T.nf; ++f) {
-                const double *r = row(k + f);
+                const double *r = row(frow[f]);
                 PairFactor &F = T.f[f];
                 F.type = (int)r[0];
-                PYSIC_REQUIRE(F.type == 1 || (F.type >= 3 && F.type <= 9), PYSIC_EINVAL,
-                              "a product takes pair potentials only");
+                const bool charged = F.type >= 12 && F.type <= 14;
+                PYSIC_REQUIRE(F.type == 1 || (F.type >= 3 && F.type <= 9) || charged,
+                              PYSIC_EINVAL, "a product takes pair potentials only");
                 F.p[0] = r[4]; F.p[1] = r[5]; F.p[2] = r[9]; F.p[3] = r[10];
+                if (charged) {
+                    const double *c = row(crow[f]);
+                    ChargeFactor Q;
+                    std::memset(&Q, 0, sizeof(Q));
+                    for (int a = 0; a < 4; ++a) Q.p[a] = F.p[a];
+                    for (int a = 0; a < 9; ++a) Q.p[4 + a] = c[2 + a];
+                    Q.orig = (int)c[1];
+                    if (F.type == 12) {
+                        // calculate_derived_parameters_charge_exp (Potentials.f90:3747-3766)
+                        const double *q = Q.p;
+                        Q.eta[0] = std::log(q[1] / (q[1] - q[2])) / std::log(q[3] / (q[3] - q[4]));
+                        Q.eta[1] = std::log(q[5] / (q[5] - q[6])) / std::log(q[7] / (q[7] - q[8]));
+                        Q.beta[0] = std::pow(q[2] - q[1], 1.0 / Q.eta[0]) / (q[3] - q[4]);
+                        Q.beta[1] = std::pow(q[6] - q[5], 1.0 / Q.eta[1]) / (q[7] - q[8]);
+                    }
+                    // the permuted copies of a potential share one entry
+                    int slot = 0;
+                    while (slot < P.ncf && std::memcmp(&P.cf[slot], &Q, sizeof(Q)) != 0) ++slot;
+                    if (slot == P.ncf) {
+                        PYSIC_REQUIRE(P.ncf < PYSIC_MAXCF, PYSIC_EINVAL,
+                                      "too many distinct charge-dependent pair factors "
+                                      "(at most 16)");
+                        P.cf[P.ncf++] = Q;
+                    }
+                    F.toff = slot;
+                    P.has_charge = 1;
+                    // the factor's value depends on which atom comes first
+                    P.needs_pick = 1;
+                }
                 if (F.type == 7) {
                     const int id = (int)r[4];
                     auto it = h->tables.find(id);
@@ -274,8 +324,8 @@ void compile_model(pysic_ctx *h)
             const double c2 = T.cut * T.cut * (1.0 + 1e-12);
             P.cut2_hi[e1][e2] = std::max(P.cut2_hi[e1][e2], c2);
             P.has_pairs = 1;
-            k += nmul;
         }
+        k = next - 1;
     }
     size_t nb = h->bonddesc.size() / 14;
     for (size_t k = 0; k < nb; ++k) {

@@ -53,13 +53,27 @@ struct Model {
 // of the reference except a bond-order-coordinated exponential, and products
 // of them (ProductPotential).  Type codes are the descriptor's:
 // 1 exponential, 3 LJ, 4 spring, 5 Buckingham, 6 power, 7 tabulated,
-// 8 shift_power, 9 step; parameters in the reference's order.
-#define PYSIC_MAXG 4      // such potentials per ordered species pair
+// 8 shift_power, 9 step, 12 charge_exp, 13 charge_abs, 14 charge_pair;
+// parameters in the reference's order.
+#define PYSIC_MAXG 8      // such potentials per ordered species pair
 #define PYSIC_MAXF 4      // factors of one product
+#define PYSIC_MAXCF 16    // charge-dependent factors per model
 struct PairFactor {
     int type;
-    int toff, tlen;       // tabulated: first row in the table buffer, rows
+    int toff, tlen;       // tabulated: first row in the table buffer, rows;
+                          // charge-dependent types: toff indexes PairModel::cf
     double p[4];
+};
+// A charge-dependent pair factor of COMB (charge_exp, charge_abs,
+// charge_pair): its
+// parameters, the species its parameters 1 belong to (the potential's
+// original first target; for a pair whose first atom is another species the
+// two halves swap, Potentials.f90:3797-3829) and charge_exp's derived
+// exponents and scales (Potentials.f90:3747-3766).
+struct ChargeFactor {
+    double p[13];
+    double eta[2], beta[2];
+    int orig, pad;
 };
 struct GenPair {
     PairFactor f[PYSIC_MAXF];  // f[0] the potential, then its multipliers
@@ -87,7 +101,9 @@ struct PairModel {
     double konst[PYSIC_MAXS];
     double kforce[PYSIC_MAXS][3];
     int has_pairs, has_single;
-    int needs_pick;       // a 'step' factor: its force depends on which atom the reference takes first
+    int needs_pick;       // a 'step' or charge factor: it depends on which atom the reference takes first
+    int has_charge, ncf;  // charge-dependent factors (cf[0 .. ncf-1])
+    ChargeFactor cf[PYSIC_MAXCF];
     // every ordered species pair holds at most one potential, a lone LJ
     // (lj_only), and its table
     int lj_only;

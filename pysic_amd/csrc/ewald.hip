@@ -1081,6 +1081,7 @@ void chi_eval(System &S, const EwaldParams &P, EwaldState &E)
         E.sk_valid = false;
         recip_atoms(S, E, 3, nullptr);
     }
+    pr_chi(S, E.chi.p);
 }
 
 }  // namespace

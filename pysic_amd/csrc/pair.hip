@@ -5,9 +5,13 @@
 // (Core.f90:4257-4385, 5371-5606) with evaluate_energy / evaluate_forces and
 // their components (Potentials.f90:2640-2890) for the types LJ, spring,
 // exponential (as a product factor), Buckingham, power, tabulated,
-// shift_power and step, products of them (ProductPotential), and the
-// one-body 'constant' and 'force' terms of core_evaluate_local_singlet
-// (Core.f90:4095-4239).
+// shift_power, step and COMB's charge-dependent charge_exp, charge_abs and
+// charge_pair,
+// products of them (ProductPotential), and the one-body 'constant' and
+// 'force' terms of core_evaluate_local_singlet (Core.f90:4095-4239).  The
+// charge-dependent factors add their -dE/dq to the electronegativities
+// (core_calculate_electronegativities' doublet loop, Core.f90:7113-7273,
+// and evaluate_electronegativity, Potentials.f90:2580-2637).
 //
 // The reference visits every pair once (pick) and scatters the force to both
 // atoms.  Here one thread owns one atom and walks its full neighbor list:
@@ -35,12 +39,97 @@ struct Radial {
     double c;
 };
 
+// The charges of a pair in the order the reference evaluates it (atom 1 the
+// atom pick takes first) and atom 1's species.
+struct QPair {
+    double q1, q2;
+    int s1;
+};
+
+// charge_exp (evaluate_energy_charge_exp and _electronegativity_charge_exp,
+// Potentials.f90:3776-3836, 3847-3903): E = eps exp((xi1 D1 + xi2 D2) / 2),
+// D_k = Rmax_k + |beta_k (Qmax_k - q_k)|^eta_k, with the parameter halves
+// swapped when atom 1 is not the species parameters 1 belong to.  With x,
+// x[k] = -dE/dq_k.
+__device__ __noinline__ double charge_exp(const ChargeFactor &c, const QPair &qp, double *x)
+{
+    const int a = qp.s1 != c.orig ? 1 : 0, b = 1 - a;
+    const double *p = c.p;
+    const double d5 = c.beta[a] * (p[3 + 4 * a] - qp.q1);
+    const double d6 = c.beta[b] * (p[3 + 4 * b] - qp.q2);
+    const double d1 = p[1 + 4 * a] + pow(fabs(d5), c.eta[a]);
+    const double d2 = p[1 + 4 * b] + pow(fabs(d6), c.eta[b]);
+    const double xi1 = p[9 + a], xi2 = p[9 + b];
+    const double e = p[0] * exp(0.5 * (xi1 * d1 + xi2 * d2));
+    if (x) {
+        x[1] = e * xi2 * 0.5 * c.eta[b] * pow(fabs(d6), c.eta[b] - 1.0) * copysign(1.0, d6) *
+               c.beta[b];
+        x[0] = e * xi1 * 0.5 * c.eta[a] * pow(fabs(d5), c.eta[a] - 1.0) * copysign(1.0, d5) *
+               c.beta[a];
+    }
+    return e;
+}
+
+// charge_abs (evaluate_energy_charge_pair_abs and
+// _electronegativity_charge_pair_abs, Potentials.f90:4571-4731):
+// E = sqrt(f1 f2), f_k = max(0, a_k + b_k |q_k - Q_k|^n_k), halves swapped
+// as for charge_exp.
+__device__ __noinline__ double charge_abs(const ChargeFactor &c, const QPair &qp, double *x)
+{
+    const int a = qp.s1 != c.orig ? 4 : 0, b = 4 - a;
+    const double *p = c.p;
+    const double a1 = p[a], b1 = p[a + 1], Q1 = p[a + 2], n1 = p[a + 3];
+    const double a2 = p[b], b2 = p[b + 1], Q2 = p[b + 2], n2 = p[b + 3];
+    if (x) {
+        const double abs1 = fabs(qp.q1 - Q1), abs2 = fabs(qp.q2 - Q2);
+        const double pow1 = pow(abs1, n1 - 1.0), pow2 = pow(abs2, n2 - 1.0);
+        double f1 = a1 + b1 * abs1 * pow1;
+        double f2 = a2 + b2 * abs2 * pow2;
+        const double dd1 = -n1 * pow1 * copysign(1.0, qp.q1 - Q1) * b1;
+        const double dd2 = -n2 * pow2 * copysign(1.0, qp.q2 - Q2) * b2;
+        double h = 0.5 / sqrt(f1 * f2);
+        if (f1 < 0.0) { f1 = 0.0; h = 0.0; }
+        if (f2 < 0.0) { f2 = 0.0; h = 0.0; }
+        x[0] = h * dd1 * f2;
+        x[1] = h * dd2 * f1;
+    }
+    double f1 = a1 + b1 * pow(fabs(qp.q1 - Q1), n1);
+    double f2 = a2 + b2 * pow(fabs(qp.q2 - Q2), n2);
+    if (f1 < 0.0) f1 = 0.0;
+    if (f2 < 0.0) f2 = 0.0;
+    return sqrt(f1 * f2);
+}
+
+// x^n for an integer n by repeated products (Fortran's real**integer)
+__device__ __forceinline__ double ipow(double x, int n)
+{
+    const int m = n < 0 ? -n : n;
+    double y = 1.0;
+    for (int k = 0; k < m; ++k) y = y * x;
+    return n < 0 ? 1.0 / y : y;
+}
+
+// charge_pair (evaluate_energy_charge_pair and
+// _electronegativity_charge_pair, Potentials.f90:4415-4506):
+// E = eps q1^n1 q2^n2 with integer exponents, swapped as for charge_exp.
+__device__ __noinline__ double charge_pair(const ChargeFactor &c, const QPair &qp, double *x)
+{
+    const bool inv = qp.s1 != c.orig;
+    const int n1 = (int)(inv ? c.p[2] : c.p[1]), n2 = (int)(inv ? c.p[1] : c.p[2]);
+    if (x) {
+        x[0] = n1 > 0 ? -c.p[0] * (n1 * ipow(qp.q1, n1 - 1) * ipow(qp.q2, n2)) : 0.0;
+        x[1] = n2 > 0 ? -c.p[0] * (n2 * ipow(qp.q1, n1) * ipow(qp.q2, n2 - 1)) : 0.0;
+    }
+    return c.p[0] * (ipow(qp.q1, n1) * ipow(qp.q2, n2));
+}
+
 // One factor at distance r > 0 (evaluate_energy_component and
 // evaluate_force_component, Potentials.f90:2717-2890; the per-type
 // routines at 2971-3085, 3643-3677, 4009-4298, 4803-4967).  cut is the
 // potential's cutoff (the spring's energy shift uses it).
 __device__ __forceinline__ Radial factor(const PairFactor &f, double r, double cut,
-                                         const double2 *__restrict__ tab, bool need_force)
+                                         const double2 *__restrict__ tab, bool need_force,
+                                         const ChargeFactor *cft, const QPair &qp)
 {
     Radial o{0.0, 0.0, 0.0};
     const double p0 = f.p[0], p1 = f.p[1], p2 = f.p[2], p3 = f.p[3];
@@ -119,6 +208,15 @@ __device__ __forceinline__ Radial factor(const PairFactor &f, double r, double c
         }
         break;
     }
+    case 12:  // charge_exp: no dependence on r (its force is the cutoff's)
+        o.v = charge_exp(cft[f.toff], qp, nullptr);
+        break;
+    case 13:  // charge_abs
+        o.v = charge_abs(cft[f.toff], qp, nullptr);
+        break;
+    case 14:  // charge_pair
+        o.v = charge_pair(cft[f.toff], qp, nullptr);
+        break;
     default:
         break;
     }
@@ -132,14 +230,15 @@ __device__ __forceinline__ Radial factor(const PairFactor &f, double r, double c
 // of the nonzero energies and the number of zero ones; the second
 // re-evaluates each factor with its force.
 __device__ __forceinline__ Radial potential(const GenPair &T, double r,
-                                            const double2 *__restrict__ tab, bool need_force)
+                                            const double2 *__restrict__ tab, bool need_force,
+                                            const ChargeFactor *cft, const QPair &qp)
 {
     const int nf = T.nf;
-    if (nf == 1) return factor(T.f[0], r, T.cut, tab, need_force);
+    if (nf == 1) return factor(T.f[0], r, T.cut, tab, need_force, cft, qp);
     double e = 1.0, enz = 1.0;
     int zeros = 0;
     for (int k = 0; k < nf; ++k) {
-        const double v = factor(T.f[k], r, T.cut, tab, false).v;
+        const double v = factor(T.f[k], r, T.cut, tab, false, cft, qp).v;
         e = e * v;
         if (v != 0.0) enz = enz * v;
         else ++zeros;
@@ -147,12 +246,49 @@ __device__ __forceinline__ Radial potential(const GenPair &T, double r,
     Radial out{e, 0.0, 0.0};
     if (!need_force) return out;
     for (int k = 0; k < nf; ++k) {
-        const Radial q = factor(T.f[k], r, T.cut, tab, true);
+        const Radial q = factor(T.f[k], r, T.cut, tab, true, cft, qp);
         const double w = q.v != 0.0 ? e / q.v : (zeros == 1 ? enz : 0.0);
         out.g += w * q.g;
         out.c += w * q.c;
     }
     return out;
+}
+
+// -dE/dq of both atoms of a potential (evaluate_electronegativity,
+// Potentials.f90:2580-2637): x = sum_k (E / E_k) x_k over the factors, or
+// prod_{j != k} E_j x_k where E_k = 0; factors without charge dependence
+// have x_k = 0.
+__device__ void potential_chi(const GenPair &T, double r, const double2 *__restrict__ tab,
+                              const ChargeFactor *cft, const QPair &qp, double *x)
+{
+    x[0] = x[1] = 0.0;
+    double me[PYSIC_MAXF], mx[PYSIC_MAXF][2], e = 1.0;
+    const int nf = T.nf;
+    for (int k = 0; k < nf; ++k) {
+        const PairFactor &f = T.f[k];
+        mx[k][0] = mx[k][1] = 0.0;
+        if (f.type == 12) me[k] = charge_exp(cft[f.toff], qp, mx[k]);
+        else if (f.type == 13) me[k] = charge_abs(cft[f.toff], qp, mx[k]);
+        else if (f.type == 14) me[k] = charge_pair(cft[f.toff], qp, mx[k]);
+        else me[k] = factor(f, r, T.cut, tab, false, cft, qp).v;
+    }
+    if (nf == 1) {
+        x[0] = mx[0][0];
+        x[1] = mx[0][1];
+        return;
+    }
+    for (int k = 0; k < nf; ++k) e = e * me[k];
+    for (int k = 0; k < nf; ++k) {
+        double w = 1.0;
+        if (me[k] != 0.0) {
+            w = e / me[k];
+        } else {
+            for (int j = 0; j < nf; ++j)
+                if (j != k) w = w * me[j];
+        }
+        x[0] = x[0] + w * mx[k][0];
+        x[1] = x[1] + w * mx[k][1];
+    }
 }
 
 // Lennard-Jones alone, the common molecular-dynamics model: the LJPair
@@ -226,7 +362,8 @@ __device__ __forceinline__ bool visit(const LJPair &q, double r2, double &v, dou
 
 template <bool FORCE>
 __device__ __forceinline__ bool visit(const PairLds<0> &L, int si, int sj, double r2, double &v,
-                                      double &G, double &C, const double2 *__restrict__ tab)
+                                      double &G, double &C, const double2 *__restrict__ tab,
+                                      const QPair &qp)
 {
     const PairModel &P = L.pm;
     const int np = P.npair[si][sj];
@@ -239,10 +376,10 @@ __device__ __forceinline__ bool visit(const PairLds<0> &L, int si, int sj, doubl
         const GenPair &T = P.pair[si][sj][c];
         if (!(T.cut > r)) continue;
         if (!FORCE) {
-            const Radial q = potential(T, r, tab, false);
+            const Radial q = potential(T, r, tab, false, P.cf, qp);
             v += T.smooth ? q.v * fc64(r, T.cut, T.soft) : q.v;
         } else if (r > 0.0) {  // the reference's forces vanish at r == 0
-            const Radial q = potential(T, r, tab, true);
+            const Radial q = potential(T, r, tab, true, P.cf, qp);
             if (T.smooth) {
                 // f1 fc + v grad(fc), grad(fc) = fc' s / r
                 const double f = fc64(r, T.cut, T.soft), df = dfc64(r, T.cut, T.soft);
@@ -275,6 +412,7 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                                               const int *__restrict__ ncnt,
                                               const PairModel *__restrict__ PM,
                                               const double2 *__restrict__ tab,
+                                              const double *__restrict__ q,
                                               double4 *__restrict__ frc, int accumulate,
                                               double4 *__restrict__ mom, double half_dt, int kick,
                                               double *__restrict__ partial)
@@ -305,8 +443,12 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
         }
         const double *cv = geo.vec(i);
         const int nf = ncnt[i];
+        // pick decides the sign of a 'step' force and which atom is atom 1
+        // of a charge factor
         const bool pick = KIND == 0 && FORCE && PM->needs_pick;
-        const int idi = pick ? id[i] : 0;
+        const bool charged = KIND == 0 && PM->has_charge;
+        const int idi = pick || charged ? id[i] : 0;
+        const double qi = charged ? q[i] : 0.0;
         LJPair one{};
         if constexpr (KIND == 2) one = PM->lj[0][0];
         // list rows t0 .. t0 + NBU - 1 of atom i (column-major: row t at t n + i)
@@ -335,7 +477,16 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                 bool on;
                 if constexpr (KIND == 2) on = visit<FORCE>(one, r2, v, G, C);
                 else if constexpr (KIND == 1) on = visit<FORCE>(lds.lj[si][(int)pv[u].w], r2, v, G, C);
-                else on = visit<FORCE>(lds, si, (int)pv[u].w, r2, v, G, C, tab);
+                else {
+                    QPair qp{0.0, 0.0, 0};
+                    if (charged) {
+                        const int j = jv[u], sj = (int)pv[u].w;
+                        const double qj = q[j];
+                        if (pick_dev(idi, id[j], iv[u])) qp = QPair{qi, qj, si};
+                        else qp = QPair{qj, qi, sj};
+                    }
+                    on = visit<FORCE>(lds, si, (int)pv[u].w, r2, v, G, C, tab, qp);
+                }
                 if (!on) continue;
                 if (!FORCE) {
                     e += 0.5 * v;
@@ -405,6 +556,58 @@ __global__ __launch_bounds__(PB) void k_pair_totals(int nblocks, const double *_
     if (tid == 0) totals[c] = red[0];
 }
 
+// The charge-dependent pair terms' share of the electronegativities:
+// chi[i] += sum over i's pairs of -dE/dq_i f_c(r).  One thread per atom
+// walks its full list; each pair is oriented as the reference takes it, and
+// each end adds its own atom's share (the reference adds both at once).
+template <class Geo>
+__global__ __launch_bounds__(PB) void k_pair_chi(int n, Geo geo, const double4 *__restrict__ pos,
+                                                 const int *__restrict__ id,
+                                                 const int *__restrict__ nbr,
+                                                 const int *__restrict__ nimg,
+                                                 const int *__restrict__ ncnt,
+                                                 const PairModel *__restrict__ PM,
+                                                 const double2 *__restrict__ tab,
+                                                 const double *__restrict__ q,
+                                                 double *__restrict__ chi)
+{
+    __shared__ PairLds<0> lds;
+    stage(lds, PM);
+    __syncthreads();
+    const int i = xcd_block(gridDim.x) * PB + threadIdx.x;
+    if (i >= n) return;
+    const PairModel &P = lds.pm;
+    const double4 pi = pos[i];
+    const int si = (int)pi.w, idi = id[i];
+    const double qi = q[i];
+    const double *cv = geo.vec(i);
+    const int nf = ncnt[i];
+    double x = 0.0;
+    for (int t = 0; t < nf; ++t) {
+        const int j = nbr[(size_t)t * n + i], img = nimg[(size_t)t * n + i];
+        const double4 pj = pos[j];
+        const int sj = (int)pj.w;
+        const int np = P.npair[si][sj];
+        if (np == 0) continue;
+        const double3 d = separation64(pi, pj, img, cv);
+        const double r2 = dot64(d);
+        if (!(r2 < P.cut2_hi[si][sj])) continue;
+        const double r = sqrt(r2);
+        const bool first = pick_dev(idi, id[j], img);
+        const double qj = q[j];
+        const QPair qp = first ? QPair{qi, qj, si} : QPair{qj, qi, sj};
+        for (int c = 0; c < np; ++c) {
+            const GenPair &T = P.pair[si][sj][c];
+            if (!(T.cut > r)) continue;
+            double xs[2];
+            potential_chi(T, r, tab, P.cf, qp, xs);
+            const double s = first ? xs[0] : xs[1];
+            x += T.smooth ? s * fc64(r, T.cut, T.soft) : s;
+        }
+    }
+    chi[i] += x;
+}
+
 template <class Geo, int KIND>
 void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt, int kick)
 {
@@ -412,8 +615,8 @@ void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt
     const int acc = accumulate ? 1 : 0;
     auto args = [&](auto kern) {
         kern<<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p,
-                                      S.dpmodel.p, S.tables.p, S.frc.p, acc, S.mom.p, half_dt,
-                                      kick, S.gpartial.p);
+                                      S.dpmodel.p, S.tables.p, S.q.p, S.frc.p, acc, S.mom.p,
+                                      half_dt, kick, S.gpartial.p);
     };
     if (mode == PAIR_ENERGY) args(k_pairs<Geo, PAIR_ENERGY, KIND>);
     else if (mode == PAIR_FORCES) args(k_pairs<Geo, PAIR_FORCES, KIND>);
@@ -447,4 +650,17 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
         HIP_TRY(hipGetLastError());
     }
     S.timer.end(name, S.stream, ev);
+}
+
+void pr_chi(System &S, double *chi)
+{
+    if (!S.pmodel.has_charge) return;
+    PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
+    const int nb = div_up(S.n, PB);
+    with_geo(S, [&](auto geo) {
+        k_pair_chi<decltype(geo)><<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p,
+                                                          S.nimg.p, S.ncnt.p, S.dpmodel.p,
+                                                          S.tables.p, S.q.p, chi);
+    });
+    HIP_TRY(hipGetLastError());
 }

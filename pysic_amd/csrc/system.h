@@ -155,6 +155,8 @@ int ts_variant_for_capacity(int need);
 // accumulate: add to frc instead of overwriting it.
 enum { PAIR_ENERGY = 0, PAIR_FORCES = 1, PAIR_FORCES_ONLY = 2 };
 void pr_eval(System &S, int mode, bool accumulate, double half_dt = 0.0, int kick = 0);
+// chi += the charge-dependent pair terms' -dE/dq (nothing without them)
+void pr_chi(System &S, double *chi);
 
 // md.hip
 void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m (no rebuild)

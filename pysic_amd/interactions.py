@@ -43,18 +43,20 @@ POTENTIAL_TYPES = {
     "step": (2, ["e1", "e2", "r1", "r2"]),
     "constant": (1, ["V"]),
     "force": (1, ["Fx", "Fy", "Fz"]),
-    # charge-dependent pair terms of COMB (Potentials.f90:3691-3903, 4524-4731)
+    # charge-dependent pair terms of COMB (Potentials.f90:3691-3903, 4384-4731)
     "charge_exp": (2, ["epsilon", "Rmax1", "Rmin1", "Qmax1", "Qmin1", "Rmax2", "Rmin2",
                        "Qmax2", "Qmin2", "xi1", "xi2"]),
     "charge_abs": (2, ["a1", "b1", "Q1", "n1", "a2", "b2", "Q2", "n2"]),
+    "charge_pair": (2, ["epsilon", "n1", "n2"]),
 }
 # descriptor type codes (include/pysic_hip.h)
 POTENTIAL_CODES = {"exponential": 1, "charge_self": 2, "LJ": 3, "spring": 4, "Buckingham": 5,
                    "power": 6, "tabulated": 7, "shift_power": 8, "step": 9, "constant": 10,
-                   "force": 11, "charge_exp": 12, "charge_abs": 13}
+                   "force": 11, "charge_exp": 12, "charge_abs": 13,
+                   "charge_pair": 14}
 # types whose energy depends on the charges and on which atom of the pair
 # the parameters belong to; their rows are followed by a continuation row
-CHARGE_PAIR_TYPES = ("charge_exp", "charge_abs")
+CHARGE_PAIR_TYPES = ("charge_exp", "charge_abs", "charge_pair")
 
 # name -> (n_targets, level, [[scaling params], [local params]])
 BOND_ORDER_TYPES = {

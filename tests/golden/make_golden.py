@@ -153,15 +153,16 @@ def run_pairs(name, struct, model, tables=None, skin=0.5):
     print(name, len(struct), "E = %.12f" % e)
 
 
-def comb_charge_pairs():
+def comb_charge_pairs(name="sio2_48_comb_charge_pairs", asymmetric=False, seed=21):
     """COMB's charge-dependent pair terms on amorphous SiO2 (48 atoms, no
     coordinators, random charges): energy, forces, stress and the
-    electronegativities -dE/dq."""
-    from helpers import comb_charge_model, comb_charges, ref_entries
-    s = st.amorphous_sio2(16, 2.2, seed=21)
-    q = comb_charges(s.symbols, seed=22)
+    electronegativities -dE/dq; asymmetric adds terms whose value depends
+    on which atom of a pair the reference takes first."""
+    from helpers import comb_asymmetric_terms, comb_charge_model, comb_charges, ref_entries
+    s = st.amorphous_sio2(16, 2.2, seed=seed)
+    q = comb_charges(s.symbols, seed=seed + 1)
     names = sorted(set(s.symbols))
-    model = comb_charge_model()
+    model = comb_charge_model() + (comb_asymmetric_terms() if asymmetric else [])
     pd, _, _ = build_descriptors(model, {x: k for k, x in enumerate(names)})
     sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
     cuts = np.full(len(s), 3.05 + 0.5)
@@ -171,11 +172,11 @@ def comb_charge_pairs():
     e = R.energy()
     f, w = R.forces()
     x = R.electronegativities()
-    np.savez_compressed(os.path.join(HERE, "sio2_48_comb_charge_pairs.npz"),
+    np.savez_compressed(os.path.join(HERE, name + ".npz"),
                         positions=s.positions, cell=s.cell, pbc=s.pbc.astype(np.int32),
                         species=sp, symbols=np.array(s.symbols), charges=q, potdesc=pd,
                         cutoffs=cuts, energy=e, forces=f, stress=w, enegs=x)
-    print("sio2_48_comb_charge_pairs", len(s), "E = %.12f" % e)
+    print(name, len(s), "E = %.12f" % e)
 
 
 def pair_cases():
@@ -221,6 +222,7 @@ def main():
     scaled_ewald()
     sic_mixed()
     comb_charge_pairs()
+    comb_charge_pairs("sio2_48_charge_terms_asymmetric", asymmetric=True, seed=23)
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
     rng = np.random.default_rng(8)
@@ -240,6 +242,7 @@ if __name__ == "__main__":
         sic_mixed()
     elif sys.argv[1:] == ["comb_charge_pairs"]:
         comb_charge_pairs()
+        comb_charge_pairs("sio2_48_charge_terms_asymmetric", asymmetric=True, seed=23)
     elif sys.argv[1:] == ["scaled_ewald"]:
         scaled_ewald()
     else:

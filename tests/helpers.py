@@ -130,6 +130,33 @@ def comb_charge_model():
     return pots
 
 
+def comb_asymmetric_terms():
+    """Charge terms whose value depends on which atom of a pair comes
+    first: a Si-Si charge_exp with unequal halves, a product written for
+    (Si, O) and (O, O) whose charge_abs multiplier keeps the first tuple's
+    original targets, so on O-O pairs its halves always swap, charge_pair
+    q1^2 q2 on Si-O and O-O, and a power x charge_pair q1 q2^3 product on
+    Si-Si."""
+    C = COMB
+    P = Potential
+    si_si = P("charge_exp", symbols=[["Si", "Si"]], cutoff=3.0, cutoff_margin=0.3,
+              parameters=[5.0, C["Si"]["rmax"], C["Si"]["rmin"], C["Si"]["qmax"], C["Si"]["qmin"],
+                          C["O"]["rmax"], C["O"]["rmin"], C["O"]["qmax"], C["O"]["qmin"],
+                          0.9, 1.6])
+    mixed = ProductPotential([
+        P("exponential", symbols=[["Si", "O"], ["O", "O"]], parameters=[3.0, 1.1], cutoff=3.05,
+          cutoff_margin=0.5),
+        P("charge_abs", symbols=[["Si", "O"], ["O", "O"]],
+          parameters=[1.2, -0.05, 0.3, 2.0, 0.8, -0.02, -0.7, 3.0])])
+    qpair = P("charge_pair", symbols=[["Si", "O"], ["O", "O"]], parameters=[0.7, 2.0, 1.0],
+              cutoff=3.05, cutoff_margin=0.5)
+    qprod = ProductPotential([
+        P("power", symbols=[["Si", "Si"]], parameters=[1.0, 2.5, 1.0], cutoff=3.0,
+          cutoff_margin=0.4),
+        P("charge_pair", symbols=[["Si", "Si"]], parameters=[-0.3, 1.0, 3.0])])
+    return [si_si, mixed, qpair, qprod]
+
+
 def comb_charges(symbols, seed):
     """Random charges in the ranges COMB SiO2 visits: Si in [0.5, 2.2],
     O in [-1.3, -0.2] (not neutral: no Ewald term here)."""

@@ -1,16 +1,20 @@
 """CPU tests that pin the oracle's charge-dependent pair terms of COMB
-(oracle/pysic_oracle.c: orc_charge_exp, orc_charge_abs, orc_product_eneg).
+(oracle/pysic_oracle.c: orc_charge_exp, orc_charge_abs, orc_charge_pair,
+orc_product_eneg).
 
 Pins, each chosen so that a plausible slip fails one of them:
-  * the golden vectors of the reference Fortran core
-    (tests/golden/sio2_48_comb_charge_pairs.npz: COMB's repulsive and
-    attractive products for Si-Si, O-O, Si-O and a lone charge_abs term on
-    amorphous SiO2 with random charges): energy, forces, stress and
+  * the golden vectors of the reference Fortran core on amorphous SiO2 with
+    random charges: tests/golden/sio2_48_comb_charge_pairs.npz (COMB's
+    repulsive and attractive products for Si-Si, O-O, Si-O and a lone
+    charge_abs term) and sio2_48_charge_terms_asymmetric.npz (the same plus
+    terms whose value depends on which atom of a pair comes first, and
+    charge_pair alone and in a product): energy, forces, stress and
     electronegativities;
   * closed forms the screening function fixes: D(Qmax) = Rmax and
     D(Qmin) = Rmin, so charge_exp at those charges is
     epsilon exp((xi1 Rmax1 + xi2 Rmin2)/2); charge_abs at q = Q is
-    sqrt(a1 a2) and at q = Q + 1 is sqrt((a1 + b1) a2);
+    sqrt(a1 a2) and at q = Q + 1 is sqrt((a1 + b1) a2); charge_pair is
+    eps q1^n1 q2^n2 with chi = -dE/dq in closed form;
   * the parameter swap when the pair's first atom is not the element the
     parameters 1 were given for: the same potential written for (O, Si)
     with its parameter halves exchanged gives the same energy;
@@ -22,21 +26,23 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from helpers import COMB, comb_charge_model, comb_charges, ref_entries, rel_maxnorm
+from helpers import (COMB, comb_asymmetric_terms, comb_charge_model, comb_charges, ref_entries,
+                     rel_maxnorm)
 from pysic_amd import structures as st
 from pysic_amd.interactions import Potential, build_descriptors
 
-NAME = "sio2_48_comb_charge_pairs"
+NAMES = ["sio2_48_comb_charge_pairs", "sio2_48_charge_terms_asymmetric"]
 
 
-def _golden(O):
-    g = load_golden(NAME)
+def _golden(O, name):
+    g = load_golden(name)
     nl = O.neighbor_list(g["positions"], g["cell"], g["pbc"], g["cutoffs"])
     return g, nl, (g["species"], g["positions"], g["charges"], g["cell"], nl, g["potdesc"])
 
 
-def test_comb_matches_reference_core(oracle):
-    g, _, args = _golden(oracle)
+@pytest.mark.parametrize("name", NAMES)
+def test_comb_matches_reference_core(oracle, name):
+    g, _, args = _golden(oracle, name)
     e = oracle.evaluate("energy", *args)
     f, s = oracle.evaluate("forces", *args)
     x = oracle.evaluate("enegs", *args)
@@ -46,8 +52,9 @@ def test_comb_matches_reference_core(oracle):
     assert rel_maxnorm(x, g["enegs"]) <= 1e-12
 
 
-def test_comb_enegs_are_charge_gradient(oracle):
-    g, nl, args = _golden(oracle)
+@pytest.mark.parametrize("name", NAMES)
+def test_comb_enegs_are_charge_gradient(oracle, name):
+    g, nl, args = _golden(oracle, name)
     sp, pos, q, cell, _, pd = args
     x = oracle.evaluate("enegs", *args)
     h = 1e-5
@@ -60,8 +67,9 @@ def test_comb_enegs_are_charge_gradient(oracle):
         assert abs(x[i] + d) <= 1e-6 * max(1.0, abs(d)), i
 
 
-def test_comb_forces_are_energy_gradient(oracle):
-    g, nl, args = _golden(oracle)
+@pytest.mark.parametrize("name", NAMES)
+def test_comb_forces_are_energy_gradient(oracle, name):
+    g, nl, args = _golden(oracle, name)
     sp, pos, q, cell, _, pd = args
     f, _ = oracle.evaluate("forces", *args)
     h = 1e-5
@@ -124,16 +132,31 @@ def test_charge_abs_closed_form(oracle, order):
         assert abs(e - want) <= 1e-13 * want, dq
 
 
-def test_reference_core_agrees_on_fresh_comb(oracle):
+@pytest.mark.parametrize("order", [("Si", "O"), ("O", "Si")])
+def test_charge_pair_closed_form(oracle, order):
+    pot = Potential("charge_pair", symbols=[["Si", "O"]], parameters=[0.7, 2.0, 1.0],
+                    cutoff=3.05, cutoff_margin=0.5)
+    q = {"Si": 1.3, "O": -0.8}
+    e, x = _pair(oracle, [pot], order[0], order[1], [q[order[0]], q[order[1]]])
+    want = 0.7 * 1.3 ** 2 * -0.8
+    assert abs(e - want) <= 1e-14 * abs(want)
+    xs = {"Si": -0.7 * 2 * 1.3 * -0.8, "O": -0.7 * 1.3 ** 2}
+    assert np.abs(x - [xs[order[0]], xs[order[1]]]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("model", ["comb", "asymmetric"])
+def test_reference_core_agrees_on_fresh_comb(oracle, model):
     """The C restatement against the reference Fortran core on a structure
     and charges none of the fixtures holds (only where the reference build
-    exists)."""
+    exists); 'asymmetric' adds terms whose value depends on which atom the
+    reference takes first."""
     if not oracle.ref_available():
         pytest.skip("reference checker not built (oracle/build_ref.sh)")
     s = st.amorphous_sio2(24, 2.4, seed=31)
     q = comb_charges(s.symbols, seed=32)
     names = sorted(set(s.symbols))
-    pd, _, _ = build_descriptors(comb_charge_model(), {x: k for k, x in enumerate(names)})
+    pots = comb_charge_model() + (comb_asymmetric_terms() if model == "asymmetric" else [])
+    pd, _, _ = build_descriptors(pots, {x: k for k, x in enumerate(names)})
     sp = np.array([names.index(x) for x in s.symbols], np.int32)
     cut = np.full(len(s), 3.55)
     R = oracle.RefCore()
