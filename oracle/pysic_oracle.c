@@ -292,7 +292,8 @@ typedef struct {
     double p[13];
     const double *tab; /* tabulated: n rows of (value, derivative) */
     int tlen;
-    int orig;          /* charge_exp / charge_abs: species of the original first target */
+    int orig;          /* charge-dependent types: species of the original first target */
+    double c0;         /* bond_bend: cos(theta_0)^n (Potentials.f90:3178-3185) */
     double eta[2], beta[2]; /* charge_exp derived parameters (Potentials.f90:3747-3766) */
 } fac_t;
 /* what a charge-dependent factor reads of its pair: the first atom's
@@ -302,7 +303,8 @@ typedef struct {
     double q1, q2;
 } qpair_t;
 typedef struct {
-    int nt, e[2], group, smooth, nf;
+    int nt, e[3], group, smooth, nf;
+    int o[3];          /* bond_bend: the original targets (o[1] the vertex) */
     double cut, soft;
     fac_t f[ORC_MAXF]; /* f[0] the potential, then its multipliers */
 } pot_t;
@@ -742,10 +744,19 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     if (!S->pot || !S->bond) return ORC_ENOMEM;
     int m = 0;
     fac_t *last = NULL;   /* the factor a continuation row completes */
+    pot_t *lastbend = NULL;
     for (int k = 0; k < npot; ++k) {
         const double *d = pd + ORC_POT_STRIDE * k;
         int type = (int)d[0];
-        if (type < 1 || type > 14) return ORC_EINVAL;
+        if (type < 1 || type > 15) return ORC_EINVAL;
+        if (d[11] == 2.0 && type == 15) { /* the third target and the original targets */
+            if (!lastbend) return ORC_EINVAL;
+            lastbend->e[2] = (int)d[1];
+            for (int q = 0; q < 3; ++q) lastbend->o[q] = (int)d[2 + q];
+            lastbend = NULL;
+            continue;
+        }
+        if (lastbend) return ORC_EINVAL;
         if (d[11] == 2.0) { /* parameters 5.. and original_elements(1) of the row above */
             if (!last || last->type != type || !orc_is_charge(type)) return ORC_EINVAL;
             for (int q = 0; q < 9; ++q) last->p[4 + q] = d[2 + q];
@@ -776,7 +787,7 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
         if (d[11] != 0.0) { /* a multiplier of the previous potential */
             if (m == 0) return ORC_EINVAL;
             pot_t *q = S->pot + m - 1;
-            if (q->nf == ORC_MAXF || q->nt != 2 || single) return ORC_EINVAL;
+            if (q->nf == ORC_MAXF || q->nt != 2 || single || type == 15) return ORC_EINVAL;
             q->f[q->nf++] = fac;
             if (orc_is_charge(type)) last = q->f + q->nf - 1;
             continue;
@@ -785,6 +796,11 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
         p->nf = 1;
         p->f[0] = fac;
         if (orc_is_charge(type)) last = p->f;
+        if (type == 15) {
+            lastbend = p;
+            /* calculate_derived_parameters_bond_bending (Potentials.f90:3178-3185) */
+            p->f[0].c0 = pow(cos(p->f[0].p[1]), p->f[0].p[2]);
+        }
         p->nt = (int)d[1];
         p->e[0] = (int)d[2];
         p->e[1] = (int)d[3];
@@ -793,9 +809,10 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
         if (d[7] > 0.0 && d[7] < d[6]) { p->soft = d[7]; p->smooth = 1; }
         else { p->soft = d[6]; p->smooth = 0; }
         p->group = (int)d[8];
-        if (p->nt != (single ? 1 : 2)) return ORC_EINVAL;
+        if (p->nt != (single ? 1 : type == 15 ? 3 : 2)) return ORC_EINVAL;
+        if (type == 15 && p->group > -1) return ORC_EINVAL;
     }
-    if (last) return ORC_EINVAL;
+    if (last || lastbend) return ORC_EINVAL;
     S->npot = m;
     for (int k = 0; k < m; ++k) /* bond-order scaling: plain exponentials only */
         if (S->pot[k].group > -1 && (S->pot[k].f[0].type != 1 || S->pot[k].nf != 1))
@@ -1058,6 +1075,86 @@ static int orc_ewald_kspace(const sys_t *S, const ewald_t *E, double volume, int
  * orc_neighbor_list.  ewald_desc = {on, rc, kc, K1, K2, K3, sigma, eps};
  * scaler may be NULL (all ones).
  */
+/* bond_bend on the triplet (a, v, b) with v the vertex, s1 = r_v - r_a,
+ * s2 = r_b - r_v (evaluate_energy_bond_bending and
+ * evaluate_force_bond_bending, Potentials.f90:3196-3328): zero unless the
+ * vertex is the original second target.  Returns E; with f, the forces on
+ * a, v, b. */
+static double orc_bend(const pot_t *P, int spv, const double *s1, double r1, const double *s2,
+                       double r2, double f[3][3])
+{
+    const fac_t *fc = P->f;
+    if (f) memset(f, 0, 9 * sizeof(double));
+    if (P->o[1] != spv) return 0.0;
+    int n = (int)fc->p[2], m = (int)fc->p[3];
+    double t1[3] = {-s1[0], -s1[1], -s1[2]};
+    double r3 = orc_dot(t1, s2);
+    double e = fc->p[0] * orc_ipow(orc_ipow(r3 / (r1 * r2), n) - fc->c0, m);
+    if (f) {
+        double t3[3], t4[3];
+        for (int a = 0; a < 3; ++a) t3[a] = s2[a] - r3 / (r1 * r1) * t1[a];
+        for (int a = 0; a < 3; ++a) t4[a] = t1[a] - r3 / (r2 * r2) * s2[a];
+        double ratio = 1.0 / (r1 * r2);
+        double cosplain = r3 * ratio;
+        double cospower = orc_ipow(cosplain, n - 1);
+        double r6 = fc->p[0] * (double)(m * n) * orc_ipow(cospower * cosplain - fc->c0, m - 1) *
+                    cospower;
+        for (int a = 0; a < 3; ++a) {
+            f[0][a] = -t3[a] * ratio * r6;
+            f[1][a] = (t3[a] + t4[a]) * ratio * r6;
+            f[2][a] = -t4[a] * ratio * r6;
+        }
+    }
+    return e;
+}
+
+/* One triplet of core_evaluate_local_triplet_B (Core.f90:6656-7028) for
+ * every many-body potential of the pair that matches it: atoms (a, v, b),
+ * s1 = r_v - r_a with direction d1, s2 = r_b - r_v with direction d2.
+ * Bond-order scaling of triplets is not supported (no coordinator). */
+static void orc_triplet(const sys_t *S, int calc, const int *mb, int nmb, int ia, int iv, int ib,
+                        const double *s1, const double *d1, double r1, const double *s2,
+                        const double *d2, double r2, double *en, double *F, double *st)
+{
+    for (int k = 0; k < nmb; ++k) {
+        const pot_t *P = S->pot + mb[k];
+        if (P->e[2] != S->sp[ib] || !(P->cut > r2)) continue;
+        double fc1 = 1.0, fc2 = 1.0;
+        if (P->smooth) {
+            fc1 = orc_fc(r1, P->cut, P->soft);
+            fc2 = orc_fc(r2, P->cut, P->soft);
+        }
+        if (calc == 1) {
+            double e = orc_bend(P, S->sp[iv], s1, r1, s2, r2, NULL);
+            *en = *en + e * fc1 * fc2 * (1.0 + 1.0 + 1.0) / 3.0;
+        } else if (calc == 2) {
+            double e = P->smooth ? orc_bend(P, S->sp[iv], s1, r1, s2, r2, NULL) : 0.0;
+            double cg1[3] = {0, 0, 0}, cg2[3] = {0, 0, 0};
+            if (P->smooth) {
+                double g1 = orc_dfc(r1, P->cut, P->soft), g2 = orc_dfc(r2, P->cut, P->soft);
+                for (int a = 0; a < 3; ++a) { cg1[a] = d1[a] * g1; cg2[a] = d2[a] * g2; }
+            }
+            double f[3][3], tf[3][3];
+            orc_bend(P, S->sp[iv], s1, r1, s2, r2, f);
+            for (int a = 0; a < 3; ++a) {
+                tf[0][a] = f[0][a] * fc1 * fc2 + cg1[a] * fc2 * e;
+                tf[1][a] = f[1][a] * fc1 * fc2 + (-cg1[a] * fc2 + cg2[a] * fc1) * e;
+                tf[2][a] = f[2][a] * fc1 * fc2 - cg2[a] * fc1 * e;
+                F[3 * ia + a] += tf[0][a];
+                F[3 * iv + a] += tf[1][a];
+                F[3 * ib + a] += tf[2][a];
+            }
+            st[0] = st[0] + s1[0] * tf[1][0] + (s1[0] + s2[0]) * tf[2][0];
+            st[1] = st[1] + s1[1] * tf[1][1] + (s1[1] + s2[1]) * tf[2][1];
+            st[2] = st[2] + s1[2] * tf[1][2] + (s1[2] + s2[2]) * tf[2][2];
+            st[3] = st[3] + s1[1] * tf[1][2] + (s1[1] + s2[1]) * tf[2][2];
+            st[4] = st[4] + s1[0] * tf[1][2] + (s1[0] + s2[0]) * tf[2][2];
+            st[5] = st[5] + s1[0] * tf[1][1] + (s1[0] + s2[0]) * tf[2][1];
+        }
+        /* bond_bend carries no charge: nothing for the electronegativities */
+    }
+}
+
 int orc_evaluate(int calc, int n, const int *species, const double *pos, const double *charge,
                  const double *cell, const double *inv, const int *nbstart, const int *nbidx,
                  const int *nboff, int npot, const double *potdesc, int nbond,
@@ -1162,6 +1259,16 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                 }
             }
             if (!(dist < S.maxpot[i1])) continue;
+            /* the many-body potentials the doublet routine finds for this
+             * pair: first target atom 1, second atom 2, cutoff > r
+             * (Core.f90:4293-4378) */
+            int mb[64], nmb = 0;
+            for (int k = 0; k < S.npot && nmb < 64; ++k) {
+                const pot_t *p = S.pot + k;
+                if (p->nt == 3 && p->e[0] == species[i1] && p->cut > dist &&
+                    p->e[1] == species[i2])
+                    mb[nmb++] = k;
+            }
             /* doublets (Core.f90:4257-4385, 5371-5606) */
             for (int k = 0; k < S.npot; ++k) {
                 const pot_t *p = S.pot + k;
@@ -1218,6 +1325,39 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                     X[i1] += x[0] * cutf;
                     X[i2] += x[1] * cutf;
                 }
+            }
+            if (nmb == 0 || calc == 3) continue;
+            /* triplets (Core.f90:3567-3610, 3760-3800): atom 1 as the vertex,
+             * (atom2, atom1, atom3) with pick(atom2, atom3); then atom 2 as
+             * the vertex, (atom1, atom2, atom3) with pick(atom1, atom3) */
+            double ms[3] = {-sep[0], -sep[1], -sep[2]}, md[3] = {-dir[0], -dir[1], -dir[2]};
+            for (int t3 = nbstart[i1]; t3 < nbstart[i1 + 1]; ++t3) {
+                int i3 = nbidx[t3];
+                const int *o3 = nboff + 3 * t3;
+                int toff[3] = {o3[0] - off[0], o3[1] - off[1], o3[2] - off[2]};
+                if (!orc_pick(i2, i3, toff)) continue;
+                double s2[3], d2[3];
+                orc_separation(pos + 3 * i1, pos + 3 * i3, o3, cell, s2);
+                double r2 = sqrt(orc_dot(s2, s2));
+                if (r2 == 0.0) d2[0] = d2[1] = d2[2] = 0.0;
+                else for (int a = 0; a < 3; ++a) d2[a] = s2[a] / r2;
+                if (r2 < S.maxpot[i1])
+                    orc_triplet(&S, calc, mb, nmb, i2, i1, i3, ms, md, dist, s2, d2, r2, &en, F,
+                                st);
+            }
+            for (int t3 = nbstart[i2]; t3 < nbstart[i2 + 1]; ++t3) {
+                int i3 = nbidx[t3];
+                const int *o3 = nboff + 3 * t3;
+                int toff[3] = {o3[0] + off[0], o3[1] + off[1], o3[2] + off[2]};
+                if (!orc_pick(i1, i3, toff)) continue;
+                double s2[3], d2[3];
+                orc_separation(pos + 3 * i2, pos + 3 * i3, o3, cell, s2);
+                double r2 = sqrt(orc_dot(s2, s2));
+                if (r2 == 0.0) d2[0] = d2[1] = d2[2] = 0.0;
+                else for (int a = 0; a < 3; ++a) d2[a] = s2[a] / r2;
+                if (r2 < S.maxpot[i2])
+                    orc_triplet(&S, calc, mb, nmb, i1, i2, i3, sep, dir, dist, s2, d2, r2, &en, F,
+                                st);
             }
         }
     }

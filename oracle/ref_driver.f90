@@ -77,7 +77,7 @@ contains
     noinds = -9
     mult = is_mult /= 0
     success = .false.
-    if (kind >= 1 .and. kind <= 14) then
+    if (kind >= 1 .and. kind <= 15) then
        call add_potential(n_targets, n_params, trim(pot_name(kind)), params, cutoff, soft, &
             elements, notags, noinds, elements, notags, noinds, group, mult, success)
     end if
@@ -101,7 +101,7 @@ contains
     noinds = -9
     mult = is_mult /= 0
     success = .false.
-    if (kind >= 1 .and. kind <= 14) then
+    if (kind >= 1 .and. kind <= 15) then
        call add_potential(n_targets, n_params, trim(pot_name(kind)), params, cutoff, soft, &
             elements, notags, noinds, orig_elements, notags, noinds, group, mult, success)
     end if
@@ -112,10 +112,10 @@ contains
   function pot_name(kind) result(name)
     integer(c_int), intent(in) :: kind
     character(len=11) :: name
-    character(len=11) :: names(14)
+    character(len=11) :: names(15)
     names = (/ 'exponential', 'charge_self', 'LJ         ', 'spring     ', 'Buckingham ', &
          'power      ', 'tabulated  ', 'shift_power', 'step       ', 'constant   ', &
-         'force      ', 'charge_exp ', 'charge_abs ', 'charge_pair' /)
+         'force      ', 'charge_exp ', 'charge_abs ', 'charge_pair', 'bond_bend  ' /)
     name = names(kind)
   end function pot_name
 

@@ -48,12 +48,14 @@ POTENTIAL_TYPES = {
                        "Qmax2", "Qmin2", "xi1", "xi2"]),
     "charge_abs": (2, ["a1", "b1", "Q1", "n1", "a2", "b2", "Q2", "n2"]),
     "charge_pair": (2, ["epsilon", "n1", "n2"]),
+    # the angle term of COMB (Potentials.f90:3150-3328)
+    "bond_bend": (3, ["epsilon", "theta_0", "n", "m"]),
 }
 # descriptor type codes (include/pysic_hip.h)
 POTENTIAL_CODES = {"exponential": 1, "charge_self": 2, "LJ": 3, "spring": 4, "Buckingham": 5,
                    "power": 6, "tabulated": 7, "shift_power": 8, "step": 9, "constant": 10,
                    "force": 11, "charge_exp": 12, "charge_abs": 13,
-                   "charge_pair": 14}
+                   "charge_pair": 14, "bond_bend": 15}
 # types whose energy depends on the charges and on which atom of the pair
 # the parameters belong to; their rows are followed by a continuation row
 CHARGE_PAIR_TYPES = ("charge_exp", "charge_abs", "charge_pair")
@@ -448,17 +450,23 @@ def build_descriptors(potentials, species_of):
         nt = pot.get_number_of_targets()
 
         def rows(f, e, link, orig):
-            """The factor's row; a charge-dependent factor adds a
-            continuation row (link 2): type, the species of its original
-            first target (its parameters 1 belong to that atom,
-            Potentials.f90:3797-3829), parameters 5-13."""
+            """The factor's row, and a continuation row (link 2) where the
+            row cannot hold the factor: for a charge-dependent factor its
+            type, the species of its original first target (its parameters
+            1 belong to that atom, Potentials.f90:3797-3829) and parameters
+            5-13; for 'bond_bend' its type, the third target and the three
+            original targets (the second is the vertex,
+            Potentials.f90:3201-3215)."""
             v = list(f.get_parameter_values())
             p = v + [0.0] * (13 - len(v))
             code = POTENTIAL_CODES[f.get_potential_type()]
             out = [[code, nt, e[0], e[1], p[0], p[1], pot.get_cutoff(), pot.get_soft_cutoff(),
                     group, p[2], p[3], link]]
+            o = [species_of.get(x, -1) for x in orig]
             if f.get_potential_type() in CHARGE_PAIR_TYPES:
-                out.append([code, orig] + p[4:13] + [2])
+                out.append([code, o[0]] + p[4:13] + [2])
+            elif f.get_potential_type() == "bond_bend":
+                out.append([code, e[2]] + o + [0.0] * 6 + [2])
             return out
 
         tuples = pot.get_symbols() or []
@@ -469,9 +477,9 @@ def build_descriptors(potentials, species_of):
                 e = [species_of[s] for s in perm] + [0] * (2 - len(perm))
                 # the reference keeps each copy's original targets; its
                 # multipliers carry the first target tuple (calculator.py:1241-1277)
-                pot_rows += rows(master, e, 0, species_of.get(targets[0], -1))
+                pot_rows += rows(master, e, 0, targets)
                 for m in mults:
-                    pot_rows += rows(m, e, 1, species_of.get(tuples[0][0], -1))
+                    pot_rows += rows(m, e, 1, tuples[0])
     for coord, g in coord_list:
         for bond in coord.get_bond_order_parameters():
             if bond.get_bond_order_type() != "tersoff":

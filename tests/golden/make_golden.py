@@ -179,6 +179,31 @@ def comb_charge_pairs(name="sio2_48_comb_charge_pairs", asymmetric=False, seed=2
     print(name, len(s), "E = %.12f" % e)
 
 
+def comb_bend():
+    """COMB's bond bending (Si-O-Si, O-Si-O) alone and with the charge
+    terms on amorphous SiO2 (48 atoms, random charges)."""
+    from helpers import comb_bend_terms, comb_charge_model, comb_charges, ref_entries
+    for name, extra in (("sio2_48_comb_bend", []), ("sio2_48_comb_bend_charge", None)):
+        s = st.amorphous_sio2(16, 2.2, seed=25)
+        q = comb_charges(s.symbols, seed=26)
+        names = sorted(set(s.symbols))
+        model = comb_bend_terms() + (comb_charge_model() if extra is None else extra)
+        pd, _, _ = build_descriptors(model, {x: k for k, x in enumerate(names)})
+        sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
+        cuts = np.full(len(s), 3.05 + 0.5)
+        R = O.RefCore()
+        R.setup(s.symbols, s.positions, s.cell, s.pbc, charges=q,
+                potentials=ref_entries(pd, names), cutoffs=cuts)
+        e = R.energy()
+        f, w = R.forces()
+        x = R.electronegativities()
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), positions=s.positions,
+                            cell=s.cell, pbc=s.pbc.astype(np.int32), species=sp,
+                            symbols=np.array(s.symbols), charges=q, potdesc=pd, cutoffs=cuts,
+                            energy=e, forces=f, stress=w, enegs=x)
+        print(name, len(s), "E = %.12f" % e)
+
+
 def pair_cases():
     from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table
     from pysic_amd.interactions import lj_argon
@@ -223,6 +248,7 @@ def main():
     sic_mixed()
     comb_charge_pairs()
     comb_charge_pairs("sio2_48_charge_terms_asymmetric", asymmetric=True, seed=23)
+    comb_bend()
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
     rng = np.random.default_rng(8)
@@ -243,6 +269,8 @@ if __name__ == "__main__":
     elif sys.argv[1:] == ["comb_charge_pairs"]:
         comb_charge_pairs()
         comb_charge_pairs("sio2_48_charge_terms_asymmetric", asymmetric=True, seed=23)
+    elif sys.argv[1:] == ["comb_bend"]:
+        comb_bend()
     elif sys.argv[1:] == ["scaled_ewald"]:
         scaled_ewald()
     else:

@@ -157,6 +157,18 @@ def comb_asymmetric_terms():
     return [si_si, mixed, qpair, qprod]
 
 
+def comb_bend_terms():
+    """COMB's bond bending with the reference builder's defaults
+    (comb.py:94-125, 636-662): Si-O-Si epsilon 2.6 eV, theta_0 2.508561734;
+    O-Si-O epsilon 0.3122 eV, theta_0 1.910611932; n = 1, m = 2; cutoff
+    3.05 A with a 0.5 A margin."""
+    P = Potential
+    return [P("bond_bend", symbols=[["Si", "O", "Si"]], cutoff=3.05, cutoff_margin=0.5,
+              parameters=[2.6, 2.508561734, 1, 2]),
+            P("bond_bend", symbols=[["O", "Si", "O"]], cutoff=3.05, cutoff_margin=0.5,
+              parameters=[0.3122, 1.910611932, 1, 2])]
+
+
 def comb_charges(symbols, seed):
     """Random charges in the ranges COMB SiO2 visits: Si in [0.5, 2.2],
     O in [-1.3, -0.2] (not neutral: no Ewald term here)."""
@@ -176,6 +188,13 @@ def ref_entries(pd, names):
     for r in np.asarray(pd, dtype=np.float64).reshape(-1, 12):
         kind = int(r[0])
         npar = len(POTENTIAL_TYPES[inv[kind]][1])
+        if r[11] == 2 and kind == POTENTIAL_CODES["bond_bend"]:
+            # the third target and the original targets
+            syms = last[2] + (names[int(r[1])],)
+            orig = tuple("Xx" if o < 0 else names[int(o)] for o in r[2:5])
+            e = out[-1]
+            out[-1] = e[:4] + (syms,) + e[5:7] + (orig,)
+            continue
         if r[11] == 2:
             params = last[1] + list(r[2:11])
             o1 = "Xx" if r[1] < 0 else names[int(r[1])]
@@ -189,7 +208,7 @@ def ref_entries(pd, names):
             continue
         params = [r[4], r[5], r[9], r[10]]
         nt = int(r[1])
-        syms = tuple(names[int(e)] for e in r[2:2 + nt])
+        syms = tuple(names[int(e)] for e in r[2:2 + min(nt, 2)])
         last = (int(r[11]), params, syms)
         if r[11] != 0:
             out[-1][6].append((kind, params[:npar]))
