@@ -15,19 +15,26 @@
  *   group (-1 none), p3, p4, link
  * type: 1 'exponential', 2 'charge_self', 3 'LJ', 4 'spring', 5 'Buckingham',
  *   6 'power', 7 'tabulated', 8 'shift_power', 9 'step', 10 'constant',
- *   11 'force', 12 'charge_exp', 13 'charge_abs', 14 'charge_pair'; p1..p4
- *   are the parameters in the reference's order (Potentials.f90:2909-4990).
- *   Only a lone 'exponential' may carry a bond-order group.
+ *   11 'force', 12 'charge_exp', 13 'charge_abs', 14 'charge_pair',
+ *   15 'bond_bend' (3 targets); p1..p4 are the parameters in the reference's
+ *   order (Potentials.f90:2909-4990).  Only a lone 'exponential' may carry a
+ *   bond-order group.
  * link: 0 starts a potential; 1 makes the entry a multiplier of the
  *   potential before it (a ProductPotential factor: add_potential with
  *   is_multiplier, calculator.py:1197-1369); a multiplier's targets, cutoffs
  *   and group are ignored (the product uses its first potential's); 2
- *   continues the entry before it, which must be of type 12-14, as
+ *   continues the entry before it, which must be of type 12-15; for 12-14
+ *   as
  *     type, s, p5, p6, ..., p13, 2
  *   with p5.. its further parameters and s the species of the potential's
  *   original first target (the element its parameters 1 belong to; -1 for
  *   one absent from the system), calculator.py:1241-1277 and
- *   Potentials.f90:3797-3829.  At most 16 distinct such factors.
+ *   Potentials.f90:3797-3829.  At most 16 distinct such factors.  A
+ *   'bond_bend' row (e1, e2 its first two targets) is followed by
+ *     15, e3, o1, o2, o3, 0, 0, 0, 0, 0, 0, 2
+ *   with e3 its third target and o1..o3 the potential's original targets
+ *   (o2, the vertex, decides where it acts, Potentials.f90:3201-3215).  At
+ *   most 12 such rows, no multipliers, no group.
  * Bond-order descriptors ('tersoff', 14 doubles per entry, no permutation as
  * calculator.py:1372):
  *   e1 e2 e3 (center, j, k), beta, eta, a, c, d, h, mu, cutoff, soft_cutoff,

@@ -76,8 +76,11 @@ void System::release_all()
                           &gtotals, &atom_ev, &cell_ev};
     for (auto *b : bd) b->release();
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
-                       &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0};
+                       &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0,
+                       &bsj, &bsimg, &bscnt};
     for (auto *b : bi) b->release();
+    bG.release();
+    bF.release();
     dU.release();
     wpos.release();
     dmodel.release();
@@ -211,21 +214,24 @@ void compile_model(pysic_ctx *h)
             ++next;
             if (next < np && row(next)[11] == 2.0) crow[nfac] = next++;
             const int ft = (int)row(frow[nfac])[0];
-            PYSIC_REQUIRE((ft >= 12 && ft <= 14) == (crow[nfac] != SIZE_MAX) &&
+            PYSIC_REQUIRE((ft >= 12 && ft <= 15) == (crow[nfac] != SIZE_MAX) &&
                               (crow[nfac] == SIZE_MAX || (int)row(crow[nfac])[0] == ft),
                           PYSIC_EINVAL,
-                          "a charge-dependent pair row needs its continuation row, and only it");
+                          "a charge-dependent or bond_bend row needs its continuation row, and "
+                          "only it");
             ++nfac;
         }
         PYSIC_REQUIRE(next >= np || row(next)[11] == 0.0, PYSIC_EINVAL,
-                      "a continuation row must follow a charge-dependent pair row");
+                      "a continuation row must follow a charge-dependent or bond_bend row");
         const size_t nmul = nfac - 1;  // the ProductPotential multipliers
         PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && (nt < 2 || (e2 >= 0 && e2 < PYSIC_MAXS)),
                       PYSIC_EINVAL, "potential species out of range");
-        PYSIC_REQUIRE(type >= 1 && type <= 14, PYSIC_EINVAL, "unsupported potential type");
+        PYSIC_REQUIRE(type >= 1 && type <= 15, PYSIC_EINVAL, "unsupported potential type");
         const bool single = type == 2 || type == 10 || type == 11;
-        PYSIC_REQUIRE(nt == (single ? 1 : 2), PYSIC_EINVAL,
-                      single ? "one-body potential needs 1 target" : "pair potential needs 2 targets");
+        PYSIC_REQUIRE(nt == (single ? 1 : type == 15 ? 3 : 2), PYSIC_EINVAL,
+                      single ? "one-body potential needs 1 target"
+                             : type == 15 ? "bond_bend needs 3 targets"
+                                          : "pair potential needs 2 targets");
         PYSIC_REQUIRE(!(single && nmul), PYSIC_EINVAL,
                       "products of one-body potentials are not supported");
         if (e1 < ns) M.maxpot[e1] = std::max(M.maxpot[e1], d[6]);
@@ -249,6 +255,27 @@ void compile_model(pysic_ctx *h)
             PYSIC_REQUIRE(nexp >= 0 && nexp <= 4, PYSIC_EINVAL, "charge_self exponent must be 0..4");
             M.self_eps[e1][nexp] += d[4];
             M.has_self = 1;
+        } else if (type == 15) {
+            PYSIC_REQUIRE(nmul == 0 && group < 0, PYSIC_EINVAL,
+                          "bond_bend takes no multipliers and no bond-order scaling");
+            PYSIC_REQUIRE(P.nbend < PYSIC_MAXB, PYSIC_EINVAL,
+                          "too many bond_bend rows (at most 12)");
+            const double *c = row(crow[0]);
+            const int e3 = (int)c[1];
+            PYSIC_REQUIRE(e3 >= 0 && e3 < PYSIC_MAXS, PYSIC_EINVAL, "potential species out of range");
+            BendTerm &B = P.bend[P.nbend++];
+            B.e[0] = e1; B.e[1] = e2; B.e[2] = e3;
+            B.vertex = (int)c[3];
+            B.eps = d[4];
+            B.n = (int)d[9];
+            B.m = (int)d[10];
+            // calculate_derived_parameters_bond_bending (Potentials.f90:3178-3185)
+            B.c0 = std::pow(std::cos(d[5]), d[9]);
+            B.cut = d[6];
+            if (d[7] > 0.0 && d[7] < d[6]) { B.soft = d[7]; B.smooth = 1; }
+            else { B.soft = d[6]; B.smooth = 0; }
+            P.bend_cut = std::max(P.bend_cut, d[6]);
+            P.has_pairs = 1;
         } else if (type == 10 || type == 11) {
             PYSIC_REQUIRE(group < 0, PYSIC_EINVAL,
                           "bond-order scaling of one-body potentials is not supported");
@@ -357,6 +384,7 @@ void compile_model(pysic_ctx *h)
     h->S.dmodel.ensure(1);
     HIP_TRY(hipMemcpyAsync(h->S.dmodel.p, &h->S.model, sizeof(Model), hipMemcpyHostToDevice,
                            h->S.stream));
+    for (int a = 0; a < PYSIC_MAXS; ++a) P.bend_maxpot[a] = M.maxpot[a];
     // the LJ table of a model whose pair potentials are lone LJs
     P.lj_only = 1;
     for (int a = 0; a < PYSIC_MAXS; ++a)
@@ -613,7 +641,8 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
             S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.nbr.p, S.nimg.p,
             S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
-            S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p,
+            S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p, S.bsj.p,
+            S.bsimg.p, S.bscnt.p, S.bG.p, S.bF.p,
             (const void *)(long)S.cap, (const void *)(long)h->variant,
             (const void *)(long)S.site_grid, (const void *)(long)S.n, (const void *)(long)S.nbins,
             (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};

@@ -90,6 +90,19 @@ struct LJPair {
     double cut, soft, eps, sig2;
     int on, smooth;
 };
+// A bond_bend row, one per target permutation as the reference adds them
+// (calculator.py:1247-1277): E = eps (cos^n theta - cos^n theta_0)^m on the
+// angle at the vertex, acting only where the vertex is the species of the
+// original second target (Potentials.f90:3196-3328).
+#define PYSIC_MAXB 12
+struct BendTerm {
+    int e[3];             // targets in this row's order
+    int vertex;           // species of the original second target
+    int n, m;             // integer exponents (the reference truncates)
+    int smooth, pad;
+    double eps, c0;       // c0 = cos(theta_0)^n (Potentials.f90:3178-3185)
+    double cut, soft;
+};
 struct PairModel {
     int npair[PYSIC_MAXS][PYSIC_MAXS];
     GenPair pair[PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXG];
@@ -104,6 +117,12 @@ struct PairModel {
     int needs_pick;       // a 'step' or charge factor: it depends on which atom the reference takes first
     int has_charge, ncf;  // charge-dependent factors (cf[0 .. ncf-1])
     ChargeFactor cf[PYSIC_MAXCF];
+    int nbend, pad_b;     // bond_bend rows (bend[0 .. nbend-1])
+    double bend_cut;      // the largest bond_bend cutoff
+    // per species, the largest cutoff of the potentials whose first target
+    // it is (the reference's max_potential_radius, Core.f90:1034-1125)
+    double bend_maxpot[PYSIC_MAXS];
+    BendTerm bend[PYSIC_MAXB];
     // every ordered species pair holds at most one potential, a lone LJ
     // (lj_only), and its table
     int lj_only;

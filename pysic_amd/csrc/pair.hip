@@ -633,23 +633,28 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
     PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
     PYSIC_REQUIRE(S.n > 0, PYSIC_ESTATE, "no atoms");
     const int nb = div_up(S.n, PB);
-    S.gpartial.ensure((size_t)nb * 8);
+    // bond_bend angles add their partials after the pair blocks and their
+    // forces after the pair forces (then the half-kick, if asked)
+    const int bb = bend_blocks(S);
+    S.gpartial.ensure((size_t)(nb + bb) * 8);
     S.gtotals.ensure(8);
-    S.gpartial_blocks = nb;
+    S.gpartial_blocks = nb + bb;
     hipEvent_t ev{};
     const char *name = mode == PAIR_ENERGY ? "pair_energy" : "pair_forces";
     S.timer.begin(name, S.stream, &ev);
     const int kind = S.pmodel.lj_only ? (S.model.nspecies == 1 ? 2 : 1) : 0;
+    const int pkick = bb ? 0 : kick;
     with_geo(S, [&](auto geo) {
-        if (kind == 2) launch<decltype(geo), 2>(S, geo, mode, accumulate, half_dt, kick);
-        else if (kind == 1) launch<decltype(geo), 1>(S, geo, mode, accumulate, half_dt, kick);
-        else launch<decltype(geo), 0>(S, geo, mode, accumulate, half_dt, kick);
+        if (kind == 2) launch<decltype(geo), 2>(S, geo, mode, accumulate, half_dt, pkick);
+        else if (kind == 1) launch<decltype(geo), 1>(S, geo, mode, accumulate, half_dt, pkick);
+        else launch<decltype(geo), 0>(S, geo, mode, accumulate, half_dt, pkick);
     });
+    S.timer.end(name, S.stream, ev);
+    if (bb) bend_eval(S, mode, nb, half_dt, kick);
     if (mode != PAIR_FORCES_ONLY) {
-        k_pair_totals<<<7, PB, 0, S.stream>>>(nb, S.gpartial.p, S.gtotals.p);
+        k_pair_totals<<<7, PB, 0, S.stream>>>(nb + bb, S.gpartial.p, S.gtotals.p);
         HIP_TRY(hipGetLastError());
     }
-    S.timer.end(name, S.stream, ev);
 }
 
 void pr_chi(System &S, double *chi)

@@ -94,6 +94,11 @@ struct System {
     DBuf<double> gpartial;      // per-block energy + virial partials of pair.hip
     int gpartial_blocks = 0;
     DBuf<double> gtotals;       // reduced energy + virial (7 doubles)
+    // bond_bend: each vertex's neighbors within the largest bend cutoff
+    // (column-major, capacity cap, as the full lists), the forces its
+    // angles put on them, and on itself
+    DBuf<int> bsj, bsimg, bscnt;
+    DBuf<double4> bG, bF;
 
     // Batch of independent periodic cells (ncells > 0).  The atoms of cell c
     // occupy slots [hatom0[c], hatom0[c+1]) in every order: the bins of each
@@ -157,6 +162,10 @@ enum { PAIR_ENERGY = 0, PAIR_FORCES = 1, PAIR_FORCES_ONLY = 2 };
 void pr_eval(System &S, int mode, bool accumulate, double half_dt = 0.0, int kick = 0);
 // chi += the charge-dependent pair terms' -dE/dq (nothing without them)
 void pr_chi(System &S, double *chi);
+// bond_bend angles: energy / virial partials into gpartial blocks
+// [blk0, blk0 + blocks), forces added into frc (then p += half_dt F if kick)
+void bend_eval(System &S, int mode, int blk0, double half_dt, int kick);
+int bend_blocks(const System &S);
 
 // md.hip
 void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m (no rebuild)
