@@ -219,3 +219,34 @@ def test_calculator_housekeeping_and_core_updates(oracle):
             other.engine.close()
     finally:
         calc.engine.close()
+
+
+def test_comb_terms_through_the_calculator(oracle):
+    """COMB's bond bending and charge-dependent pair terms set up as a pysic
+    script does (Potential / ProductPotential objects on a Pysic
+    calculator, charges on the atoms), against the oracle."""
+    from helpers import comb_bend_terms, comb_charge_model, comb_charges
+    from pysic_amd import Pysic
+    from pysic_amd.interactions import build_descriptors
+    s = st.amorphous_sio2(64, 2.2, seed=111)
+    q = comb_charges(s.get_chemical_symbols(), seed=112)
+    s.set_initial_charges(q)
+    pots = comb_bend_terms() + comb_charge_model()
+    calc = Pysic(s, pots)
+    e = calc.get_potential_energy()
+    f = calc.get_forces()
+    x = calc.get_electronegativities()
+    names = sorted(set(s.get_chemical_symbols()))
+    pd, _, _ = build_descriptors(pots, {y: k for k, y in enumerate(names)})
+    sp = np.array([names.index(y) for y in s.get_chemical_symbols()], np.int32)
+    n = len(s)
+    nl = oracle.neighbor_list(s.get_positions(), s.get_cell(), [1, 1, 1], np.full(n, 3.55))
+    args = (sp, s.get_positions(), q, s.get_cell(), nl, pd)
+    e_ref = oracle.evaluate("energy", *args)
+    f_ref, _ = oracle.evaluate("forces", *args)
+    x_ref = oracle.evaluate("enegs", *args)
+    print("  |dE|/N %.2e rel|dF| %.2e rel|dchi| %.2e" % (
+        abs(e - e_ref) / n, rel_maxnorm(f, f_ref), rel_maxnorm(x, x_ref)), flush=True)
+    assert abs(e - e_ref) / n <= 1e-9
+    assert rel_maxnorm(f, f_ref) <= 1e-9
+    assert rel_maxnorm(x, x_ref) <= 1e-9
