@@ -478,6 +478,91 @@ def lj_benchmark(steps=100, warmup=10, ref_cells=6, reference=True):
     return out
 
 
+def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
+    """COMB's local SiO2 terms (bond bending, charge-dependent repulsion
+    and attraction; bend.hip and pair.hip) in molecular dynamics: random
+    amorphous SiO2 at 2.2 g/cm^3, 3 x 8192 = 24,576 atoms, fixed seeded
+    charges (Si in [0.5, 2.2] e, O in [-1.3, -0.2] e), Maxwell-Boltzmann
+    momenta at 300 K, velocity Verlet with dt = 0.5 fs, lists (3.05 + 0.5 A)
+    rebuilt every step, FP64.  The reference core is timed on a 384-atom
+    sample of the same kind: list build + forces per step."""
+    from pysic_amd import structures as st
+    from pysic_amd.engine import Engine
+    from pysic_amd.interactions import build_descriptors, comb_sio2
+
+    def system(k, seed):
+        s = st.amorphous_sio2(k, 2.2, seed=seed)
+        rng = np.random.default_rng(seed + 1)
+        q = np.where(np.array(s.symbols) == "Si", rng.uniform(0.5, 2.2, len(s)),
+                     rng.uniform(-1.3, -0.2, len(s)))
+        return s, q
+
+    s, q = system(n_si, 5)
+    st.maxwell_boltzmann(s, 300.0, seed=6)
+    n = len(s)
+    names = ["O", "Si"]
+    pd, _, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+    sp = np.array([names.index(x) for x in s.symbols], np.int32)
+    cut = np.full(n, 3.55)
+    eng = Engine(0)
+    eng.create_atoms(sp, s.positions, masses=s.masses, charges=q, momenta=s.momenta)
+    eng.create_cell(s.cell, s.pbc)
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(np.zeros((0, 14)))
+    eng.ewald_off()
+    eng.generate_neighbor_lists(cut)
+    dt = 0.5 * st.ASE_FS
+    eng.md_run(warmup, dt, rebuild=1)
+    m = s.masses[:, None]
+    e0 = eng.energy() + float((eng.state()[1] ** 2 / (2 * m)).sum())
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.md_run(steps, dt, rebuild=1)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    e1 = eng.energy() + float((eng.state()[1] ** 2 / (2 * m)).sum())
+    eng.timing(True)
+    eng.md_run(10, dt, rebuild=1)
+    eng.synchronize()
+    tim = eng.timing_read()
+    eng.timing(False)
+    eng.close()
+    out = {"workload": "COMB local SiO2 terms (bond_bend, charge_exp, charge_abs), amorphous "
+                       "SiO2 24,576 atoms, fixed charges, VV dt = 0.5 fs, 300 K, lists rebuilt "
+                       "every step, FP64",
+           "atoms": n, "steps": steps, "value": round(n * steps / wall, 1),
+           "unit": "atom-steps/s", "ms_per_step": round(wall / steps * 1e3, 4),
+           "total_energy_drift_eV_per_atom": (e1 - e0) / n,
+           "kernels_us": {k: round(v[0] / max(v[1], 1) * 1e3, 1) for k, v in tim.items()}}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if O.ref_available():
+            r, qr = system(ref_si, 7)
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from helpers import ref_entries
+            pr, _, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+            rcut = np.full(len(r), 3.55)
+            R = O.RefCore()
+            R.setup(r.symbols, r.positions, r.cell, r.pbc, charges=qr,
+                    potentials=ref_entries(pr, names), cutoffs=rcut)
+            R.forces()
+            k = 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < 5.0 and k < 50:
+                R.rebuild(r.positions, rcut)
+                R.forces()
+                k += 1
+            t_ref = (time.perf_counter() - t0) / k
+            out["reference_sample"] = {
+                "atoms": len(r), "seconds_per_step": round(t_ref, 5), "cores": 1,
+                "kind": "reference", "atom_steps_per_s": round(len(r) / t_ref, 1),
+                "note": "list rebuild + forces per step; serial amdflang -O2 build"}
+    except Exception as exc:  # the checker is optional on the GPU host
+        out["reference_sample"] = {"error": str(exc)}
+    return out
+
+
 def cpu_baseline(seconds_budget=20.0):
     """Reference Fortran core on the host: VV steps (rebuild + forces) on a
     16x16x16-cell sample (32,768 atoms) of the same workload."""
@@ -533,6 +618,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cells measurement")
     ap.add_argument("--no-ewald", action="store_true", help="skip the Ewald rocksalt measurement")
     ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon measurement")
+    ap.add_argument("--no-comb", action="store_true", help="skip the COMB SiO2 measurement")
     ap.add_argument("--only", choices=["qeq", "batch", "ewald", "lj"],
                     help="run one secondary measurement alone and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -638,6 +724,7 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     lj = None if args.no_lj else lj_benchmark()
     ewald = None if args.no_ewald else ewald_benchmark()
+    comb = None if args.no_comb else comb_benchmark()
     out = {"metric": "atom-steps/s Tersoff 1M-atom Si; QEq time-to-1e-6 learned vs cold start",
            "value": round(value, 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -650,7 +737,7 @@ def main():
                       "kernel_variant": variant_name,
                       "total_energy_drift_eV_per_atom": drift},
            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "qeq": qeq,
-           "batched": batched, "ewald": ewald, "lj": lj}
+           "batched": batched, "ewald": ewald, "lj": lj, "comb": comb}
     print(json.dumps(out))
 
 

@@ -10,7 +10,8 @@ Coordinator, CoulombSummation, ChargeRelaxation.
 __version__ = "0.1.0"
 
 from .interactions import (BondOrderParameters, Coordinator, InvalidParametersError,  # noqa: F401
-                           InvalidPotentialError, Potential, ProductPotential, tersoff_si)
+                           InvalidPotentialError, Potential, ProductPotential, comb_sio2,
+                           tersoff_si)
 from .coulomb import ChargeRelaxation, CoulombSummation, estimate_ewald_parameters  # noqa: F401
 from .structures import Structure  # noqa: F401
 

@@ -11,12 +11,13 @@
 // when pick(v, a) holds (targets matched as (v, a, b)), else from (a, v)
 // (targets matched as (a, v, b)).
 //
-// Here one thread owns one vertex v.  Its neighbors within the largest bend
-// cutoff form a short list; every unordered pair of the list is oriented and
-// matched against the rows as the reference does it.  The forces an angle
-// puts on its two outer atoms are stored per short-list slot and summed by
-// those atoms' own threads afterwards (no atomics, fixed order, bit-for-bit
-// repeatable).  All arithmetic is FP64.
+// Here each vertex v gets a short list of its neighbors within the largest
+// bend cutoff, then G lanes: each lane takes some of the legs and, for each,
+// every angle it forms with another leg, oriented and matched against the
+// rows as the reference does it.  The force on a leg's atom is stored per
+// short-list slot and summed by that atom's own thread afterwards, with the
+// vertex's own (minus the sum over its slots): no atomics, fixed order,
+// bit-for-bit repeatable.  All arithmetic is FP64.
 #include "system.h"
 
 namespace {
@@ -83,75 +84,88 @@ __device__ __forceinline__ double bend_term(const BendTerm &P, const double3 &s1
     return e;
 }
 
-__device__ __forceinline__ void add3(double4 *p, const double3 &f)
-{
-    double4 g = *p;
-    g.x += f.x;
-    g.y += f.y;
-    g.z += f.z;
-    *p = g;
-}
-
-template <class Geo, int MODE>
-__global__ __launch_bounds__(BB) void k_bend_site(int n, Geo geo, const double4 *__restrict__ pos,
-                                                  const int *__restrict__ id,
+// The short list of each vertex: its neighbors within the largest bend
+// cutoff, widened by 1e-10 relative so that a pair inside a cutoff seen from
+// either end is in both atoms' lists.
+template <class Geo>
+__global__ __launch_bounds__(BB) void k_bend_list(int n, Geo geo, const double4 *__restrict__ pos,
                                                   const int *__restrict__ nbr,
                                                   const int *__restrict__ nimg,
                                                   const int *__restrict__ ncnt,
                                                   const PairModel *__restrict__ PM,
                                                   int *__restrict__ bsj, int *__restrict__ bsimg,
-                                                  int *__restrict__ bscnt,
-                                                  double4 *__restrict__ bG,
-                                                  double4 *__restrict__ bF,
-                                                  double *__restrict__ partial, int blk0)
+                                                  int *__restrict__ bscnt)
+{
+    const int v = xcd_block(gridDim.x) * BB + threadIdx.x;
+    if (v >= n) return;
+    const double4 pv = pos[v];
+    const double *cv = geo.vec(v);
+    const size_t N = (size_t)n;
+    const double bc2 = PM->bend_cut * PM->bend_cut * (1.0 + 1e-10);
+    const int nf = ncnt[v];
+    int c = 0;
+    for (int t = 0; t < nf; ++t) {
+        const int j = nbr[(size_t)t * N + v], img = nimg[(size_t)t * N + v];
+        const double3 d = separation64(pv, pos[j], img, cv);
+        if (!(dot64(d) < bc2)) continue;
+        bsj[(size_t)c * N + v] = j;
+        bsimg[(size_t)c * N + v] = img;
+        ++c;
+    }
+    bscnt[v] = c;
+}
+
+// G lanes per vertex v; lane l takes the short-list legs ta = l, l + G, ...
+// and every angle (ta, tb), tb != ta, so each angle is evaluated by the two
+// lanes of its legs.  A lane keeps only its own leg's share: the force on
+// that atom (stored per slot), half the energy, and d_ta (x) F_ta of the
+// virial (the three forces of an angle sum to zero, so sum r (x) F over its
+// atoms is d_a (x) F_a + d_b (x) F_b, taking r_v as the origin).
+template <class Geo, int MODE, int G>
+__global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double4 *__restrict__ pos,
+                                                    const int *__restrict__ id,
+                                                    const PairModel *__restrict__ PM,
+                                                    const int *__restrict__ bsj,
+                                                    const int *__restrict__ bsimg,
+                                                    const int *__restrict__ bscnt,
+                                                    double4 *__restrict__ bG,
+                                                    double *__restrict__ partial, int blk0)
 {
     constexpr bool FORCE = MODE != PAIR_ENERGY;
     constexpr bool VIR = MODE == PAIR_FORCES;
     __shared__ double red[7][BB];
     const int blk = xcd_block(gridDim.x);
-    const int v = blk * BB + threadIdx.x;
+    const int gid = blk * BB + threadIdx.x;
+    const int v = gid / G, lane = gid % G;
     double e = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     if (v < n) {
         const double4 pv = pos[v];
         const int sv = (int)pv.w, idv = id[v];
         const double *cv = geo.vec(v);
         const size_t N = (size_t)n;
-        // the short list; its radius is widened by 1e-10 relative so that a
-        // pair inside a cutoff seen from either end is in both atoms' lists
-        const double bc2 = PM->bend_cut * PM->bend_cut * (1.0 + 1e-10);
         const double vmax = PM->bend_maxpot[sv];
-        const int nf = ncnt[v];
-        int c = 0;
-        for (int t = 0; t < nf; ++t) {
-            const int j = nbr[(size_t)t * N + v], img = nimg[(size_t)t * N + v];
-            const double3 d = separation64(pv, pos[j], img, cv);
-            if (!(dot64(d) < bc2)) continue;
-            bsj[(size_t)c * N + v] = j;
-            bsimg[(size_t)c * N + v] = img;
-            if (FORCE) bG[(size_t)c * N + v] = make_double4(0.0, 0.0, 0.0, 0.0);
-            ++c;
-        }
-        bscnt[v] = c;
-        double3 fv = make_double3(0.0, 0.0, 0.0);
+        const int c = bscnt[v];
         const int nbend = PM->nbend;
-        for (int ta = 0; ta < c; ++ta) {
+        for (int ta = lane; ta < c; ta += G) {
             const Leg A = load_leg(pv, bsj[(size_t)ta * N + v], bsimg[(size_t)ta * N + v], pos, id,
                                    cv);
-            for (int tb = ta + 1; tb < c; ++tb) {
+            double3 fa = make_double3(0.0, 0.0, 0.0);
+            for (int tb = 0; tb < c; ++tb) {
+                if (tb == ta) continue;
                 const Leg B = load_leg(pv, bsj[(size_t)tb * N + v], bsimg[(size_t)tb * N + v],
                                        pos, id, cv);
-                // the reference's orientation: first leg a with pick(a, b)
-                const int dimg = pack_img(img_x(B.img) - img_x(A.img), img_y(B.img) - img_y(A.img),
-                                          img_z(B.img) - img_z(A.img));
-                const bool ab = pick_dev(A.id, B.id, dimg);
+                // the reference's orientation: first leg with pick(first, last)
+                const bool ab = pick_dev(A.id, B.id,
+                                         pack_img(img_x(B.img) - img_x(A.img),
+                                                  img_y(B.img) - img_y(A.img),
+                                                  img_z(B.img) - img_z(A.img)));
                 const Leg &Fl = ab ? A : B;
                 const Leg &Ll = ab ? B : A;
-                const int tf_slot = ab ? ta : tb, tl_slot = ab ? tb : ta;
-                // from the pair (v, a) [v atom 1] or (a, v) [a atom 1]
+                // from the pair (v, first) [v atom 1] or (first, v)
                 const bool vfirst = pick_dev(idv, Fl.id, Fl.img);
                 const int e0 = vfirst ? sv : Fl.s, e1 = vfirst ? Fl.s : sv;
                 // the second leg is searched within atom 1's radius, which
-                // for (a, v) is the vertex's own largest potential cutoff
+                // for (first, v) is the vertex's own largest potential cutoff
                 if (!vfirst && !(Ll.r < vmax)) continue;
                 const double3 s1 = make_double3(-Fl.d.x, -Fl.d.y, -Fl.d.z), s2 = Ll.d;
                 const double r1 = Fl.r, r2 = Ll.r;
@@ -166,50 +180,48 @@ __global__ __launch_bounds__(BB) void k_bend_site(int n, Geo geo, const double4 
                     }
                     if (!FORCE) {
                         const double ev = bend_term<false>(P, s1, r1, s2, r2, nullptr);
-                        e += ev * fc1 * fc2 * (1.0 + 1.0 + 1.0) / 3.0;
+                        e += 0.5 * (ev * fc1 * fc2 * (1.0 + 1.0 + 1.0) / 3.0);
                         continue;
                     }
                     double3 f[3];
                     const double ev = bend_term<true>(P, s1, r1, s2, r2, f);
-                    double3 g1 = make_double3(0.0, 0.0, 0.0), g2 = g1;
-                    double es = 0.0;
-                    if (P.smooth) {
-                        es = ev;
-                        const double d1 = dfc64(r1, P.cut, P.soft) / r1;
-                        const double d2 = r2 == 0.0 ? 0.0 : dfc64(r2, P.cut, P.soft) / r2;
-                        g1 = make_double3(s1.x * d1, s1.y * d1, s1.z * d1);
-                        g2 = make_double3(s2.x * d2, s2.y * d2, s2.z * d2);
-                    }
                     const double ff = fc1 * fc2;
-                    const double3 t0 = make_double3(f[0].x * ff + g1.x * fc2 * es,
-                                                    f[0].y * ff + g1.y * fc2 * es,
-                                                    f[0].z * ff + g1.z * fc2 * es);
-                    const double3 t1 = make_double3(
-                        f[1].x * ff + (-g1.x * fc2 + g2.x * fc1) * es,
-                        f[1].y * ff + (-g1.y * fc2 + g2.y * fc1) * es,
-                        f[1].z * ff + (-g1.z * fc2 + g2.z * fc1) * es);
-                    const double3 t2 = make_double3(f[2].x * ff - g2.x * fc1 * es,
-                                                    f[2].y * ff - g2.y * fc1 * es,
-                                                    f[2].z * ff - g2.z * fc1 * es);
-                    fv.x += t1.x;
-                    fv.y += t1.y;
-                    fv.z += t1.z;
-                    add3(bG + (size_t)tf_slot * N + v, t0);
-                    add3(bG + (size_t)tl_slot * N + v, t2);
+                    double3 t;
+                    if (ab) {  // this lane's leg is the first: f[0] and fc(r1)'s gradient
+                        double3 g1 = make_double3(0.0, 0.0, 0.0);
+                        if (P.smooth) {
+                            const double d1 = dfc64(r1, P.cut, P.soft) / r1;
+                            g1 = make_double3(s1.x * d1 * fc2 * ev, s1.y * d1 * fc2 * ev,
+                                              s1.z * d1 * fc2 * ev);
+                        }
+                        t = make_double3(f[0].x * ff + g1.x, f[0].y * ff + g1.y,
+                                         f[0].z * ff + g1.z);
+                    } else {  // the last: f[2] and fc(r2)'s gradient
+                        double3 g2 = make_double3(0.0, 0.0, 0.0);
+                        if (P.smooth && r2 != 0.0) {
+                            const double d2 = dfc64(r2, P.cut, P.soft) / r2;
+                            g2 = make_double3(s2.x * d2 * fc1 * ev, s2.y * d2 * fc1 * ev,
+                                              s2.z * d2 * fc1 * ev);
+                        }
+                        t = make_double3(f[2].x * ff - g2.x, f[2].y * ff - g2.y,
+                                         f[2].z * ff - g2.z);
+                    }
+                    fa.x += t.x;
+                    fa.y += t.y;
+                    fa.z += t.z;
                     if (VIR) {
-                        // sum of r (x) f over the three atoms, relative to a
-                        const double3 s12 = make_double3(s1.x + s2.x, s1.y + s2.y, s1.z + s2.z);
-                        w0 += s1.x * t1.x + s12.x * t2.x;
-                        w1 += s1.y * t1.y + s12.y * t2.y;
-                        w2 += s1.z * t1.z + s12.z * t2.z;
-                        w3 += s1.y * t1.z + s12.y * t2.z;
-                        w4 += s1.x * t1.z + s12.x * t2.z;
-                        w5 += s1.x * t1.y + s12.x * t2.y;
+                        const double3 d = A.d;
+                        w0 += d.x * t.x;
+                        w1 += d.y * t.y;
+                        w2 += d.z * t.z;
+                        w3 += d.y * t.z;
+                        w4 += d.x * t.z;
+                        w5 += d.x * t.y;
                     }
                 }
             }
+            if (FORCE) bG[(size_t)ta * N + v] = make_double4(fa.x, fa.y, fa.z, 0.0);
         }
-        if (FORCE) bF[v] = make_double4(fv.x, fv.y, fv.z, 0.0);
     }
     if (MODE == PAIR_FORCES_ONLY) return;
     const int tid = threadIdx.x;
@@ -225,13 +237,13 @@ __global__ __launch_bounds__(BB) void k_bend_site(int n, Geo geo, const double4 
     if (tid < 7) partial[(size_t)(blk0 + blk) * 8 + tid] = red[tid][0];
 }
 
-// F_a += (force of a's own angles on a) + sum over a's short-list neighbors
-// v of the force v's angles put on a, found in v's short list at (a, -o).
+// F_a += -(the forces of a's own angles on their outer atoms) + the forces
+// a's short-list neighbors' angles put on a (found in each neighbor's short
+// list at (a, -o)).
 __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict__ bsj,
                                                     const int *__restrict__ bsimg,
                                                     const int *__restrict__ bscnt,
                                                     const double4 *__restrict__ bG,
-                                                    const double4 *__restrict__ bF,
                                                     double4 *__restrict__ frc,
                                                     double4 *__restrict__ mom, double half_dt,
                                                     int kick)
@@ -239,9 +251,14 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
     const int a = xcd_block(gridDim.x) * BB + threadIdx.x;
     if (a >= n) return;
     const size_t N = (size_t)n;
-    const double4 own = bF[a];
-    double fx = own.x, fy = own.y, fz = own.z;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     const int c = bscnt[a];
+    for (int t = 0; t < c; ++t) {
+        const double4 own = bG[(size_t)t * N + a];
+        fx -= own.x;
+        fy -= own.y;
+        fz -= own.z;
+    }
     for (int t = 0; t < c; ++t) {
         const int v = bsj[(size_t)t * N + a];
         const int want = neg_img(bsimg[(size_t)t * N + a]);
@@ -272,35 +289,42 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
 
 }  // namespace
 
-int bend_blocks(const System &S) { return S.pmodel.nbend > 0 ? div_up(S.n, BB) : 0; }
+// lanes per vertex: 8 covers the angles of a 3 A cutoff in oxides in one
+// pass per lane and spreads a 25k-atom system over 768 blocks
+constexpr int BG_LANES = 8;
+
+int bend_blocks(const System &S)
+{
+    return S.pmodel.nbend > 0 ? div_up(S.n * BG_LANES, BB) : 0;
+}
 
 void bend_eval(System &S, int mode, int blk0, double half_dt, int kick)
 {
     if (S.pmodel.nbend == 0) return;
-    const int n = S.n, nb = div_up(n, BB);
+    const int n = S.n, nb = div_up(n, BB), na = div_up(n * BG_LANES, BB);
     const size_t cap = (size_t)S.cap * n;
     S.bsj.ensure(cap);
     S.bsimg.ensure(cap);
     S.bscnt.ensure(n);
     S.bG.ensure(cap);
-    S.bF.ensure(n);
     hipEvent_t ev{};
     S.timer.begin("bend", S.stream, &ev);
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
+        k_bend_list<Geo><<<nb, BB, 0, S.stream>>>(n, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p,
+                                                  S.dpmodel.p, S.bsj.p, S.bsimg.p, S.bscnt.p);
         auto go = [&](auto kern) {
-            kern<<<nb, BB, 0, S.stream>>>(n, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p,
-                                          S.dpmodel.p, S.bsj.p, S.bsimg.p, S.bscnt.p, S.bG.p,
-                                          S.bF.p, S.gpartial.p, blk0);
+            kern<<<na, BB, 0, S.stream>>>(n, geo, S.pos.p, S.id.p, S.dpmodel.p, S.bsj.p,
+                                          S.bsimg.p, S.bscnt.p, S.bG.p, S.gpartial.p, blk0);
         };
-        if (mode == PAIR_ENERGY) go(k_bend_site<Geo, PAIR_ENERGY>);
-        else if (mode == PAIR_FORCES) go(k_bend_site<Geo, PAIR_FORCES>);
-        else go(k_bend_site<Geo, PAIR_FORCES_ONLY>);
+        if (mode == PAIR_ENERGY) go(k_bend_angles<Geo, PAIR_ENERGY, BG_LANES>);
+        else if (mode == PAIR_FORCES) go(k_bend_angles<Geo, PAIR_FORCES, BG_LANES>);
+        else go(k_bend_angles<Geo, PAIR_FORCES_ONLY, BG_LANES>);
     });
     HIP_TRY(hipGetLastError());
     if (mode != PAIR_ENERGY) {
-        k_bend_gather<<<nb, BB, 0, S.stream>>>(n, S.bsj.p, S.bsimg.p, S.bscnt.p, S.bG.p, S.bF.p,
-                                               S.frc.p, S.mom.p, half_dt, kick);
+        k_bend_gather<<<nb, BB, 0, S.stream>>>(n, S.bsj.p, S.bsimg.p, S.bscnt.p, S.bG.p, S.frc.p,
+                                               S.mom.p, half_dt, kick);
         HIP_TRY(hipGetLastError());
     }
     S.timer.end("bend", S.stream, ev);

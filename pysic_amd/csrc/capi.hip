@@ -80,7 +80,6 @@ void System::release_all()
                        &bsj, &bsimg, &bscnt};
     for (auto *b : bi) b->release();
     bG.release();
-    bF.release();
     dU.release();
     wpos.release();
     dmodel.release();
@@ -642,7 +641,7 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
             S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
             S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p, S.bsj.p,
-            S.bsimg.p, S.bscnt.p, S.bG.p, S.bF.p,
+            S.bsimg.p, S.bscnt.p, S.bG.p,
             (const void *)(long)S.cap, (const void *)(long)h->variant,
             (const void *)(long)S.site_grid, (const void *)(long)S.n, (const void *)(long)S.nbins,
             (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};

@@ -403,7 +403,14 @@ __device__ __forceinline__ bool visit(const PairLds<0> &L, int si, int sj, doubl
 #endif
 
 // KIND 0: any pair potentials; 1: every pair potential is a lone LJ;
-// 2: a lone LJ and one species.
+// 2: a lone LJ and one species.  G lanes per atom (consecutive threads):
+// lane l takes the list rows l NBU .. l NBU + NBU - 1, then every G NBU-th
+// batch; the lanes' forces are summed by shuffles and lane 0 writes them.
+// The general kind runs 4 lanes per atom: its per-pair work (exp, pow and
+// the products) is long enough that one thread per atom leaves a system of
+// some ten thousand atoms on a fraction of the chip.
+template <int KIND> struct PairLanes { static constexpr int G = KIND == 0 ? 4 : 1; };
+
 template <class Geo, int MODE, int KIND>
 __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__restrict__ pos,
                                               const int *__restrict__ id,
@@ -420,18 +427,20 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
     constexpr bool FORCE = MODE != PAIR_ENERGY;
     constexpr bool VIR = MODE == PAIR_FORCES;
     constexpr int NBU = KIND == 0 ? 2 : PYSIC_PAIR_NBU;
+    constexpr int G = PairLanes<KIND>::G;
     __shared__ PairLds<KIND> lds;
     __shared__ double red[7][PB];
     stage(lds, PM);
     __syncthreads();
     const int blk = xcd_block(gridDim.x);
-    const int i = blk * PB + threadIdx.x;
+    const int gid = blk * PB + threadIdx.x;
+    const int i = gid / G, lane = gid % G;
     double e = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     if (i < n) {
         const double4 pi = pos[i];
         const int si = (int)pi.w;
-        if (PM->has_single) {
+        if (PM->has_single && lane == 0) {
             const double kfx = PM->kforce[si][0], kfy = PM->kforce[si][1],
                          kfz = PM->kforce[si][2];
             if (FORCE) {
@@ -453,8 +462,8 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
         if constexpr (KIND == 2) one = PM->lj[0][0];
         // list rows t0 .. t0 + NBU - 1 of atom i (column-major: row t at t n + i)
         const size_t stride = (size_t)n;
-        const int *pn = nbr + i, *pm = nimg + i;
-        for (int t0 = 0; t0 < nf; t0 += NBU, pn += NBU * stride, pm += NBU * stride) {
+        for (int t0 = lane * NBU; t0 < nf; t0 += G * NBU) {
+            const int *pn = nbr + (size_t)t0 * stride + i, *pm = nimg + (size_t)t0 * stride + i;
             int jv[NBU], iv[NBU];
 #pragma unroll
             for (int u = 0; u < NBU; ++u) {
@@ -473,10 +482,10 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                 if (t0 + u >= nf) break;
                 const double3 d = separation64(pi, pv[u], iv[u], cv);
                 const double r2 = dot64(d);
-                double v, G, C;
+                double v, G2, C;
                 bool on;
-                if constexpr (KIND == 2) on = visit<FORCE>(one, r2, v, G, C);
-                else if constexpr (KIND == 1) on = visit<FORCE>(lds.lj[si][(int)pv[u].w], r2, v, G, C);
+                if constexpr (KIND == 2) on = visit<FORCE>(one, r2, v, G2, C);
+                else if constexpr (KIND == 1) on = visit<FORCE>(lds.lj[si][(int)pv[u].w], r2, v, G2, C);
                 else {
                     QPair qp{0.0, 0.0, 0};
                     if (charged) {
@@ -485,7 +494,7 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                         if (pick_dev(idi, id[j], iv[u])) qp = QPair{qi, qj, si};
                         else qp = QPair{qj, qi, sj};
                     }
-                    on = visit<FORCE>(lds, si, (int)pv[u].w, r2, v, G, C, tab, qp);
+                    on = visit<FORCE>(lds, si, (int)pv[u].w, r2, v, G2, C, tab, qp);
                 }
                 if (!on) continue;
                 if (!FORCE) {
@@ -495,7 +504,7 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                     // with s = r_ij; otherwise g_i = -f1 with s = -r_ij
                     double cs = C;
                     if (pick && C != 0.0 && !pick_dev(idi, id[jv[u]], iv[u])) cs = -C;
-                    const double gx = G * d.x + cs, gy = G * d.y + cs, gz = G * d.z + cs;
+                    const double gx = G2 * d.x + cs, gy = G2 * d.y + cs, gz = G2 * d.z + cs;
                     fx += gx; fy += gy; fz += gz;
                     if (VIR) {
                         w0 -= 0.5 * d.x * gx;
@@ -508,7 +517,19 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
                 }
             }
         }
-        if (FORCE) {
+    }
+    if (FORCE) {
+        // the lanes of one atom are consecutive threads of one wave; every
+        // thread of the block reaches these shuffles
+        if constexpr (G > 1) {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) {
+                fx += __shfl_xor(fx, o);
+                fy += __shfl_xor(fy, o);
+                fz += __shfl_xor(fz, o);
+            }
+        }
+        if (i < n && lane == 0) {
             double4 f = make_double4(fx, fy, fz, 0.0);
             if (accumulate) {
                 const double4 a = frc[i];
@@ -611,7 +632,7 @@ __global__ __launch_bounds__(PB) void k_pair_chi(int n, Geo geo, const double4 *
 template <class Geo, int KIND>
 void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt, int kick)
 {
-    const int nb = div_up(S.n, PB);
+    const int nb = div_up(S.n * PairLanes<KIND>::G, PB);
     const int acc = accumulate ? 1 : 0;
     auto args = [&](auto kern) {
         kern<<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p,
@@ -632,7 +653,8 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
 {
     PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
     PYSIC_REQUIRE(S.n > 0, PYSIC_ESTATE, "no atoms");
-    const int nb = div_up(S.n, PB);
+    const int kind = S.pmodel.lj_only ? (S.model.nspecies == 1 ? 2 : 1) : 0;
+    const int nb = div_up(S.n * (kind == 0 ? PairLanes<0>::G : 1), PB);
     // bond_bend angles add their partials after the pair blocks and their
     // forces after the pair forces (then the half-kick, if asked)
     const int bb = bend_blocks(S);
@@ -642,7 +664,6 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
     hipEvent_t ev{};
     const char *name = mode == PAIR_ENERGY ? "pair_energy" : "pair_forces";
     S.timer.begin(name, S.stream, &ev);
-    const int kind = S.pmodel.lj_only ? (S.model.nspecies == 1 ? 2 : 1) : 0;
     const int pkick = bb ? 0 : kick;
     with_geo(S, [&](auto geo) {
         if (kind == 2) launch<decltype(geo), 2>(S, geo, mode, accumulate, half_dt, pkick);

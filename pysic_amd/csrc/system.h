@@ -95,10 +95,10 @@ struct System {
     int gpartial_blocks = 0;
     DBuf<double> gtotals;       // reduced energy + virial (7 doubles)
     // bond_bend: each vertex's neighbors within the largest bend cutoff
-    // (column-major, capacity cap, as the full lists), the forces its
-    // angles put on them, and on itself
+    // (column-major, capacity cap, as the full lists) and the forces its
+    // angles put on them
     DBuf<int> bsj, bsimg, bscnt;
-    DBuf<double4> bG, bF;
+    DBuf<double4> bG;
 
     // Batch of independent periodic cells (ncells > 0).  The atoms of cell c
     // occupy slots [hatom0[c], hatom0[c+1]) in every order: the bins of each

@@ -10,6 +10,7 @@ charge_self, constant, force) and the 'tersoff' bond-order factor.
 Parameter names and orders follow the reference's descriptor registry
 (Potentials.f90:2909-4990, 5685-5736).
 """
+import math
 import os
 from itertools import permutations
 
@@ -531,3 +532,76 @@ def lj_argon(cutoff=2.5 * 3.405, margin=0.0, symbol="Ar"):
     return [Potential("LJ", symbols=[[symbol, symbol]],
                       parameters=[ARGON_LJ["epsilon"], ARGON_LJ["sigma"]],
                       cutoff=cutoff, cutoff_margin=margin)]
+
+
+# COMB charge-dependent pair terms with the defaults of the reference's COMB
+# builder (pysic/interactions/comb.py:126-444, 665-990), without its
+# bond-order coordinators: per-element charge-screening radii (rmax, rmin at
+# qmax, qmin), attractive alpha and charge_abs a, b, Q, n; attractive B and
+# repulsive A, lambda mix as sqrt(B_i B_j), (alpha_i + alpha_j)/2,
+# sqrt(A_i A_j), (lambda_i + lambda_j)/2.
+COMB = {"Si": dict(rmax=-1.658949, rmin=1.651725, qmax=4.0, qmin=-4.0, alpha=1.7322, a=1.0,
+                   b=-9.5367431640625e-07, q=0.0, n=10.0, B=471.18, A=1830.8, lam=2.4799,
+                   cut=3.0, margin=0.2),
+        "O": dict(rmax=-0.00112, rmin=0.00148, qmax=5.5046, qmin=-1.8349, alpha=2.68, a=1.00098,
+                  b=-2.26e-6, q=1.83485, n=10.0, B=260.8931, A=3326.699, lam=5.36, cut=3.0,
+                  margin=0.4)}
+
+
+def comb_charge_model():
+    """Repulsive charge_exp x exponential and attractive charge_exp x
+    charge_abs x exponential products for Si-Si, O-O and Si-O (the order of
+    comb.py's pieces), plus a lone charge_abs O-O term."""
+    C = COMB
+    P = Potential
+
+    def cexp(i, j, eps, xi, cut, margin):
+        return P("charge_exp", symbols=[[i, j]], cutoff=cut, cutoff_margin=margin,
+                 parameters=[eps, C[i]["rmax"], C[i]["rmin"], C[i]["qmax"], C[i]["qmin"],
+                             C[j]["rmax"], C[j]["rmin"], C[j]["qmax"], C[j]["qmin"],
+                             C[i][xi], C[j][xi]])
+
+    def cabs(i, j):
+        return P("charge_abs", symbols=[[i, j]],
+                 parameters=[C[i]["a"], C[i]["b"], C[i]["q"], C[i]["n"],
+                             C[j]["a"], C[j]["b"], C[j]["q"], C[j]["n"]])
+
+    pots = []
+    for i, j in (("Si", "Si"), ("O", "O"), ("Si", "O")):
+        cut, margin = (C[i]["cut"], C[i]["margin"]) if i == j else (3.05, 0.5)
+        B = math.sqrt(C[i]["B"] * C[j]["B"])
+        zeta = 0.5 * (C[i]["alpha"] + C[j]["alpha"])
+        pots.append(ProductPotential([
+            cexp(i, j, B, "alpha", cut, margin), cabs(i, j),
+            P("exponential", symbols=[[i, j]], parameters=[-1.0, zeta])]))
+    for i, j in (("Si", "Si"), ("O", "O"), ("Si", "O")):
+        cut, margin = (C[i]["cut"], C[i]["margin"]) if i == j else (3.05, 0.5)
+        A = math.sqrt(C[i]["A"] * C[j]["A"])
+        lam = 0.5 * (C[i]["lam"] + C[j]["lam"])
+        pots.append(ProductPotential([
+            cexp(i, j, A, "lam", cut, margin),
+            P("exponential", symbols=[[i, j]], parameters=[1.0, lam])]))
+    lone = cabs("O", "O")
+    lone.set_cutoff(3.0)
+    lone.set_cutoff_margin(0.5)
+    pots.append(lone)
+    return pots
+
+
+def comb_bend_terms():
+    """COMB's bond bending with the reference builder's defaults
+    (comb.py:94-125, 636-662): Si-O-Si epsilon 2.6 eV, theta_0 2.508561734;
+    O-Si-O epsilon 0.3122 eV, theta_0 1.910611932; n = 1, m = 2; cutoff
+    3.05 A with a 0.5 A margin."""
+    P = Potential
+    return [P("bond_bend", symbols=[["Si", "O", "Si"]], cutoff=3.05, cutoff_margin=0.5,
+              parameters=[2.6, 2.508561734, 1, 2]),
+            P("bond_bend", symbols=[["O", "Si", "O"]], cutoff=3.05, cutoff_margin=0.5,
+              parameters=[0.3122, 1.910611932, 1, 2])]
+
+
+def comb_sio2():
+    """COMB's local SiO2 terms with the reference builder's defaults, its
+    bond-order coordinators left out: the Si-O-Si and O-Si-O bending and
+    the charge-dependent repulsion and attraction (comb.py:630-990)."""
+    return comb_bend_terms() + comb_charge_model()
