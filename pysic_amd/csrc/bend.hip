@@ -289,9 +289,10 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
 
 }  // namespace
 
-// lanes per vertex: 8 covers the angles of a 3 A cutoff in oxides in one
-// pass per lane and spreads a 25k-atom system over 768 blocks
-constexpr int BG_LANES = 8;
+// lanes per vertex: 16 covers the legs of a 3 A cutoff in oxides with one
+// leg per lane (24,576 atoms of COMB SiO2: angles + gather 337 us with 4
+// lanes, 247 with 8, 196 with 16)
+constexpr int BG_LANES = 16;
 
 int bend_blocks(const System &S)
 {

@@ -406,12 +406,16 @@ __device__ __forceinline__ bool visit(const PairLds<0> &L, int si, int sj, doubl
 // 2: a lone LJ and one species.  G lanes per atom (consecutive threads):
 // lane l takes the list rows l NBU .. l NBU + NBU - 1, then every G NBU-th
 // batch; the lanes' forces are summed by shuffles and lane 0 writes them.
-// The general kind runs 4 lanes per atom: its per-pair work (exp, pow and
-// the products) is long enough that one thread per atom leaves a system of
-// some ten thousand atoms on a fraction of the chip.
-template <int KIND> struct PairLanes { static constexpr int G = KIND == 0 ? 4 : 1; };
+// The general kind runs 8 lanes per atom up to 2^18 atoms: its per-pair
+// work (exp, pow and the products) is long enough that one thread per atom
+// leaves a system of some ten thousand atoms on a fraction of the chip
+// (24,576 atoms of COMB SiO2: 381 us with 1 lane, 209 with 2, 144 with 4,
+// 140 with 8).  Above that, one thread per atom already fills the chip, and
+// each extra block would stage the whole model into LDS once more.
+constexpr int PAIR_LANES_SMALL = 8;
+constexpr int PAIR_LANES_MAX_ATOMS = 1 << 18;
 
-template <class Geo, int MODE, int KIND>
+template <class Geo, int MODE, int KIND, int G>
 __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__restrict__ pos,
                                               const int *__restrict__ id,
                                               const int *__restrict__ nbr,
@@ -427,7 +431,6 @@ __global__ __launch_bounds__(PB) void k_pairs(int n, Geo geo, const double4 *__r
     constexpr bool FORCE = MODE != PAIR_ENERGY;
     constexpr bool VIR = MODE == PAIR_FORCES;
     constexpr int NBU = KIND == 0 ? 2 : PYSIC_PAIR_NBU;
-    constexpr int G = PairLanes<KIND>::G;
     __shared__ PairLds<KIND> lds;
     __shared__ double red[7][PB];
     stage(lds, PM);
@@ -629,19 +632,19 @@ __global__ __launch_bounds__(PB) void k_pair_chi(int n, Geo geo, const double4 *
     chi[i] += x;
 }
 
-template <class Geo, int KIND>
+template <class Geo, int KIND, int G>
 void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt, int kick)
 {
-    const int nb = div_up(S.n * PairLanes<KIND>::G, PB);
+    const int nb = div_up(S.n * G, PB);
     const int acc = accumulate ? 1 : 0;
     auto args = [&](auto kern) {
         kern<<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p,
                                       S.dpmodel.p, S.tables.p, S.q.p, S.frc.p, acc, S.mom.p,
                                       half_dt, kick, S.gpartial.p);
     };
-    if (mode == PAIR_ENERGY) args(k_pairs<Geo, PAIR_ENERGY, KIND>);
-    else if (mode == PAIR_FORCES) args(k_pairs<Geo, PAIR_FORCES, KIND>);
-    else args(k_pairs<Geo, PAIR_FORCES_ONLY, KIND>);
+    if (mode == PAIR_ENERGY) args(k_pairs<Geo, PAIR_ENERGY, KIND, G>);
+    else if (mode == PAIR_FORCES) args(k_pairs<Geo, PAIR_FORCES, KIND, G>);
+    else args(k_pairs<Geo, PAIR_FORCES_ONLY, KIND, G>);
     HIP_TRY(hipGetLastError());
 }
 
@@ -654,7 +657,8 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
     PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
     PYSIC_REQUIRE(S.n > 0, PYSIC_ESTATE, "no atoms");
     const int kind = S.pmodel.lj_only ? (S.model.nspecies == 1 ? 2 : 1) : 0;
-    const int nb = div_up(S.n * (kind == 0 ? PairLanes<0>::G : 1), PB);
+    const bool lanes = kind == 0 && S.n <= PAIR_LANES_MAX_ATOMS;
+    const int nb = div_up(S.n * (lanes ? PAIR_LANES_SMALL : 1), PB);
     // bond_bend angles add their partials after the pair blocks and their
     // forces after the pair forces (then the half-kick, if asked)
     const int bb = bend_blocks(S);
@@ -666,9 +670,11 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
     S.timer.begin(name, S.stream, &ev);
     const int pkick = bb ? 0 : kick;
     with_geo(S, [&](auto geo) {
-        if (kind == 2) launch<decltype(geo), 2>(S, geo, mode, accumulate, half_dt, pkick);
-        else if (kind == 1) launch<decltype(geo), 1>(S, geo, mode, accumulate, half_dt, pkick);
-        else launch<decltype(geo), 0>(S, geo, mode, accumulate, half_dt, pkick);
+        using Geo = decltype(geo);
+        if (kind == 2) launch<Geo, 2, 1>(S, geo, mode, accumulate, half_dt, pkick);
+        else if (kind == 1) launch<Geo, 1, 1>(S, geo, mode, accumulate, half_dt, pkick);
+        else if (lanes) launch<Geo, 0, PAIR_LANES_SMALL>(S, geo, mode, accumulate, half_dt, pkick);
+        else launch<Geo, 0, 1>(S, geo, mode, accumulate, half_dt, pkick);
     });
     S.timer.end(name, S.stream, ev);
     if (bb) bend_eval(S, mode, nb, half_dt, kick);

@@ -17,7 +17,8 @@ import pytest
 from conftest import load_golden
 from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table, rel_maxnorm
 from pysic_amd import structures as st
-from pysic_amd.interactions import Potential, build_descriptors, lj_argon, tersoff_si
+from pysic_amd.interactions import (Potential, ProductPotential, build_descriptors, lj_argon,
+                                    tersoff_si)
 
 pytestmark = pytest.mark.gpu
 
@@ -87,6 +88,20 @@ def test_mixture_2000_atoms(eng, oracle):
     print("case: Ar/Kr mixture, 2000 atoms, every pair type", flush=True)
     s = st.random_mixture(2000, 46.0, 2.9, ["Kr", "Ar", "Ar"], seed=11)
     _vs_oracle(eng, oracle, s, mixture_model(), 8.5, {MIXTURE_TABLE_ID: mixture_table()})
+
+
+def test_general_kernel_one_lane_per_atom(eng, oracle):
+    """Above 2^18 atoms the general kind runs one thread per atom (below,
+    8 lanes per atom, which every smaller case here exercises): Buckingham
+    times a power term on 275,684 fcc atoms."""
+    s = st.fcc((41, 41, 41), a=5.6, sigma=0.1, seed=3)
+    assert len(s) > 1 << 18
+    model = [ProductPotential([
+        Potential("Buckingham", symbols=[["Ar", "Ar"]], parameters=[500.0, 5e3, 0.35],
+                  cutoff=5.0, cutoff_margin=0.5),
+        Potential("power", symbols=[["Ar", "Ar"]], parameters=[1.0, 4.0, 0.5])])]
+    print("case: general kernel, n=%d" % len(s), flush=True)
+    _vs_oracle(eng, oracle, s, model, 5.0)
 
 
 def test_lj_argon_4000_atoms(eng, oracle):
