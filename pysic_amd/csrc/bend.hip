@@ -239,7 +239,8 @@ __global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double
 
 // F_a += -(the forces of a's own angles on their outer atoms) + the forces
 // a's short-list neighbors' angles put on a (found in each neighbor's short
-// list at (a, -o)).
+// list at (a, -o)).  G lanes per atom split a's slots; shuffles sum them.
+template <int G>
 __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict__ bsj,
                                                     const int *__restrict__ bsimg,
                                                     const int *__restrict__ bscnt,
@@ -248,31 +249,40 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
                                                     double4 *__restrict__ mom, double half_dt,
                                                     int kick)
 {
-    const int a = xcd_block(gridDim.x) * BB + threadIdx.x;
-    if (a >= n) return;
+    const int gid = xcd_block(gridDim.x) * BB + threadIdx.x;
+    const int a = gid / G, lane = gid % G;
     const size_t N = (size_t)n;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int c = bscnt[a];
-    for (int t = 0; t < c; ++t) {
-        const double4 own = bG[(size_t)t * N + a];
-        fx -= own.x;
-        fy -= own.y;
-        fz -= own.z;
-    }
-    for (int t = 0; t < c; ++t) {
-        const int v = bsj[(size_t)t * N + a];
-        const int want = neg_img(bsimg[(size_t)t * N + a]);
-        const int cv = bscnt[v];
-        for (int u = 0; u < cv; ++u) {
-            if (bsj[(size_t)u * N + v] == a && bsimg[(size_t)u * N + v] == want) {
-                const double4 g = bG[(size_t)u * N + v];
-                fx += g.x;
-                fy += g.y;
-                fz += g.z;
-                break;
+    if (a < n) {
+        const int c = bscnt[a];
+        for (int t = lane; t < c; t += G) {
+            const double4 own = bG[(size_t)t * N + a];
+            fx -= own.x;
+            fy -= own.y;
+            fz -= own.z;
+            const int v = bsj[(size_t)t * N + a];
+            const int want = neg_img(bsimg[(size_t)t * N + a]);
+            const int cv = bscnt[v];
+            for (int u = 0; u < cv; ++u) {
+                if (bsj[(size_t)u * N + v] == a && bsimg[(size_t)u * N + v] == want) {
+                    const double4 g = bG[(size_t)u * N + v];
+                    fx += g.x;
+                    fy += g.y;
+                    fz += g.z;
+                    break;
+                }
             }
         }
     }
+    // the lanes of one atom are consecutive threads of one wave; every
+    // thread reaches the shuffles
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o);
+        fy += __shfl_xor(fy, o);
+        fz += __shfl_xor(fz, o);
+    }
+    if (a >= n || lane != 0) return;
     double4 f = frc[a];
     f.x += fx;
     f.y += fy;
@@ -293,6 +303,8 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
 // leg per lane (24,576 atoms of COMB SiO2: angles + gather 337 us with 4
 // lanes, 247 with 8, 196 with 16)
 constexpr int BG_LANES = 16;
+// lanes per atom in the gather: its slots split over 8 lanes
+constexpr int BG_GATHER_LANES = 8;
 
 int bend_blocks(const System &S)
 {
@@ -324,8 +336,8 @@ void bend_eval(System &S, int mode, int blk0, double half_dt, int kick)
     });
     HIP_TRY(hipGetLastError());
     if (mode != PAIR_ENERGY) {
-        k_bend_gather<<<nb, BB, 0, S.stream>>>(n, S.bsj.p, S.bsimg.p, S.bscnt.p, S.bG.p, S.frc.p,
-                                               S.mom.p, half_dt, kick);
+        k_bend_gather<BG_GATHER_LANES><<<div_up(n * BG_GATHER_LANES, BB), BB, 0, S.stream>>>(
+            n, S.bsj.p, S.bsimg.p, S.bscnt.p, S.bG.p, S.frc.p, S.mom.p, half_dt, kick);
         HIP_TRY(hipGetLastError());
     }
     S.timer.end("bend", S.stream, ev);

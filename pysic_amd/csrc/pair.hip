@@ -246,6 +246,8 @@ __device__ __forceinline__ Radial potential(const GenPair &T, double r,
     Radial out{e, 0.0, 0.0};
     if (!need_force) return out;
     for (int k = 0; k < nf; ++k) {
+        // a charge-dependent factor has no r dependence: nothing to add
+        if (T.f[k].type >= 12) continue;
         const Radial q = factor(T.f[k], r, T.cut, tab, true, cft, qp);
         const double w = q.v != 0.0 ? e / q.v : (zeros == 1 ? enz : 0.0);
         out.g += w * q.g;
