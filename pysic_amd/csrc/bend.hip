@@ -121,7 +121,7 @@ __global__ __launch_bounds__(BB) void k_bend_list(int n, Geo geo, const double4 
 // that atom (stored per slot), half the energy, and d_ta (x) F_ta of the
 // virial (the three forces of an angle sum to zero, so sum r (x) F over its
 // atoms is d_a (x) F_a + d_b (x) F_b, taking r_v as the origin).
-template <class Geo, int MODE, int G>
+template <class Geo, int MODE, int G, int H>
 __global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double4 *__restrict__ pos,
                                                     const int *__restrict__ id,
                                                     const PairModel *__restrict__ PM,
@@ -136,7 +136,9 @@ __global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double
     __shared__ double red[7][BB];
     const int blk = xcd_block(gridDim.x);
     const int gid = blk * BB + threadIdx.x;
-    const int v = gid / G, lane = gid % G;
+    // G * H consecutive lanes per vertex: lane l + G h takes the legs
+    // l, l + G, ... and, of each, the partner legs h, h + H, ...
+    const int v = gid / (G * H), lane = gid % G, part = (gid / G) % H;
     double e = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     if (v < n) {
         const double4 pv = pos[v];
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double
             const Leg A = load_leg(pv, bsj[(size_t)ta * N + v], bsimg[(size_t)ta * N + v], pos, id,
                                    cv);
             double3 fa = make_double3(0.0, 0.0, 0.0);
-            for (int tb = 0; tb < c; ++tb) {
+            for (int tb = part; tb < c; tb += H) {
                 if (tb == ta) continue;
                 const Leg B = load_leg(pv, bsj[(size_t)tb * N + v], bsimg[(size_t)tb * N + v],
                                        pos, id, cv);
@@ -220,7 +222,17 @@ __global__ __launch_bounds__(BB) void k_bend_angles(int n, Geo geo, const double
                     }
                 }
             }
-            if (FORCE) bG[(size_t)ta * N + v] = make_double4(fa.x, fa.y, fa.z, 0.0);
+            if (FORCE) {
+                // the H parts of one leg run the same ta loop, so they are
+                // active together here
+#pragma unroll
+                for (int o = G; o < G * H; o <<= 1) {
+                    fa.x += __shfl_xor(fa.x, o);
+                    fa.y += __shfl_xor(fa.y, o);
+                    fa.z += __shfl_xor(fa.z, o);
+                }
+                if (part == 0) bG[(size_t)ta * N + v] = make_double4(fa.x, fa.y, fa.z, 0.0);
+            }
         }
     }
     if (MODE == PAIR_FORCES_ONLY) return;
@@ -303,18 +315,20 @@ __global__ __launch_bounds__(BB) void k_bend_gather(int n, const int *__restrict
 // leg per lane (24,576 atoms of COMB SiO2: angles + gather 337 us with 4
 // lanes, 247 with 8, 196 with 16)
 constexpr int BG_LANES = 16;
+// and each leg's partners split over BG_PARTS lanes
+constexpr int BG_PARTS = 2;
 // lanes per atom in the gather: its slots split over 8 lanes
 constexpr int BG_GATHER_LANES = 8;
 
 int bend_blocks(const System &S)
 {
-    return S.pmodel.nbend > 0 ? div_up(S.n * BG_LANES, BB) : 0;
+    return S.pmodel.nbend > 0 ? div_up(S.n * BG_LANES * BG_PARTS, BB) : 0;
 }
 
 void bend_eval(System &S, int mode, int blk0, double half_dt, int kick)
 {
     if (S.pmodel.nbend == 0) return;
-    const int n = S.n, nb = div_up(n, BB), na = div_up(n * BG_LANES, BB);
+    const int n = S.n, nb = div_up(n, BB), na = bend_blocks(S);
     const size_t cap = (size_t)S.cap * n;
     S.bsj.ensure(cap);
     S.bsimg.ensure(cap);
@@ -330,9 +344,9 @@ void bend_eval(System &S, int mode, int blk0, double half_dt, int kick)
             kern<<<na, BB, 0, S.stream>>>(n, geo, S.pos.p, S.id.p, S.dpmodel.p, S.bsj.p,
                                           S.bsimg.p, S.bscnt.p, S.bG.p, S.gpartial.p, blk0);
         };
-        if (mode == PAIR_ENERGY) go(k_bend_angles<Geo, PAIR_ENERGY, BG_LANES>);
-        else if (mode == PAIR_FORCES) go(k_bend_angles<Geo, PAIR_FORCES, BG_LANES>);
-        else go(k_bend_angles<Geo, PAIR_FORCES_ONLY, BG_LANES>);
+        if (mode == PAIR_ENERGY) go(k_bend_angles<Geo, PAIR_ENERGY, BG_LANES, BG_PARTS>);
+        else if (mode == PAIR_FORCES) go(k_bend_angles<Geo, PAIR_FORCES, BG_LANES, BG_PARTS>);
+        else go(k_bend_angles<Geo, PAIR_FORCES_ONLY, BG_LANES, BG_PARTS>);
     });
     HIP_TRY(hipGetLastError());
     if (mode != PAIR_ENERGY) {
