@@ -583,10 +583,11 @@ __global__ __launch_bounds__(PB) void k_pair_totals(int nblocks, const double *_
 }
 
 // The charge-dependent pair terms' share of the electronegativities:
-// chi[i] += sum over i's pairs of -dE/dq_i f_c(r).  One thread per atom
-// walks its full list; each pair is oriented as the reference takes it, and
-// each end adds its own atom's share (the reference adds both at once).
-template <class Geo>
+// chi[i] += sum over i's pairs of -dE/dq_i f_c(r).  G lanes per atom walk
+// its full list (every G-th row each, summed by shuffles); each pair is
+// oriented as the reference takes it, and each end adds its own atom's
+// share (the reference adds both at once).
+template <class Geo, int G>
 __global__ __launch_bounds__(PB) void k_pair_chi(int n, Geo geo, const double4 *__restrict__ pos,
                                                  const int *__restrict__ id,
                                                  const int *__restrict__ nbr,
@@ -600,38 +601,44 @@ __global__ __launch_bounds__(PB) void k_pair_chi(int n, Geo geo, const double4 *
     __shared__ PairLds<0> lds;
     stage(lds, PM);
     __syncthreads();
-    const int i = xcd_block(gridDim.x) * PB + threadIdx.x;
-    if (i >= n) return;
-    const PairModel &P = lds.pm;
-    const double4 pi = pos[i];
-    const int si = (int)pi.w, idi = id[i];
-    const double qi = q[i];
-    const double *cv = geo.vec(i);
-    const int nf = ncnt[i];
+    const int gid = xcd_block(gridDim.x) * PB + threadIdx.x;
+    const int i = gid / G, lane = gid % G;
     double x = 0.0;
-    for (int t = 0; t < nf; ++t) {
-        const int j = nbr[(size_t)t * n + i], img = nimg[(size_t)t * n + i];
-        const double4 pj = pos[j];
-        const int sj = (int)pj.w;
-        const int np = P.npair[si][sj];
-        if (np == 0) continue;
-        const double3 d = separation64(pi, pj, img, cv);
-        const double r2 = dot64(d);
-        if (!(r2 < P.cut2_hi[si][sj])) continue;
-        const double r = sqrt(r2);
-        const bool first = pick_dev(idi, id[j], img);
-        const double qj = q[j];
-        const QPair qp = first ? QPair{qi, qj, si} : QPair{qj, qi, sj};
-        for (int c = 0; c < np; ++c) {
-            const GenPair &T = P.pair[si][sj][c];
-            if (!(T.cut > r)) continue;
-            double xs[2];
-            potential_chi(T, r, tab, P.cf, qp, xs);
-            const double s = first ? xs[0] : xs[1];
-            x += T.smooth ? s * fc64(r, T.cut, T.soft) : s;
+    if (i < n) {
+        const PairModel &P = lds.pm;
+        const double4 pi = pos[i];
+        const int si = (int)pi.w, idi = id[i];
+        const double qi = q[i];
+        const double *cv = geo.vec(i);
+        const int nf = ncnt[i];
+        for (int t = lane; t < nf; t += G) {
+            const int j = nbr[(size_t)t * n + i], img = nimg[(size_t)t * n + i];
+            const double4 pj = pos[j];
+            const int sj = (int)pj.w;
+            const int np = P.npair[si][sj];
+            if (np == 0) continue;
+            const double3 d = separation64(pi, pj, img, cv);
+            const double r2 = dot64(d);
+            if (!(r2 < P.cut2_hi[si][sj])) continue;
+            const double r = sqrt(r2);
+            const bool first = pick_dev(idi, id[j], img);
+            const double qj = q[j];
+            const QPair qp = first ? QPair{qi, qj, si} : QPair{qj, qi, sj};
+            for (int c = 0; c < np; ++c) {
+                const GenPair &T = P.pair[si][sj][c];
+                if (!(T.cut > r)) continue;
+                double xs[2];
+                potential_chi(T, r, tab, P.cf, qp, xs);
+                const double s = first ? xs[0] : xs[1];
+                x += T.smooth ? s * fc64(r, T.cut, T.soft) : s;
+            }
         }
     }
-    chi[i] += x;
+    // the lanes of one atom are consecutive threads of one wave; every
+    // thread reaches the shuffles
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (i < n && lane == 0) chi[i] += x;
 }
 
 template <class Geo, int KIND, int G>
@@ -690,11 +697,12 @@ void pr_chi(System &S, double *chi)
 {
     if (!S.pmodel.has_charge) return;
     PYSIC_REQUIRE(S.dpmodel.p && S.tables.p, PYSIC_ESTATE, "pair model not compiled");
-    const int nb = div_up(S.n, PB);
+    constexpr int G = PAIR_LANES_SMALL;
+    const int nb = div_up(S.n * G, PB);
     with_geo(S, [&](auto geo) {
-        k_pair_chi<decltype(geo)><<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p,
-                                                          S.nimg.p, S.ncnt.p, S.dpmodel.p,
-                                                          S.tables.p, S.q.p, chi);
+        k_pair_chi<decltype(geo), G><<<nb, PB, 0, S.stream>>>(S.n, geo, S.pos.p, S.id.p, S.nbr.p,
+                                                             S.nimg.p, S.ncnt.p, S.dpmodel.p,
+                                                             S.tables.p, S.q.p, chi);
     });
     HIP_TRY(hipGetLastError());
 }
