@@ -37,9 +37,21 @@ def _save(d):
 
 
 def time_variants(eng, steps=3, dt=0.0982269479):
-    """Average site-kernel time (us) of every variant that runs on this system."""
+    """Average site-kernel time (us) of every variant that runs on this
+    system.  The timing steps run on the loaded neighbor lists without a
+    rebuild; positions and momenta are put back afterwards, so the system
+    is left as it was found (same state, same lists)."""
     out = {}
     names = eng.variants()
+    pos, mom = eng.state()
+    try:
+        _time_each(eng, names, steps, dt, out)
+    finally:
+        eng.update_coordinates(pos, mom)
+    return out
+
+
+def _time_each(eng, names, steps, dt, out):
     for v, name in enumerate(names):
         try:
             eng.set_variant(v)
@@ -54,12 +66,11 @@ def time_variants(eng, steps=3, dt=0.0982269479):
         ms, calls = t.get("tersoff_site", (0.0, 0))
         if calls:
             out[name] = ms / calls * 1e3
-    return out
 
 
 def best_variant(eng, key, retune=False):
-    """Index of the fastest site-kernel variant for this workload.  Note:
-    tuning advances the MD state by a few steps of rebuild-free dynamics."""
+    """Index of the fastest site-kernel variant for this workload (the
+    system's state is unchanged by the tuning)."""
     names = eng.variants()
     ck = "%s/n%d" % (key, _bucket(eng.n))
     cache = _load()

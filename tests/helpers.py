@@ -37,6 +37,23 @@ def ewald_descriptors(g):
     return sp, pd, ew
 
 
+def engine_pair_set(eng):
+    """The engine's neighbor lists as sorted (i, j, o1, o2, o3) rows."""
+    counts, idx, off = eng.neighbor_lists()
+    i = np.repeat(np.arange(len(counts), dtype=np.int64), counts)
+    rows = np.column_stack([i, idx.astype(np.int64), off.astype(np.int64).reshape(-1, 3)])
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def assert_same_pairs(eng, nl):
+    """Neighbor lists bit-exact as sorted (i, j, offset) sets against the
+    oracle's lists nl (Core.f90:8148-8280): the whole list radius, the
+    shell between the cutoff and cutoff + skin included."""
+    got, want = engine_pair_set(eng), nl.pair_set()
+    print("  pairs: %d (oracle %d)" % (len(got), len(want)), flush=True)
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
 def rel_maxnorm(a, b, floor=1e-300):
     """max|a-b| / max(max|b|, floor)"""
     return float(np.abs(a - b).max() / max(np.abs(b).max(), floor))

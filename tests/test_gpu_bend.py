@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from helpers import comb_bend_terms, comb_charge_model, comb_charges, rel_maxnorm
+from helpers import (assert_same_pairs, comb_bend_terms, comb_charge_model, comb_charges,
+                     rel_maxnorm)
 from pysic_amd import structures as st
 from pysic_amd.interactions import build_descriptors
 
@@ -66,8 +67,9 @@ def _case(n_si, seed, charged):
     return s, q, sp, pd, np.full(len(s), 3.55)
 
 
-def oracle_ref(oracle, sp, pos, q, cell, pd, cuts):
+def oracle_ref(oracle, eng, sp, pos, q, cell, pd, cuts):
     nl = oracle.neighbor_list(pos, cell, [1, 1, 1], cuts)
+    assert_same_pairs(eng, nl)
     args = (sp, pos, q, cell, nl, pd)
     f, w = oracle.evaluate("forces", *args)
     return (oracle.evaluate("energy", *args), f, w, oracle.evaluate("enegs", *args))
@@ -80,7 +82,7 @@ def test_bend_matches_oracle(eng, oracle, n_si, charged):
     s, q, sp, pd, cuts = _case(n_si, 91 + n_si, charged)
     print("case: amorphous SiO2 n=%d charged=%s" % (len(s), charged), flush=True)
     load(eng, sp, s.positions, q, s.cell, pd, cuts)
-    compare(eng, oracle_ref(oracle, sp, s.positions, q, s.cell, pd, cuts), len(s),
+    compare(eng, oracle_ref(oracle, eng, sp, s.positions, q, s.cell, pd, cuts), len(s),
             "n=%d" % len(s))
 
 

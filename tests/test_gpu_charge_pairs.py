@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from helpers import comb_asymmetric_terms, comb_charge_model, comb_charges, rel_maxnorm
+from helpers import (assert_same_pairs, comb_asymmetric_terms, comb_charge_model, comb_charges,
+                     rel_maxnorm)
 from pysic_amd import structures as st
 from pysic_amd.interactions import build_descriptors
 
@@ -55,8 +56,9 @@ def compare(eng, ref, n, what):
     assert rel_maxnorm(x, x_ref) <= 1e-9
 
 
-def oracle_ref(oracle, sp, pos, q, cell, pd, cuts, ew=None):
+def oracle_ref(oracle, eng, sp, pos, q, cell, pd, cuts, ew=None):
     nl = oracle.neighbor_list(pos, cell, [1, 1, 1], cuts)
+    assert_same_pairs(eng, nl)
     ewa = None if ew is None else np.array([1, ew[0], ew[1], *ew[2], ew[3], ew[4]])
     args = (sp, pos, q, cell, nl, pd, None, ewa)
     f, w = oracle.evaluate("forces", *args)
@@ -89,7 +91,7 @@ def test_comb_matches_oracle(eng, oracle, asymmetric):
     cuts = np.full(len(s), 3.55)
     print("case: amorphous SiO2 n=%d asymmetric=%s" % (len(s), asymmetric), flush=True)
     load(eng, sp, s.positions, q, s.cell, pd, cuts)
-    compare(eng, oracle_ref(oracle, sp, s.positions, q, s.cell, pd, cuts), len(s), "n=1536")
+    compare(eng, oracle_ref(oracle, eng, sp, s.positions, q, s.cell, pd, cuts), len(s), "n=1536")
 
 
 def kcells(cell, kc):
@@ -115,7 +117,8 @@ def test_comb_under_ewald(eng, oracle):
     cuts = np.full(len(s), ew[0] + 0.5)
     print("case: COMB + Ewald n=%d" % len(s), flush=True)
     load(eng, sp, s.positions, q, s.cell, pd, cuts, ew)
-    compare(eng, oracle_ref(oracle, sp, s.positions, q, s.cell, pd, cuts, ew), len(s), "ewald")
+    compare(eng, oracle_ref(oracle, eng, sp, s.positions, q, s.cell, pd, cuts, ew), len(s),
+            "ewald")
 
 
 def test_comb_relaxation_matches_oracle(eng, oracle):

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import EWALD_CASES, load_golden
-from helpers import ewald_descriptors, rel_maxnorm
+from helpers import assert_same_pairs, ewald_descriptors, rel_maxnorm
 from pysic_amd import structures as st
 
 pytestmark = pytest.mark.gpu
@@ -47,8 +47,9 @@ def check(eng, oracle, species, pos, q, cell, pd, ew, ref=None, scaler=None):
     n = len(pos)
     print("case: n=%d" % n, flush=True)
     load_ewald(eng, species, pos, q, cell, pd, ew, scaler)
+    nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(n, ew[1] + 0.5))
+    assert_same_pairs(eng, nl)
     if ref is None:
-        nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(n, ew[1] + 0.5))
         args = (species, pos, q, cell, nl, pd, None, ew, scaler)
         e_ref = oracle.evaluate("energy", *args)
         f_ref, s_ref = oracle.evaluate("forces", *args)
@@ -84,6 +85,34 @@ def test_rocksalt_8x8x8_rc10(eng, oracle):
     ew = np.array([1, rc, kc, K[0], K[1], K[2], sigma, 0.00552635])
     sp = (s.charges < 0).astype(np.int32)
     check(eng, oracle, sp, s.positions, s.charges, s.cell, np.zeros((0, 10)), ew)
+
+
+def test_dense_bin_neighbor_scan(eng, oracle):
+    """The neighbor scan taken for large dense cells (Ewald and LJ lists):
+    the per-atom FP32-prefiltered scan with its wave-uniform stencil union
+    (method 5; picked by default from 2^17 atoms).  Rocksalt 8^3 at
+    rc + skin = 10.5 A has 4 bins per axis and 64 ions per bin, which selects
+    the dense instance.  The sorted (i, j, offset) set must equal the
+    oracle's bit for bit, and E, F, W and chi must agree to 1e-9."""
+    s = st.rocksalt((8, 8, 8), sigma=0.08, seed=15)
+    rc, sigma, kc = 10.0, 2.04430, 2.5713
+    K = kcells(s.cell, kc)
+    ew = np.array([1, rc, kc, K[0], K[1], K[2], sigma, 0.00552635])
+    sp = (s.charges < 0).astype(np.int32)
+    L = np.linalg.norm(s.cell, axis=1)
+    bins = np.floor(L / (rc + 0.5))
+    assert bins.min() >= 3 and len(s) / bins.prod() >= 6  # the dense instance's condition
+    eng.set_neighbor_method(5)
+    try:
+        check(eng, oracle, sp, s.positions, s.charges, s.cell, np.zeros((0, 10)), ew)
+        # a triclinic replica: bins and wrapped images along sheared axes
+        shear = np.array([[1.0, 0.0, 0.0], [0.15, 1.0, 0.0], [-0.1, 0.1, 1.0]])
+        cell, pos = s.cell @ shear, s.positions @ shear
+        K = kcells(cell, kc)
+        ew = np.array([1, rc, kc, K[0], K[1], K[2], sigma, 0.00552635])
+        check(eng, oracle, sp, pos, s.charges, cell, np.zeros((0, 10)), ew)
+    finally:
+        eng.set_neighbor_method(0)
 
 
 def test_structure_factors_follow_the_state(eng, oracle):

@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table, rel_maxnorm
+from helpers import (MIXTURE_TABLE_ID, assert_same_pairs, mixture_model, mixture_table,
+                     rel_maxnorm)
 from pysic_amd import structures as st
 from pysic_amd.interactions import (Potential, ProductPotential, build_descriptors, lj_argon,
                                     tersoff_si)
@@ -75,6 +76,7 @@ def _vs_oracle(eng, oracle, s, model, cutoff, tables=None, tol=TOL):
         oracle.set_table(tid, rows)
     setup(eng, s.positions, s.cell, s.pbc, sp, pd, cut, tables, bd)
     nl = oracle.neighbor_list(s.positions, s.cell, s.pbc, cut)
+    assert_same_pairs(eng, nl)
     args = (sp, s.positions, None, s.cell, nl, pd, bd)
     e_ref = oracle.evaluate("energy", *args)
     f_ref, s_ref = oracle.evaluate("forces", *args)
@@ -122,6 +124,7 @@ def test_lj_next_to_tersoff(eng, oracle):
     cut = np.full(len(s), 5.5)
     setup(eng, s.positions, s.cell, s.pbc, sp, pd, cut, None, bd)
     nl = oracle.neighbor_list(s.positions, s.cell, s.pbc, cut)
+    assert_same_pairs(eng, nl)
     args = (sp, s.positions, None, s.cell, nl, pd, bd)
     e_ref = oracle.evaluate("energy", *args)
     f_ref, w_ref = oracle.evaluate("forces", *args)

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from conftest import TERSOFF_CASES, load_golden
-from helpers import rel_maxnorm, tersoff_descriptors
+from helpers import assert_same_pairs, rel_maxnorm, tersoff_descriptors
 from pysic_amd import structures as st
 
 pytestmark = pytest.mark.gpu
@@ -232,3 +232,68 @@ def test_neighbor_methods_agree(eng, oracle, method):
         check_tersoff(eng, oracle, t.positions, t.cell, [1, 1, 1], pd, bd, np.full(len(t), 3.5))
     finally:
         eng.set_neighbor_method(0)
+
+
+def test_full_size_md_workload(oracle):
+    """The 1,048,576-atom Tersoff MD workload (64 x 64 x 32 diamond cells,
+    sigma 0.1 A, Maxwell-Boltzmann 600 K), set up as the benchmark sets it
+    up: the default neighbor scan for a cell of this size, the site-kernel
+    shape the autotuner picks, lists rebuilt every step.  Checked against
+    the oracle on the host: the sorted (i, j, offset) set, the FP64 energy,
+    the forces of the API call, and two device MD steps (the graph-free
+    first step and the fused gather + drift + binning of the next), whose
+    momenta carry the forces of the MD path's own site kernel."""
+    import bench
+    from pysic_amd import autotune
+    from pysic_amd.engine import Engine
+    s = bench.build_workload((64, 64, 32))
+    n = len(s)
+    assert n == 1 << 20
+    pd, bd = tersoff_descriptors()
+    cut = np.full(n, 3.5)
+    eng = Engine(0)
+    try:
+        eng.create_atoms(np.zeros(n, np.int32), s.positions, masses=s.masses, momenta=s.momenta)
+        eng.create_cell(s.cell, s.pbc)
+        eng.set_potentials(pd)
+        eng.set_bond_order_factors(bd)
+        eng.ewald_off()
+        eng.set_neighbor_capacity(16)
+        eng.generate_neighbor_lists(cut)
+        variant = autotune.best_variant(eng, "tersoff_si")
+        eng.set_variant(variant)
+        print("case: full-size workload n=%d variant %s" % (n, eng.variants()[variant]),
+              flush=True)
+        sp = np.zeros(n, np.int32)
+        nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], cut)
+        assert_same_pairs(eng, nl)
+        e_ref = oracle.evaluate("energy", sp, s.positions, None, s.cell, nl, pd, bd)
+        f0, _ = oracle.evaluate("forces", sp, s.positions, None, s.cell, nl, pd, bd)
+        e = eng.energy()
+        f, _ = eng.forces()
+        err_e, err_f = abs(e - e_ref) / n, rel_maxnorm(f, f0, 1e-2)
+        print("  |dE|/N %.2e  rel|dF| %.2e" % (err_e, err_f), flush=True)
+        assert err_e <= 1e-7
+        assert err_f <= 1e-5
+        # two velocity-Verlet steps on the device against the oracle's
+        dt = st.ASE_FS
+        eng.md_run(2, dt, rebuild=1)
+        pos_g, mom_g = eng.state()
+        m = s.masses[:, None]
+        r, p, f_or = s.positions.copy(), s.momenta.copy(), f0
+        for step in range(2):
+            p = p + 0.5 * dt * f_or
+            r = r + dt * p / m
+            nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+            f_or, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+            p = p + 0.5 * dt * f_or
+            print("  oracle step %d done" % (step + 1), flush=True)
+        # each step's momentum change is dt/2 (F_old + F_new): a force error
+        # of 1e-5 max|F| per evaluation bounds the difference by 2e-5 dt max|F|
+        dp = np.abs(mom_g - p).max() / (dt * np.abs(f0).max())
+        dr = np.abs(pos_g - r).max()
+        print("  MD: |dr| %.2e A  |dp| / (dt max|F|) %.2e" % (dr, dp), flush=True)
+        assert dp <= 2e-5
+        assert dr <= 2e-5 * dt * dt * np.abs(f0).max() / s.masses.min() + 1e-12
+    finally:
+        eng.close()

@@ -302,3 +302,58 @@ def test_relaxation_single_mode_is_damped_oscillator(oracle):
     assert steps == nstep
     assert abs(q[0] - x) < 2e-3 * 0.3
     assert abs(q[0] + q[1]) < 1e-14  # total charge is conserved
+
+
+def test_relaxation_exact_discrete_dispersion(oracle):
+    """The damped dynamics at zero friction, pinned by the exact solution of
+    its own discrete map (relaxation.py:426-448).  With a single mode of
+    stiffness w^2 (two atoms, E = q^2 each, so dchi_1 = -2 q_1 on the neutral
+    pair; inertia 2 gives w = 1) and h = w dt, one step maps (x, u = v dt)
+    by
+        x' = (1 - h^2) x + u                (the update carries the full dt^2)
+        u' = (1 - h^2/2) u - h^2 (1 - h^2/2) x
+    so the trace is 2 - 3h^2/2 and the determinant D = 1 - h^2/2: not
+    velocity Verlet (det 1).  At 9h^4 - 22h^2 + 12 = 0 the eigenvalues are
+    sqrt(D) exp(+-i pi/3), hence A^3 = -D^(3/2) I and A^6 = D^3 I exactly.
+    A swapped half-kick, a dt^2/2 position update or a wrong friction
+    factor breaks these."""
+    h2 = (22.0 - math.sqrt(52.0)) / 18.0
+    dt = math.sqrt(h2)
+    D = 1.0 - 0.5 * h2
+    sp = np.array([0, 0], np.int32)
+    pd = np.array([[2, 1, 0, 0, 1.0, 2.0, 0, 0, -1, 0]], float)  # E = q^2, chi = -2q
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 5.0, 5.0]])
+    cell = np.eye(3) * 10.0
+    nl = oracle.neighbor_list(pos, cell, [1, 1, 1], np.full(2, 1.0))
+    x0 = 0.3
+    expect = {1: ((1.0 - h2) * x0, -h2 * (1.0 - 0.5 * h2) * x0),
+              3: (-D ** 1.5 * x0, 0.0),
+              6: (D ** 3 * x0, 0.0)}
+    for n, (x, u) in expect.items():
+        q, r, steps, _ = oracle.relax_dynamic(sp, pos, np.array([x0, -x0]), cell, nl, pd, None,
+                                              n, dt, 2.0, 0.0, 1e-300)
+        assert steps == n
+        assert abs(q[0] - x) < 1e-14 and abs(q[0] + q[1]) < 1e-15
+        assert abs(r[0] * dt - u) < 1e-14 and abs(r[0] + r[1]) < 1e-15
+    # with friction g = friction dt the determinant of the map becomes
+    # D (1 - g/2) / (1 + g/2): the half-kick before the new forces scales
+    # the rate by (1 - g/2), the one after by 1 / (1 + g/2).  The map's two
+    # columns come from a unit-rate start and a charge start.
+    g = 0.4
+    for n in (1, 2):
+        q, r, steps, _ = oracle.relax_dynamic(sp, pos, np.zeros(2), cell, nl, pd, None, n, dt,
+                                              2.0, g / dt, 1e-300, rates=np.array([1.0, -1.0]))
+        if n == 1:
+            x1, u1 = q[0], r[0] * dt
+        else:
+            x2, u2 = q[0], r[0] * dt
+    qx, rx, _, _ = oracle.relax_dynamic(sp, pos, np.array([x0, -x0]), cell, nl, pd, None, 1, dt,
+                                        2.0, g / dt, 1e-300)
+    a11, a21 = qx[0] / x0, rx[0] * dt / x0
+    a12, a22 = x1 / dt, u1 / dt
+    det = a11 * a22 - a12 * a21
+    assert abs(det - D * (1 - 0.5 * g) / (1 + 0.5 * g)) < 1e-14
+    # Cayley-Hamilton over two steps: A^2 = tr(A) A - det(A) I
+    tr = a11 + a22
+    assert abs(x2 - (tr * x1 - det * 0.0)) < 1e-13
+    assert abs(u2 - (tr * u1 - det * dt)) < 1e-13
