@@ -106,6 +106,23 @@ int pysic_get_neighbor_lists(pysic_ctx *h, long nnz, int *idx, int *offsets);
 
 /* calculate_energy (PyInterface.f90:652) */
 int pysic_calculate_energy(pysic_ctx *h, double *energy);
+/* No counterpart in the reference: pysic_calculate_energy for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Allocates nothing: the energy
+ * pass's per-block partial sums and its total live in a fixed device array
+ * of the library, 8 doubles per site-kernel tile for PYSIC_NOALLOC_MAX_TILES
+ * tiles plus one row (2 MiB per GPU, held until the process ends); a system
+ * needing more tiles (more than 2,097,152 atoms at 4 lanes per atom: tiles =
+ * ceil(N * G / 256) for kernel variant g<G>_s<S>) gets PYSIC_ERANGE.  Takes
+ * one cell of one species with Tersoff and exponential terms only, after its
+ * neighbor lists are generated (potentials set before the atoms exist are
+ * compiled by the first evaluation: until pysic_calculate_energy has run
+ * once, the call refuses with PYSIC_ESTATE); the result equals
+ * pysic_calculate_energy's, bit for bit.  Every call on one
+ * GPU shares the array (it keeps the last call's contents; nothing in it
+ * needs to start at zero): calls that overlap on one GPU race. */
+#define PYSIC_NOALLOC_MAX_TILES 32768
+int pysic_calculate_energy_noalloc(pysic_ctx *h, double *energy);
 /* calculate_forces (PyInterface.f90:668): forces (N,3), stress(6) =
  * sum r (x) f in the order xx yy zz yz xz xy */
 int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress);

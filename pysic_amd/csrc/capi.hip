@@ -20,6 +20,48 @@
 #include "../../include/pysic_learned.h"
 #include "../../include/pysic_batch.h"
 
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of pysic_calculate_energy_noalloc: the per-block
+// partial sums of the energy pass (8 doubles per site-kernel tile) and the
+// totals row.  Each GPU holds its own copy for the life of the process; host
+// code uses the address hipGetSymbolAddress returns on that device (the
+// array's name is only a host-side stand-in), looked up once per device when
+// the first handle on it is created.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoallocTiles = PYSIC_NOALLOC_MAX_TILES;
+constexpr size_t kScratchDoubles = (size_t)kNoallocTiles * 8 + 8;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+// with `device` current
+void scratch_setup(int device)
+{
+    PYSIC_REQUIRE(device < kMaxDevices, PYSIC_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void *p = nullptr;
+    hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    PYSIC_REQUIRE(rc == hipSuccess && p, PYSIC_EHIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    g_scratch[device] = (double *)p;
+}
+
+double *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
+}  // namespace
+
+
 // ---------------------------------------------------------------------------
 // kernel timing
 // ---------------------------------------------------------------------------
@@ -825,6 +867,7 @@ int pysic_create(pysic_ctx **out, int device)
         HIP_TRY(hipGetDeviceCount(&count));
         PYSIC_REQUIRE(device >= 0 && device < count, PYSIC_EINVAL, "no such HIP device");
         HIP_TRY(hipSetDevice(device));
+        scratch_setup(device);
         HIP_TRY(hipStreamCreate(&h->S.stream));
         h->S.flags.ensure(4);
         HIP_TRY(hipMemset(h->S.flags.p, 0, sizeof(int) * 4));
@@ -1071,6 +1114,47 @@ int pysic_calculate_energy(pysic_ctx *h, double *energy)
         e += t;
     }
     if (S.model.has_self || h->ewald.on) e += ewald_eem_energy(S, h->ewald, h->ew);
+    *energy = e;
+    API_END(h)
+}
+
+int pysic_calculate_energy_noalloc(pysic_ctx *h, double *energy)
+{
+    API_BEGIN(h)
+    PYSIC_REQUIRE(energy, PYSIC_EINVAL, "calculate_energy_noalloc: null output");
+    PYSIC_REQUIRE(h->atoms_ready && h->cell_ready && h->lists_valid, PYSIC_ESTATE,
+                  "calculate_energy_noalloc: atoms, cell and neighbor lists must be set up first");
+    System &S = h->S;
+    PYSIC_REQUIRE(!S.model_dirty && S.dmodel.p, PYSIC_ESTATE,
+                  "calculate_energy_noalloc: the potentials changed after the last evaluation; "
+                  "evaluate once with pysic_calculate_energy first");
+    PYSIC_REQUIRE(S.ncells == 0 && S.model.nspecies == 1 && S.model.has_pairs && !has_gen(S) &&
+                      !S.model.has_self && !h->ewald.on,
+                  PYSIC_EINVAL,
+                  "calculate_energy_noalloc takes one cell of one species with Tersoff and "
+                  "exponential terms only");
+    PYSIC_REQUIRE(S.flags.p && S.flags.cap >= 4, PYSIC_ESTATE, "calculate_energy_noalloc: no flags");
+    double *ptr = scratch_of(h->device);
+    PYSIC_REQUIRE(ptr, PYSIC_ESTATE, "calculate_energy_noalloc: scratch not set up on this device");
+    const int tiles = ts_energy_tiles(S, h->variant);
+    PYSIC_REQUIRE(tiles <= kNoallocTiles, PYSIC_ERANGE,
+                  "calculate_energy_noalloc: " + std::to_string(S.n) + " atoms need " +
+                      std::to_string(tiles) + " site-kernel tiles of kernel variant " +
+                      ts_variant_name(h->variant) + "; the fixed scratch holds " +
+                      std::to_string(kNoallocTiles));
+    double *partial = ptr, *totals = ptr + (size_t)kNoallocTiles * 8;
+    ts_energy_into(S, h->variant, partial, totals);
+    double e = 0.0;
+    int f[4];
+    HIP_TRY(hipMemcpyAsync(&e, totals, sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipMemcpyAsync(f, S.flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+    PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
+                  "calculate_energy_noalloc: " + std::to_string(f[1]) +
+                      " short-range neighbors exceed kernel variant " +
+                      ts_variant_name(h->variant) + "; pick a larger one (pysic_calculate_energy "
+                      "switches by itself)");
     *energy = e;
     API_END(h)
 }

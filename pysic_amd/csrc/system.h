@@ -148,6 +148,10 @@ void ts_site(System &S, int variant, int mode);
 void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
 void ts_reduce(System &S);                    // totals[0..6] = E, W(6)
 void ts_cell_sums(System &S);                 // batch: cell_ev[c] = E, W(6) of cell c
+int ts_energy_tiles(const System &S, int variant);  // partial rows of the energy pass
+// energy pass into caller scratch (one cell, one species; allocates nothing):
+// partial[ts_energy_tiles * 8], totals[0] = E
+void ts_energy_into(System &S, int variant, double *partial, double *totals);
 int ts_variant_count();
 const char *ts_variant_name(int v);
 int ts_variant_capacity(int v);

@@ -870,6 +870,49 @@ void ts_site(System &S, int variant, int mode)
     S.partial_blocks = div_up(S.n, BLOCK / kVariants[variant].G);
 }
 
+// Site-kernel tiles of the single-species FP64 energy pass for this system
+// and variant: the number of per-block partial rows it writes.
+int ts_energy_tiles(const System &S, int variant)
+{
+    PYSIC_REQUIRE(variant >= 0 && variant < kNumVariants, PYSIC_EINVAL, "bad Tersoff kernel variant");
+    return div_up(S.n, BLOCK / kVariants[variant].G);
+}
+
+namespace {
+template <int G, int SCAP>
+void launch_energy_into(System &S, double *partial)
+{
+    const int ntiles = div_up(S.n, BLOCK / G);
+    k_site1e<G, SCAP, GeoOne><<<ntiles, BLOCK, 0, S.stream>>>(
+        S.n, ntiles, GeoOne{S.geom}, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, partial,
+        nullptr, S.flags.p);
+}
+}  // namespace
+
+// The single-species FP64 energy pass of one cell with caller-provided
+// scratch: partial[ts_energy_tiles * 8] receives the per-block sums and
+// totals[0] the energy.  Allocates nothing and records no timing events;
+// the flags are cleared by a kernel first (short-list overflow lands in
+// flags[1], as in ts_site).
+void ts_energy_into(System &S, int variant, double *partial, double *totals)
+{
+    PYSIC_REQUIRE(S.ncells == 0 && S.model.nspecies == 1, PYSIC_EINVAL,
+                  "the scratch energy pass takes one cell of one species");
+    nb_clear_flags(S);
+    switch (variant) {
+    case 0: launch_energy_into<4, 8>(S, partial); break;
+    case 1: launch_energy_into<4, 16>(S, partial); break;
+    case 2: launch_energy_into<8, 8>(S, partial); break;
+    case 3: launch_energy_into<8, 16>(S, partial); break;
+    case 4: launch_energy_into<2, 8>(S, partial); break;
+    case 5: launch_energy_into<16, 32>(S, partial); break;
+    default: throw PysicError{PYSIC_EINVAL, "bad Tersoff kernel variant"};
+    }
+    HIP_TRY(hipGetLastError());
+    k_reduce_partials<<<1, 1024, 0, S.stream>>>(ts_energy_tiles(S, variant), partial, totals);
+    HIP_TRY(hipGetLastError());
+}
+
 void ts_gather(System &S, double half_dt, int kick)
 {
     hipEvent_t ev;

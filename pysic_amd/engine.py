@@ -41,6 +41,7 @@ SIGNATURES = {
     "pysic_get_neighbor_counts": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "pysic_get_neighbor_lists": (ctypes.c_int, [_vp, ctypes.c_long, _ip, _ip]),
     "pysic_calculate_energy": (ctypes.c_int, [_vp, _dp]),
+    "pysic_calculate_energy_noalloc": (ctypes.c_int, [_vp, _dp]),
     "pysic_calculate_forces": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "pysic_calculate_electronegativities": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pysic_relax_charges": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int,
@@ -244,6 +245,14 @@ class Engine:
     def energy(self):
         e = np.zeros(1)
         self._check(self.L.pysic_calculate_energy(self.h, _d(e)), "calculate_energy")
+        return float(e[0])
+
+    def energy_noalloc(self):
+        """pysic_calculate_energy_noalloc: the energy with no device
+        allocation (fixed library scratch; see include/pysic_hip.h)."""
+        e = np.zeros(1)
+        self._check(self.L.pysic_calculate_energy_noalloc(self.h, _d(e)),
+                    "calculate_energy_noalloc")
         return float(e[0])
 
     def forces(self):
