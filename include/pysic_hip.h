@@ -130,6 +130,31 @@ int pysic_calculate_forces(pysic_ctx *h, int n, double *forces, double *stress);
  * charge_self, Ewald and charge-dependent pair terms */
 int pysic_calculate_electronegativities(pysic_ctx *h, int n, double *enegs);
 
+/* calculate_bond_order_factors (PyInterface.f90:579): the per-atom factors
+ * of bond-order group `group` (a Coordinator's index).  The reference sums
+ * only per-atom (level-1) factors into them; 'tersoff', the factor type
+ * this engine takes, is per-bond (level 2), so they are zero, as the
+ * reference's are. */
+int pysic_calculate_bond_order_factors(pysic_ctx *h, int n, int group, double *factors);
+/* calculate_bond_order_gradients (PyInterface.f90:605): gradients (N,3) of
+ * every per-atom factor of the group with respect to moving atom `atom`
+ * (0-based; the reference's f2py call takes it 1-based); zero here, as
+ * above. */
+int pysic_calculate_bond_order_gradients(pysic_ctx *h, int n, int group, int atom,
+                                         double *gradients);
+/* calculate_bond_order_gradients_of_factor (PyInterface.f90:634): gradients
+ * (N,3) of atom `atom`'s factor with respect to moving every atom; zero here,
+ * as above. */
+int pysic_calculate_bond_order_gradients_of_factor(pysic_ctx *h, int n, int group, int atom,
+                                                   double *gradients);
+/* get_ewald_energy (PyInterface.f90:1050, Core.f90:8077-8089): the Ewald
+ * sum of the current charges with these parameters -- real space over the
+ * neighbor lists (which must reach real_cut), self, background, reciprocal
+ * and the dipole term |sum q r|^2 / (6 V epsilon) -- with unit charge
+ * scalers.  The Ewald parameters set by pysic_set_ewald are not changed. */
+int pysic_get_ewald_energy(pysic_ctx *h, double real_cut, double k_cut, const int *kcells,
+                           double sigma, double epsilon, double *energy);
+
 /* ChargeRelaxation.charge_relaxation 'dynamic' (charges/relaxation.py:417-450)
  * run on the device; charges/rates in and out (original order).  Returns the
  * step count in *steps and the final max(dchi) in *error. */

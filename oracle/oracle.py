@@ -260,6 +260,18 @@ class RefCore:
             R.ref_energy.argtypes = [_dp]
             R.ref_forces.argtypes = [ctypes.c_int, _dp, _dp]
             R.ref_electronegativities.argtypes = [ctypes.c_int, _dp]
+            R.ref_number_of_types.argtypes = [_ip, _ip]
+            R.ref_list_potentials.argtypes = [ctypes.c_int, _ip]
+            R.ref_list_bond_order_factors.argtypes = [ctypes.c_int, _ip]
+            R.ref_potential_info.argtypes = [ctypes.c_int, ctypes.c_char_p, _ip, _ip, _ip]
+            R.ref_bond_order_factor_info.argtypes = [ctypes.c_int, ctypes.c_char_p, _ip, _ip,
+                                                     _ip, _ip]
+            R.ref_bond_order_factors.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+            R.ref_bond_order_gradients.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+            R.ref_bond_order_gradients_of_factor.argtypes = [ctypes.c_int, ctypes.c_int,
+                                                             ctypes.c_int, _dp]
+            R.ref_get_ewald_energy.argtypes = [ctypes.c_double, ctypes.c_double, _ip,
+                                               ctypes.c_double, ctypes.c_double, _dp]
             R.ref_start()
             _ref = R
         self.R = _ref
@@ -379,3 +391,77 @@ class RefCore:
         x = np.zeros(self.n)
         self.R.ref_electronegativities(self.n, _d(x))
         return x
+
+    # -- the reference's descriptor registry (PyInterface.f90:706-1011) --
+    def registry(self):
+        """{'potentials': [(name, n_targets, [parameter names])],
+        'bond_order_factors': [(name, n_targets, level, [[names] per target
+        class 1, 2])]}, as the reference's own registry reports them."""
+        R = self.R
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        R.ref_number_of_types(ctypes.byref(a), ctypes.byref(b))
+        pc = np.zeros((a.value, 11), np.int32)
+        R.ref_list_potentials(a.value, _i(pc))
+        bc = np.zeros((b.value, 11), np.int32)
+        R.ref_list_bond_order_factors(b.value, _i(bc))
+        pots, bonds = [], []
+        for code in pc:
+            name = ints2str(code)
+            nm = name.encode()
+            nt, npar = ctypes.c_int(0), ctypes.c_int(0)
+            names = np.zeros((12, 10), np.int32)  # Fortran (10,12)
+            R.ref_potential_info(len(nm), nm, ctypes.byref(nt), ctypes.byref(npar), _i(names))
+            pots.append((name, nt.value, [ints2str(c) for c in names[:npar.value]]))
+        for code in bc:
+            name = ints2str(code)
+            nm = name.encode()
+            nt, lv = ctypes.c_int(0), ctypes.c_int(0)
+            npar = np.zeros(2, np.int32)
+            names = np.zeros((2, 12, 10), np.int32)  # Fortran (10,12,2)
+            R.ref_bond_order_factor_info(len(nm), nm, ctypes.byref(nt), ctypes.byref(lv),
+                                         _i(npar), _i(names))
+            bonds.append((name, nt.value, lv.value,
+                          [[ints2str(c) for c in names[k][:npar[k]]] for k in range(2)]))
+        return {"potentials": pots, "bond_order_factors": bonds}
+
+    # -- bond-order factor queries (PyInterface.f90:579-634) --
+    def bond_order_factors(self, group):
+        x = np.zeros(self.n)
+        self.R.ref_bond_order_factors(self.n, group, _d(x))
+        return x
+
+    def bond_order_gradients(self, group, atom):
+        """Gradients (N,3) of every factor of the group with respect to
+        moving atom (0-based)."""
+        g = np.zeros((self.n, 3))  # Fortran (3,n) == C (n,3)
+        self.R.ref_bond_order_gradients(self.n, group, atom + 1, _d(g))
+        return g
+
+    def bond_order_gradients_of_factor(self, group, atom):
+        """Gradients (N,3) of atom's factor with respect to moving every atom."""
+        g = np.zeros((self.n, 3))
+        self.R.ref_bond_order_gradients_of_factor(self.n, group, atom + 1, _d(g))
+        return g
+
+    def ewald_energy(self, rc, kc, kcells, sigma, eps):
+        """get_ewald_energy (PyInterface.f90:1050): the whole Ewald sum with
+        the given parameters, unit scalers and the dipole term."""
+        e = np.zeros(1)
+        kk = np.asarray(kcells, dtype=np.int32)
+        self.R.ref_get_ewald_energy(rc, kc, _i(kk), sigma, eps, _d(e))
+        return float(e[0])
+
+
+_I2S = {-10: "0", 201: "_"}
+for _k in range(1, 10):
+    _I2S[-_k] = str(_k)
+for _k, _ch in enumerate("abcdefghijklmnopqrstuvwxyz"):
+    _I2S[_k + 1] = _ch
+for _k, _ch in enumerate("ABCDEFGHIJKLMNOPQRSTUVWXYZ"):
+    _I2S[101 + _k] = _ch
+
+
+def ints2str(codes):
+    """The reference's integer codes back to text (Utility.f90:205-250;
+    0 and unknown codes are blanks), stripped."""
+    return "".join(_I2S.get(int(c), " ") for c in codes).strip()

@@ -203,6 +203,109 @@ contains
     call calculate_electronegativities(n, enegs)
   end subroutine ref_electronegativities
 
+  ! --- descriptor registry (PyInterface.f90:706-1011) -------------------
+  ! Names travel as the reference's integer codes (Utility.f90:205-250).
+
+  subroutine ref_number_of_types(npot, nbond) bind(C, name="ref_number_of_types")
+    integer(c_int), intent(out) :: npot, nbond
+    integer :: a, b
+    call number_of_potentials(a)
+    call number_of_bond_order_factors(b)
+    npot = a
+    nbond = b
+  end subroutine ref_number_of_types
+
+  subroutine ref_list_potentials(npot, codes) bind(C, name="ref_list_potentials")
+    integer(c_int), value :: npot
+    integer(c_int), intent(out) :: codes(11, npot)
+    integer :: tmp(11, npot)
+    call list_valid_potentials(npot, tmp)
+    codes = tmp
+  end subroutine ref_list_potentials
+
+  subroutine ref_list_bond_order_factors(nbond, codes) bind(C, name="ref_list_bond_order_factors")
+    integer(c_int), value :: nbond
+    integer(c_int), intent(out) :: codes(11, nbond)
+    integer :: tmp(11, nbond)
+    call list_valid_bond_order_factors(nbond, tmp)
+    codes = tmp
+  end subroutine ref_list_bond_order_factors
+
+  function c_name(n, chars) result(name)
+    integer(c_int), intent(in) :: n
+    character(kind=c_char), intent(in) :: chars(n)
+    character(len=n) :: name
+    integer :: i
+    do i = 1, n
+       name(i:i) = chars(i)
+    end do
+  end function c_name
+
+  ! targets, parameter count and parameter-name codes of a potential
+  subroutine ref_potential_info(n, chars, n_targets, n_params, names) &
+       bind(C, name="ref_potential_info")
+    integer(c_int), value :: n
+    character(kind=c_char), intent(in) :: chars(n)
+    integer(c_int), intent(out) :: n_targets, n_params, names(10, 12)
+    integer :: a, b, tmp(10, 12)
+    call number_of_targets_of_potential(c_name(n, chars), a)
+    call number_of_parameters_of_potential(c_name(n, chars), b)
+    call names_of_parameters_of_potential(c_name(n, chars), tmp)
+    n_targets = a
+    n_params = b
+    names = tmp
+  end subroutine ref_potential_info
+
+  ! targets, level, parameter counts and names per target class (1, 2)
+  subroutine ref_bond_order_factor_info(n, chars, n_targets, level, n_params, names) &
+       bind(C, name="ref_bond_order_factor_info")
+    integer(c_int), value :: n
+    character(kind=c_char), intent(in) :: chars(n)
+    integer(c_int), intent(out) :: n_targets, level, n_params(2), names(10, 12, 2)
+    integer :: a, b, c, k, tmp(10, 12)
+    call number_of_targets_of_bond_order_factor(c_name(n, chars), a)
+    call level_of_bond_order_factor(c_name(n, chars), b)
+    n_targets = a
+    level = b
+    do k = 1, 2
+       call number_of_parameters_of_bond_order_factor(c_name(n, chars), k, c)
+       n_params(k) = c
+       call names_of_parameters_of_bond_order_factor(c_name(n, chars), k, tmp)
+       names(:, :, k) = tmp
+    end do
+  end subroutine ref_bond_order_factor_info
+
+  ! --- bond-order factor queries (PyInterface.f90:579-634) ---------------
+  subroutine ref_bond_order_factors(n, group, factors) bind(C, name="ref_bond_order_factors")
+    integer(c_int), value :: n, group
+    real(c_double), intent(out) :: factors(n)
+    call calculate_bond_order_factors(n, group, factors)
+  end subroutine ref_bond_order_factors
+
+  ! atom: 1-based, as bondorder.py:518 passes it
+  subroutine ref_bond_order_gradients(n, group, atom, grads) bind(C, name="ref_bond_order_gradients")
+    integer(c_int), value :: n, group, atom
+    real(c_double), intent(out) :: grads(3, n)
+    call calculate_bond_order_gradients(n, group, atom, grads)
+  end subroutine ref_bond_order_gradients
+
+  subroutine ref_bond_order_gradients_of_factor(n, group, atom, grads) &
+       bind(C, name="ref_bond_order_gradients_of_factor")
+    integer(c_int), value :: n, group, atom
+    real(c_double), intent(out) :: grads(3, n)
+    call calculate_bond_order_gradients_of_factor(n, group, atom, grads)
+  end subroutine ref_bond_order_gradients_of_factor
+
+  ! --- the stand-alone Ewald energy (PyInterface.f90:1050) ----------------
+  subroutine ref_get_ewald_energy(rc, kc, kcells, sigma, eps, e) bind(C, name="ref_get_ewald_energy")
+    real(c_double), value :: rc, kc, sigma, eps
+    integer(c_int), intent(in) :: kcells(3)
+    real(c_double), intent(out) :: e
+    integer :: kk(3)
+    kk = kcells
+    call get_ewald_energy(rc, kc, kk, sigma, eps, e)
+  end subroutine ref_get_ewald_energy
+
   subroutine ref_release() bind(C, name="ref_release")
     call release()
   end subroutine ref_release

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libpysic_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYSIC_ARCH", "gfx950")
 SOURCES = ["neighbor.hip", "tersoff.hip", "pair.hip", "bend.hip", "md.hip", "ewald.hip", "mlp.hip",
-           "capi.hip"]
+           "capi.hip", "registry.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-I" + CSRC]

@@ -61,7 +61,6 @@ double *scratch_of(int device)
 }
 }  // namespace
 
-
 // ---------------------------------------------------------------------------
 // kernel timing
 // ---------------------------------------------------------------------------
@@ -1115,6 +1114,73 @@ int pysic_calculate_energy(pysic_ctx *h, double *energy)
     }
     if (S.model.has_self || h->ewald.on) e += ewald_eem_energy(S, h->ewald, h->ew);
     *energy = e;
+    API_END(h)
+}
+
+// Bond-order factor queries.  Every factor type this engine takes ('tersoff')
+// is a per-bond factor (level 2); the per-atom factors and their gradients
+// that the reference returns for a group are sums over level-1 factors only
+// (Core.f90:2015-2404, the n_level == 1 filter), so for these groups they
+// are zero, as the reference's are.
+int pysic_calculate_bond_order_factors(pysic_ctx *h, int n, int group, double *factors)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    PYSIC_REQUIRE(n == h->S.n && factors && group >= 0, PYSIC_EINVAL,
+                  "calculate_bond_order_factors: bad arguments");
+    std::fill(factors, factors + n, 0.0);
+    API_END(h)
+}
+
+int pysic_calculate_bond_order_gradients(pysic_ctx *h, int n, int group, int atom,
+                                         double *gradients)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    PYSIC_REQUIRE(n == h->S.n && gradients && group >= 0 && atom >= 0 && atom < n, PYSIC_EINVAL,
+                  "calculate_bond_order_gradients: bad arguments");
+    std::fill(gradients, gradients + 3 * (size_t)n, 0.0);
+    API_END(h)
+}
+
+int pysic_calculate_bond_order_gradients_of_factor(pysic_ctx *h, int n, int group, int atom,
+                                                   double *gradients)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    PYSIC_REQUIRE(n == h->S.n && gradients && group >= 0 && atom >= 0 && atom < n, PYSIC_EINVAL,
+                  "calculate_bond_order_gradients_of_factor: bad arguments");
+    std::fill(gradients, gradients + 3 * (size_t)n, 0.0);
+    API_END(h)
+}
+
+int pysic_get_ewald_energy(pysic_ctx *h, double real_cut, double k_cut, const int *kcells,
+                           double sigma, double epsilon, double *energy)
+{
+    API_BEGIN(h)
+    require_ready(h);
+    PYSIC_REQUIRE(kcells && energy && sigma > 0.0 && epsilon > 0.0 && real_cut >= 0.0,
+                  PYSIC_EINVAL, "get_ewald_energy: bad arguments");
+    for (int a = 0; a < 3; ++a)
+        PYSIC_REQUIRE(kcells[a] >= 0 && kcells[a] < 1024, PYSIC_EINVAL,
+                      "get_ewald_energy: reciprocal cells per axis must lie in 0..1023");
+    EwaldParams P;
+    P.on = 1;
+    P.rc = real_cut;
+    P.kc = k_cut;
+    P.sigma = sigma;
+    P.eps = epsilon;
+    for (int a = 0; a < 3; ++a) P.K[a] = kcells[a];
+    // unit scalers (Core.f90:8084) and a state of its own: the model's
+    // Ewald parameters and cached structure factors are left as they are
+    EwaldState E;
+    try {
+        *energy = ewald_standalone_energy(h->S, P, E);
+    } catch (...) {
+        ewald_release(E);
+        throw;
+    }
+    ewald_release(E);
     API_END(h)
 }
 

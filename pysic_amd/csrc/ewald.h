@@ -50,6 +50,10 @@ struct EwaldState {
 void ewald_release(EwaldState &E);
 // Energy of the charge_self singlets + Ewald (real, reciprocal, self, background).
 double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E);
+// get_ewald_energy (Core.f90:8077-8089): the Ewald sum alone, real space
+// included, with the dipole term (the caller passes unit scalers and a state
+// of its own, so the cached structure factors of the model's Ewald stay).
+double ewald_standalone_energy(System &S, const EwaldParams &P, EwaldState &E);
 // Adds Ewald forces into S.frc; stress[6] receives the Ewald part.
 void ewald_add_forces(System &S, const EwaldParams &P, EwaldState &E, double *stress);
 // Electronegativities (charge_self + Ewald) into E.chi, sorted order.

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
                                              double inv_eps_4pi, double inv_sig_sqrt2,
                                              double inv_sig_sqrt2pi, double inv_sig_sq2,
                                              double bg_chi, const double *__restrict__ bg_dev,
-                                             int mode, double *__restrict__ out1,
+                                             int mode, int self_on, double *__restrict__ out1,
                                              double4 *__restrict__ frc, double *__restrict__ part)
 {
     __shared__ double red[EB][7];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(EB) void k_real(int n, CellGeom g, const double4 *_
         int si = (int)pi.w;
         double qi = q[i];
         // charge_self singlets: E = eps q^n, chi = -n eps q^(n-1)
-        if (M->has_self && sub == 0) {
+        if (self_on && M->has_self && sub == 0) {
             double qp = 1.0;  // q^(nn-1)
             e += M->self_eps[si][0];
             for (int nn = 1; nn <= 4; ++nn) {
@@ -659,6 +659,28 @@ __global__ void k_charge_sums(int n, const double *__restrict__ q, const double 
     }
 }
 
+// block partials of the dipole sum_i q_i s_i r_i (positions as stored:
+// absolute, unwrapped), components 0..2 of each 8-double row
+__global__ void k_dipole_sums(int n, const double4 *__restrict__ pos,
+                              const double *__restrict__ q, const double *__restrict__ sc,
+                              double *__restrict__ part)
+{
+    __shared__ double r[3][EB];
+    int i = blockIdx.x * EB + threadIdx.x;
+    double v = i < n ? q[i] * sc[i] : 0.0;
+    double4 p = i < n ? pos[i] : make_double4(0, 0, 0, 0);
+    r[0][threadIdx.x] = v * p.x;
+    r[1][threadIdx.x] = v * p.y;
+    r[2][threadIdx.x] = v * p.z;
+    __syncthreads();
+    for (int w = EB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 3; ++c) r[c][threadIdx.x] += r[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[(size_t)blockIdx.x * 8 + threadIdx.x] = r[threadIdx.x][0];
+}
+
 // background electronegativity from the reduced charge sum (same operations
 // as the host form - inv_eps_2v * (-sigma^2 Q), Potentials.f90:2166-2167)
 __global__ void k_background(const double *__restrict__ qsum, double *__restrict__ out,
@@ -1026,7 +1048,7 @@ void charge_sums(System &S, EwaldState &E, double *out)
 
 // real-space + singlet pass; returns the energy (mode 1) or stress (mode 2)
 void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double bg_chi,
-               double *out7, const double *bg_dev = nullptr)
+               double *out7, const double *bg_dev = nullptr, bool with_self = true)
 {
     const int n = S.n;
     Consts c = consts_of(P, E.volume);
@@ -1040,7 +1062,8 @@ void real_pass(System &S, const EwaldParams &P, EwaldState &E, int mode, double 
         kern<<<nb, EB, 0, S.stream>>>(n, S.geom, S.pos.p, S.q.p, E.scaler.p, S.nbr.p, S.nimg.p,
                                       S.ncnt.p, S.dmodel.p, P.on, P.rc, c.inv_eps_4pi,
                                       c.inv_sig_sqrt2, c.inv_sig_sqrt2pi, c.inv_sig_sq2, bg_chi,
-                                      bg_dev, mode, E.chi.p, S.frc.p, E.part.p);
+                                      bg_dev, mode, with_self ? 1 : 0, E.chi.p, S.frc.p,
+                                      E.part.p);
     };
     switch (lanes) {
     case 1: launch(k_real<1>); break;
@@ -1108,13 +1131,29 @@ void ewald_release(EwaldState &E)
     E.sk_valid = false;
 }
 
-double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
+namespace {
+// The energy terms: [charge_self singlets +] Ewald real, self, background and
+// reciprocal parts [+ the dipole term].
+double energy_terms(System &S, const EwaldParams &P, EwaldState &E, bool with_self,
+                    bool with_dipole)
 {
     prepare(S, P, E);
     if (!P.on) ensure_scaler_identity(S, E);
     double r7[7];
-    real_pass(S, P, E, 1, 0.0, r7);
+    real_pass(S, P, E, 1, 0.0, r7, nullptr, with_self);
     double e = r7[0];
+    if (P.on && with_dipole) {
+        // |sum q s r|^2 inv_eps_2v / 3 (Potentials.f90:1563-1565, 1647)
+        Consts c = consts_of(P, E.volume);
+        int nb = div_up(S.n, EB);
+        E.part.ensure((size_t)nb * 8);
+        k_dipole_sums<<<nb, EB, 0, S.stream>>>(S.n, S.pos.p, S.q.p, E.scaler.p, E.part.p);
+        k_sum_partials<<<1, EB, 0, S.stream>>>(nb, 8, 3, E.part.p, E.red.p);
+        double d[3];
+        HIP_TRY(hipMemcpyAsync(d, E.red.p, sizeof(double) * 3, hipMemcpyDeviceToHost, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));
+        e += (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * c.inv_eps_2v / 3.0;
+    }
     if (P.on) {
         Consts c = consts_of(P, E.volume);
         double qs[2];
@@ -1134,6 +1173,18 @@ double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
         }
     }
     return e;
+}
+}  // namespace
+
+double ewald_eem_energy(System &S, const EwaldParams &P, EwaldState &E)
+{
+    return energy_terms(S, P, E, true, false);
+}
+
+double ewald_standalone_energy(System &S, const EwaldParams &P, EwaldState &E)
+{
+    PYSIC_REQUIRE(P.on, PYSIC_EINVAL, "get_ewald_energy: no Ewald parameters");
+    return energy_terms(S, P, E, false, true);
 }
 
 void ewald_add_forces(System &S, const EwaldParams &P, EwaldState &E, double *stress)

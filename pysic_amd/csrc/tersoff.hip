@@ -432,6 +432,225 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
     }
 }
 
+// e^x for the pair exponentials of the MD force pass, x = -zeta r formed in
+// FP64: x log2(e) is split in FP64 into an integer and a fraction |f| <= 1/2,
+// and only 2^f is taken in FP32 (v_exp_f32), so the result carries FP32
+// rounding (~1e-7 relative) without the 3e-7 that rounding r or x to FP32
+// first would add.
+__device__ __forceinline__ float exp_split(double x)
+{
+    const double y = x * 1.4426950408889634;
+    const double nd = rint(y);
+    return ldexpf(__builtin_amdgcn_exp2f((float)(y - nd)), (int)nd);
+}
+
+// The force pass of a velocity-Verlet step (forces only, no virial), one
+// species.  Same structure as site1f_tile, with the pair arithmetic in FP32
+// as the three-body arithmetic already is: separations and distances are
+// formed in FP64 (the short-list and cutoff decisions stay exact), then
+// everything per bond is FP32 -- the exponentials through exp_split, the
+// smoothing functions from precomputed 1/(cut - soft), the reciprocal
+// distance from v_rcp_f32 and one Newton step -- and D_ij is accumulated and
+// stored in FP32.  The energy pass (site1e_tile) stays FP64.
+template <int G, int SCAP, class Geo>
+__device__ __forceinline__ void site1m_tile(int blk, int n, const Geo &geo,
+                                            const double4 *__restrict__ pos,
+                                            const int *__restrict__ nbr,
+                                            const int *__restrict__ nimg,
+                                            const int *__restrict__ ncnt,
+                                            const Model *__restrict__ M,
+                                            int *__restrict__ sj, int *__restrict__ simg,
+                                            int *__restrict__ scnt, float4 *__restrict__ dU,
+                                            int *__restrict__ flags)
+{
+    constexpr int APB = BLOCK / G;
+    constexpr int PER = (SCAP + G - 1) / G;
+    // [slot][atom]: the atoms of a wave sit on consecutive LDS banks
+    __shared__ float4 s_u[SCAP][APB];      // x, y, z, r
+    __shared__ float4 s_aux[SCAP][APB];    // 1/r, f_c(r), f_c'(r), A (then P)
+    __shared__ float2 s_pair[SCAP][APB];   // 1/2 d(f v_R)/dr / r, 1/2 d(f v_A)/dr / r
+
+    const int a = threadIdx.x / G;
+    const int l = threadIdx.x % G;
+    const int i = blk * APB + a;
+    const bool valid = i < n;
+    const double *cv = geo.vec(valid ? i : 0);  // vectors of this atom's cell
+    const int gbase = ((threadIdx.x & 63) / G) * G;
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+
+    const BondTerm B = M->bond[0][0][0];
+    const bool bond_on = M->group >= 0 && B.valid;
+    const double maxbond = M->maxbond[0], maxpot = M->maxpot[0], rshort = M->rshort;
+    const int npair = M->npair[0][0];
+    const float hf = (float)B.h, ddf = (float)(B.d * B.d), ccf = (float)(B.c * B.c);
+    const float cd2f = (float)(B.c * B.c / (B.d * B.d));
+    const float betaf = (float)B.beta, etaf = (float)B.eta;
+    const float inv_eta = (float)(-0.5 / B.eta);
+    // bond cutoff: f_c = (1 + cos(pi t)) / 2, t = (r - soft) / (cut - soft);
+    // v_cos_f32 / v_sin_f32 take revolutions, so pi t = 2 pi (t / 2)
+    const bool bsmooth = B.soft > 0.0 && B.soft < B.cut;
+    const float bsoft = (float)B.soft, bhalfw = bsmooth ? (float)(0.5 / (B.cut - B.soft)) : 0.f;
+    const float bdk = bsmooth ? (float)(-0.5 * M_PI / (B.cut - B.soft)) : 0.f;
+
+    double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+
+    // ---- 1. short list; per neighbor: cutoff functions and pair terms -------
+    int nf = valid ? ncnt[i] : 0;
+    int cnt = 0;
+    for (int base = 0; base < nf; base += G) {
+        int e = base + l;
+        bool keep = false;
+        double3 sp3 = make_double3(0, 0, 0);
+        double r = 0.0;
+        int j = 0, im = 0;
+        if (e < nf) {
+            j = nbr[(size_t)e * n + i];
+            im = nimg[(size_t)e * n + i];
+            sp3 = separation64(pi, pos[j], im, cv);
+            r = sqrt(dot64(sp3));
+            keep = r < rshort;
+        }
+        unsigned long long b = __ballot(keep);
+        unsigned long long gb = (b >> gbase) & gmask;
+        int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
+        if (keep && slot < SCAP) {
+            const float rf = (float)r;
+            float dR = 0.f, dA = 0.f, A = 0.f;
+            if (r < maxpot) {
+                for (int t = 0; t < npair; ++t) {
+                    const PairTerm &pt = M->pair[0][0][t];
+                    if (!(pt.cut > r)) continue;
+                    const float v = (float)pt.eps * exp_split(-pt.zeta * r);
+                    float f = 1.f, df = 0.f;
+                    if (pt.smooth && r > pt.soft) {
+                        const float hw = (float)(0.5 / (pt.cut - pt.soft));
+                        const float x = (rf - (float)pt.soft) * hw;
+                        f = 0.5f + 0.5f * __builtin_amdgcn_cosf(x);
+                        df = -(float)M_PI * hw * __builtin_amdgcn_sinf(x);
+                    }
+                    const float dv = 0.5f * v * (df - (float)pt.zeta * f);
+                    if (pt.group >= 0) { dA += dv; A += 0.5f * v * f; }
+                    else dR += dv;
+                }
+            }
+            float ir = __builtin_amdgcn_rcpf(rf);
+            ir = ir * (2.0f - rf * ir);
+            float fb = 0.f, dfb = 0.f;
+            if (bond_on && r < maxbond) {
+                fb = 1.f;
+                if (bsmooth && r > B.soft) {
+                    const float x = (rf - bsoft) * bhalfw;
+                    fb = 0.5f + 0.5f * __builtin_amdgcn_cosf(x);
+                    dfb = bdk * __builtin_amdgcn_sinf(x);
+                }
+            }
+            s_u[slot][a] = make_float4((float)sp3.x, (float)sp3.y, (float)sp3.z, rf);
+            s_aux[slot][a] = make_float4(ir, fb, dfb, A);
+            s_pair[slot][a] = make_float2(dR * ir, dA * ir);  // divided by r once
+            size_t o = (size_t)slot * n + i;
+            sj[o] = j;
+            simg[o] = im;
+        }
+        cnt += __popcll(gb);
+    }
+    if (cnt > SCAP && l == 0) atomicMax(flags + 1, cnt);
+    const int ns = cnt < SCAP ? cnt : SCAP;
+    __syncthreads();
+
+    // ---- 2. zeta, b~, b~' ---------------------------------------------------
+    float cpair[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        cpair[q] = 0.f;
+        int jj = l + q * G;
+        if (jj >= ns) continue;
+        const float4 u = s_u[jj][a];
+        const float4 au = s_aux[jj][a];
+        float zeta = 0.f;
+        if (bond_on) {
+            for (int k = 0; k < ns; ++k) {
+                if (k == jj) continue;
+                const float4 ak = s_aux[k][a];
+                if (ak.y == 0.f) continue;
+                const float4 v = s_u[k][a];
+                const float cs = (u.x * v.x + u.y * v.y + u.z * v.z) * au.x * ak.x;
+                const float t = hf - cs;
+                const float t2 = t * t;
+                float xi = ak.y;
+                if (!B.a_zero) xi *= __expf(powf((float)B.a * (u.w - v.w), (float)B.mu));
+                zeta += xi * (1.0f + cd2f * t2 * __builtin_amdgcn_rcpf(ddf + t2));
+            }
+        }
+        float bt = 1.f, dbt = 0.f;
+        if (bond_on) {
+            // b~ = (1 + (beta zeta)^eta)^(-1/(2 eta)); b~' = b~ / (1 + pw) *
+            // (-1/2) (beta zeta)^(eta - 1) beta
+            const float be = betaf * zeta;
+            const float pw = be > 0.f ? __powf(be, etaf) : 0.f;
+            bt = __powf(1.0f + pw, inv_eta);
+            if (etaf == 0.f)
+                dbt = 0.f;  // Potentials.f90:5940-5941
+            else if (be > 0.f)
+                dbt = inv_eta * etaf * betaf * bt * pw *
+                      __builtin_amdgcn_rcpf((1.0f + pw) * be);
+            else if (etaf == 1.f)
+                dbt = inv_eta * betaf;  // (beta zeta)^0 = 1 at zeta = 0
+            // eta < 1 at zeta = 0: the reference's b~' = 0 (Potentials.f90:5946-5953)
+        }
+        const float2 pp = s_pair[jj][a];
+        cpair[q] = pp.x + pp.y * bt;   // (d/dr of 1/2 f (v_R + v_A b~) at fixed b~) / r
+        s_aux[jj][a].w = au.w * dbt;   // P = 1/2 f v_A db~/dzeta
+    }
+    __syncthreads();
+
+    // ---- 3. D_ij ------------------------------------------------------------
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int jj = l + q * G;
+        if (jj >= ns) continue;
+        const float4 u = s_u[jj][a];
+        const float4 aj = s_aux[jj][a];
+        const float ir1 = aj.x, fj = aj.y, dfj = aj.z, Pj = aj.w;
+        float cu = 0.f, dcx = 0.f, dcy = 0.f, dcz = 0.f;
+        if (bond_on) {
+            for (int k = 0; k < ns; ++k) {
+                if (k == jj) continue;
+                const float4 ak = s_aux[k][a];
+                const float fk = ak.y, Pk = ak.w;
+                if (fk == 0.f && fj == 0.f) continue;
+                const float4 v = s_u[k][a];
+                const float ir12 = ir1 * ak.x;
+                const float cs = (u.x * v.x + u.y * v.y + u.z * v.z) * ir12;
+                const float t = hf - cs;
+                const float iden = __builtin_amdgcn_rcpf(ddf + t * t);
+                const float gee = 1.0f + cd2f * t * t * iden;
+                const float gp = -2.0f * ccf * t * iden * iden;
+                float coef_u, coef_c;
+                if (B.a_zero) {
+                    coef_u = Pk * gee * dfj;
+                    coef_c = gp * (Pj * fk + Pk * fj);
+                } else {
+                    const float av = (float)B.a, mu = (float)B.mu;
+                    const float x1 = av * (u.w - v.w), x2 = -x1;
+                    const float p1 = powf(x1, mu - 1.0f), p2 = powf(x2, mu - 1.0f);
+                    const float e1 = __expf(x1 * p1), e2 = __expf(x2 * p2);
+                    const float d1 = e1 * av * mu * p1, d2 = e2 * av * mu * p2;
+                    coef_u = Pj * gee * fk * d1 + Pk * gee * (dfj * e2 - fj * d2);
+                    coef_c = gp * (Pj * fk * e1 + Pk * fj * e2);
+                }
+                const float cu2 = cs * ir1 * ir1;
+                dcx += coef_c * (v.x * ir12 - cu2 * u.x);
+                dcy += coef_c * (v.y * ir12 - cu2 * u.y);
+                dcz += coef_c * (v.z * ir12 - cu2 * u.z);
+                cu += coef_u * ir1;
+            }
+        }
+        const float c = cpair[q] + cu;
+        dU[(size_t)jj * n + i] = make_float4(c * u.x + dcx, c * u.y + dcy, c * u.z + dcz, 0.f);
+    }
+    if (valid && l == 0) scnt[i] = ns;
+}
+
 template <int G, int SCAP, class Geo, bool VIR>
 __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                                             const double4 *__restrict__ pos,
@@ -715,8 +934,12 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
     int t0, t1;
     tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
-        site1f_tile<G, SCAP, Geo, VIR>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU,
-                                       partial, atom_ev, flags);
+        if constexpr (VIR)
+            site1f_tile<G, SCAP, Geo, true>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt,
+                                            dU, partial, atom_ev, flags);
+        else
+            site1m_tile<G, SCAP, Geo>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU,
+                                      flags);
         __syncthreads();
     }
 }

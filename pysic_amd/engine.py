@@ -58,6 +58,38 @@ SIGNATURES = {
     "pysic_timing_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                          _lp]),
     "pysic_synchronize": (ctypes.c_int, [_vp]),
+    "pysic_calculate_bond_order_factors": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "pysic_calculate_bond_order_gradients": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                                            ctypes.c_int, _dp]),
+    "pysic_calculate_bond_order_gradients_of_factor": (ctypes.c_int, [_vp, ctypes.c_int,
+                                                                      ctypes.c_int, ctypes.c_int,
+                                                                      _dp]),
+    "pysic_get_ewald_energy": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _ip,
+                                              ctypes.c_double, ctypes.c_double, _dp]),
+    # descriptor registry (include/pysic_registry.h), host only
+    "pysic_number_of_potentials": (ctypes.c_int, []),
+    "pysic_number_of_bond_order_factors": (ctypes.c_int, []),
+    "pysic_potential_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "pysic_bond_order_factor_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "pysic_is_potential": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_is_bond_order_factor": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_potential_type_code": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_number_of_targets_of_potential": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_number_of_targets_of_bond_order_factor": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_level_of_bond_order_factor": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_number_of_parameters_of_potential": (ctypes.c_int, [ctypes.c_char_p]),
+    "pysic_number_of_parameters_of_bond_order_factor": (ctypes.c_int, [ctypes.c_char_p,
+                                                                       ctypes.c_int]),
+    "pysic_parameter_name_of_potential": (ctypes.c_char_p, [ctypes.c_char_p, ctypes.c_int]),
+    "pysic_parameter_name_of_bond_order_factor": (ctypes.c_char_p, [ctypes.c_char_p, ctypes.c_int,
+                                                                    ctypes.c_int]),
+    "pysic_parameter_description_of_potential": (ctypes.c_char_p, [ctypes.c_char_p,
+                                                                   ctypes.c_int]),
+    "pysic_parameter_description_of_bond_order_factor": (ctypes.c_char_p, [ctypes.c_char_p,
+                                                                           ctypes.c_int,
+                                                                           ctypes.c_int]),
+    "pysic_description_of_potential": (ctypes.c_char_p, [ctypes.c_char_p]),
+    "pysic_description_of_bond_order_factor": (ctypes.c_char_p, [ctypes.c_char_p]),
     # batches of cells (include/pysic_batch.h)
     "pysic_set_cells": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _ip]),
     "pysic_get_number_of_cells": (ctypes.c_int, [_vp, _ip]),
@@ -234,6 +266,32 @@ class Engine:
         self._check(self.L.pysic_get_neighbor_lists(self.h, nnz, _i(idx), _i(off)),
                     "get_neighbor_lists")
         return counts, idx[:nnz], off[:nnz]
+
+    def bond_order_factors(self, group):
+        """calculate_bond_order_factors (PyInterface.f90:579)."""
+        x = np.zeros(self.n)
+        self._check(self.L.pysic_calculate_bond_order_factors(self.h, self.n, int(group), _d(x)),
+                    "calculate_bond_order_factors")
+        return x
+
+    def bond_order_gradients(self, group, atom, of_factor=False):
+        """calculate_bond_order_gradients[_of_factor] (PyInterface.f90:605,
+        634): (N,3); atom 0-based."""
+        g = np.zeros((self.n, 3))
+        fn = (self.L.pysic_calculate_bond_order_gradients_of_factor if of_factor else
+              self.L.pysic_calculate_bond_order_gradients)
+        self._check(fn(self.h, self.n, int(group), int(atom), _d(g)),
+                    "calculate_bond_order_gradients")
+        return g
+
+    def ewald_energy(self, real_cut, k_cut, kcells, sigma, epsilon):
+        """get_ewald_energy (PyInterface.f90:1050)."""
+        k = np.ascontiguousarray(kcells, dtype=np.int32)
+        e = np.zeros(1)
+        self._check(self.L.pysic_get_ewald_energy(self.h, float(real_cut), float(k_cut), _i(k),
+                                                  float(sigma), float(epsilon), _d(e)),
+                    "get_ewald_energy")
+        return float(e[0])
 
     def set_neighbor_method(self, method):
         self._check(self.L.pysic_set_neighbor_method(self.h, int(method)), "set_neighbor_method")

@@ -217,6 +217,63 @@ def pair_cases():
     run_pairs("lj_argon_fcc_108_smooth", ar, lj_argon(margin=1.0))
 
 
+def boundary_cases():
+    """The rest of the f2py boundary pysic calls on this path, as the
+    reference core answers them: its descriptor registry
+    (PyInterface.f90:706-1011), the bond-order factor queries of a Tersoff
+    coordinator (579-634) and the stand-alone Ewald energy (1050)."""
+    R = O.RefCore()
+    reg = R.registry()
+    pots, bonds = reg["potentials"], reg["bond_order_factors"]
+
+    def pad(names):
+        return list(names) + [""] * (12 - len(names))
+    np.savez_compressed(
+        os.path.join(HERE, "registry_reference.npz"),
+        pot_names=np.array([p[0] for p in pots]), pot_targets=np.array([p[1] for p in pots]),
+        pot_nparams=np.array([len(p[2]) for p in pots]),
+        pot_param_names=np.array([pad(p[2]) for p in pots]),
+        bond_names=np.array([b[0] for b in bonds]), bond_targets=np.array([b[1] for b in bonds]),
+        bond_levels=np.array([b[2] for b in bonds]),
+        bond_nparams=np.array([[len(b[3][0]), len(b[3][1])] for b in bonds]),
+        bond_param_names=np.array([[pad(b[3][0]), pad(b[3][1])] for b in bonds]))
+    print("registry_reference", len(pots), "potentials", len(bonds), "bond-order factors")
+    # bond-order queries of the Tersoff group on a small displaced diamond
+    s = st.diamond_si((2, 2, 2), sigma=0.05, seed=1)
+    p_t, b_t, cuts = tersoff_setup(s)
+    R.setup(s.symbols, s.positions, s.cell, s.pbc, potentials=p_t, tersoff=b_t, ngroups=1,
+            cutoffs=cuts)
+    atom = 3
+    np.savez_compressed(os.path.join(HERE, "tersoff_bond_order_queries.npz"),
+                        positions=s.positions, cell=s.cell, cutoffs=cuts, group=0, atom=atom,
+                        factors=R.bond_order_factors(0),
+                        gradients=R.bond_order_gradients(0, atom),
+                        gradients_of_factor=R.bond_order_gradients_of_factor(0, atom))
+    print("tersoff_bond_order_queries", len(s))
+    # get_ewald_energy on the sheared rocksalt cell: two parameter sets,
+    # neither the one set with set_ewald_parameters (scalers there are not 1)
+    r = st.rocksalt((2, 2, 2), sigma=0.08, seed=6)
+    shear = np.array([[1.0, 0.0, 0.0], [0.2, 1.0, 0.0], [-0.1, 0.15, 1.0]])
+    r.cell = r.cell @ shear
+    r.positions = r.positions @ shear
+    eps = 0.00552635
+    rc = 5.0
+    sc = np.random.default_rng(12).uniform(0.5, 1.5, len(r))
+    R.setup(r.symbols, r.positions, r.cell, r.pbc, charges=r.charges, potentials=[],
+            tersoff=[], ngroups=0, cutoffs=np.full(len(r), rc + 0.5),
+            ewald=(rc, 1.8, kcells(r.cell, 1.8), rc / 3.0, eps), scaler=sc)
+    params, energies = [], []
+    for sigma in (rc / 3.0, 1.4):
+        kc = 3.0 / sigma
+        K = kcells(r.cell, kc)
+        params.append([rc, kc, K[0], K[1], K[2], sigma, eps])
+        energies.append(R.ewald_energy(rc, kc, K, sigma, eps))
+    np.savez_compressed(os.path.join(HERE, "ewald_energy_reference.npz"), positions=r.positions,
+                        cell=r.cell, charges=r.charges, symbols=np.array(r.symbols), scaler=sc,
+                        params=np.array(params), energies=np.array(energies))
+    print("ewald_energy_reference", len(r), energies)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference checker first: oracle/build_ref.sh")
@@ -258,6 +315,7 @@ def main():
     run_ewald("sio2_48_eem_rc5", sio2, rc=5.0,
               charge_self={"Si": (4.77, 8.0), "O": (8.74, 13.4)})
     pair_cases()
+    boundary_cases()
 
 
 if __name__ == "__main__":
@@ -273,5 +331,7 @@ if __name__ == "__main__":
         comb_bend()
     elif sys.argv[1:] == ["scaled_ewald"]:
         scaled_ewald()
+    elif sys.argv[1:] == ["boundary"]:
+        boundary_cases()
     else:
         main()

@@ -10,7 +10,7 @@ import pytest
 from conftest import ROOT
 
 HEADERS = [os.path.join(ROOT, "include", h)
-           for h in ("pysic_hip.h", "pysic_learned.h", "pysic_batch.h")]
+           for h in ("pysic_hip.h", "pysic_learned.h", "pysic_batch.h", "pysic_registry.h")]
 LIB = os.path.join(ROOT, "pysic_amd", "libpysic_hip.so")
 
 
