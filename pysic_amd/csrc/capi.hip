@@ -117,6 +117,7 @@ void System::release_all()
                           &gtotals, &atom_ev, &cell_ev};
     for (auto *b : bd) b->release();
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
+                       &prevstart, &dirty, &scanloc,
                        &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0,
                        &bsj, &bsimg, &bscnt};
     for (auto *b : bi) b->release();
@@ -127,6 +128,8 @@ void System::release_all()
     dpmodel.release();
     tables.release();
     cells.release();
+    order_valid = false;
+    bins_clean = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -524,11 +527,20 @@ void require_ready(pysic_ctx *h)
     if (h->S.model_dirty) compile_model(h);
 }
 
+// flags[2]: bit 0 an atom lies more than 255 cells from its box; bit 1 the
+// cell sort found an inconsistent permutation (its guards kept every access
+// in bounds; the state is not to be trusted)
+void require_range_flag(int f2)
+{
+    PYSIC_REQUIRE(!(f2 & 2), PYSIC_ESTATE, "internal error: inconsistent cell-list sort");
+    PYSIC_REQUIRE(f2 == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from its box");
+}
+
 void check_flags(pysic_ctx *h, bool lists_only)
 {
     int f[4];
     nb_read_flags(h->S, f);
-    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+    require_range_flag(f[2]);
     if (!lists_only) {
         PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
                       "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
@@ -546,7 +558,7 @@ void build_lists(pysic_ctx *h)
         nb_build_lists(S);
         int f[4];
         nb_read_flags(S, f);
-        PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+        require_range_flag(f[2]);
         if (f[0] <= S.cap) {
             h->lists_valid = true;
             return;
@@ -678,7 +690,8 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
 {
     System &S = h->S;
     return {S.pos.p, S.pos2.p, S.mom.p, S.q.p, S.cut.p, S.id.p, S.key.p, S.wrap.p, S.perm.p,
-            S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.nbr.p, S.nimg.p,
+            S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.prevstart.p,
+            S.dirty.p, S.scanloc.p, (const void *)(long)(S.order_valid ? 1 : 0), S.nbr.p, S.nimg.p,
             S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
             S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p, S.bsj.p,
@@ -688,12 +701,7 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
             (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};
 }
 
-void swap_state(System &S)
-{
-    std::swap(S.pos, S.pos2); std::swap(S.mom, S.mom2); std::swap(S.q, S.q2);
-    std::swap(S.cut, S.cut2); std::swap(S.id, S.id2); std::swap(S.key, S.key2);
-    std::swap(S.wrap, S.wrap2);
-}
+void swap_state(System &S) { nb_swap_state(S); }
 
 // Capture batch_sequence for the current buffer state into slot k.  Nothing
 // runs; the host-side buffer swap that the sort performs during capture is
@@ -909,6 +917,7 @@ int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *
     h->masses.assign(n, 1.0);
     if (masses) h->masses.assign(masses, masses + n);
     S.ncells = 0;
+    S.order_valid = false;  // atoms in original order: no previous sort to follow
     drop_graphs(h);
     S.pos.ensure(n); S.mom.ensure(n); S.frc.ensure(n); S.q.ensure(n); S.cut.ensure(n); S.id.ensure(n);
     k_iota<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p);
@@ -1215,7 +1224,7 @@ int pysic_calculate_energy_noalloc(pysic_ctx *h, double *energy)
     HIP_TRY(hipMemcpyAsync(&e, totals, sizeof(double), hipMemcpyDeviceToHost, S.stream));
     HIP_TRY(hipMemcpyAsync(f, S.flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, S.stream));
     HIP_TRY(hipStreamSynchronize(S.stream));
-    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from the box");
+    require_range_flag(f[2]);
     PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
                   "calculate_energy_noalloc: " + std::to_string(f[1]) +
                       " short-range neighbors exceed kernel variant " +
@@ -1429,6 +1438,7 @@ int pysic_set_cells(pysic_ctx *h, int ncells, const int *natoms, const double *v
                            S.stream));
     HIP_TRY(hipStreamSynchronize(S.stream));
     S.ncells = ncells;
+    S.order_valid = false;
     drop_graphs(h);
     h->cell_ready = true;
     h->lists_valid = false;
@@ -1490,7 +1500,7 @@ int pysic_get_cell_results(pysic_ctx *h, int ncells, double *energies, double *v
                       std::to_string(S.cap) + "); raise the capacity and evaluate again");
     PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
                   "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
-    PYSIC_REQUIRE(f[2] == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from its box");
+    require_range_flag(f[2]);
     std::vector<double> ev((size_t)8 * ncells);
     HIP_TRY(hipMemcpyAsync(ev.data(), S.cell_ev.p, sizeof(double) * ev.size(),
                            hipMemcpyDeviceToHost, S.stream));

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #pragma clang fp contract(off)
 
@@ -67,9 +68,30 @@ __device__ __forceinline__ void bin_atom(const CellGeom &g, const double4 &p, in
     *bad_out = bad;
 }
 
+// Records the new bin k of the atom in slot i.  With track (the slot order
+// is the last sort's, so key[i] still holds the bin the atom was sorted
+// into), a change of bin marks both bins dirty: the sort then re-sorts only
+// the dirty bins and moves the atoms of the others as blocks.
+__device__ __forceinline__ void record_bin(int i, int k, int w, int nbins, int track,
+                                           int *__restrict__ key, int *__restrict__ wrap,
+                                           int *__restrict__ cellcount, int *__restrict__ dirty)
+{
+    if (track) {
+        const int old = key[i];
+        if (old != k) {
+            dirty[k] = 1;
+            if ((unsigned)old < (unsigned)nbins) dirty[old] = 1;
+        }
+    }
+    key[i] = k;
+    wrap[i] = w;
+    atomicAdd(cellcount + k, 1);
+}
+
 template <class Geo>
 __global__ void k_bin(int n, Geo geo, const double4 *__restrict__ pos, int *__restrict__ key,
-                      int *__restrict__ wrap, int *__restrict__ cellcount, int *__restrict__ flags)
+                      int *__restrict__ wrap, int *__restrict__ cellcount, int *__restrict__ flags,
+                      int nbins, int track, int *__restrict__ dirty)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -78,9 +100,7 @@ __global__ void k_bin(int n, Geo geo, const double4 *__restrict__ pos, int *__re
     bin_atom(geo.of(i), pos[i], &k, &w, &bad);
     k += geo.bin0(i);
     if (bad) atomicOr(flags + 2, 1);
-    key[i] = k;
-    wrap[i] = w;
-    atomicAdd(cellcount + k, 1);
+    record_bin(i, k, w, nbins, track, key, wrap, cellcount, dirty);
 }
 
 // velocity-Verlet first half (p += dt/2 F; r += dt p/m) fused with binning
@@ -97,7 +117,8 @@ __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
                                    const int *__restrict__ scnt, const float4 *__restrict__ dU,
                                    double4 *__restrict__ mom, double4 *__restrict__ pos,
                                    int *__restrict__ key, int *__restrict__ wrap,
-                                   int *__restrict__ cellcount, int *__restrict__ flags)
+                                   int *__restrict__ cellcount, int *__restrict__ flags,
+                                   int nbins, int track, int *__restrict__ dirty)
 {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -123,9 +144,7 @@ __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
     bin_atom(geo.of(i), r, &k, &w, &bad);
     k += geo.bin0(i);
     if (bad) atomicOr(flags + 2, 1);
-    key[i] = k;
-    wrap[i] = w;
-    atomicAdd(cellcount + k, 1);
+    record_bin(i, k, w, nbins, track, key, wrap, cellcount, dirty);
 }
 
 template <class Geo>
@@ -133,7 +152,8 @@ __global__ void k_drift_bin(int n, Geo geo, double dt, double half_dt,
                             const double4 *__restrict__ frc, double4 *__restrict__ mom,
                             double4 *__restrict__ pos, int *__restrict__ key,
                             int *__restrict__ wrap, int *__restrict__ cellcount,
-                            int *__restrict__ flags)
+                            int *__restrict__ flags, int nbins, int track,
+                            int *__restrict__ dirty)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -154,12 +174,11 @@ __global__ void k_drift_bin(int n, Geo geo, double dt, double half_dt,
     bin_atom(geo.of(i), r, &k, &w, &bad);
     k += geo.bin0(i);
     if (bad) atomicOr(flags + 2, 1);
-    key[i] = k;
-    wrap[i] = w;
-    atomicAdd(cellcount + k, 1);
+    record_bin(i, k, w, nbins, track, key, wrap, cellcount, dirty);
 }
 
-// --- exclusive scan of the cell counts (3 phases, 1024 elements per block) --
+// --- exclusive scan of the cell counts (1024 elements per block, then the
+// block totals; a bin's start is loc[k] + blk[k / SCAN_BLK]) ---------------
 constexpr int SCAN_T = 256;
 constexpr int SCAN_PER = 4;
 constexpr int SCAN_BLK = SCAN_T * SCAN_PER;
@@ -181,14 +200,20 @@ __device__ int block_exclusive_scan(int v, int *lds, int *total)
     return incl - v;
 }
 
-__global__ void k_scan_local(int m, const int *__restrict__ in, int *__restrict__ out,
-                             int *__restrict__ blk)
+// Also zeroes the counts it reads and the scatter's fill counters, so the
+// next binning and this sort's scatter start from zero without a fill pass.
+__global__ void k_scan_local(int m, int *__restrict__ in, int *__restrict__ out,
+                             int *__restrict__ blk, int *__restrict__ fill)
 {
     __shared__ int lds[SCAN_T];
     int base = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
     int v[SCAN_PER], s = 0;
     for (int q = 0; q < SCAN_PER; ++q) {
         v[q] = base + q < m ? in[base + q] : 0;
+        if (base + q < m) {
+            in[base + q] = 0;
+            fill[base + q] = 0;
+        }
         s += v[q];
     }
     int tot;
@@ -216,12 +241,10 @@ __global__ void k_scan_blocks(int nb, int *__restrict__ blk, int *__restrict__ o
     if (threadIdx.x == 0) out[m] = carry;
 }
 
-__global__ void k_scan_add(int m, int *__restrict__ out, const int *__restrict__ blk)
+__device__ __forceinline__ int bin_start(int k, const int *__restrict__ loc,
+                                         const int *__restrict__ blk)
 {
-    int base = blockIdx.x * SCAN_BLK;
-    int add = blk[blockIdx.x];
-    for (int q = threadIdx.x; q < SCAN_BLK; q += SCAN_T)
-        if (base + q < m) out[base + q] += add;
+    return loc[k] + blk[k / SCAN_BLK];
 }
 
 // zero two bin arrays of length m (a kernel, so that graphs hold no memset)
@@ -237,24 +260,42 @@ __global__ void k_clear_flags(int *__restrict__ flags)
     if (threadIdx.x < 4 && blockIdx.x == 0) flags[threadIdx.x] = 0;
 }
 
-__global__ void k_scatter(int n, const int *__restrict__ key, const int *__restrict__ start,
-                          int *__restrict__ fill, int *__restrict__ perm)
+// New slot of the atom in old slot i.  A bin no atom entered or left keeps
+// its atoms in the same order (sorted by original index), so they move as a
+// block: new start + (i - old start).  The atoms of a dirty bin claim slots
+// with atomics and k_cellsort orders them.  all_dirty: every bin is dirty
+// (the old order is not a previous sort's).
+__global__ void k_scatter(int n, const int *__restrict__ key, const int *__restrict__ loc,
+                          const int *__restrict__ blk, const int *__restrict__ prevstart,
+                          const int *__restrict__ dirty, int all_dirty, int *__restrict__ fill,
+                          int *__restrict__ perm, int *__restrict__ flags)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int k = key[i];
-    int slot = start[k] + atomicAdd(fill + k, 1);
-    perm[slot] = i;
+    int start = bin_start(k, loc, blk);
+    int slot = (all_dirty || dirty[k]) ? start + atomicAdd(fill + k, 1)
+                                       : start + (i - prevstart[k]);
+    if ((unsigned)slot < (unsigned)n) perm[slot] = i;
+    else atomicOr(flags + 2, 2);   // an inconsistent sort: reported, never written
 }
 
-// Within each cell, order atoms by original index: the atomic slot claims
-// above are unordered, this makes the whole permutation deterministic.
-__global__ void k_cellsort(int ncell, const int *__restrict__ start, int *__restrict__ perm,
-                           const int *__restrict__ id)
+// Within each dirty cell, order atoms by original index: the atomic slot
+// claims above are unordered, this makes the whole permutation
+// deterministic (the same as sorting every cell).  Writes the final cell
+// starts and clears the dirty flags for the next sort.
+__global__ void k_cellsort(int ncell, int n, const int *__restrict__ loc,
+                           const int *__restrict__ blk, int *__restrict__ perm,
+                           const int *__restrict__ id, int *__restrict__ dirty, int all_dirty,
+                           int *__restrict__ start)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    int a = start[c], b = start[c + 1];
+    int a = bin_start(c, loc, blk), b = c + 1 < ncell ? bin_start(c + 1, loc, blk) : n;
+    start[c] = a;
+    if (c == ncell - 1) start[ncell] = n;
+    if (!(all_dirty || dirty[c])) return;
+    dirty[c] = 0;
     for (int x = a + 1; x < b; ++x) {
         int v = perm[x];
         int kv = id[v];
@@ -276,12 +317,16 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
                           double4 *__restrict__ pos2, double4 *__restrict__ mom2,
                           double *__restrict__ q2, double *__restrict__ cut2,
                           int *__restrict__ id2, int *__restrict__ key2, int *__restrict__ wrap2,
-                          Geo geo, float4 *__restrict__ wpos)
+                          Geo geo, float4 *__restrict__ wpos, int *__restrict__ flags)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const CellGeom &g = geo.of(s);   // slot s stays in the cell of slot perm[s]
     int o = perm[s];
+    if ((unsigned)o >= (unsigned)n) {
+        atomicOr(flags + 2, 2);   // an inconsistent sort (reported): keep the slot
+        o = s;
+    }
     double4 p = pos[o];
     int w = wrap[o];
     pos2[s] = p;
@@ -888,10 +933,15 @@ void nb_configure_grid(System &S, double max_cut)
         for (int i = 0; i < 3; ++i) g.dim[i] = g.pbc[i] ? g.nc[i] : g.nc[i] + 2;
         g.ncell = g.dim[0] * g.dim[1] * g.dim[2];
     };
+    const int old_nbins = S.nbins;
+    const CellGeom old_geom = S.geom;
     if (S.ncells == 0) {
         configure(S.geom, S.n);
         S.nbins = S.geom.ncell;
+        if (S.nbins != old_nbins || std::memcmp(S.geom.dim, old_geom.dim, sizeof(int) * 3) != 0)
+            S.order_valid = false;
     } else {
+        S.order_valid = false;
         std::vector<int> fb(S.ncells);
         long total = 0;
         for (int c = 0; c < S.ncells; ++c) {
@@ -908,9 +958,12 @@ void nb_configure_grid(System &S, double max_cut)
         HIP_TRY(hipMemcpy(S.first_bin.p, fb.data(), sizeof(int) * S.ncells, hipMemcpyHostToDevice));
     }
     S.max_cut = max_cut;
+    if (!S.order_valid) S.bins_clean = false;  // bins changed: counts and flags refilled
 }
 
-static void swap_bufs(System &S)
+// The buffers a sort alternates: the per-atom arrays and the cell starts
+// (the last sort's starts are the next sort's old ones).
+void nb_swap_state(System &S)
 {
     std::swap(S.pos, S.pos2);
     std::swap(S.mom, S.mom2);
@@ -919,6 +972,7 @@ static void swap_bufs(System &S)
     std::swap(S.id, S.id2);
     std::swap(S.key, S.key2);
     std::swap(S.wrap, S.wrap2);
+    std::swap(S.cellstart, S.prevstart);
 }
 
 // The counting sort after binning: scan of the bin counts, scatter, per-bin
@@ -929,19 +983,27 @@ static void sort_binned(System &S)
     const int nbins = S.nbins;
     hipStream_t st = S.stream;
     const int nb = div_up(nbins, SCAN_BLK);
-    k_scan_local<<<nb, SCAN_T, 0, st>>>(nbins, S.cellcount.p, S.cellstart.p, S.scanblk.p);
-    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.cellstart.p, nbins);
-    k_scan_add<<<nb, SCAN_T, 0, st>>>(nbins, S.cellstart.p, S.scanblk.p);
-    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.cellstart.p, S.cellfill.p, S.perm.p);
-    k_cellsort<<<div_up(nbins, 256), 256, 0, st>>>(nbins, S.cellstart.p, S.perm.p, S.id.p);
+    const int all_dirty = S.order_valid ? 0 : 1;
+    k_scan_local<<<nb, SCAN_T, 0, st>>>(nbins, S.cellcount.p, S.scanloc.p, S.scanblk.p,
+                                        S.cellfill.p);
+    k_scan_blocks<<<1, SCAN_T, 0, st>>>(nb, S.scanblk.p, S.scanloc.p, nbins);
+    // S.cellstart holds the last sort's starts here (the old slot ranges)
+    k_scatter<<<div_up(n, 256), 256, 0, st>>>(n, S.key.p, S.scanloc.p, S.scanblk.p,
+                                              S.cellstart.p, S.dirty.p, all_dirty, S.cellfill.p,
+                                              S.perm.p, S.flags.p);
+    k_cellsort<<<div_up(nbins, 256), 256, 0, st>>>(nbins, n, S.scanloc.p, S.scanblk.p, S.perm.p,
+                                                   S.id.p, S.dirty.p, all_dirty, S.prevstart.p);
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         k_permute<Geo><<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p,
                                                        S.cut.p, S.id.p, S.key.p, S.wrap.p,
                                                        S.pos2.p, S.mom2.p, S.q2.p, S.cut2.p,
-                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p);
+                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p,
+                                                       S.flags.p);
     });
     HIP_TRY(hipGetLastError());
+    // the new starts went to prevstart, which nb_swap_state makes cellstart
+    S.order_valid = true;
 }
 
 static void ensure_sort_buffers(System &S)
@@ -951,14 +1013,25 @@ static void ensure_sort_buffers(System &S)
     S.key.ensure(n); S.key2.ensure(n);
     S.wrap.ensure(n); S.wrap2.ensure(n);
     S.perm.ensure(n);
+    const bool fresh = S.cellcount.cap < (size_t)nbins || S.cellfill.cap < (size_t)nbins ||
+                       S.dirty.cap < (size_t)nbins || !S.cellcount.p || !S.dirty.p;
     S.cellcount.ensure(nbins);
     S.cellfill.ensure(nbins);
+    S.dirty.ensure(nbins);
     S.cellstart.ensure((size_t)nbins + 1);
-    S.scanblk.ensure(div_up(nbins, SCAN_BLK));
+    S.prevstart.ensure((size_t)nbins + 1);
+    S.scanloc.ensure((size_t)nbins + 1);
+    S.scanblk.ensure(div_up(nbins, SCAN_BLK) + 1);
     S.pos2.ensure(n); S.mom2.ensure(n); S.q2.ensure(n); S.cut2.ensure(n);
     S.id2.ensure(n);
     S.wpos.ensure(n);
     S.flags.ensure(4);
+    if (fresh || !S.bins_clean) {
+        // counts and dirty flags start at zero (a kernel: this may be captured)
+        k_fill_int<<<div_up(2 * nbins, 256), 256, 0, S.stream>>>(nbins, S.cellcount.p, S.dirty.p);
+        S.order_valid = false;
+        S.bins_clean = true;
+    }
 }
 
 void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
@@ -970,22 +1043,22 @@ void nb_sort_atoms(System &S, bool drift, double dt, double half_dt)
 
     hipEvent_t ev;
     S.timer.begin(drift ? "drift_bin_sort" : "bin_sort", st, &ev);
-    // fill kernels rather than memset nodes: this sequence may be captured
-    // into a graph
-    k_fill_int<<<div_up(2 * nbins, 256), 256, 0, st>>>(nbins, S.cellcount.p, S.cellfill.p);
+    const int track = S.order_valid ? 1 : 0;
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         if (drift)
             k_drift_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(n, geo, dt, half_dt, S.frc.p, S.mom.p,
                                                              S.pos.p, S.key.p, S.wrap.p,
-                                                             S.cellcount.p, S.flags.p);
+                                                             S.cellcount.p, S.flags.p, nbins,
+                                                             track, S.dirty.p);
         else
             k_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(n, geo, S.pos.p, S.key.p, S.wrap.p,
-                                                       S.cellcount.p, S.flags.p);
+                                                       S.cellcount.p, S.flags.p, nbins, track,
+                                                       S.dirty.p);
     });
     sort_binned(S);
     S.timer.end(drift ? "drift_bin_sort" : "bin_sort", st, ev);
-    swap_bufs(S);
+    nb_swap_state(S);
 }
 
 void nb_gather_drift_sort(System &S, double dt, double half_dt)
@@ -998,16 +1071,16 @@ void nb_gather_drift_sort(System &S, double dt, double half_dt)
     ensure_sort_buffers(S);
     hipEvent_t ev;
     S.timer.begin("gather_drift_bin_sort", st, &ev);
-    k_fill_int<<<div_up(2 * nbins, 256), 256, 0, st>>>(nbins, S.cellcount.p, S.cellfill.p);
+    const int track = S.order_valid ? 1 : 0;
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         k_gather_drift_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(
             n, geo, dt, half_dt, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.mom.p, S.pos.p, S.key.p,
-            S.wrap.p, S.cellcount.p, S.flags.p);
+            S.wrap.p, S.cellcount.p, S.flags.p, nbins, track, S.dirty.p);
     });
     sort_binned(S);
     S.timer.end("gather_drift_bin_sort", st, ev);
-    swap_bufs(S);
+    nb_swap_state(S);
 }
 
 void nb_clear_flags(System &S)

@@ -62,6 +62,11 @@ struct System {
     DBuf<int> perm;             // new slot -> old slot
     DBuf<float4> wpos;          // wrapped positions, FP32 (neighbor prefilter)
     DBuf<int> cellcount, cellstart, cellfill, scanblk;
+    DBuf<int> prevstart;        // the previous sort's cell starts (its slot ranges)
+    DBuf<int> dirty;            // per bin: an atom entered or left it since the last sort
+    DBuf<int> scanloc;          // cell starts within blocks of 1024 bins
+    bool order_valid = false;   // the slot order is the last sort's (key[] holds its bins)
+    bool bins_clean = false;    // cellcount and dirty are all zero between sorts
     double max_cut = 0.0;
 
     // neighbor lists (column-major: entry e of atom i at e*n + i)
@@ -136,6 +141,7 @@ void nb_sort_atoms(System &S, bool drift = false, double dt = 0.0,
 // (the forces are not stored)
 void nb_gather_drift_sort(System &S, double dt, double half_dt);
 void nb_build_lists(System &S);               // cell-list search, reference semantics
+void nb_swap_state(System &S);                // the buffers a sort alternates
 int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
 void nb_clear_flags(System &S);               // zero the flags (kernel; graph-safe)
 
