@@ -117,7 +117,7 @@ void System::release_all()
                           &gtotals, &atom_ev, &cell_ev};
     for (auto *b : bd) b->release();
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
-                       &prevstart, &dirty, &scanloc,
+                       &prevstart, &dirty, &scanloc, &srow,
                        &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0,
                        &bsj, &bsimg, &bscnt};
     for (auto *b : bi) b->release();
@@ -692,7 +692,9 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
     return {S.pos.p, S.pos2.p, S.mom.p, S.q.p, S.cut.p, S.id.p, S.key.p, S.wrap.p, S.perm.p,
             S.wpos.p, S.cellcount.p, S.cellfill.p, S.cellstart.p, S.scanblk.p, S.prevstart.p,
             S.dirty.p, S.scanloc.p, (const void *)(long)(S.order_valid ? 1 : 0), S.nbr.p, S.nimg.p,
-            S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.partial.p, S.atom_ev.p, S.cell_ev.p,
+            S.ncnt.p, S.sj.p, S.simg.p, S.scnt.p, S.srow.p,
+            (const void *)(long)(S.unique_images ? 1 : 0), S.dU.p, S.partial.p, S.atom_ev.p,
+            S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
             S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p, S.bsj.p,
             S.bsimg.p, S.bscnt.p, S.bG.p,

@@ -231,6 +231,76 @@ __device__ __forceinline__ double3 gather_bond_force(int i, int n, const int *__
     return make_double3(fx, fy, fz);
 }
 
+// The same sum when every neighbor appears once in a list (no atom reaches
+// two images of another within the list radius): each atom's short list
+// also sits in a row srow[i * SCAP + t] (-1 past its end), so D_ji is found
+// by comparing j's row against i in registers.  Four bonds go at a time,
+// each stage's loads issued together -- the neighbors' rows, then the own
+// and reverse gradients -- so a wave waits on three memory latencies per
+// four bonds rather than several per bond.  The bonds are
+// summed in the same order as gather_bond_force, so the result is the same.
+template <int SCAP>
+__device__ __forceinline__ double3 gather_bond_force_rows(int i, int n,
+                                                          const int *__restrict__ srow,
+                                                          const float4 *__restrict__ dU,
+                                                          bool &miss)
+{
+    constexpr int B = 4;   // bonds per batch (a crystal atom has four)
+    const int *row = srow + (size_t)i * SCAP;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll 1
+    for (int c0 = 0; c0 < SCAP; c0 += B) {
+        const int4 jv = *reinterpret_cast<const int4 *>(row + c0);
+        const int j[B] = {jv.x, jv.y, jv.z, jv.w};
+        if (!__any(j[0] >= 0)) break;   // rows are -1 past their end
+        int tr[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) tr[t] = -1;
+#pragma unroll(SCAP == 8 ? 2 : 1)
+        for (int s0 = 0; s0 < SCAP; s0 += 4) {
+            int4 kv[B];
+#pragma unroll
+            for (int t = 0; t < B; ++t)
+                kv[t] = j[t] >= 0 ? *reinterpret_cast<const int4 *>(srow + (size_t)j[t] * SCAP + s0)
+                                  : make_int4(-1, -1, -1, -1);
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                tr[t] = kv[t].x == i ? s0 : tr[t];
+                tr[t] = kv[t].y == i ? s0 + 1 : tr[t];
+                tr[t] = kv[t].z == i ? s0 + 2 : tr[t];
+                tr[t] = kv[t].w == i ? s0 + 3 : tr[t];
+            }
+        }
+        float4 own[B], rev[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            own[t] = j[t] >= 0 ? dU[(size_t)(c0 + t) * n + i] : make_float4(0, 0, 0, 0);
+            rev[t] = (j[t] >= 0 && tr[t] >= 0) ? dU[(size_t)tr[t] * n + j[t]]
+                                               : make_float4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            if (j[t] < 0) continue;
+            miss |= tr[t] < 0;
+            fx += (double)own[t].x - (double)rev[t].x;
+            fy += (double)own[t].y - (double)rev[t].y;
+            fz += (double)own[t].z - (double)rev[t].z;
+        }
+    }
+    return make_double3(fx, fy, fz);
+}
+
+// The gather for the short-list capacity of the site kernel that wrote the
+// rows (8, 16 or 32).
+__device__ __forceinline__ double3 gather_rows_any(int i, int n, int scap,
+                                                   const int *__restrict__ srow,
+                                                   const float4 *__restrict__ dU, bool &miss)
+{
+    if (scap == 8) return gather_bond_force_rows<8>(i, n, srow, dU, miss);
+    if (scap == 16) return gather_bond_force_rows<16>(i, n, srow, dU, miss);
+    return gather_bond_force_rows<32>(i, n, srow, dU, miss);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Remap so that XCD x runs one contiguous range of logical blocks:
 // spatially adjacent atoms (the cell-sorted order) then share an L2.

@@ -114,7 +114,8 @@ __global__ void k_bin(int n, Geo geo, const double4 *__restrict__ pos, int *__re
 template <class Geo>
 __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
                                    const int *__restrict__ sj, const int *__restrict__ simg,
-                                   const int *__restrict__ scnt, const float4 *__restrict__ dU,
+                                   const int *__restrict__ scnt, const int *__restrict__ srow,
+                                   int scap, int unique, const float4 *__restrict__ dU,
                                    double4 *__restrict__ mom, double4 *__restrict__ pos,
                                    int *__restrict__ key, int *__restrict__ wrap,
                                    int *__restrict__ cellcount, int *__restrict__ flags,
@@ -122,10 +123,12 @@ __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
 {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    double4 p = mom[i];   // loaded first: in flight during the gather
+    double4 r = pos[i];
     bool miss = false;
-    const double3 f = gather_bond_force(i, n, sj, simg, scnt, dU, miss);
+    const double3 f = unique ? gather_rows_any(i, n, scap, srow, dU, miss)
+                             : gather_bond_force(i, n, sj, simg, scnt, dU, miss);
     if (miss) atomicOr(flags + 3, 1);
-    double4 p = mom[i];
     p.x = fma(half_dt, f.x, p.x);
     p.y = fma(half_dt, f.y, p.y);
     p.z = fma(half_dt, f.z, p.z);
@@ -133,7 +136,6 @@ __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
     p.y += half_dt * f.y;
     p.z += half_dt * f.z;
     mom[i] = p;
-    double4 r = pos[i];
     double s = dt / p.w;  // p.w carries the mass
     r.x += s * p.x;
     r.y += s * p.y;
@@ -959,6 +961,27 @@ void nb_configure_grid(System &S, double max_cut)
     }
     S.max_cut = max_cut;
     if (!S.order_valid) S.bins_clean = false;  // bins changed: counts and flags refilled
+    // every periodic perpendicular width above twice the list radius: a
+    // lattice vector is at least as long as the smallest such width, so no
+    // atom reaches two images of another within the radius and each
+    // neighbor appears once per list (the gather then matches rows by index)
+    auto unique_in = [&](const CellGeom &g) {
+        for (int a = 0; a < 3; ++a) {
+            if (!g.pbc[a]) continue;
+            const double *u = g.v + 3 * ((a + 1) % 3), *w = g.v + 3 * ((a + 2) % 3),
+                         *v = g.v + 3 * a;
+            double nx = u[1] * w[2] - u[2] * w[1], ny = u[2] * w[0] - u[0] * w[2],
+                   nz = u[0] * w[1] - u[1] * w[0];
+            double width = std::fabs(v[0] * nx + v[1] * ny + v[2] * nz) /
+                           std::sqrt(nx * nx + ny * ny + nz * nz);
+            if (!(width > 2.0 * max_cut)) return false;
+        }
+        return true;
+    };
+    bool uq = true;
+    if (S.ncells == 0) uq = unique_in(S.geom);
+    for (int c = 0; c < S.ncells; ++c) uq = uq && unique_in(S.hcells[c]);
+    S.unique_images = uq;
 }
 
 // The buffers a sort alternates: the per-atom arrays and the cell starts
@@ -1066,7 +1089,7 @@ void nb_gather_drift_sort(System &S, double dt, double half_dt)
     const int n = S.n;
     const int nbins = S.nbins;
     hipStream_t st = S.stream;
-    PYSIC_REQUIRE(S.sj.p && S.simg.p && S.scnt.p && S.dU.p, PYSIC_ESTATE,
+    PYSIC_REQUIRE(S.sj.p && S.simg.p && S.scnt.p && S.srow.p && S.dU.p, PYSIC_ESTATE,
                   "no per-bond gradients to gather");
     ensure_sort_buffers(S);
     hipEvent_t ev;
@@ -1075,8 +1098,9 @@ void nb_gather_drift_sort(System &S, double dt, double half_dt)
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         k_gather_drift_bin<Geo><<<div_up(n, 256), 256, 0, st>>>(
-            n, geo, dt, half_dt, S.sj.p, S.simg.p, S.scnt.p, S.dU.p, S.mom.p, S.pos.p, S.key.p,
-            S.wrap.p, S.cellcount.p, S.flags.p, nbins, track, S.dirty.p);
+            n, geo, dt, half_dt, S.sj.p, S.simg.p, S.scnt.p, S.srow.p, S.scap,
+            S.unique_images ? 1 : 0, S.dU.p, S.mom.p, S.pos.p, S.key.p, S.wrap.p, S.cellcount.p,
+            S.flags.p, nbins, track, S.dirty.p);
     });
     sort_binned(S);
     S.timer.end("gather_drift_bin_sort", st, ev);

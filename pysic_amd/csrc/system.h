@@ -66,6 +66,7 @@ struct System {
     DBuf<int> dirty;            // per bin: an atom entered or left it since the last sort
     DBuf<int> scanloc;          // cell starts within blocks of 1024 bins
     bool order_valid = false;   // the slot order is the last sort's (key[] holds its bins)
+    bool unique_images = false; // no atom reaches two images of another within the list radius
     bool bins_clean = false;    // cellcount and dirty are all zero between sorts
     double max_cut = 0.0;
 
@@ -79,6 +80,7 @@ struct System {
     // Tersoff short lists and per-bond gradients
     int scap = 0;
     DBuf<int> sj, simg, scnt;
+    DBuf<int> srow;             // short lists as rows: srow[i * scap + t], -1 past the end
     DBuf<float4> dU;            // dU_i / dr_ij per short-list entry
     DBuf<double> partial;       // per-block energy + virial partials
     int partial_blocks = 0;

@@ -94,7 +94,8 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
                                           const int *__restrict__ ncnt,
                                           const Model *__restrict__ M,
                                           int *__restrict__ sj, int *__restrict__ simg,
-                                          int *__restrict__ scnt, float4 *__restrict__ dU,
+                                          int *__restrict__ scnt, int *__restrict__ srow,
+                                          int unique, float4 *__restrict__ dU,
                                           double *__restrict__ partial,
                                           double *__restrict__ atom_ev,
                                           int *__restrict__ flags)
@@ -282,8 +283,11 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
         double Dz = cp * ud.z + (double)(cu * uz + dcz);
         size_t o = (size_t)jj * n + i;
         dU[o] = make_float4((float)Dx, (float)Dy, (float)Dz, 0.f);
-        sj[o] = sh.jdx[jj][a];
-        simg[o] = sh.img[jj][a];
+        if (!unique) {
+            sj[o] = sh.jdx[jj][a];
+            simg[o] = sh.img[jj][a];
+        }
+        srow[(size_t)i * SCAP + jj] = sh.jdx[jj][a];
         // virial W = -sum r_ij (x) D_ij ; order xx yy zz yz xz xy
         w6[0] -= ud.x * Dx;
         w6[1] -= ud.y * Dy;
@@ -292,7 +296,10 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
         w6[4] -= ud.x * Dz;
         w6[5] -= ud.x * Dy;
     }
-    if (valid && l == 0) scnt[i] = ns;
+    if (valid) {
+        if (l == 0) scnt[i] = ns;
+        for (int t = ns + l; t < SCAP; t += G) srow[(size_t)i * SCAP + t] = -1;
+    }
 
     // ---- 4. deterministic reduction: group lanes, then atoms of the block ----
     double vals[7] = {e_lane, w6[0], w6[1], w6[2], w6[3], w6[4], w6[5]};
@@ -460,7 +467,8 @@ __device__ __forceinline__ void site1m_tile(int blk, int n, const Geo &geo,
                                             const int *__restrict__ ncnt,
                                             const Model *__restrict__ M,
                                             int *__restrict__ sj, int *__restrict__ simg,
-                                            int *__restrict__ scnt, float4 *__restrict__ dU,
+                                            int *__restrict__ scnt, int *__restrict__ srow,
+                                            int unique, float4 *__restrict__ dU,
                                             int *__restrict__ flags)
 {
     constexpr int APB = BLOCK / G;
@@ -547,9 +555,12 @@ __device__ __forceinline__ void site1m_tile(int blk, int n, const Geo &geo,
             s_u[slot][a] = make_float4((float)sp3.x, (float)sp3.y, (float)sp3.z, rf);
             s_aux[slot][a] = make_float4(ir, fb, dfb, A);
             s_pair[slot][a] = make_float2(dR * ir, dA * ir);  // divided by r once
-            size_t o = (size_t)slot * n + i;
-            sj[o] = j;
-            simg[o] = im;
+            if (!unique) {
+                size_t o = (size_t)slot * n + i;
+                sj[o] = j;
+                simg[o] = im;
+            }
+            srow[(size_t)i * SCAP + slot] = j;
         }
         cnt += __popcll(gb);
     }
@@ -648,7 +659,10 @@ __device__ __forceinline__ void site1m_tile(int blk, int n, const Geo &geo,
         const float c = cpair[q] + cu;
         dU[(size_t)jj * n + i] = make_float4(c * u.x + dcx, c * u.y + dcy, c * u.z + dcz, 0.f);
     }
-    if (valid && l == 0) scnt[i] = ns;
+    if (valid) {
+        if (l == 0) scnt[i] = ns;
+        for (int t = ns + l; t < SCAP; t += G) srow[(size_t)i * SCAP + t] = -1;
+    }
 }
 
 template <int G, int SCAP, class Geo, bool VIR>
@@ -659,7 +673,8 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                                             const int *__restrict__ ncnt,
                                             const Model *__restrict__ M,
                                             int *__restrict__ sj, int *__restrict__ simg,
-                                            int *__restrict__ scnt, float4 *__restrict__ dU,
+                                            int *__restrict__ scnt, int *__restrict__ srow,
+                                            int unique, float4 *__restrict__ dU,
                                             double *__restrict__ partial,
                                             double *__restrict__ atom_ev,
                                             int *__restrict__ flags)
@@ -729,9 +744,12 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
             s_aux[slot][a] = make_float4((float)ir, inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
                                          inb ? (float)dfc64(r, B.cut, B.soft) : 0.f, (float)A);
             s_pair[slot][a] = make_double2(dR * ir, dA * ir);  // divided by r once
-            size_t o = (size_t)slot * n + i;
-            sj[o] = j;
-            simg[o] = im;
+            if (!unique) {
+                size_t o = (size_t)slot * n + i;
+                sj[o] = j;
+                simg[o] = im;
+            }
+            srow[(size_t)i * SCAP + slot] = j;
         }
         cnt += __popcll(gb);
     }
@@ -838,7 +856,10 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
             w6[5] -= ud.x * Dy;
         }
     }
-    if (valid && l == 0) scnt[i] = ns;
+    if (valid) {
+        if (l == 0) scnt[i] = ns;
+        for (int t = ns + l; t < SCAP; t += G) srow[(size_t)i * SCAP + t] = -1;
+    }
 
     if constexpr (VIR) {
 #pragma unroll
@@ -885,7 +906,8 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, int ntiles, Geo geo,
                                                 const int *__restrict__ ncnt,
                                                 const Model *__restrict__ M,
                                                 int *__restrict__ sj, int *__restrict__ simg,
-                                                int *__restrict__ scnt, float4 *__restrict__ dU,
+                                                int *__restrict__ scnt, int *__restrict__ srow,
+                                                int unique, float4 *__restrict__ dU,
                                                 double *__restrict__ partial,
                                                 double *__restrict__ atom_ev,
                                                 int *__restrict__ flags)
@@ -893,8 +915,8 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, int ntiles, Geo geo,
     int t0, t1;
     tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
-        site_tile<G, SCAP>(t, n, geo, pos, id, nbr, nimg, ncnt, M, sj, simg, scnt, dU, partial,
-                           atom_ev, flags);
+        site_tile<G, SCAP>(t, n, geo, pos, id, nbr, nimg, ncnt, M, sj, simg, scnt, srow, unique,
+                           dU, partial, atom_ev, flags);
         __syncthreads();
     }
 }
@@ -926,7 +948,8 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
                                                   const int *__restrict__ ncnt,
                                                   const Model *__restrict__ M,
                                                   int *__restrict__ sj, int *__restrict__ simg,
-                                                  int *__restrict__ scnt, float4 *__restrict__ dU,
+                                                  int *__restrict__ scnt, int *__restrict__ srow,
+                                                  int unique, float4 *__restrict__ dU,
                                                   double *__restrict__ partial,
                                                   double *__restrict__ atom_ev,
                                                   int *__restrict__ flags)
@@ -936,10 +959,10 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
     for (int t = t0; t < t1; ++t) {
         if constexpr (VIR)
             site1f_tile<G, SCAP, Geo, true>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt,
-                                            dU, partial, atom_ev, flags);
+                                            srow, unique, dU, partial, atom_ev, flags);
         else
-            site1m_tile<G, SCAP, Geo>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, dU,
-                                      flags);
+            site1m_tile<G, SCAP, Geo>(t, n, geo, pos, nbr, nimg, ncnt, M, sj, simg, scnt, srow,
+                                      unique, dU, flags);
         __syncthreads();
     }
 }
@@ -971,7 +994,8 @@ __global__ __launch_bounds__(256) void k_cell_sums(int n, const int *__restrict_
 __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ sj,
                                                 const int *__restrict__ simg,
                                                 const int *__restrict__ scnt,
-                                                const float4 *__restrict__ dU,
+                                                const int *__restrict__ srow, int scap,
+                                                int unique, const float4 *__restrict__ dU,
                                                 double4 *__restrict__ frc,
                                                 double4 *__restrict__ mom, double half_dt,
                                                 int kick, int *__restrict__ flags)
@@ -979,7 +1003,8 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ s
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     bool miss = false;
-    const double3 f = gather_bond_force(i, n, sj, simg, scnt, dU, miss);
+    const double3 f = unique ? gather_rows_any(i, n, scap, srow, dU, miss)
+                             : gather_bond_force(i, n, sj, simg, scnt, dU, miss);
     if (miss) atomicOr(flags + 3, 1);
     frc[i] = make_double4(f.x, f.y, f.z, 0.0);
     if (kick) {
@@ -1028,6 +1053,8 @@ void launch_site(System &S, int mode)
     const int grid =
         (S.ncells > 0 && S.site_grid > 0 && S.site_grid < ntiles) ? S.site_grid : ntiles;
     S.partial.ensure((size_t)ntiles * 8);
+    // rows only: the column-major short list is not written (nor read)
+    const int uq = S.unique_images ? 1 : 0;
     double *aev = nullptr;
     if (S.ncells > 0) {
         S.atom_ev.ensure((size_t)7 * S.n);
@@ -1042,15 +1069,15 @@ void launch_site(System &S, int mode)
         else if (S.model.nspecies == 1 && mode == SITE_FORCES)
             k_site1f<G, SCAP, Geo, true><<<grid, BLOCK, 0, S.stream>>>(
                 S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
-                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+                S.simg.p, S.scnt.p, S.srow.p, uq, S.dU.p, S.partial.p, aev, S.flags.p);
         else if (S.model.nspecies == 1 && mode == SITE_FORCES_ONLY)
             k_site1f<G, SCAP, Geo, false><<<grid, BLOCK, 0, S.stream>>>(
                 S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
-                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+                S.simg.p, S.scnt.p, S.srow.p, uq, S.dU.p, S.partial.p, aev, S.flags.p);
         else
             k_site<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
                 S.n, ntiles, geo, S.pos.p, S.id.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
-                S.simg.p, S.scnt.p, S.dU.p, S.partial.p, aev, S.flags.p);
+                S.simg.p, S.scnt.p, S.srow.p, uq, S.dU.p, S.partial.p, aev, S.flags.p);
     });
 }
 
@@ -1077,6 +1104,7 @@ void ts_site(System &S, int variant, int mode)
     S.simg.ensure((size_t)S.scap * S.n);
     S.scnt.ensure(S.n);
     S.dU.ensure((size_t)S.scap * S.n);
+    S.srow.ensure((size_t)S.scap * S.n);
     hipEvent_t ev;
     const char *name = mode == SITE_ENERGY ? "tersoff_energy" : "tersoff_site";
     S.timer.begin(name, S.stream, &ev);
@@ -1140,7 +1168,8 @@ void ts_gather(System &S, double half_dt, int kick)
 {
     hipEvent_t ev;
     S.timer.begin("tersoff_gather", S.stream, &ev);
-    k_gather<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.sj.p, S.simg.p, S.scnt.p, S.dU.p,
+    k_gather<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.sj.p, S.simg.p, S.scnt.p, S.srow.p,
+                                                     S.scap, S.unique_images ? 1 : 0, S.dU.p,
                                                      S.frc.p, S.mom.p, half_dt, kick, S.flags.p);
     HIP_TRY(hipGetLastError());
     S.timer.end("tersoff_gather", S.stream, ev);

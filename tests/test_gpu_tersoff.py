@@ -330,3 +330,39 @@ def test_full_size_md_workload(oracle):
         assert dr <= 2e-5 * dt * dt * np.abs(f0).max() / s.masses.min() + 1e-12
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("cells", [(1, 1, 1), (2, 1, 1)])
+def test_small_cells_column_gather(eng, oracle, cells):
+    """Cells narrower than twice the list radius (5.431 A across, 3.5 A
+    radius): the engine cannot rule out an atom reaching two images of
+    another, so its force gather matches bonds by (index, image) through the
+    column-major short lists instead of the index rows.  API forces and a
+    device MD run against the oracle."""
+    s = st.diamond_si(cells, sigma=0.1, seed=23)
+    st.maxwell_boltzmann(s, 600.0, seed=24)
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    cut = np.full(n, 3.5)
+    check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, cut)
+    load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, cut, masses=s.masses,
+                momenta=s.momenta)
+    dt, steps = st.ASE_FS, 10
+    eng.md_run(steps, dt, rebuild=1)
+    pos_g, mom_g = eng.state()
+    sp = np.zeros(n, np.int32)
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+    for _ in range(steps):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+        p = p + 0.5 * dt * f
+    travel = np.abs(r - s.positions).max()
+    dr, dp = np.abs(pos_g - r).max(), np.abs(mom_g - p).max() / np.abs(p).max()
+    print("  small cell MD: travel %.3f A |dr| %.2e A rel|dp| %.2e" % (travel, dr, dp), flush=True)
+    assert dr < 1e-5 * travel
+    assert dp < 1e-5
