@@ -409,6 +409,30 @@ def read_table(table_id, directory="."):
     return np.ascontiguousarray(rows[:, :2])
 
 
+def factor_rows(name, params, n_targets, species, cutoff, soft_cutoff, group, link, orig):
+    """The engine descriptor rows of one potential factor applied to the
+    species tuple `species` (include/pysic_hip.h): its 12-double row, and a
+    continuation row (link 2) where the row cannot hold the factor -- for a
+    charge-dependent factor its type, the species of its original first
+    target (its parameters 1 belong to that atom, Potentials.f90:3797-3829)
+    and parameters 5-13; for 'bond_bend' its type, the third target and the
+    three original targets (the second is the vertex,
+    Potentials.f90:3201-3215).  orig: the species of the original targets
+    (-1 for an element absent from the system)."""
+    v = [float(x) for x in params]
+    p = v + [0.0] * (13 - len(v))
+    e = list(species) + [0] * (2 - len(species))
+    code = POTENTIAL_CODES[name]
+    out = [[code, n_targets, e[0], e[1], p[0], p[1], cutoff, soft_cutoff, group, p[2], p[3],
+            link]]
+    o = list(orig) + [-1] * (3 - len(orig))
+    if name in CHARGE_PAIR_TYPES:
+        out.append([code, o[0]] + p[4:13] + [2])
+    elif name == "bond_bend":
+        out.append([code, e[2]] + o[:3] + [0.0] * 6 + [2])
+    return out
+
+
 def build_descriptors(potentials, species_of):
     """Translate potentials into the engine's descriptor rows, the way
     update_core_potentials does for the Fortran core (calculator.py:1063-1446):
@@ -451,24 +475,9 @@ def build_descriptors(potentials, species_of):
         nt = pot.get_number_of_targets()
 
         def rows(f, e, link, orig):
-            """The factor's row, and a continuation row (link 2) where the
-            row cannot hold the factor: for a charge-dependent factor its
-            type, the species of its original first target (its parameters
-            1 belong to that atom, Potentials.f90:3797-3829) and parameters
-            5-13; for 'bond_bend' its type, the third target and the three
-            original targets (the second is the vertex,
-            Potentials.f90:3201-3215)."""
-            v = list(f.get_parameter_values())
-            p = v + [0.0] * (13 - len(v))
-            code = POTENTIAL_CODES[f.get_potential_type()]
-            out = [[code, nt, e[0], e[1], p[0], p[1], pot.get_cutoff(), pot.get_soft_cutoff(),
-                    group, p[2], p[3], link]]
-            o = [species_of.get(x, -1) for x in orig]
-            if f.get_potential_type() in CHARGE_PAIR_TYPES:
-                out.append([code, o[0]] + p[4:13] + [2])
-            elif f.get_potential_type() == "bond_bend":
-                out.append([code, e[2]] + o + [0.0] * 6 + [2])
-            return out
+            return factor_rows(f.get_potential_type(), f.get_parameter_values(), nt, e,
+                               pot.get_cutoff(), pot.get_soft_cutoff(), group, link,
+                               [species_of.get(x, -1) for x in orig])
 
         tuples = pot.get_symbols() or []
         for targets in tuples:
