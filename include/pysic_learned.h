@@ -59,6 +59,42 @@ int pysic_mlp_descriptors(pysic_mlp *m, pysic_ctx *h, int n, float *feat, int *s
 int pysic_mlp_predict_charges(pysic_mlp *m, pysic_ctx *h, int n, int neutralize,
                               double *charges);
 
+/* Residual corrector.  Stages that move the predicted charges towards the
+ * relaxed ones, each fed by the electronegativities chi (the quantity the
+ * relaxation drives to uniformity) of the current charges, evaluated by the
+ * system's own model (EEM terms + Ewald), and by the previous stage's change
+ * d (zero before the first stage).  With the descriptor Gaussians
+ * phi_m(r) = exp(-eta_m (r - mu_m)^2) (cos(pi r / rc) + 1) / 2 and the pair
+ * class p = species_i + species_j (0, 1 or 2), stage k computes
+ *   dq_i = sum_{j != i, r_ij < rc} sum_m phi_m(r_ij) [ a[p][m] (chi_j - chi_i)
+ *                                                   + b[p][m] (d_j - d_i) ]
+ *          + c d_i,
+ *   q_i += dq_i,   d_i = dq_i,
+ * with the stage's PYSIC_CORRECTOR_NCOEF coefficients stored as
+ * a[3][16] | b[3][16] | c.  The weights are symmetric in i and j: the total
+ * charge is kept, and chi enters only through differences.  Each stage costs
+ * one evaluation of chi. */
+#define PYSIC_CORRECTOR_NCOEF 97
+#define PYSIC_CORRECTOR_MAX_STAGES 16
+
+/* The corrector's stages: coef[n_stages][PYSIC_CORRECTOR_NCOEF]; 0 stages
+ * removes it. */
+int pysic_mlp_set_corrector(pysic_mlp *m, int n_stages, const double *coef);
+/* Number of stages held (or a negative code for a null handle). */
+int pysic_mlp_corrector_stages(pysic_mlp *m);
+/* Training features of one stage on the system held by h: sets its charges
+ * to `charges`, evaluates chi, and writes, per atom (original order),
+ * features[n][PYSIC_CORRECTOR_NCOEF] such that dq_i = features_i . coef of
+ * the stage formula above with d = `previous`.  enegs[n] (may be NULL)
+ * receives chi. */
+int pysic_mlp_corrector_features(pysic_mlp *m, pysic_ctx *h, int n, const double *charges,
+                                 const double *previous, double *features, double *enegs);
+/* The warm start: predicted charges, made neutral (mean subtracted), then the
+ * first n_stages corrector stages, all on the device.  The system's charges
+ * are left at the result, which is also written to charges[n] (original
+ * order).  n_stages = 0 gives pysic_mlp_predict_charges with neutralize. */
+int pysic_mlp_warm_start(pysic_mlp *m, pysic_ctx *h, int n, int n_stages, double *charges);
+
 /* Training set: n samples of 32 features, species and target charge, kept on
  * the device until replaced. */
 int pysic_mlp_set_training_data(pysic_mlp *m, int n, const float *feat, const int *species,

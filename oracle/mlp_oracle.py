@@ -27,6 +27,16 @@ How it is pinned (tests/test_oracle_mlp.py):
     FP64 loss;
   * descriptors: the isolated-dimer closed form, translation and
     permutation invariance, and a periodic image count check.
+
+The residual corrector (include/pysic_learned.h), per stage:
+  dq_i = sum_{j != i, r_ij < rc} sum_m phi_m(r_ij) [ a[p][m] (chi_j - chi_i)
+                                                  + b[p][m] (d_j - d_i) ] + c d_i
+with phi_m the descriptor Gaussians and p = species_i + species_j.  Pinned
+by: the isolated-dimer closed form of every feature, invariance of the
+features under a uniform shift of chi, sum_i dq_i = c sum_i d_i (symmetric
+weights), permutation equivariance, and the charge error of a trained
+corrector on held-out cells (relaxed by a dense solve of the linear EEM +
+Ewald equations) shrinking stage by stage.
 """
 import math
 
@@ -141,3 +151,60 @@ def descriptors(positions, cell, pbc, species, nspecies, rc, mu, eta):
         for s in range(nspecies):
             out[i, s * G:(s + 1) * G] = g[sp[jj] == s].sum(axis=0)
     return out
+
+
+NCOEF = 2 * 3 * G + 1
+
+
+def _pairs(positions, cell, pbc, rc):
+    """(i, j, r) of every ordered pair j != i (all periodic images) with r < rc."""
+    pos = np.asarray(positions, np.float64)
+    cell = np.asarray(cell, np.float64)
+    n = len(pos)
+    vol = abs(np.linalg.det(cell))
+    reach = []
+    for a in range(3):
+        if pbc[a]:
+            height = vol / np.linalg.norm(np.cross(cell[(a + 1) % 3], cell[(a + 2) % 3]))
+            reach.append(int(math.ceil(rc / height)))
+        else:
+            reach.append(0)
+    shifts = np.array([[i, j, k] for i in range(-reach[0], reach[0] + 1)
+                       for j in range(-reach[1], reach[1] + 1)
+                       for k in range(-reach[2], reach[2] + 1)], dtype=np.float64) @ cell
+    I, J, R = [], [], []
+    for i in range(n):
+        d = pos[None, :, :] + shifts[:, None, :] - pos[i][None, None, :]
+        r = np.sqrt((d * d).sum(axis=2))
+        ok = (r < rc) & (r > 0.0)
+        jj = np.broadcast_to(np.arange(n)[None, :], r.shape)[ok]
+        I.append(np.full(len(jj), i))
+        J.append(jj)
+        R.append(r[ok])
+    return np.concatenate(I), np.concatenate(J), np.concatenate(R)
+
+
+def corrector_features(positions, cell, pbc, species, rc, mu, eta, chi, prev):
+    """features[n][97] of one corrector stage, so that dq = features @ coef."""
+    sp = np.asarray(species, np.int64)
+    chi = np.asarray(chi, np.float64)
+    prev = np.asarray(prev, np.float64)
+    mu = np.asarray(mu, np.float64)
+    eta = np.asarray(eta, np.float64)
+    n = len(sp)
+    I, J, R = _pairs(positions, cell, pbc, rc)
+    fc = 0.5 * (np.cos(np.pi * R / rc) + 1.0)
+    phi = np.exp(-eta[None, :] * (R[:, None] - mu[None, :]) ** 2) * fc[:, None]
+    p = sp[I] + sp[J]
+    out = np.zeros((n, NCOEF))
+    for m in range(G):
+        np.add.at(out, (I, p * G + m), phi[:, m] * (chi[J] - chi[I]))
+        np.add.at(out, (I, 3 * G + p * G + m), phi[:, m] * (prev[J] - prev[I]))
+    out[:, NCOEF - 1] = prev
+    return out
+
+
+def corrector_stage(positions, cell, pbc, species, rc, mu, eta, chi, prev, coef):
+    """dq[n] of one stage with coefficients coef[97]."""
+    return corrector_features(positions, cell, pbc, species, rc, mu, eta, chi, prev) @ \
+        np.asarray(coef, np.float64)

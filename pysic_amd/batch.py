@@ -11,7 +11,6 @@ graph and replayed.
     cells.evaluate(use_graph=True)
     energies, virials, forces = cells.results()
 """
-import multiprocessing as mp
 import time
 
 import numpy as np
@@ -20,24 +19,15 @@ from .interactions import build_descriptors
 from . import structures as st
 
 
-def _amorphous(args):
-    n, dens, seed, dmin = args
-    return st.amorphous_si(n, dens, seed, dmin)
-
-
-def batched_amorphous_si(ncells=512, nmin=64, nmax=2048, dmin=2.1, dens=(2.25, 2.35), seed=0,
-                         workers=1):
+def batched_amorphous_si(ncells=512, nmin=64, nmax=2048, dmin=2.1, dens=(2.25, 2.35), seed=0):
     """The batched workload's cells (sizes, densities and seeds drawn from one
-    seeded generator), optionally built in worker processes."""
+    seeded generator), built in this process."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(nmin, nmax + 1, size=ncells)
     densities = rng.uniform(dens[0], dens[1], size=ncells)
     seeds = rng.integers(0, 2 ** 31 - 1, size=ncells)
-    jobs = [(int(a), float(b), int(c), dmin) for a, b, c in zip(sizes, densities, seeds)]
-    if workers <= 1:
-        return [_amorphous(j) for j in jobs]
-    with mp.get_context("spawn").Pool(workers) as pool:
-        return pool.map(_amorphous, jobs, chunksize=max(1, ncells // (4 * workers)))
+    return [st.amorphous_si(int(a), float(b), int(c), dmin)
+            for a, b, c in zip(sizes, densities, seeds)]
 
 
 class BatchedCells:

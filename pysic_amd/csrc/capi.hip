@@ -22,6 +22,9 @@
 
 #include <mutex>
 
+static_assert(PYSIC_CORRECTOR_NCOEF == CORR_NC, "corrector coefficient count");
+static_assert(PYSIC_CORRECTOR_MAX_STAGES == CORR_MAX_STAGES, "corrector stage limit");
+
 // ---------------------------------------------------------------------------
 // Fixed device scratch of pysic_calculate_energy_noalloc: the per-block
 // partial sums of the energy pass (8 doubles per site-kernel tile) and the
@@ -1731,6 +1734,98 @@ int pysic_mlp_predict_charges(pysic_mlp *m, pysic_ctx *h, int n, int neutralize,
         mean /= n;
         for (int i = 0; i < n; ++i) charges[i] -= mean;
     }
+    MLP_END(m)
+}
+
+int pysic_mlp_set_corrector(pysic_mlp *m, int n_stages, const double *coef)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(n_stages >= 0 && n_stages <= CORR_MAX_STAGES && (coef || n_stages == 0),
+                  PYSIC_EINVAL, "set_corrector: bad arguments");
+    MlpState &M = m->M;
+    const size_t count = (size_t)n_stages * CORR_NC;
+    for (size_t k = 0; k < count; ++k)
+        PYSIC_REQUIRE(std::isfinite(coef[k]), PYSIC_EINVAL,
+                      "set_corrector: non-finite coefficient");
+    M.corr_stages = 0;
+    if (n_stages > 0) {
+        M.ccoef.ensure(count);
+        HIP_TRY(hipMemcpyAsync(M.ccoef.p, coef, sizeof(double) * count, hipMemcpyHostToDevice,
+                               M.stream));
+        HIP_TRY(hipStreamSynchronize(M.stream));
+    }
+    M.corr_stages = n_stages;
+    MLP_END(m)
+}
+
+int pysic_mlp_corrector_stages(pysic_mlp *m)
+{
+    return m ? m->M.corr_stages : PYSIC_EINVAL;
+}
+
+int pysic_mlp_corrector_features(pysic_mlp *m, pysic_ctx *h, int n, const double *charges,
+                                 const double *previous, double *features, double *enegs)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(h && charges && previous && features && n == h->S.n, PYSIC_EINVAL,
+                  "corrector_features: bad arguments");
+    MlpState &M = m->M;
+    try {
+        // the same checks as the descriptors: configured, lists reach rc
+        mlp_system_descriptors(M, h);
+    } catch (const PysicError &e) {
+        h->err = e.msg;
+        throw;
+    }
+    System &S = h->S;
+    upload_scalar(S, charges, S.q);
+    M.cprev.ensure(n);
+    upload_scalar(S, previous, M.cprev);
+    h->forces_valid = false;
+    DBuf<double> &chi = ewald_eem_chi(S, h->ewald, h->ew);
+    M.cfeat.ensure((size_t)n * CORR_NC);
+    mlp_corrector_features(S, M, chi.p, M.cprev.p, S.stream);
+    std::vector<double> f((size_t)n * CORR_NC);
+    std::vector<int> id(n);
+    HIP_TRY(hipMemcpyAsync(f.data(), M.cfeat.p, sizeof(double) * f.size(), hipMemcpyDeviceToHost,
+                           S.stream));
+    HIP_TRY(hipMemcpyAsync(id.data(), S.id.p, sizeof(int) * n, hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    for (int s = 0; s < n; ++s)
+        std::memcpy(features + (size_t)id[s] * CORR_NC, f.data() + (size_t)s * CORR_NC,
+                    sizeof(double) * CORR_NC);
+    if (enegs) download_scalar(S, chi.p, enegs);
+    MLP_END(m)
+}
+
+int pysic_mlp_warm_start(pysic_mlp *m, pysic_ctx *h, int n, int n_stages, double *charges)
+{
+    MLP_BEGIN(m)
+    PYSIC_REQUIRE(h && charges && n == h->S.n, PYSIC_EINVAL, "warm_start: bad arguments");
+    MlpState &M = m->M;
+    mlp_require_params(M);
+    PYSIC_REQUIRE(n_stages >= 0 && n_stages <= M.corr_stages, PYSIC_EINVAL,
+                  "warm_start: more stages asked for than the corrector holds");
+    try {
+        mlp_system_descriptors(M, h);
+    } catch (const PysicError &e) {
+        h->err = e.msg;
+        throw;
+    }
+    System &S = h->S;
+    M.y.ensure(n);
+    M.cprev.ensure(n);
+    M.cnext.ensure(n);
+    mlp_forward(M, n, nullptr, M.sfeat.p, M.sspec.p, M.y.p, S.stream);
+    mlp_neutral_charges(S, M, M.y.p, M.cprev.p, S.stream);
+    h->forces_valid = false;
+    double *prev = M.cprev.p, *next = M.cnext.p;
+    for (int k = 0; k < n_stages; ++k) {
+        DBuf<double> &chi = ewald_eem_chi(S, h->ewald, h->ew);
+        mlp_corrector_stage(S, M, M.ccoef.p + (size_t)k * CORR_NC, chi.p, prev, next, S.stream);
+        std::swap(prev, next);
+    }
+    download_scalar(S, S.q.p, charges);
     MLP_END(m)
 }
 

@@ -44,6 +44,21 @@ constexpr int MLP_NPK = MLP_PK_W3 + MLP_H / 2;               // 3104 words
 
 constexpr int MLP_TB = 64;  // samples per training block (one wavefront)
 
+// Residual corrector: stages that move the predicted charges towards the
+// relaxed ones, each fed by the electronegativities chi of the current
+// charges and by the previous stage's change d.  With the descriptor
+// Gaussians phi_m (same rc, mu, eta) and the pair class p = s_i + s_j of the
+// species (0, 1 or 2), stage k computes
+//   dq_i = sum_{j != i, r_ij < rc} sum_m phi_m(r_ij) [ a_k[p][m] (chi_j - chi_i)
+//                                                   + b_k[p][m] (d_j - d_i) ]
+//          + c_k d_i
+//   q_i += dq_i,  d_i = dq_i            (d = 0 before the first stage)
+// The weights are symmetric in i and j, so sum_i dq_i = c_k sum_i d_i = 0:
+// the total charge is kept, and a uniform shift of chi changes nothing.
+constexpr int CORR_G = MLP_G;
+constexpr int CORR_NC = 2 * 3 * CORR_G + 1;  // a[3][16] | b[3][16] | c = 97 per stage
+constexpr int CORR_MAX_STAGES = 16;
+
 struct DescParams {
     float rc;
     float mu[MLP_G];
@@ -76,6 +91,11 @@ struct MlpState {
     DBuf<float> sfeat;       // [n][F], current (cell-sorted) order of the system
     DBuf<int> sspec;
     DBuf<double> q;          // predicted charges, original order
+    // residual corrector
+    int corr_stages = 0;
+    DBuf<double> ccoef;          // [corr_stages][CORR_NC]
+    DBuf<double> cprev, cnext;   // d of the previous stage, current order (ping-pong)
+    DBuf<double> cfeat;          // [n][CORR_NC] corrector features, current order
     void release_all();
 };
 
@@ -89,3 +109,11 @@ void mlp_gradient(MlpState &m, int count, const int *order, float *loss_dev);
 void mlp_adam(MlpState &m, float lr, float beta1, float beta2, float eps);
 // y (current order, FP32) -> q[id[s]] (original order, FP64)
 void mlp_scatter_charges(System &S, const float *y, double *q, hipStream_t st);
+// y (current order) -> S.q = y - mean(y) (current order, FP64); zeroes prev
+void mlp_neutral_charges(System &S, MlpState &m, const float *y, double *prev, hipStream_t st);
+// corrector features of chi and prev on S (current order) into m.cfeat
+void mlp_corrector_features(System &S, MlpState &m, const double *chi, const double *prev,
+                            hipStream_t st);
+// one corrector stage: S.q += dq, next = dq
+void mlp_corrector_stage(System &S, const MlpState &m, const double *coef, const double *chi,
+                         const double *prev, double *next, hipStream_t st);

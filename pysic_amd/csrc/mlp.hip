@@ -396,6 +396,152 @@ __global__ void k_scatter_charges(int n, const int *__restrict__ id, const float
     if (s < n) q[id[s]] = (double)y[s];
 }
 
+// ---- residual corrector (learned.h) -----------------------------------------
+
+// The corrector Gaussians of one pair, as k_descriptors forms them: FP64
+// separation, FP32 distance, cutoff function and exponentials.
+__device__ __forceinline__ void corr_basis(float r, const DescParams &dp, float pirc,
+                                           float *phi)
+{
+    const float fc = 0.5f * (cosf(r * pirc) + 1.0f);
+#pragma unroll
+    for (int m = 0; m < CORR_G; ++m) {
+        const float dr = r - dp.mu[m];
+        phi[m] = expf(-dp.eta[m] * dr * dr) * fc;
+    }
+}
+
+// One thread per atom: the 97 features of the stage formula, so that
+// dq_i = features_i . coefficients.  Training-only (FP64 accumulators).
+template <int NS>
+__global__ __launch_bounds__(128) void k_corr_features(int n, CellGeom g,
+                                                       const double4 *__restrict__ pos,
+                                                       const int *__restrict__ nbr,
+                                                       const int *__restrict__ nimg,
+                                                       const int *__restrict__ ncnt, DescParams dp,
+                                                       const double *__restrict__ chi,
+                                                       const double *__restrict__ prev,
+                                                       double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ar[NS][CORR_G], ad[NS][CORR_G];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int m = 0; m < CORR_G; ++m) ar[s][m] = ad[s][m] = 0.0;
+    const double4 pi = pos[i];
+    const double rc2 = (double)dp.rc * (double)dp.rc;
+    const float pirc = 3.14159265358979f / dp.rc;
+    const double ci = chi[i], di = prev[i];
+    const int nf = ncnt[i];
+    for (int t = 0; t < nf; ++t) {
+        const int j = nbr[(size_t)t * n + i];
+        const int im = nimg[(size_t)t * n + i];
+        const double4 pj = pos[j];
+        const double r2 = dot64(separation64(pi, pj, im, g.v));
+        if (!(r2 < rc2)) continue;
+        float phi[CORR_G];
+        corr_basis((float)sqrt(r2), dp, pirc, phi);
+        const double dc = chi[j] - ci, dd = prev[j] - di;
+        const int sj = (int)pj.w;
+#pragma unroll
+        for (int m = 0; m < CORR_G; ++m) {
+            const double v = (double)phi[m];
+            if (NS == 1) {
+                ar[0][m] += v * dc;
+                ad[0][m] += v * dd;
+            } else {
+                ar[0][m] += (sj == 0) ? v * dc : 0.0;
+                ad[0][m] += (sj == 0) ? v * dd : 0.0;
+                ar[NS - 1][m] += (sj == 1) ? v * dc : 0.0;
+                ad[NS - 1][m] += (sj == 1) ? v * dd : 0.0;
+            }
+        }
+    }
+    // pair class p = s_i + s_j: the row holds ar/ad at p in {s_i, s_i + 1}
+    const int si = (int)pi.w;
+    double *o = out + (size_t)i * CORR_NC;
+    for (int k = 0; k < CORR_NC - 1; ++k) o[k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int m = 0; m < CORR_G; ++m) {
+            o[(si + s) * CORR_G + m] = ar[s][m];
+            o[3 * CORR_G + (si + s) * CORR_G + m] = ad[s][m];
+        }
+    o[CORR_NC - 1] = di;
+}
+
+// One corrector stage, one thread per atom; the stage's coefficients are
+// staged in LDS and read per pair class.
+__global__ __launch_bounds__(256) void k_corr_stage(int n, CellGeom g,
+                                                    const double4 *__restrict__ pos,
+                                                    const int *__restrict__ nbr,
+                                                    const int *__restrict__ nimg,
+                                                    const int *__restrict__ ncnt, DescParams dp,
+                                                    const double *__restrict__ coef,
+                                                    const double *__restrict__ chi,
+                                                    const double *__restrict__ prev,
+                                                    double *__restrict__ q,
+                                                    double *__restrict__ next)
+{
+    __shared__ double sc[CORR_NC];
+    for (int k = threadIdx.x; k < CORR_NC; k += blockDim.x) sc[k] = coef[k];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    const int si = (int)pi.w;
+    const double rc2 = (double)dp.rc * (double)dp.rc;
+    const float pirc = 3.14159265358979f / dp.rc;
+    const double ci = chi[i], di = prev[i];
+    double acc = 0.0;
+    const int nf = ncnt[i];
+    for (int t = 0; t < nf; ++t) {
+        const int j = nbr[(size_t)t * n + i];
+        const int im = nimg[(size_t)t * n + i];
+        const double4 pj = pos[j];
+        const double r2 = dot64(separation64(pi, pj, im, g.v));
+        if (!(r2 < rc2)) continue;
+        float phi[CORR_G];
+        corr_basis((float)sqrt(r2), dp, pirc, phi);
+        const double *a = sc + (si + (int)pj.w) * CORR_G;
+        const double *b = a + 3 * CORR_G;
+        double wa = 0.0, wb = 0.0;
+#pragma unroll
+        for (int m = 0; m < CORR_G; ++m) {
+            wa += a[m] * (double)phi[m];
+            wb += b[m] * (double)phi[m];
+        }
+        acc += wa * (chi[j] - ci) + wb * (prev[j] - di);
+    }
+    const double dq = acc + sc[CORR_NC - 1] * di;
+    q[i] += dq;
+    next[i] = dq;
+}
+
+// S.q = y - mean(y) and prev = 0, in one block (the mean in FP64, fixed order)
+__global__ __launch_bounds__(1024) void k_neutral_charges(int n, const float *__restrict__ y,
+                                                          double *__restrict__ q,
+                                                          double *__restrict__ prev)
+{
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 1024) s += (double)y[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double mean = red[0] / n;
+    for (int k = threadIdx.x; k < n; k += 1024) {
+        q[k] = (double)y[k] - mean;
+        prev[k] = 0.0;
+    }
+}
+
 }  // namespace
 
 void MlpState::release_all()
@@ -417,6 +563,11 @@ void MlpState::release_all()
     sfeat.release();
     sspec.release();
     q.release();
+    ccoef.release();
+    cprev.release();
+    cnext.release();
+    cfeat.release();
+    corr_stages = 0;
 }
 
 void mlp_descriptors(System &S, const MlpState &m, hipStream_t st)
@@ -475,5 +626,34 @@ void mlp_adam(MlpState &m, float lr, float beta1, float beta2, float eps)
 void mlp_scatter_charges(System &S, const float *y, double *q, hipStream_t st)
 {
     k_scatter_charges<<<div_up(S.n, 256), 256, 0, st>>>(S.n, S.id.p, y, q);
+    HIP_TRY(hipGetLastError());
+}
+
+void mlp_neutral_charges(System &S, MlpState &m, const float *y, double *prev, hipStream_t st)
+{
+    (void)m;
+    k_neutral_charges<<<1, 1024, 0, st>>>(S.n, y, S.q.p, prev);
+    HIP_TRY(hipGetLastError());
+}
+
+void mlp_corrector_features(System &S, MlpState &m, const double *chi, const double *prev,
+                            hipStream_t st)
+{
+    const int n = S.n;
+    const int blocks = div_up(n, 128);
+    if (m.nspecies == 1)
+        k_corr_features<1><<<blocks, 128, 0, st>>>(n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+                                                    S.ncnt.p, m.dp, chi, prev, m.cfeat.p);
+    else
+        k_corr_features<2><<<blocks, 128, 0, st>>>(n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+                                                    S.ncnt.p, m.dp, chi, prev, m.cfeat.p);
+    HIP_TRY(hipGetLastError());
+}
+
+void mlp_corrector_stage(System &S, const MlpState &m, const double *coef, const double *chi,
+                         const double *prev, double *next, hipStream_t st)
+{
+    k_corr_stage<<<div_up(S.n, 256), 256, 0, st>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+                                                   S.ncnt.p, m.dp, coef, chi, prev, S.q.p, next);
     HIP_TRY(hipGetLastError());
 }

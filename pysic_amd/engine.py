@@ -108,6 +108,10 @@ SIGNATURES = {
     "pysic_mlp_get_params": (ctypes.c_int, [_vp, _fp]),
     "pysic_mlp_descriptors": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _fp, _ip]),
     "pysic_mlp_predict_charges": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "pysic_mlp_set_corrector": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pysic_mlp_corrector_stages": (ctypes.c_int, [_vp]),
+    "pysic_mlp_corrector_features": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "pysic_mlp_warm_start": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
     "pysic_mlp_set_training_data": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _ip, _fp]),
     "pysic_mlp_evaluate": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _fp]),
     "pysic_mlp_gradients": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _fp, _fp]),
@@ -469,6 +473,37 @@ class Mlp:
         q = np.zeros(eng.n)
         self._check(self.L.pysic_mlp_predict_charges(self.m, eng.h, eng.n, 1 if neutralize else 0,
                                                      _d(q)), "mlp_predict_charges")
+        return q
+
+    CORRECTOR_NCOEF = 97
+
+    def set_corrector(self, coef):
+        """Corrector stages, coef[n_stages][97] (empty: none)."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, self.CORRECTOR_NCOEF)
+        self._check(self.L.pysic_mlp_set_corrector(self.m, len(c), _d(c)), "mlp_set_corrector")
+
+    def corrector_stages(self):
+        return int(self.L.pysic_mlp_corrector_stages(self.m))
+
+    def corrector_features(self, eng, charges, previous):
+        """(features[n][97], chi[n]) of one corrector stage at the given
+        charges and previous change; leaves the engine's charges at `charges`."""
+        q = _f64(charges)
+        d = _f64(previous)
+        assert len(q) == eng.n and len(d) == eng.n
+        feat = np.zeros((eng.n, self.CORRECTOR_NCOEF))
+        chi = np.zeros(eng.n)
+        self._check(self.L.pysic_mlp_corrector_features(self.m, eng.h, eng.n, _d(q), _d(d),
+                                                        _d(feat), _d(chi)),
+                    "mlp_corrector_features")
+        return feat, chi
+
+    def warm_start(self, eng, n_stages=None):
+        """Neutral predicted charges followed by n_stages corrector stages
+        (all held ones by default), computed on the device."""
+        k = self.corrector_stages() if n_stages is None else int(n_stages)
+        q = np.zeros(eng.n)
+        self._check(self.L.pysic_mlp_warm_start(self.m, eng.h, eng.n, k, _d(q)), "mlp_warm_start")
         return q
 
     def set_training_data(self, feat, species, targets):

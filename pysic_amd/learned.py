@@ -15,11 +15,23 @@ only the number of steps does.
 builds the training set (seeded random-addition SiO2 cells relaxed on the
 GPU), trains, and saves the model (FP32 master parameters, their BF16
 images as uint16, the descriptor set-up and the normalisation) as .npz.
+
+The damped dynamics shrinks the charge error by about the same factor per
+step in every mode (all of them are underdamped at the workload's friction),
+so a start k times closer saves only ln k / ln(e0 / tol) of the steps.  The
+warm start therefore continues the network's prediction with a residual
+corrector: a few learned stages, each fed by the electronegativities of the
+current charges (one evaluation of the model), that move the charges along
+a local, symmetric, zero-sum kernel of the electronegativity differences.
+
+    python -m pysic_amd.learned corrector --model model.npz --out model.npz
+
+fits the stages on further relaxed cells by linear least squares over
+features the device computes.
 """
 import argparse
 import json
 import math
-import multiprocessing as mp
 import os
 import sys
 import time
@@ -56,6 +68,12 @@ DESC_MU = np.linspace(1.0, 5.5, 16)
 DESC_ETA = np.full(16, 1.0 / (2.0 * 0.3 ** 2))
 
 NPARAM = 6532
+# Residual corrector (include/pysic_learned.h): coefficients per stage and
+# the number of stages trained.  Each stage cuts the charge error of the
+# held-out cells by ~2.6x for one evaluation of chi, where one step of the
+# damped dynamics cuts it by ~1.06x.
+CORR_NCOEF = 97
+CORR_STAGES = 8
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 2048, 2304, 6400, 6464, 6528
 
 
@@ -110,19 +128,9 @@ class EEMSystem:
         self.eng.close()
 
 
-def _make_structure(args):
-    n_si, seed = args
-    return st.amorphous_sio2(n_si, DENSITY, seed)
-
-
-def generate_structures(n_configs, n_si, seed0, workers=8):
-    """Seeded random-addition SiO2 cells (seeds seed0 .. seed0 + n_configs - 1),
-    made in worker processes that never touch the GPU."""
-    jobs = [(n_si, seed0 + k) for k in range(n_configs)]
-    if workers <= 1:
-        return [_make_structure(j) for j in jobs]
-    with mp.get_context("spawn").Pool(workers) as pool:
-        return pool.map(_make_structure, jobs, chunksize=max(1, n_configs // (4 * workers)))
+def generate_structures(n_configs, n_si, seed0):
+    """Seeded random-addition SiO2 cells (seeds seed0 .. seed0 + n_configs - 1)."""
+    return [st.amorphous_sio2(n_si, DENSITY, seed0 + k) for k in range(n_configs)]
 
 
 class ChargePredictor:
@@ -140,6 +148,7 @@ class ChargePredictor:
         self.mean = np.zeros(32, dtype=np.float32)
         self.inv_scale = np.ones(32, dtype=np.float32)
         self.meta = {}
+        self.corrector = np.zeros((0, CORR_NCOEF))
         self._configure()
 
     def _configure(self):
@@ -169,8 +178,20 @@ class ChargePredictor:
         return self.mlp.descriptors(eng)
 
     def predict_engine(self, eng, neutralize=True):
-        """Charges for the system an Engine holds (its lists must reach rc)."""
+        """Network charges for the system an Engine holds (its lists must
+        reach rc)."""
         return self.mlp.predict_charges(eng, neutralize)
+
+    def set_corrector(self, coef):
+        self.corrector = np.asarray(coef, dtype=np.float64).reshape(-1, CORR_NCOEF)
+        self.mlp.set_corrector(self.corrector)
+
+    def warm_start(self, eng, stages=None):
+        """Starting charges for the relaxation of the system an Engine holds:
+        the network's charges made neutral, then the corrector's stages (all
+        of them by default), each fed by the electronegativities of the
+        engine's own model at the current charges."""
+        return self.mlp.warm_start(eng, stages)
 
     def predict_for_calculator(self, calc):
         calc.set_core()
@@ -178,7 +199,7 @@ class ChargePredictor:
         if names != self.species[:len(names)]:
             raise ValueError("the network was trained for species %s, the system has %s"
                              % (self.species, names))
-        return self.predict_engine(calc.engine)
+        return self.warm_start(calc.engine)
 
     def fit(self, feat, species, targets, epochs=60, batch=1024, lr=2e-3, lr_final=5e-5, seed=0,
             log=None):
@@ -202,7 +223,7 @@ class ChargePredictor:
         rounded = ((bits.astype(np.uint64) + 0x7FFF + ((bits >> 16) & 1)) >> 16).astype(np.uint16)
         np.savez(path, params=P, params_bf16=rounded, species=np.array(self.species),
                  rc=self.rc, mu=self.mu, eta=self.eta, mean=self.mean, inv_scale=self.inv_scale,
-                 meta=json.dumps(self.meta))
+                 corrector=self.corrector, meta=json.dumps(self.meta))
 
     @classmethod
     def load(cls, path, device=0):
@@ -213,6 +234,8 @@ class ChargePredictor:
         p.inv_scale = z["inv_scale"].astype(np.float32)
         p._configure()
         p.mlp.set_params(z["params"])
+        if "corrector" in z.files:
+            p.set_corrector(z["corrector"])
         p.meta = json.loads(str(z["meta"]))
         return p
 
@@ -247,10 +270,64 @@ def build_training_set(structures, device=0, tolerance=1e-9, log=None, system=No
             np.array(steps))
 
 
-def train_model(n_configs=2000, n_si=256, seed0=100000, epochs=60, batch=1024, device=0,
-                tolerance=1e-9, workers=8, log=print):
+def train_corrector(pred, structures, n_stages=CORR_STAGES, device=0, tolerance=1e-10,
+                    relax=TRAIN_RELAX, log=None, system=None):
+    """Fit the corrector's stages one after another by linear least squares.
+
+    Every structure is relaxed from zero charges by the plain method to
+    tolerance (q*); its start is the network's neutral prediction.  Stage k's
+    features are computed on the device at each cell's current charges and
+    previous change (pysic_mlp_corrector_features); its coefficients minimise
+    sum over all atoms of (q* - q - features . coef)^2; the stage is then
+    applied to every cell before stage k + 1 is fitted.  Returns the charge
+    RMSE after each stage."""
+    sysm = system or EEMSystem(device)
+    cells = []
     t0 = time.perf_counter()
-    structs = generate_structures(n_configs, n_si, seed0, workers)
+    for k, s in enumerate(structures):
+        sysm.load(s)
+        q_star, _, err, _ = sysm.relax(np.zeros(len(s)), tolerance, relax=relax)
+        if err > tolerance:
+            raise RuntimeError("corrector label relaxation %d did not converge (%.3e)" % (k, err))
+        q0 = pred.mlp.warm_start(sysm.eng, 0)
+        cells.append([s, q_star, q0, np.zeros(len(s))])
+    rmse = [float(np.sqrt(np.mean(np.concatenate([c[1] - c[2] for c in cells]) ** 2)))]
+    if log:
+        log("corrector: %d cells relaxed (%.1f s), start rmse %.3e e"
+            % (len(cells), time.perf_counter() - t0, rmse[0]))
+    coefs = []
+    for k in range(n_stages):
+        feats = []
+        for c in cells:
+            sysm.load(c[0])
+            f, _ = pred.mlp.corrector_features(sysm.eng, c[2], c[3])
+            feats.append(f)
+        X = np.concatenate(feats)
+        y = np.concatenate([c[1] - c[2] for c in cells])
+        # columns scaled to unit norm: chi differences (eV) and charge
+        # changes (e) differ by orders of magnitude
+        scale = np.sqrt((X * X).sum(axis=0))
+        scale[scale == 0.0] = 1.0
+        sol, *_ = np.linalg.lstsq(X / scale, y, rcond=None)
+        coef = sol / scale
+        coefs.append(coef)
+        for c, f in zip(cells, feats):
+            dq = f @ coef
+            c[2] = c[2] + dq
+            c[3] = dq
+        rmse.append(float(np.sqrt(np.mean(np.concatenate([c[1] - c[2] for c in cells]) ** 2))))
+        if log:
+            log("corrector stage %d: rmse %.3e e" % (k, rmse[-1]))
+    if system is None:
+        sysm.close()
+    pred.set_corrector(np.array(coefs))
+    return rmse
+
+
+def train_model(n_configs=2000, n_si=256, seed0=100000, epochs=60, batch=1024, device=0,
+                tolerance=1e-9, log=print):
+    t0 = time.perf_counter()
+    structs = generate_structures(n_configs, n_si, seed0)
     log("generated %d structures of %d atoms (%.1f s)" % (n_configs, 3 * n_si,
                                                           time.perf_counter() - t0))
     pred = ChargePredictor(device)
@@ -286,15 +363,31 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=60)
     t.add_argument("--batch", type=int, default=1024)
     t.add_argument("--tolerance", type=float, default=1e-9)
-    t.add_argument("--workers", type=int, default=8)
     t.add_argument("--out", required=True)
+    c = sub.add_parser("corrector", help="fit the residual corrector of a trained model")
+    c.add_argument("--model", required=True)
+    c.add_argument("--configs", type=int, default=200)
+    c.add_argument("--n-si", type=int, default=256)
+    c.add_argument("--seed0", type=int, default=300000)
+    c.add_argument("--stages", type=int, default=CORR_STAGES)
+    c.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
-    pred = train_model(a.configs, a.n_si, a.seed0, a.epochs, a.batch, 0, a.tolerance, a.workers,
-                       log)
+    if a.cmd == "train":
+        pred = train_model(a.configs, a.n_si, a.seed0, a.epochs, a.batch, 0, a.tolerance, log)
+    else:
+        t0 = time.perf_counter()
+        pred = ChargePredictor.load(a.model, 0)
+        structs = generate_structures(a.configs, a.n_si, a.seed0)
+        rmse = train_corrector(pred, structs, a.stages, log=log)
+        pred.meta["corrector"] = {"configs": a.configs, "atoms_per_config": 3 * a.n_si,
+                                  "seed0": a.seed0, "stages": a.stages,
+                                  "label_tolerance": 1e-10, "label_relax": TRAIN_RELAX,
+                                  "rmse_per_stage": rmse,
+                                  "seconds": time.perf_counter() - t0}
     pred.save(a.out)
     log("saved %s: %s" % (a.out, json.dumps(pred.meta)))
     return 0

@@ -170,41 +170,68 @@ def rocksalt(reps=(4, 4, 4), a=5.64, sigma=0.0, seed=0, cation="Na", anion="Cl")
     return Structure(sym, pos, cell, charges=q)
 
 
-def random_sequential_addition(n, box, dmin, rng, max_tries=2000000):
+def random_sequential_addition(n, box, dmin, rng, max_tries=2000000, batch=256):
     """Uniform random positions in a cubic periodic box with a hard minimum
-    separation (cell-list accelerated rejection sampling)."""
+    separation: candidates p = rng.random(3) * box are tried one after another
+    and kept when no kept position lies closer than dmin (minimum image).
+
+    Candidates are drawn and tested in batches against a cell grid of the
+    kept positions, then resolved in order among themselves, so the result
+    and the generator's final state are those of trying them one at a time."""
     ncell = max(1, int(box // dmin))
     w = box / ncell
-    grid = {}
+    d2 = dmin * dmin
+    cap = 4
+    grid = np.zeros((ncell, ncell, ncell, cap, 3))
+    count = np.zeros((ncell, ncell, ncell), dtype=np.int64)
+    offs = np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)])
     pos = np.zeros((n, 3))
     placed = 0
     tries = 0
-    d2 = dmin * dmin
     while placed < n:
-        tries += 1
-        if tries > max_tries:
+        if tries >= max_tries:
             raise RuntimeError("random sequential addition jammed at %d of %d" % (placed, n))
-        p = rng.random(3) * box
-        c = tuple((p // w).astype(int) % ncell)
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    key = ((c[0] + dx) % ncell, (c[1] + dy) % ncell, (c[2] + dz) % ncell)
-                    for j in grid.get(key, ()):
-                        d = pos[j] - p
-                        d -= box * np.round(d / box)
-                        if d @ d < d2:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-        if ok:
-            pos[placed] = p
-            grid.setdefault(c, []).append(placed)
+        state = rng.bit_generator.state
+        nb = min(batch, max_tries - tries)
+        cand = rng.random((nb, 3)) * box
+        cells = (cand // w).astype(int) % ncell
+        around = (cells[:, None, :] + offs[None, :, :]) % ncell            # [nb, 27, 3]
+        gx, gy, gz = around[..., 0], around[..., 1], around[..., 2]
+        kept = grid[gx, gy, gz]                                            # [nb, 27, cap, 3]
+        valid = np.arange(cap)[None, None, :] < count[gx, gy, gz][..., None]
+        d = kept - cand[:, None, None, :]
+        d -= box * np.round(d / box)
+        clash = ((d * d).sum(axis=-1) < d2) & valid
+        free = np.nonzero(~clash.any(axis=(1, 2)))[0]
+        # candidates that pass the kept positions, resolved in order
+        fc = cand[free]
+        dd = fc[None, :, :] - fc[:, None, :]
+        dd -= box * np.round(dd / box)
+        near = (dd * dd).sum(axis=-1) < d2                                 # [m, m]
+        blocked = np.zeros(len(free), dtype=bool)
+        used = nb
+        for a in range(len(free)):
+            if blocked[a]:
+                continue
+            blocked |= near[a]
+            k = free[a]
+            cx, cy, cz = cells[k]
+            slot = count[cx, cy, cz]
+            if slot == cap:
+                grid = np.concatenate([grid, np.zeros_like(grid)], axis=3)
+                cap *= 2
+            grid[cx, cy, cz, slot] = cand[k]
+            count[cx, cy, cz] = slot + 1
+            pos[placed] = cand[k]
             placed += 1
+            if placed == n:
+                used = k + 1
+                break
+        tries += used
+        if used < nb:
+            # leave the generator where one-at-a-time trials would stop
+            rng.bit_generator.state = state
+            rng.random((used, 3))
     return pos
 
 

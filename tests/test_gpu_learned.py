@@ -7,7 +7,12 @@ Tolerances (each written where it is checked):
   * forward: the device accumulates in FP32 where the oracle sums exactly,
     so a hidden activation can round to the neighbouring BF16 value in rare
     cases; median |dy| <= 1e-5 * rms(y) and max |dy| <= 1e-2 * rms(y);
-  * gradient: max |dg| <= 1e-3 * max |g|, loss within 1e-4 relative.
+  * gradient: max |dg| <= 1e-3 * max |g|, loss within 1e-4 relative;
+  * corrector features: FP32 Gaussians, FP64 differences and sums, max |df|
+    <= 2e-5 * max |f| per block (chi part, previous-change part); chi within
+    1e-9 relative of the reference core's electronegativities;
+  * corrector stages: max |dq| <= 1e-6 e after all stages against the
+    oracle's stages fed by the reference core's chi.
 """
 import os
 import sys
@@ -19,6 +24,7 @@ from conftest import ROOT
 
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import mlp_oracle as MO  # noqa: E402
+import oracle  # noqa: E402
 
 from pysic_amd import structures as st  # noqa: E402
 
@@ -194,4 +200,100 @@ def test_warm_start_through_charge_relaxation():
     print("calculator path: steps cold %d warm %d" % (nc, nw), flush=True)
     assert np.abs(qc - qw).max() < 1e-7
     assert nw < nc
+    p.close()
+
+
+def _oracle_chi_setup(s):
+    from pysic_amd.interactions import build_descriptors
+    from pysic_amd.learned import SIO2_EWALD, SPECIES, eem_potentials, kcells
+    sp = np.array([SPECIES.index(x) for x in s.symbols], np.int32)
+    pd, _, _ = build_descriptors(eem_potentials(), {x: k for k, x in enumerate(SPECIES)})
+    p = SIO2_EWALD
+    K = kcells(s.cell, p["k_cutoff"])
+    ew = np.array([1, p["real_cutoff"], p["k_cutoff"], K[0], K[1], K[2], p["sigma"],
+                   p["epsilon"]])
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1],
+                              np.full(len(s), p["real_cutoff"] + 0.5))
+    return lambda q: oracle.evaluate("enegs", sp, s.positions, q, s.cell, nl, pd, None, ew), sp
+
+
+@pytest.mark.parametrize("n_si,seed", [(32, 21), (256, 22)])
+def test_corrector_features_match_oracle(eem, n_si, seed):
+    from pysic_amd.learned import ChargePredictor, DESC_ETA, DESC_MU, DESC_RC
+    s = st.amorphous_sio2(n_si, 2.2, seed)
+    n = len(s)
+    print("case: corrector features n=%d seed=%d" % (n, seed), flush=True)
+    p = ChargePredictor(0)
+    eem.load(s)
+    rng = np.random.default_rng(seed)
+    q = rng.normal(0, 0.3, n)
+    q -= q.mean()
+    d = rng.normal(0, 0.01, n)
+    f, chi = p.mlp.corrector_features(eem.eng, q, d)
+    chi_of, sp = _oracle_chi_setup(s)
+    chi_ref = chi_of(q)
+    e_chi = np.abs(chi - chi_ref).max() / np.abs(chi_ref).max()
+    fref = MO.corrector_features(s.positions, s.cell, [True] * 3, sp, DESC_RC, DESC_MU, DESC_ETA,
+                                 chi, d)
+    e_a = np.abs(f[:, :48] - fref[:, :48]).max() / np.abs(fref[:, :48]).max()
+    e_b = np.abs(f[:, 48:96] - fref[:, 48:96]).max() / np.abs(fref[:, 48:96]).max()
+    print("  rel |dchi| %.2e  chi block %.2e  change block %.2e" % (e_chi, e_a, e_b), flush=True)
+    assert e_chi <= 1e-9
+    assert e_a <= 2e-5 and e_b <= 2e-5
+    assert np.array_equal(f[:, 96], d)
+    p.close()
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="trained model not present")
+def test_warm_start_stages_match_oracle(eem):
+    from pysic_amd.learned import ChargePredictor, DESC_ETA, DESC_MU, DESC_RC
+    p = ChargePredictor.load(MODEL, 0)
+    k = p.mlp.corrector_stages()
+    if k == 0:
+        pytest.skip("model has no corrector")
+    s = st.amorphous_sio2(64, 2.2, seed=31)
+    print("case: warm start stages n=%d stages=%d" % (len(s), k), flush=True)
+    eem.load(s)
+    q0 = p.warm_start(eem.eng, 0)
+    q_dev = p.warm_start(eem.eng)
+    assert abs(q0.sum()) < 1e-9 and abs(q_dev.sum()) < 1e-9
+    chi_of, sp = _oracle_chi_setup(s)
+    q, d = q0.copy(), np.zeros(len(s))
+    for c in p.corrector:
+        dq = MO.corrector_stage(s.positions, s.cell, [True] * 3, sp, DESC_RC, DESC_MU, DESC_ETA,
+                                chi_of(q), d, c)
+        q, d = q + dq, dq
+    err = np.abs(q_dev - q).max()
+    print("  |q - q_oracle| max %.2e; total change %.3e" % (err, np.abs(q - q0).max()), flush=True)
+    assert err <= 1e-6
+    p.close()
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="trained model not present")
+@pytest.mark.parametrize("n_si,seed", [(256, 7), (1024, 8)])
+def test_corrected_warm_start_halves_the_relaxation(eem, n_si, seed):
+    # held-out configurations (the training seeds start at 100000 and
+    # 300000), one of the training cells' size and one with 4x the atoms: the
+    # warm start with its corrector stages plus the unchanged relaxation takes
+    # at most half the chi evaluations of the cold start, to the same charges
+    from pysic_amd.learned import ChargePredictor, SIO2_RELAX
+    p = ChargePredictor.load(MODEL, 0)
+    k = p.mlp.corrector_stages()
+    if k == 0:
+        pytest.skip("model has no corrector")
+    s = st.amorphous_sio2(n_si, 2.2, seed=seed)
+    print("case: corrected warm start n=%d seed=%d stages=%d" % (len(s), seed, k), flush=True)
+    eem.load(s)
+    tol = 1e-6
+    q_star, _, e_star, _ = eem.relax(np.zeros(len(s)), 1e-11)
+    assert e_star <= 1e-11
+    q_cold, n_cold, e_cold, _ = eem.relax(np.zeros(len(s)), tol, relax=SIO2_RELAX)
+    q0 = p.warm_start(eem.eng)
+    q_warm, n_warm, e_warm, _ = eem.relax(q0, tol, relax=SIO2_RELAX)
+    print("  steps cold %d, warm %d + %d stages; start rmse %.3e e; |q-q*| cold %.2e warm %.2e"
+          % (n_cold, n_warm, k, np.sqrt(np.mean((q0 - q_star) ** 2)),
+             np.abs(q_cold - q_star).max(), np.abs(q_warm - q_star).max()), flush=True)
+    assert e_cold <= tol and e_warm <= tol
+    assert np.abs(q_warm - q_star).max() <= 1e-6 and np.abs(q_cold - q_star).max() <= 1e-6
+    assert n_warm + k <= 0.5 * n_cold
     p.close()

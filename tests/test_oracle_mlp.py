@@ -134,3 +134,63 @@ def test_descriptor_invariances_and_images():
                           for a in (0, 1) for b in (0, 1) for c in (0, 1)])
     Gb = M.descriptors(big, 2 * s.cell, [True] * 3, np.tile(sp, 8), 2, 6.0, mu, eta)
     assert np.allclose(Gb[:len(s)], G, atol=1e-10)
+
+
+def test_corrector_dimer_closed_form():
+    rc = 6.0
+    mu = np.linspace(1.0, 5.5, 16)
+    eta = np.full(16, 1.0 / (2 * 0.3 ** 2))
+    r = 3.21
+    pos = np.array([[10.0, 10.0, 10.0], [10.0, 10.0 + r, 10.0]])
+    cell = np.eye(3) * 40.0
+    chi = np.array([-1.25, 2.5])
+    prev = np.array([0.125, -0.5])
+    f = M.corrector_features(pos, cell, [True] * 3, [0, 1], rc, mu, eta, chi, prev)
+    fc = 0.5 * (math.cos(math.pi * r / rc) + 1)
+    phi = np.array([math.exp(-e * (r - m) ** 2) * fc for m, e in zip(mu, eta)])
+    # the mixed pair has class 1 for both atoms
+    expect = np.zeros((2, M.NCOEF))
+    expect[0, 16:32] = phi * 3.75
+    expect[0, 64:80] = phi * -0.625
+    expect[0, 96] = 0.125
+    expect[1, 16:32] = phi * -3.75
+    expect[1, 64:80] = phi * 0.625
+    expect[1, 96] = -0.5
+    assert np.allclose(f, expect, rtol=0, atol=1e-14)
+    # same species: class 0 (both species 0) or 2 (both species 1)
+    f0 = M.corrector_features(pos, cell, [True] * 3, [0, 0], rc, mu, eta, chi, prev)
+    f2 = M.corrector_features(pos, cell, [True] * 3, [1, 1], rc, mu, eta, chi, prev)
+    assert np.allclose(f0[0, 0:16], phi * 3.75, atol=1e-14) and not f0[0, 16:48].any()
+    assert np.allclose(f2[0, 32:48], phi * 3.75, atol=1e-14) and not f2[0, 0:32].any()
+
+
+def test_corrector_shift_sum_rule_and_permutation():
+    from pysic_amd import structures as st
+    s = st.amorphous_sio2(16, 2.2, seed=3)
+    sp = np.array([["O", "Si"].index(x) for x in s.symbols])
+    rng = np.random.default_rng(5)
+    n = len(s)
+    chi = rng.normal(size=n)
+    prev = rng.normal(size=n) * 0.01
+    coef = rng.normal(size=M.NCOEF)
+    mu = np.linspace(1.0, 5.5, 16)
+    eta = np.full(16, 1.0 / (2 * 0.3 ** 2))
+    args = (s.positions, s.cell, [True] * 3, sp, 6.0, mu, eta)
+    f = M.corrector_features(*args, chi, prev)
+    # chi enters through differences only
+    assert np.allclose(M.corrector_features(*args, chi + 3.7, prev), f, rtol=0, atol=1e-12)
+    # symmetric pair weights: the pair terms sum to zero over the atoms
+    dq = f @ coef
+    assert abs(dq.sum() - coef[-1] * prev.sum()) <= 1e-12 * np.abs(dq).sum()
+    # relabelling the atoms relabels the result
+    perm = rng.permutation(n)
+    fp = M.corrector_features(s.positions[perm], s.cell, [True] * 3, sp[perm], 6.0, mu, eta,
+                              chi[perm], prev[perm])
+    assert np.allclose(fp, f[perm], rtol=0, atol=1e-12)
+    # the cell (L ~ 8.5 A) is narrower than 2 rc: a 2x2x2 replica of the same
+    # charges gives the same rows
+    big = np.concatenate([s.positions + s.cell[0, 0] * np.array([a, b, c])
+                          for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+    fb = M.corrector_features(big, 2 * s.cell, [True] * 3, np.tile(sp, 8), 6.0, mu, eta,
+                              np.tile(chi, 8), np.tile(prev, 8))
+    assert np.allclose(fb[:n], f, rtol=0, atol=1e-10)
