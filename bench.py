@@ -572,47 +572,66 @@ def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
     return out
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """Reference Fortran core on the host: VV steps (rebuild + forces) on a
-    16x16x16-cell sample (32,768 atoms) of the same workload."""
+def cpu_baseline(seconds_budget=20.0, ranks=16, mpi_steps=20):
+    """Reference Fortran core on the host, on a 16x16x16-cell sample (32,768
+    atoms) of the same workload: VV steps with neighbor rebuild + forces each.
+    The MPI build (-DMPI, MPICH) runs on `ranks` processes under mpiexec, as
+    pysic runs under mpirun; the serial build is timed beside it in this
+    process.  The MPI figure is the baseline when it ran."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     if not O.ref_available():
         return None
-    from pysic_amd import structures as st
-    reps = 16
-    s = st.diamond_si((reps, reps, reps), a=5.431, sigma=0.1, seed=0)
-    st.maxwell_boltzmann(s, 600.0, seed=1)
+    import ref_md
+    R, s, cut = ref_md.setup(16)
     n = len(s)
-    T = dict(A=1830.8, l1=2.4799, B=471.18, l2=1.7322)
-    ters = [1.1e-6, 0.78734, 0.0, 100390.0, 16.217, -0.59825, 3.0]
-    pots = [(1, [T["A"], T["l1"]], 3.0, 2.7, ("Si", "Si"), -1),
-            (1, [-T["B"], T["l2"]], 3.0, 2.7, ("Si", "Si"), 0)]
-    cut = np.full(n, 3.5)
-    R = O.RefCore()
-    R.setup(s.symbols, s.positions, s.cell, s.pbc, masses=s.masses, potentials=pots,
-            tersoff=[(ters, 3.0, 2.7, ("Si", "Si", "Si"), 0)], ngroups=1, cutoffs=cut)
-    dt = st.ASE_FS
-    m = s.masses[:, None]
-    r, p = s.positions.copy(), s.momenta.copy()
-    f, _ = R.forces()
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        p = p + 0.5 * dt * f
-        r = r + dt * p / m
-        R.rebuild(r, cut)
-        f, _ = R.forces()
-        p = p + 0.5 * dt * f
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_budget or steps >= 50:
-            break
-    return {"value": n * steps / el, "unit": "atom-steps/s", "cores": 1, "kind": "reference",
-            "sample": "reference Fortran core (serial, amdflang -O2), %d atoms (16^3 cells of the "
-                      "same Si workload), %d velocity-Verlet steps with neighbor rebuild + "
-                      "forces each step, %.1f s; its force loop is O(N^2) in N, so this "
-                      "sample overstates its 1M-atom rate" % (n, steps, el)}
+    steps, el = ref_md.run_vv(R, s, cut, seconds=seconds_budget)
+    serial = {"value": n * steps / el, "unit": "atom-steps/s", "cores": 1, "kind": "reference",
+              "sample": "reference Fortran core (serial, amdflang -O2), %d atoms (16^3 cells of "
+                        "the same Si workload), %d velocity-Verlet steps with neighbor rebuild + "
+                        "forces each step, %.1f s; its force loop is O(N^2) in N, so this "
+                        "sample overstates its 1M-atom rate" % (n, steps, el)}
+    mpi = reference_mpi_sample(ranks, mpi_steps)
+    if "error" in mpi:
+        serial["mpi"] = mpi
+        return serial
+    out = {"value": mpi["atoms"] * mpi["steps"] / mpi["seconds"], "unit": "atom-steps/s",
+           "cores": mpi["ranks"], "kind": "reference",
+           "sample": "reference Fortran core built with -DMPI (amdflang -O2, MPICH 3.3.2), "
+                     "mpiexec -n %d, one rank per core, %d atoms (16^3 cells of the same Si "
+                     "workload), %d velocity-Verlet steps with neighbor rebuild + forces each "
+                     "step, %.2f s; its force loop is O(N^2) in N, so this sample overstates "
+                     "its 1M-atom rate" % (mpi["ranks"], mpi["atoms"], mpi["steps"],
+                                           mpi["seconds"]),
+           "serial": serial}
+    return out
+
+
+def reference_mpi_sample(ranks, steps, timeout=300):
+    """oracle/ref_md.py under mpiexec (child processes that never touch the
+    GPU); its rank-0 JSON line, or {"error": ...}."""
+    import shutil
+    import subprocess
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    lib = os.path.join(ROOT, "oracle", "_ref", "libpysic_ref_mpi.so")
+    if not (os.path.exists(mpiexec) and os.path.exists(lib)):
+        return {"error": "no MPI build of the reference core (oracle/_ref) or no mpiexec"}
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    # the reference's mpi_stack keeps a 4 x max-neighbors x N integer array
+    # on the stack (MPI.f90:416), 50 MB here: each rank raises its stack
+    # limit to the hard limit first
+    rank_cmd = "ulimit -s $(ulimit -H -s) && exec %s %s --reps 16 --steps %d" % (
+        sys.executable, os.path.join(ROOT, "oracle", "ref_md.py"), steps)
+    cmd = [mpiexec, "-n", str(ranks), "-launcher", "fork", "bash", "-c", rank_cmd]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env,
+                           cwd="/tmp")
+    except subprocess.TimeoutExpired:
+        return {"error": "mpiexec timed out after %d s" % timeout}
+    lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "mpiexec exit %d: %s" % (r.returncode, (r.stderr or r.stdout)[-300:])}
+    return json.loads(lines[-1])
 
 
 def main():

@@ -17,6 +17,7 @@ Descriptor formats shared with the engine's C-ABI:
   ewald (8 doubles): on, real_cutoff, k_cutoff, K1, K2, K3, sigma, epsilon
 """
 import ctypes
+import math
 import os
 import tempfile
 
@@ -25,6 +26,18 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libpysic_ref.so")
+# the reference core built with -DMPI (build_ref.sh), for the multi-rank
+# CPU baseline (ref_mpi_md.py); selected by use_mpi_reference()
+REF_MPI_PATH = os.path.join(HERE, "_ref", "libpysic_ref_mpi.so")
+
+
+def use_mpi_reference():
+    """Load the MPI build of the reference core in this process (before the
+    first RefCore); every rank of an mpiexec launch calls it."""
+    global REF_PATH
+    if _ref is not None:
+        raise RuntimeError("the reference core is already loaded")
+    REF_PATH = REF_MPI_PATH
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -116,6 +129,56 @@ def inverse_of(cell):
     with the bits the reference's caller produces (calculator.py:1557-1561
     passes numpy's inv(cell).T, i.e. M^-1, in Fortran order)."""
     return np.ascontiguousarray(np.linalg.inv(np.asarray(cell, dtype=np.float64)).T)
+
+
+def image_neighbor_list(pos, cell, pbc, cutoffs):
+    """Neighbor lists by enumeration of periodic images, for any cell: every
+    (i, j, n) with |r_j - r_i + n . cell| < cutoffs[i] and (j, n) != (i, 0),
+    the offsets n relative to the given (unwrapped) positions.  Where a cell
+    is thinner than the cutoff the reference hands the search to ASE's
+    NeighborList (calculator.py:798-850), whose lists hold the pairs within
+    the sum of two radii plus the skin; the energies and forces only need the
+    pairs within each potential's cutoff, which this list contains.
+    O(N^2 * images): small cells only."""
+    pos = np.asarray(pos, dtype=np.float64)
+    cell = np.asarray(cell, dtype=np.float64)
+    cut = np.asarray(cutoffs, dtype=np.float64)
+    n = len(pos)
+    inv = np.linalg.inv(cell)
+    frac = pos @ inv
+    wrap = np.where(np.asarray(pbc, bool)[None, :], np.floor(frac), 0.0)
+    vol = abs(np.linalg.det(cell))
+    reach = []
+    for a in range(3):
+        if pbc[a]:
+            width = vol / np.linalg.norm(np.cross(cell[(a + 1) % 3], cell[(a + 2) % 3]))
+            reach.append(int(math.ceil(cut.max() / width)) + 1)
+        else:
+            reach.append(0)
+    shifts = np.array([[x, y, z] for x in range(-reach[0], reach[0] + 1)
+                       for y in range(-reach[1], reach[1] + 1)
+                       for z in range(-reach[2], reach[2] + 1)], dtype=np.int64)
+    counts = np.zeros(n, dtype=np.int32)
+    idx, off = [], []
+    for i in range(n):
+        # images of every j around i's wrapped position
+        base = (wrap[i][None, :] - wrap).astype(np.int64)            # [j, 3]
+        o = base[None, :, :] + shifts[:, None, :]                     # [shift, j, 3]
+        # separation_vector's order (Geometry.f90:449-466): (r_j - r_i),
+        # then + o_1 v_1, + o_2 v_2, + o_3 v_3
+        d = np.broadcast_to(pos[None, :, :] - pos[i][None, None, :], o.shape).copy()
+        of = o.astype(np.float64)
+        for a in range(3):
+            d = d + of[:, :, a:a + 1] * cell[a][None, None, :]
+        r2 = d[:, :, 0] * d[:, :, 0] + d[:, :, 1] * d[:, :, 1] + d[:, :, 2] * d[:, :, 2]
+        ok = r2 < cut[i] * cut[i]
+        ok &= ~((np.arange(n)[None, :] == i) & (o == 0).all(axis=2))
+        sidx, jidx = np.nonzero(ok)
+        counts[i] = len(jidx)
+        idx.append(jidx)
+        off.append(o[sidx, jidx])
+    return NeighborLists(counts, np.concatenate(idx) if n else np.zeros(0, np.int32),
+                         np.concatenate(off) if n else np.zeros((0, 3), np.int32))
 
 
 def neighbor_list(pos, cell, pbc, cutoffs):

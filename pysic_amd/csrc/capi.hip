@@ -530,13 +530,23 @@ void require_ready(pysic_ctx *h)
     if (h->S.model_dirty) compile_model(h);
 }
 
-// flags[2]: bit 0 an atom lies more than 255 cells from its box; bit 1 the
+// flags[2]: bit 0 an atom lies more than 223 cells from its box; bit 1 the
 // cell sort found an inconsistent permutation (its guards kept every access
 // in bounds; the state is not to be trusted)
 void require_range_flag(int f2)
 {
     PYSIC_REQUIRE(!(f2 & 2), PYSIC_ESTATE, "internal error: inconsistent cell-list sort");
-    PYSIC_REQUIRE(f2 == 0, PYSIC_ERANGE, "an atom lies more than 255 cells from its box");
+    PYSIC_REQUIRE(f2 == 0, PYSIC_ERANGE, "an atom lies more than 223 cells from its box");
+}
+
+// flags[3]: bit 0 a Tersoff bond without its reverse in the short lists,
+// bit 1 a non-finite force from the gather
+void require_force_flags(int f3)
+{
+    PYSIC_REQUIRE(!(f3 & 1), PYSIC_EINVAL, "neighbor list is not symmetric");
+    PYSIC_REQUIRE(!(f3 & 2), PYSIC_ERANGE,
+                  "non-finite force (atoms on top of each other, or a time step too long for "
+                  "the dynamics)");
 }
 
 void check_flags(pysic_ctx *h, bool lists_only)
@@ -547,7 +557,7 @@ void check_flags(pysic_ctx *h, bool lists_only)
     if (!lists_only) {
         PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
                       "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
-        PYSIC_REQUIRE(f[3] == 0, PYSIC_EINVAL, "neighbor list is not symmetric");
+        require_force_flags(f[3]);
     }
 }
 
@@ -1506,6 +1516,7 @@ int pysic_get_cell_results(pysic_ctx *h, int ncells, double *energies, double *v
     PYSIC_REQUIRE(f[1] == 0, PYSIC_ECAP,
                   "Tersoff short list overflow: " + std::to_string(f[1]) + " neighbors");
     require_range_flag(f[2]);
+    require_force_flags(f[3]);
     std::vector<double> ev((size_t)8 * ncells);
     HIP_TRY(hipMemcpyAsync(ev.data(), S.cell_ev.p, sizeof(double) * ev.size(),
                            hipMemcpyDeviceToHost, S.stream));

@@ -144,6 +144,11 @@ __host__ __device__ inline int neg_img(int p)
     return pack_img(-img_x(p), -img_y(p), -img_z(p));
 }
 #define PYSIC_IMG_MAX 511
+// Largest wrap offset of an atom (cells from its box) and largest scan reach
+// (bins = cells of a thin axis): a pair image is a bin shift plus the
+// difference of two wraps, 2 * 223 + 64 <= PYSIC_IMG_MAX.
+#define PYSIC_WRAP_MAX 223
+#define PYSIC_REACH_MAX 64
 
 // separation_vector (Geometry.f90:449-466): r2 - r1 + sum_i o_i v_i in that
 // order, no contraction, so distances match the reference's bits.
@@ -329,6 +334,8 @@ struct CellGeom {
     int nc[3];       // cell-list bins per axis (interior)
     int dim[3];      // bins per axis incl. padding on open axes
     int ncell;       // dim[0]*dim[1]*dim[2]
+    int reach[3];    // bins searched on each side: 1 unless a bin is narrower
+                     // than the list radius (a cell thinner than the cutoff)
 };
 
 // Where a kernel finds the cell of an atom slot.  GeoOne: one periodic cell

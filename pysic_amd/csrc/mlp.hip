@@ -473,8 +473,11 @@ __global__ __launch_bounds__(128) void k_corr_features(int n, CellGeom g,
     o[CORR_NC - 1] = di;
 }
 
-// One corrector stage, one thread per atom; the stage's coefficients are
-// staged in LDS and read per pair class.
+// One corrector stage, CORR_LANES lanes per atom (the atom's neighbor list
+// split over the lanes, partial sums reduced by cross-lane shuffles); the
+// stage's coefficients are staged in LDS and read per pair class.
+constexpr int CORR_LANES = 16;
+
 __global__ __launch_bounds__(256) void k_corr_stage(int n, CellGeom g,
                                                     const double4 *__restrict__ pos,
                                                     const int *__restrict__ nbr,
@@ -489,36 +492,43 @@ __global__ __launch_bounds__(256) void k_corr_stage(int n, CellGeom g,
     __shared__ double sc[CORR_NC];
     for (int k = threadIdx.x; k < CORR_NC; k += blockDim.x) sc[k] = coef[k];
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double4 pi = pos[i];
-    const int si = (int)pi.w;
-    const double rc2 = (double)dp.rc * (double)dp.rc;
-    const float pirc = 3.14159265358979f / dp.rc;
-    const double ci = chi[i], di = prev[i];
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / CORR_LANES;
+    const int lane = threadIdx.x % CORR_LANES;
     double acc = 0.0;
-    const int nf = ncnt[i];
-    for (int t = 0; t < nf; ++t) {
-        const int j = nbr[(size_t)t * n + i];
-        const int im = nimg[(size_t)t * n + i];
-        const double4 pj = pos[j];
-        const double r2 = dot64(separation64(pi, pj, im, g.v));
-        if (!(r2 < rc2)) continue;
-        float phi[CORR_G];
-        corr_basis((float)sqrt(r2), dp, pirc, phi);
-        const double *a = sc + (si + (int)pj.w) * CORR_G;
-        const double *b = a + 3 * CORR_G;
-        double wa = 0.0, wb = 0.0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int si = (int)pi.w;
+        const double rc2 = (double)dp.rc * (double)dp.rc;
+        const float pirc = 3.14159265358979f / dp.rc;
+        const double ci = chi[i], di = prev[i];
+        const int nf = ncnt[i];
+        for (int t = lane; t < nf; t += CORR_LANES) {
+            const int j = nbr[(size_t)t * n + i];
+            const int im = nimg[(size_t)t * n + i];
+            const double4 pj = pos[j];
+            const double r2 = dot64(separation64(pi, pj, im, g.v));
+            if (!(r2 < rc2)) continue;
+            float phi[CORR_G];
+            corr_basis((float)sqrt(r2), dp, pirc, phi);
+            const double *a = sc + (si + (int)pj.w) * CORR_G;
+            const double *b = a + 3 * CORR_G;
+            double wa = 0.0, wb = 0.0;
 #pragma unroll
-        for (int m = 0; m < CORR_G; ++m) {
-            wa += a[m] * (double)phi[m];
-            wb += b[m] * (double)phi[m];
+            for (int m = 0; m < CORR_G; ++m) {
+                wa += a[m] * (double)phi[m];
+                wb += b[m] * (double)phi[m];
+            }
+            acc += wa * (chi[j] - ci) + wb * (prev[j] - di);
         }
-        acc += wa * (chi[j] - ci) + wb * (prev[j] - di);
     }
-    const double dq = acc + sc[CORR_NC - 1] * di;
-    q[i] += dq;
-    next[i] = dq;
+    // every lane of the group takes part in the shuffles (no early return)
+#pragma unroll
+    for (int w = CORR_LANES / 2; w > 0; w >>= 1) acc += __shfl_xor(acc, w, CORR_LANES);
+    if (i < n && lane == 0) {
+        const double dq = acc + sc[CORR_NC - 1] * prev[i];
+        q[i] += dq;
+        next[i] = dq;
+    }
 }
 
 // S.q = y - mean(y) and prev = 0, in one block (the mean in FP64, fixed order)
@@ -653,7 +663,8 @@ void mlp_corrector_features(System &S, MlpState &m, const double *chi, const dou
 void mlp_corrector_stage(System &S, const MlpState &m, const double *coef, const double *chi,
                          const double *prev, double *next, hipStream_t st)
 {
-    k_corr_stage<<<div_up(S.n, 256), 256, 0, st>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
+    const long threads = (long)S.n * CORR_LANES;
+    k_corr_stage<<<(int)div_up(threads, 256L), 256, 0, st>>>(S.n, S.geom, S.pos.p, S.nbr.p, S.nimg.p,
                                                    S.ncnt.p, m.dp, coef, chi, prev, S.q.p, next);
     HIP_TRY(hipGetLastError());
 }

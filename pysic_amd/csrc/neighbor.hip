@@ -48,7 +48,7 @@ __device__ __forceinline__ void bin_atom(const CellGeom &g, const double4 &p, in
     for (int a = 0; a < 3; ++a) {
         if (g.pbc[a]) {
             double fl = floor(rel[a]);
-            bool ok = fabs(fl) < (double)(PYSIC_IMG_MAX / 2);
+            bool ok = fabs(fl) <= (double)PYSIC_WRAP_MAX;
             bad |= !ok;
             w[a] = ok ? -(int)fl : 0;
             double fb = ok ? floor((rel[a] + w[a]) * g.nc[a]) : 0.0;
@@ -128,7 +128,10 @@ __global__ void k_gather_drift_bin(int n, Geo geo, double dt, double half_dt,
     bool miss = false;
     const double3 f = unique ? gather_rows_any(i, n, scap, srow, dU, miss)
                              : gather_bond_force(i, n, sj, simg, scnt, dU, miss);
-    if (miss) atomicOr(flags + 3, 1);
+    // flags[3]: bit 0 a bond without its reverse, bit 1 a non-finite force
+    const int fbad = (miss ? 1 : 0) |
+                     ((isfinite(f.x) && isfinite(f.y) && isfinite(f.z)) ? 0 : 2);
+    if (fbad) atomicOr(flags + 3, fbad);
     p.x = fma(half_dt, f.x, p.x);
     p.y = fma(half_dt, f.y, p.y);
     p.z = fma(half_dt, f.z, p.z);
@@ -798,13 +801,24 @@ __global__ __launch_bounds__(256) void k_neighbors_brick(int n, CellGeom g, Bric
 }
 
 // ---------------------------------------------------------------------------
-// Lane-parallel scan: G lanes per atom.  The three bins of each (dy, dz)
-// column are consecutive in the cell-sorted atom array (bin index runs x
-// fastest), so every column is at most three contiguous segments; the lanes
-// of a group test consecutive candidates of a segment in parallel and the
-// hits are compacted with a ballot, which keeps the sequential (dz, dy, dx,
-// atom) order of the per-atom scan.
+// Lane-parallel scan: G lanes per atom.  The bins of each (dy, dz) column
+// are consecutive in the cell-sorted atom array (bin index runs x fastest),
+// so a column is a few contiguous segments, one per periodic x shift; the
+// lanes of a group test consecutive candidates of a segment in parallel and
+// the hits are compacted with a ballot, which keeps the sequential (dz, dy,
+// dx, atom) order of the per-atom scan.  The scan covers g.reach bins on
+// each side: one, or more when a bin is narrower than the list radius (a
+// cell thinner than the cutoff, whose atoms then meet several periodic
+// images of one another and of themselves).
 // ---------------------------------------------------------------------------
+
+// bin b + shift of a periodic axis with nc bins: b in [0, nc), shift in cells
+__device__ __forceinline__ void wrap_bin(int &b, int &shift, int nc)
+{
+    shift = b >= 0 ? b / nc : -((-b + nc - 1) / nc);
+    b -= shift * nc;
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_neighbors_group(int n, CellGeom g,
                                                          const double4 *__restrict__ pos,
@@ -836,65 +850,62 @@ __global__ __launch_bounds__(256) void k_neighbors_group(int n, CellGeom g,
     double csq = valid ? cut[s] : 0.0;
     csq = csq * csq;
     int count = 0;
-    for (int dz = valid ? -1 : 2; dz <= 1; ++dz) {
+    // test the candidates [b, e) with x shift sx (all lanes of the group)
+    auto scan = [&](int b, int e, int sx, int sy, int sz) {
+        for (int t0 = b; t0 < e; t0 += G) {
+            int t = t0 + l;
+            bool hit = false;
+            int im = 0;
+            if (t < e) {
+                int wt = wrap[t];
+                int ox = sx + img_x(wt) - wsx;
+                int oy = sy + img_y(wt) - wsy;
+                int oz = sz + img_z(wt) - wsz;
+                im = pack_img(ox, oy, oz);
+                if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
+                    double d2 = dot64(separation64(ps, pos[t], im, g.v));
+                    hit = d2 < csq;
+                }
+            }
+            unsigned long long bal = __ballot(hit);
+            unsigned long long gb = (bal >> gbase) & gmask;
+            int slot = count + __popcll(gb & below);
+            if (hit && slot < cap) {
+                nbr[(size_t)slot * n + s] = t;
+                nimg[(size_t)slot * n + s] = im;
+            }
+            count += __popcll(gb);
+        }
+    };
+    const int R0 = g.reach[0], R1 = g.reach[1], R2 = g.reach[2];
+    for (int dz = valid ? -R2 : R2 + 1; dz <= R2; ++dz) {
         int bz = c2 + dz, sz = 0;
-        if (g.pbc[2]) {
-            if (bz < 0) { bz += g.nc[2]; sz = -1; }
-            else if (bz >= g.nc[2]) { bz -= g.nc[2]; sz = 1; }
-        } else if (bz < 0 || bz >= g.dim[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
+        if (g.pbc[2]) wrap_bin(bz, sz, g.nc[2]);
+        else if (bz < 0 || bz >= g.dim[2]) continue;
+        for (int dy = -R1; dy <= R1; ++dy) {
             int by = c1 + dy, sy = 0;
-            if (g.pbc[1]) {
-                if (by < 0) { by += g.nc[1]; sy = -1; }
-                else if (by >= g.nc[1]) { by -= g.nc[1]; sy = 1; }
-            } else if (by < 0 || by >= g.dim[1]) continue;
+            if (g.pbc[1]) wrap_bin(by, sy, g.nc[1]);
+            else if (by < 0 || by >= g.dim[1]) continue;
             int row = g.dim[0] * (by + g.dim[1] * bz);
-            // up to three segments of consecutive atoms sharing one x shift
-            int seg_b[3], seg_e[3], seg_s[3], nseg = 0;
-            for (int dx = -1; dx <= 1; ++dx) {
+            // consecutive bins sharing one x shift merge into one segment
+            int sb = 0, se = 0, ss = 0;
+            bool open = false;
+            for (int dx = -R0; dx <= R0; ++dx) {
                 int bx = c0 + dx, sx = 0;
-                if (g.pbc[0]) {
-                    if (bx < 0) { bx += g.nc[0]; sx = -1; }
-                    else if (bx >= g.nc[0]) { bx -= g.nc[0]; sx = 1; }
-                } else if (bx < 0 || bx >= g.dim[0]) continue;
+                if (g.pbc[0]) wrap_bin(bx, sx, g.nc[0]);
+                else if (bx < 0 || bx >= g.dim[0]) continue;
                 int b0 = start[row + bx], b1 = start[row + bx + 1];
-                if (nseg > 0 && seg_e[nseg - 1] == b0 && seg_s[nseg - 1] == sx) {
-                    seg_e[nseg - 1] = b1;
+                if (open && se == b0 && ss == sx) {
+                    se = b1;
                 } else {
-                    seg_b[nseg] = b0;
-                    seg_e[nseg] = b1;
-                    seg_s[nseg] = sx;
-                    ++nseg;
+                    if (open) scan(sb, se, ss, sy, sz);
+                    sb = b0;
+                    se = b1;
+                    ss = sx;
+                    open = true;
                 }
             }
-            for (int q = 0; q < nseg; ++q) {
-                int e1 = seg_e[q];
-                int sxq = seg_s[q];
-                for (int t0 = seg_b[q]; t0 < e1; t0 += G) {
-                    int t = t0 + l;
-                    bool hit = false;
-                    int im = 0;
-                    if (t < e1) {
-                        int wt = wrap[t];
-                        int ox = sxq + img_x(wt) - wsx;
-                        int oy = sy + img_y(wt) - wsy;
-                        int oz = sz + img_z(wt) - wsz;
-                        im = pack_img(ox, oy, oz);
-                        if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
-                            double d2 = dot64(separation64(ps, pos[t], im, g.v));
-                            hit = d2 < csq;
-                        }
-                    }
-                    unsigned long long b = __ballot(hit);
-                    unsigned long long gb = (b >> gbase) & gmask;
-                    int slot = count + __popcll(gb & below);
-                    if (hit && slot < cap) {
-                        nbr[(size_t)slot * n + s] = t;
-                        nimg[(size_t)slot * n + s] = im;
-                    }
-                    count += __popcll(gb);
-                }
-            }
+            if (open) scan(sb, se, ss, sy, sz);
         }
     }
     if (valid && l == 0) {
@@ -921,9 +932,9 @@ void nb_configure_grid(System &S, double max_cut)
                    nz = a[0] * b[1] - a[1] * b[0];
             w[i] = std::fabs(v[0] * nx + v[1] * ny + v[2] * nz) /
                    std::sqrt(nx * nx + ny * ny + nz * nz);
-            g.nc[i] = (int)std::floor(w[i] / max_cut);
-            PYSIC_REQUIRE(g.nc[i] >= 1, PYSIC_ERANGE,
-                          "cell narrower than the neighbor cutoff; the cell list needs width >= cutoff");
+            // a cell thinner than the list radius gets one bin on that axis
+            // and a scan over several bins (its periodic images)
+            g.nc[i] = std::max(1, (int)std::floor(w[i] / max_cut));
         }
         long limit = std::max<long>(64, 2L * natoms);
         while ((long)g.nc[0] * g.nc[1] * g.nc[2] > limit) {
@@ -932,7 +943,16 @@ void nb_configure_grid(System &S, double max_cut)
                 if (g.nc[i] > g.nc[a]) a = i;
             g.nc[a] = std::max(1, (int)(g.nc[a] * 0.8));
         }
-        for (int i = 0; i < 3; ++i) g.dim[i] = g.pbc[i] ? g.nc[i] : g.nc[i] + 2;
+        for (int i = 0; i < 3; ++i) {
+            g.dim[i] = g.pbc[i] ? g.nc[i] : g.nc[i] + 2;
+            // every atom within max_cut lies within ceil(max_cut / bin width)
+            // bins: the fractional coordinates of two atoms r apart differ by
+            // at most r / width along each axis
+            const double bw = w[i] / g.nc[i];
+            g.reach[i] = bw >= max_cut ? 1 : (int)std::ceil(max_cut / bw);
+            PYSIC_REQUIRE(g.reach[i] <= PYSIC_REACH_MAX, PYSIC_ERANGE,
+                          "cell more than 64 times thinner than the neighbor cutoff");
+        }
         g.ncell = g.dim[0] * g.dim[1] * g.dim[2];
     };
     const int old_nbins = S.nbins;
@@ -953,6 +973,13 @@ void nb_configure_grid(System &S, double max_cut)
         }
         PYSIC_REQUIRE(total < (1L << 30), PYSIC_ERANGE, "too many cell-list bins in the batch");
         S.nbins = (int)total;
+        for (int c = 0; c < S.ncells; ++c)
+            PYSIC_REQUIRE(S.hcells[c].reach[0] == 1 && S.hcells[c].reach[1] == 1 &&
+                              S.hcells[c].reach[2] == 1,
+                          PYSIC_ERANGE,
+                          "cell " + std::to_string(c) +
+                              " of the batch is thinner than the neighbor cutoff; a batch "
+                              "needs cells at least as wide as the cutoff");
         S.cells.ensure(S.ncells);
         S.first_bin.ensure(S.ncells);
         HIP_TRY(hipMemcpy(S.cells.p, S.hcells.data(), sizeof(CellGeom) * S.ncells,
@@ -1129,7 +1156,11 @@ void nb_build_lists(System &S)
     // the default for one cell below 2^17 atoms is the lane-parallel scan:
     // one thread per atom would leave most of the chip idle (measured on
     // 24,576 atoms of SiO2 at 3.55 A: 91 us per thread, 34 us per group)
-    const int method = (S.nb_method == 0 && S.ncells == 0 && n < (1 << 17)) ? 4 : S.nb_method;
+    // a cell thinner than the cutoff: the lane-parallel scan, the one that
+    // searches several bins on each side
+    const bool thin = S.ncells == 0 && (g.reach[0] > 1 || g.reach[1] > 1 || g.reach[2] > 1);
+    const int method =
+        ((S.nb_method == 0 && S.ncells == 0 && n < (1 << 17)) || thin) ? 4 : S.nb_method;
     bool brick = method == 2 || method == 3;
     hipEvent_t ev;
     S.timer.begin("neighbors", st, &ev);

@@ -1005,7 +1005,10 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ s
     bool miss = false;
     const double3 f = unique ? gather_rows_any(i, n, scap, srow, dU, miss)
                              : gather_bond_force(i, n, sj, simg, scnt, dU, miss);
-    if (miss) atomicOr(flags + 3, 1);
+    // flags[3]: bit 0 a bond without its reverse, bit 1 a non-finite force
+    const int fbad = (miss ? 1 : 0) |
+                     ((isfinite(f.x) && isfinite(f.y) && isfinite(f.z)) ? 0 : 2);
+    if (fbad) atomicOr(flags + 3, fbad);
     frc[i] = make_double4(f.x, f.y, f.z, 0.0);
     if (kick) {
         // the second half-kick of velocity Verlet (fma: the fused

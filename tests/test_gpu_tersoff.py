@@ -366,3 +366,21 @@ def test_small_cells_column_gather(eng, oracle, cells):
     print("  small cell MD: travel %.3f A |dr| %.2e A rel|dp| %.2e" % (travel, dr, dp), flush=True)
     assert dr < 1e-5 * travel
     assert dp < 1e-5
+
+
+def test_coincident_atoms_report_non_finite_forces(eng):
+    # two atoms at one position make the bond direction 0/0: the gather flags
+    # the non-finite force and the call fails instead of returning NaN
+    from pysic_amd.engine import PysicEngineError
+    s = st.diamond_si((2, 2, 2), sigma=0.0, seed=0)
+    pos = s.positions.copy()
+    pos[5] = pos[3]
+    pd, bd = tersoff_descriptors()
+    print("case: coincident atoms 3 and 5 of a 2x2x2 Si cell", flush=True)
+    load_system(eng, pos, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+    with pytest.raises(PysicEngineError, match="non-finite force"):
+        eng.forces()
+    # the engine recovers: a sound structure evaluates as before
+    load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+    f, _ = eng.forces()
+    assert np.isfinite(f).all()

@@ -357,3 +357,63 @@ def test_relaxation_exact_discrete_dispersion(oracle):
     tr = a11 + a22
     assert abs(x2 - (tr * x1 - det * 0.0)) < 1e-13
     assert abs(u2 - (tr * u1 - det * dt)) < 1e-13
+
+
+def test_image_neighbor_list_pins(oracle):
+    # (1) on cells at least as wide as the cutoff it is the reference's
+    # subcell search, as a set
+    from pysic_amd import structures as st
+    for s, rc in [(st.diamond_si((3, 3, 3), sigma=0.1, seed=4), 3.5),
+                  (st.amorphous_sio2(32, 2.2, seed=3), 4.0)]:
+        n = len(s)
+        a = oracle.image_neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, rc))
+        b = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, rc))
+        assert np.array_equal(a.pair_set(), b.pair_set())
+    # (2) a cell thinner than the cutoff (2-atom primitive Si, one atom
+    # outside the box) has, atom by atom, the neighbor distances of the same
+    # atoms in a 4x4x4 replica that the subcell search handles
+    a0 = 5.431
+    cell = 0.5 * a0 * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]])
+    # off the cell boundaries (the subcell search bins an atom at a fractional
+    # coordinate of -1e-17 at the wrong end of its cell)
+    pos = np.array([[0.0, 0.0, 0.0], [0.25 * a0 + 0.03, 0.25 * a0, 0.25 * a0 - 0.02]]) + 0.1234
+    pos[1] += cell[0] * 2.0
+    for rc in (3.5, 6.0):
+        nl = oracle.image_neighbor_list(pos, cell, [1, 1, 1], np.full(2, rc))
+        reps = np.array([[x, y, z] for x in range(4) for y in range(4) for z in range(4)])
+        inside = pos - np.array([[0.0, 0.0, 0.0], 2.0 * cell[0]])
+        big = (inside[None, :, :] + (reps @ cell)[:, None, :]).reshape(-1, 3)
+        nb = oracle.neighbor_list(big, 4 * cell, [1, 1, 1], np.full(len(big), rc))
+        for i in range(2):
+            s0, s1 = nl.start[i], nl.start[i + 1]
+            d = pos[nl.idx[s0:s1]] + nl.off[s0:s1] @ cell - pos[i]
+            mine = np.sort(np.linalg.norm(d, axis=1))
+            t0, t1 = nb.start[i], nb.start[i + 1]
+            bi = big[i]
+            db = big[nb.idx[t0:t1]] + nb.off[t0:t1] @ (4 * cell) - bi
+            theirs = np.sort(np.linalg.norm(db, axis=1))
+            assert len(mine) == len(theirs) and np.allclose(mine, theirs, atol=1e-9)
+        # self images appear once the cutoff passes the lattice vector length
+        has_self = any((nl.idx[nl.start[i]:nl.start[i + 1]] == i).any() for i in range(2))
+        assert has_self == (rc > np.linalg.norm(cell[0]))
+
+
+def test_image_neighbor_list_self_images(oracle):
+    # one atom in a 2.5 A simple cubic cell with a 6 A cutoff: its neighbors
+    # are its own images, those of a 5x5x5 replica within 6 A
+    pos = np.array([[0.31, 0.72, 1.13]])
+    cell = np.eye(3) * 2.5
+    nl = oracle.image_neighbor_list(pos, cell, [1, 1, 1], [6.0])
+    assert (nl.idx == 0).all()
+    d = np.linalg.norm(nl.off @ cell, axis=1)
+    reps = np.array([[x, y, z] for x in range(5) for y in range(5) for z in range(5)])
+    big = pos + reps @ cell
+    nb = oracle.neighbor_list(big, 5 * cell, [1, 1, 1], np.full(len(big), 6.0))
+    db = np.linalg.norm(big[nb.idx[:nb.counts[0]]] + nb.off[:nb.counts[0]] @ (5 * cell) - big[0],
+                        axis=1)
+    assert len(d) == len(db) == nb.counts[0]
+    assert np.allclose(np.sort(d), np.sort(db), atol=1e-12)
+    # the count is the number of lattice points within 6 A, by enumeration
+    g = np.array([[x, y, z] for x in range(-3, 4) for y in range(-3, 4) for z in range(-3, 4)])
+    r = np.linalg.norm(g * 2.5, axis=1)
+    assert len(d) == int(((r > 0) & (r < 6.0)).sum())
