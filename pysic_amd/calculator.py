@@ -11,6 +11,7 @@ import warnings
 
 import numpy as np
 
+from .core import CoreMirror
 from .coulomb import CoulombSummation
 from .engine import Engine
 from .interactions import build_descriptors, read_table, table_ids
@@ -123,8 +124,36 @@ class FastNeighborList:
         return np.sort(d) if sort else d
 
 
+class _InitMode:
+    """The full-initialisation switch.  The reference keeps it in an
+    attribute named like the method that sets it (calculator.py:214, 303-312);
+    this object reads as the flag and calls as the setter, so both
+    `if calc.force_core_initialization:` and
+    `calc.force_core_initialization(True)` work."""
+
+    def __init__(self, value):
+        self.value = bool(value)
+
+    def __call__(self, new_mode):
+        self.value = bool(new_mode)
+
+    def __bool__(self):
+        return self.value
+
+    def __eq__(self, other):
+        return bool(other) == self.value
+
+    def __repr__(self):
+        return str(self.value)
+
+
 class Pysic:
-    """ASE calculator over the MI355X engine (calculator.py:192-1832)."""
+    """ASE calculator over the MI355X engine (calculator.py:192-1832).
+
+    Each calculator owns an engine and a CoreMirror of what it holds;
+    Pysic.core is the mirror of the calculator that synchronised last."""
+
+    core = CoreMirror()
 
     def __init__(self, atoms=None, potentials=None, charge_relaxation=None, coulomb=None,
                  full_initialization=False, device=0):
@@ -134,10 +163,11 @@ class Pysic:
         self.charge_relaxation = None
         self.coulomb = None
         self.neighbor_list = None
-        self.force_core_initialization = full_initialization
+        self.force_core_initialization = _InitMode(full_initialization)
         self.extra_calculators = []
         self.forces = self.stress = self.energy = self.electronegativities = None
-        self._core = None  # what the engine currently holds
+        self._mirror = CoreMirror()   # what this calculator's engine holds
+        self._mirror.engine = self.engine
         self.set_atoms(atoms)
         self.set_potentials(potentials)
         self.set_charge_relaxation(charge_relaxation)
@@ -167,7 +197,7 @@ class Pysic:
     def core_initialization_is_forced(self):
         """True if the engine is fully re-initialised before every
         calculation (calculator.py:297-300)."""
-        return self.force_core_initialization
+        return bool(self.force_core_initialization)
 
     # --- configuration (calculator.py:537-820) -------------------------------
     def _invalidate(self):
@@ -198,14 +228,12 @@ class Pysic:
 
     def set_potentials(self, potentials):
         self.potentials = None if potentials is None else list(potentials)
-        self._core = None
         self._invalidate()
 
     def add_potential(self, potential):
         if self.potentials is None:
             self.potentials = []
         self.potentials.append(potential)
-        self._core = None
         self._invalidate()
 
     def get_potentials(self):
@@ -213,7 +241,6 @@ class Pysic:
 
     def set_coulomb_summation(self, coulomb):
         self.coulomb = coulomb
-        self._core = None
         self._invalidate()
 
     def get_coulomb_summation(self):
@@ -337,31 +364,41 @@ class Pysic:
             eng.ewald_off()
         self.create_neighbor_lists()
         self.neighbor_list.build(s)
-        self._core = dict(n=len(s), symbols=_symbols(s), cell=s.get_cell().copy(),
-                          pbc=np.array(s.get_pbc()).copy(), positions=s.get_positions().copy(),
-                          charges=_charges(s).copy(), momenta=s.get_momenta().copy())
+        m = self._mirror
+        m.set_atoms(s)
+        m.set_potentials(self.potentials)
+        m.set_coulomb(self.coulomb)
+        m.set_neighbor_lists(self.neighbor_list)
+        m.potential_lists_ready = m.bond_order_factor_lists_ready = True
+        m.cutoffs = list(self.neighbor_list.cutoffs)
+        Pysic.core = m
 
     def set_core(self):
+        """Brings the engine up to date with the calculator
+        (calculator.py:998-1046): from scratch when forced, on the first call,
+        or when the atom count, elements or cell changed; otherwise it
+        pushes only the coordinates, charges, potentials or Coulomb
+        summation that differ from the mirror."""
         s = self.structure
         if s is None:
             raise MissingAtomsError("no atoms assigned to the calculator")
-        c = self._core
-        if (self.force_core_initialization or c is None or c["n"] != len(s)
-                or c["symbols"] != _symbols(s) or not np.array_equal(c["cell"], s.get_cell())
-                or not np.array_equal(c["pbc"], np.array(s.get_pbc()))):
+        m = self._mirror
+        if (self.force_core_initialization or m.structure is None or len(m.structure) != len(s)
+                or _symbols(m.structure) != _symbols(s) or not m.cell_ready(s)
+                or not m.potentials_ready(self.potentials)
+                or (self.coulomb is not None and not m.coulomb_summation_ready(self.coulomb))
+                or (self.coulomb is None and m.coulomb is not None)):
+            # a new model or cell changes the species tables, the cutoffs
+            # and the cell lists: the engine is set up again
             self._full_init()
             return
-        if not (np.array_equal(c["positions"], s.get_positions())
-                and np.array_equal(c["momenta"], s.get_momenta())):
-            self.engine.update_coordinates(s.get_positions(), s.get_momenta())
-            c["positions"] = s.get_positions().copy()
-            c["momenta"] = s.get_momenta().copy()
-            self.neighbor_list.update(s)
-        if not np.array_equal(c["charges"], _charges(s)):
-            q = _charges(s)
-            _warn_charge(q)
-            self.engine.update_charges(q)
-            c["charges"] = q.copy()
+        if not m.atoms_ready(s):
+            self.update_core_coordinates()
+        if not m.charges_ready(s):
+            self.update_core_charges()
+        if not m.neighbor_lists_ready(self.neighbor_list):
+            self.update_core_neighbor_lists()
+        Pysic.core = m
 
     # The reference pushes each part of the calculator's state into its core
     # with one call per part (calculator.py:998-1482); set_core() does what
@@ -372,7 +409,7 @@ class Pysic:
         self._full_init()
 
     def _core_ready(self):
-        if self._core is None:
+        if self._mirror.structure is None:
             self._full_init()
             return False
         return True
@@ -382,8 +419,8 @@ class Pysic:
         if self._core_ready():
             s = self.structure
             self.engine.update_coordinates(s.get_positions(), s.get_momenta())
-            self._core["positions"] = s.get_positions().copy()
-            self._core["momenta"] = s.get_momenta().copy()
+            self._mirror.set_atomic_positions(s)
+            self._mirror.set_atomic_momenta(s)
             self.neighbor_list.update(s)
 
     def update_core_charges(self):
@@ -393,7 +430,7 @@ class Pysic:
             q = _charges(self.structure)
             _warn_charge(q)
             self.engine.update_charges(q)
-            self._core["charges"] = q.copy()
+            self._mirror.set_charges(q)
             self._invalidate()
 
     def update_core_supercell(self):
@@ -411,13 +448,24 @@ class Pysic:
                 self.engine.set_potential_table(tid, read_table(tid))
             self.engine.set_potentials(pd)
             self.engine.set_bond_order_factors(bd)
+            self._mirror.set_potentials(self.potentials)
+            new_cut = self.get_individual_cutoffs(1.0)
+            nl = self.neighbor_list
+            if nl is None or not np.array_equal(nl.cutoffs, np.asarray(new_cut) + nl.skin):
+                # the list radius follows the cutoffs (calculator.py:1431-1446)
+                self.create_neighbor_lists(new_cut)
+                self.neighbor_list.build(self.structure)
+                self._mirror.set_neighbor_lists(self.neighbor_list)
+            self._mirror.potential_lists_ready = True
             self._invalidate()
 
     def update_core_potential_lists(self):
         """The per-atom potential lists (calculator.py:1048-1061): the engine
         compiles its species tables with the potentials, so nothing is left
         to do once update_core_potentials has run."""
-        self._core_ready()
+        if self._core_ready():
+            self._mirror.potential_lists_ready = True
+            self._mirror.bond_order_factor_lists_ready = True
 
     def update_core_coulomb(self):
         """Ewald parameters and scaling factors (calculator.py:1450-1483)."""
@@ -429,27 +477,39 @@ class Pysic:
                                       p["sigma"], p["epsilon"], sc)
             else:
                 self.engine.ewald_off()
+            self._mirror.set_coulomb(self.coulomb)
             self._invalidate()
 
     def update_core_neighbor_lists(self):
         """Rebuild the neighbor lists (calculator.py:1567-1593)."""
         if self._core_ready():
             self.neighbor_list.build(self.structure)
+            self._mirror.set_neighbor_lists(self.neighbor_list)
 
     # --- calculations (calculator.py:315-534, 918-990) ---------------------
     def calculation_required(self, atoms=None, quantities=("forces", "energy", "stress",
                                                            "electronegativities")):
+        """True if a quantity is not cached, or if the structure, charges,
+        cell or potentials differ from what the engine holds
+        (calculator.py:315-372): a calculator whose atoms were changed in
+        place recomputes."""
         if isinstance(quantities, str):
             quantities = [quantities]
         table = {"energy": self.energy, "forces": self.forces, "stress": self.stress,
                  "electronegativities": self.electronegativities}
-        return any(table.get(q, 0) is None for q in quantities)
+        if any(table.get(q, 0) is None for q in quantities):
+            return True
+        m, s = self._mirror, self.structure
+        if s is None:
+            return True
+        return not (m.atoms_ready(s) and m.charges_ready(s) and m.cell_ready(s)
+                    and m.potentials_ready(self.potentials))
 
     def relax_charges(self, charges, rates, n_steps, timestep, inertia, friction, tolerance):
         self.set_core()
         out = self.engine.relax_charges(charges, rates, n_steps, timestep, inertia, friction,
                                         tolerance)
-        self._core["charges"] = out[0].copy()
+        self._mirror.set_charges(out[0].copy())   # the engine holds the relaxed charges
         return out
 
     def _relax_if_needed(self, skip):
@@ -548,6 +608,31 @@ class Pysic:
         self.set_atoms(s)
         return -(e[0] - e[1]) / (2 * shift)
 
+    def get_numerical_bond_order_gradient(self, coordinator, atom_index, moved_index,
+                                          shift=0.001, atoms=None):
+        """Central differences of atom atom_index's bond-order factor in
+        coordinator's group with respect to the position of atom
+        moved_index (calculator.py:1721-1778); a debugging aid for the
+        analytic gradients.  The calculator's structure is restored."""
+        system = (atoms if atoms is not None else self.structure).copy()
+        orig = system.copy()
+        out = []
+        for c in range(3):
+            vals = []
+            for sg in (1.0, -1.0):
+                t = system.copy()
+                p = t.get_positions()
+                p[moved_index, c] += sg * shift
+                t.set_positions(p)
+                self.set_atoms(t)
+                self.set_core()
+                coordinator.calculate_bond_order_factors()
+                vals.append(coordinator.get_bond_order_factors()[atom_index])
+            out.append((vals[0] - vals[1]) / (2.0 * shift))
+        self.set_atoms(orig)
+        self.set_core()
+        return out
+
     # --- device-resident dynamics -----------------------------------------
     def run_md(self, steps, dt, rebuild=1, energies=False):
         """Velocity Verlet on the device (ASE VelocityVerlet semantics);
@@ -557,8 +642,8 @@ class Pysic:
         pos, mom = self.engine.state()
         self.structure.set_positions(pos)
         self.structure.set_momenta(mom)
-        self._core["positions"] = pos.copy()
-        self._core["momenta"] = mom.copy()
+        self._mirror.set_atomic_positions(self.structure)
+        self._mirror.set_atomic_momenta(self.structure)
         self.neighbor_list.positions = pos.copy()
         self._invalidate()
         return e

@@ -145,6 +145,35 @@ class BondOrderParameters:
     def set_symbols(self, symbols):
         self.symbols = _normalise_targets(symbols, self.n_targets)
 
+    def accepts_target_list(self, targets):
+        """True if targets is a list of lists of get_number_of_targets()
+        symbols each (bondorder.py:321-344); the symbols are not checked."""
+        try:
+            return all(isinstance(t, list) and len(t) == self.n_targets for t in targets)
+        except TypeError:
+            return False
+
+    def add_symbols(self, symbols):
+        """Adds target lists: a list of them, or a single one
+        (bondorder.py:165-184)."""
+        if self.accepts_target_list(symbols):
+            new = [list(t) for t in symbols]
+        elif self.accepts_target_list([symbols]):
+            new = [list(symbols)]
+        else:
+            raise InvalidParametersError("Invalid number of targets.")
+        self.symbols = (self.symbols or []) + new
+
+    def get_different_symbols(self):
+        """Each symbol the targets name, once, in order of appearance
+        (bondorder.py:193-203)."""
+        out = []
+        for targets in self.symbols or []:
+            for sym in targets:
+                if sym not in out:
+                    out.append(sym)
+        return out
+
     def get_cutoff(self):
         return self.cutoff
 
@@ -155,7 +184,19 @@ class BondOrderParameters:
         return self.cutoff_margin
 
     def set_cutoff_margin(self, m):
-        self.cutoff_margin = float(m)
+        """bondorder.py:285-298: the margin may not exceed the cutoff."""
+        m = float(m)
+        if m > self.cutoff:
+            raise InvalidParametersError(
+                "A cutoff margin (%s) longer than the hard cutoff (%s) was given."
+                % (m, self.cutoff))
+        self.cutoff_margin = m
+
+    def set_soft_cutoff(self, cutoff):
+        """Sets the soft cutoff through the margin (cutoff - soft), so a later
+        change of the hard cutoff moves the soft one too (bondorder.py:
+        306-318)."""
+        self.set_cutoff_margin(self.cutoff - float(cutoff))
 
     def get_soft_cutoff(self):
         """cutoff - margin (bondorder.py:241-243)."""
@@ -191,6 +232,7 @@ class Coordinator:
         if isinstance(bond_order_parameters, BondOrderParameters):
             bond_order_parameters = [bond_order_parameters]
         self.bond_order_params = list(bond_order_parameters)
+        self.bond_order_factors = None
         self.group_index = -1
 
     def get_bond_order_parameters(self):
@@ -208,9 +250,46 @@ class Coordinator:
     def get_group_index(self):
         return self.group_index
 
+    # The reference's Coordinator asks the one Fortran core
+    # (bondorder.py:488-532); here the engine is that of the calculator
+    # that synchronised last (Pysic.core), with this coordinator's group.
+    @staticmethod
+    def _engine():
+        from .calculator import Pysic
+        eng = Pysic.core.engine
+        if eng is None or Pysic.core.structure is None:
+            raise InvalidPotentialError("no calculator has set up a system for the coordinator")
+        return eng
+
+    def calculate_bond_order_factors(self):
+        """Computes and stores the per-atom bond-order factors of this
+        coordinator's group (bondorder.py:488-497)."""
+        self.bond_order_factors = self._engine().bond_order_factors(self.group_index)
+
+    def get_bond_order_factors(self):
+        """The factors stored by calculate_bond_order_factors."""
+        return self.bond_order_factors
+
+    def get_bond_order_gradients(self, atom_index):
+        """(N, 3): the gradients of every atom's factor with respect to the
+        position of atom atom_index (bondorder.py:507-518)."""
+        return self._engine().bond_order_gradients(self.group_index, int(atom_index))
+
+    def get_bond_order_gradients_of_factor(self, atom_index):
+        """(N, 3): the gradient of atom atom_index's factor with respect to
+        the position of every atom (bondorder.py:521-532)."""
+        return self._engine().bond_order_gradients(self.group_index, int(atom_index),
+                                                   of_factor=True)
+
     def __eq__(self, other):
         return isinstance(other, Coordinator) and \
             self.bond_order_params == other.bond_order_params
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+    def __repr__(self):
+        return "Coordinator(bond_order_parameters=%s)" % (self.bond_order_params,)
 
 
 class Potential:

@@ -78,8 +78,14 @@ class Structure:
     def get_cell(self):
         return self.cell.copy()
 
+    def set_cell(self, cell):
+        self.cell = np.array(cell, dtype=np.float64).reshape(3, 3)
+
     def get_pbc(self):
         return self.pbc.copy()
+
+    def set_pbc(self, pbc):
+        self.pbc = np.array([bool(p) for p in np.broadcast_to(pbc, (3,))])
 
     def get_chemical_symbols(self):
         return list(self.symbols)
