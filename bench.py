@@ -488,13 +488,15 @@ def lj_benchmark(steps=100, warmup=10, ref_cells=6, reference=True):
 
 
 def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
-    """COMB's local SiO2 terms (bond bending, charge-dependent repulsion
-    and attraction; bend.hip and pair.hip) in molecular dynamics: random
-    amorphous SiO2 at 2.2 g/cm^3, 3 x 8192 = 24,576 atoms, fixed seeded
-    charges (Si in [0.5, 2.2] e, O in [-1.3, -0.2] e), Maxwell-Boltzmann
-    momenta at 300 K, velocity Verlet with dt = 0.5 fs, lists (3.05 + 0.5 A)
-    rebuilt every step, FP64.  The reference core is timed on a 384-atom
-    sample of the same kind: list build + forces per step."""
+    """COMB's local SiO2 terms as the reference's builder makes them (bond
+    bending, charge-dependent repulsion, and charge-dependent attraction
+    scaled by three coordinators' 'tersoff' bond-order factors; bend.hip
+    and pair.hip) in molecular dynamics: random amorphous SiO2 at
+    2.2 g/cm^3, 3 x 8192 = 24,576 atoms, fixed seeded charges (Si in
+    [0.5, 2.2] e, O in [-1.3, -0.2] e), Maxwell-Boltzmann momenta at 300 K,
+    velocity Verlet with dt = 0.5 fs, lists (3.05 + 0.5 A) rebuilt every
+    step, FP64.  The reference core is timed on a 384-atom sample of the
+    same kind: list build + forces per step."""
     from pysic_amd import structures as st
     from pysic_amd.engine import Engine
     from pysic_amd.interactions import build_descriptors, comb_sio2
@@ -510,14 +512,14 @@ def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
     st.maxwell_boltzmann(s, 300.0, seed=6)
     n = len(s)
     names = ["O", "Si"]
-    pd, _, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+    pd, bd, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
     sp = np.array([names.index(x) for x in s.symbols], np.int32)
     cut = np.full(n, 3.55)
     eng = Engine(0)
     eng.create_atoms(sp, s.positions, masses=s.masses, charges=q, momenta=s.momenta)
     eng.create_cell(s.cell, s.pbc)
     eng.set_potentials(pd)
-    eng.set_bond_order_factors(np.zeros((0, 14)))
+    eng.set_bond_order_factors(bd)
     eng.ewald_off()
     eng.generate_neighbor_lists(cut)
     dt = 0.5 * st.ASE_FS
@@ -536,9 +538,10 @@ def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
     tim = eng.timing_read()
     eng.timing(False)
     eng.close()
-    out = {"workload": "COMB local SiO2 terms (bond_bend, charge_exp, charge_abs), amorphous "
-                       "SiO2 24,576 atoms, fixed charges, VV dt = 0.5 fs, 300 K, lists rebuilt "
-                       "every step, FP64",
+    out = {"workload": "COMB local SiO2 terms as comb.py builds them (bond_bend, charge_exp, "
+                       "charge_abs, attraction scaled by three 'tersoff' coordinators), "
+                       "amorphous SiO2 24,576 atoms, fixed charges, VV dt = 0.5 fs, 300 K, "
+                       "lists rebuilt every step, FP64",
            "atoms": n, "steps": steps, "value": round(n * steps / wall, 1),
            "unit": "atom-steps/s", "ms_per_step": round(wall / steps * 1e3, 4),
            "total_energy_drift_eV_per_atom": (e1 - e0) / n,
@@ -549,12 +552,13 @@ def comb_benchmark(n_si=8192, steps=100, warmup=10, ref_si=128):
         if O.ref_available():
             r, qr = system(ref_si, 7)
             sys.path.insert(0, os.path.join(ROOT, "tests"))
-            from helpers import ref_entries
-            pr, _, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+            from helpers import ref_bonds, ref_entries
+            pr, br, ng = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
             rcut = np.full(len(r), 3.55)
             R = O.RefCore()
             R.setup(r.symbols, r.positions, r.cell, r.pbc, charges=qr,
-                    potentials=ref_entries(pr, names), cutoffs=rcut)
+                    potentials=ref_entries(pr, names), tersoff=ref_bonds(br, names),
+                    ngroups=ng, cutoffs=rcut)
             R.forces()
             k = 0
             t0 = time.perf_counter()
@@ -647,7 +651,7 @@ def main():
     ap.add_argument("--no-ewald", action="store_true", help="skip the Ewald rocksalt measurement")
     ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon measurement")
     ap.add_argument("--no-comb", action="store_true", help="skip the COMB SiO2 measurement")
-    ap.add_argument("--only", choices=["qeq", "batch", "ewald", "lj"],
+    ap.add_argument("--only", choices=["qeq", "batch", "ewald", "lj", "comb"],
                     help="run one secondary measurement alone and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (PMC traffic calibration)")
@@ -662,6 +666,8 @@ def main():
             res = batched_benchmark(*build_batched_cells())
         elif args.only == "qeq":
             res = qeq_benchmark(os.path.join(ROOT, "pysic_amd", "data", "sio2_eem_mlp.npz"))
+        elif args.only == "comb":
+            res = comb_benchmark()
         elif args.only == "lj":
             res = lj_benchmark()
         else:

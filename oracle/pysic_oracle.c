@@ -814,9 +814,8 @@ static int orc_unpack(sys_t *S, int npot, const double *pd, int nbond, const dou
     }
     if (last || lastbend) return ORC_EINVAL;
     S->npot = m;
-    for (int k = 0; k < m; ++k) /* bond-order scaling: plain exponentials only */
-        if (S->pot[k].group > -1 && (S->pot[k].f[0].type != 1 || S->pot[k].nf != 1))
-            return ORC_EINVAL;
+    for (int k = 0; k < m; ++k) /* bond-order scaling: pair potentials (and products) */
+        if (S->pot[k].group > -1 && S->pot[k].nt != 2) return ORC_EINVAL;
     for (int k = 0; k < nbond; ++k) {
         const double *d = bd + ORC_BOND_STRIDE * k;
         bond_t *b = S->bond + k;
@@ -1318,12 +1317,15 @@ int orc_evaluate(int calc, int n, const int *species, const double *pos, const d
                 } else if (calc == 3) {
                     /* core_evaluate_local_doublet_electronegativities_B
                      * (Core.f90:5622-5758): charge_exp / charge_abs terms,
-                     * times the smoothing factor (bond-order scaling applies
-                     * to lone exponentials only, which carry no charge) */
+                     * times the smoothing factor and, for a coordinated
+                     * potential, the pair's bond-order factor
+                     * (b_i + b_j + B_ij + B_ji) / 2 with b_i = 0 for the
+                     * level-2 'tersoff' factor (Core.f90:5717-5729) */
                     double x[2];
                     orc_product_eneg(p, dist, &qp, x);
-                    X[i1] += x[0] * cutf;
-                    X[i2] += x[1] * cutf;
+                    double bsum = p->group > -1 ? (bo[0] + bo[1]) * 0.5 : 1.0;
+                    X[i1] += x[0] * cutf * bsum;
+                    X[i2] += x[1] * cutf * bsum;
                 }
             }
             if (nmb == 0 || calc == 3) continue;

@@ -122,9 +122,13 @@ void System::release_all()
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
                        &prevstart, &dirty, &scanloc, &srow,
                        &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0,
-                       &bsj, &bsimg, &bscnt};
+                       &bsj, &bsimg, &bscnt, &osj, &osimg, &oscnt};
     for (auto *b : bi) b->release();
     bG.release();
+    oD.release();
+    oW.release();
+    oown.release();
+    dbmodel.release();
     dU.release();
     wpos.release();
     dmodel.release();
@@ -231,6 +235,8 @@ void compile_model(pysic_ctx *h)
     PYSIC_REQUIRE(ns <= PYSIC_MAXS, PYSIC_EINVAL, "at most 4 species per system");
     M.nspecies = ns;
     M.group = -1;
+    BopModel B;
+    std::memset(&B, 0, sizeof(B));
     auto use_group = [&](int g) {
         if (g < 0) return;
         PYSIC_REQUIRE(M.group < 0 || M.group == g, PYSIC_EINVAL,
@@ -329,12 +335,26 @@ void compile_model(pysic_ctx *h)
             else { P.kforce[e1][0] += d[4]; P.kforce[e1][1] += d[5]; P.kforce[e1][2] += d[9]; }
             P.has_single = 1;
         } else {
-            PYSIC_REQUIRE(group < 0, PYSIC_EINVAL,
-                          "bond-order scaling applies to a lone 'exponential' potential only");
             PYSIC_REQUIRE(nmul + 1 <= PYSIC_MAXF, PYSIC_EINVAL, "at most 4 factors in a product");
-            int &c = P.npair[e1][e2];
-            PYSIC_REQUIRE(c < PYSIC_MAXG, PYSIC_EINVAL, "too many pair potentials for one species pair (at most 8)");
-            GenPair &T = P.pair[e1][e2][c++];
+            // a bond-order-scaled potential goes to the bond-order pass
+            // (one per ordered species pair), any other to the pair kernel
+            const bool scaled = group >= 0;
+            if (scaled) {
+                PYSIC_REQUIRE(group < PYSIC_MAXGRP, PYSIC_EINVAL,
+                              "bond-order-scaled pair potentials take coordinator groups 0..3");
+                PYSIC_REQUIRE(B.nbop[e1][e2] == 0, PYSIC_EINVAL,
+                              "at most one bond-order-scaled potential per species pair");
+            } else {
+                PYSIC_REQUIRE(P.npair[e1][e2] < PYSIC_MAXG, PYSIC_EINVAL,
+                              "too many pair potentials for one species pair (at most 8)");
+            }
+            GenPair &T = scaled ? B.pair[e1][e2] : P.pair[e1][e2][P.npair[e1][e2]++];
+            if (scaled) {
+                B.nbop[e1][e2] = 1;
+                B.group[e1][e2] = group;
+                B.on = 1;
+                B.radius = std::max(B.radius, d[6]);
+            }
             T.cut = d[6];
             if (d[7] > 0.0 && d[7] < d[6]) { T.soft = d[7]; T.smooth = 1; }
             else { T.soft = d[6]; T.smooth = 0; }
@@ -393,9 +413,14 @@ void compile_model(pysic_ctx *h)
                     F.tlen = (int)(it->second.size() / 2);
                 }
                 if (F.type == 9) P.needs_pick = 1;
+                // the bond-order pass takes central pair forces only
+                PYSIC_REQUIRE(!(scaled && F.type == 9), PYSIC_EINVAL,
+                              "a bond-order-scaled potential cannot hold a 'step' factor");
             }
-            const double c2 = T.cut * T.cut * (1.0 + 1e-12);
-            P.cut2_hi[e1][e2] = std::max(P.cut2_hi[e1][e2], c2);
+            if (!scaled) {
+                const double c2 = T.cut * T.cut * (1.0 + 1e-12);
+                P.cut2_hi[e1][e2] = std::max(P.cut2_hi[e1][e2], c2);
+            }
             P.has_pairs = 1;
         }
         k = next - 1;
@@ -407,15 +432,31 @@ void compile_model(pysic_ctx *h)
         PYSIC_REQUIRE(e1 >= 0 && e1 < PYSIC_MAXS && e2 >= 0 && e2 < PYSIC_MAXS && e3 >= 0 &&
                           e3 < PYSIC_MAXS,
                       PYSIC_EINVAL, "bond-order species out of range");
-        BondTerm &b = M.bond[e1][e2][e3];
-        PYSIC_REQUIRE(!b.valid, PYSIC_EINVAL, "duplicate tersoff entry for one species triplet");
+        BondTerm b;
+        std::memset(&b, 0, sizeof(b));
         b.beta = d[3]; b.eta = d[4]; b.a = d[5]; b.c = d[6]; b.d = d[7]; b.h = d[8];
         b.mu = d[9]; b.cut = d[10]; b.soft = d[11];
         b.valid = 1;
         b.a_zero = (d[5] == 0.0 && d[9] > 0.0) ? 1 : 0;
-        use_group((int)d[12]);
+        const int g = (int)d[12];
+        // a third atom counts within the largest bond cutoff of the center
+        // species over every factor (atom%max_bond_radius, Core.f90:1034-1070)
         M.maxbond[e1] = std::max(M.maxbond[e1], d[10]);
-        if (!M.post[e1].valid) M.post[e1] = b;  // first factor of the center species
+        // the Tersoff kernels' group: that of the bond-order-scaled lone
+        // exponentials (with none, the first group named)
+        if (M.group < 0 && !M.has_pairs) use_group(g);
+        if (g == M.group) {
+            PYSIC_REQUIRE(!M.bond[e1][e2][e3].valid, PYSIC_EINVAL,
+                          "duplicate tersoff entry for one species triplet");
+            M.bond[e1][e2][e3] = b;
+            if (!M.post[e1].valid) M.post[e1] = b;  // first factor of the center species
+        }
+        if (g >= 0 && g < PYSIC_MAXGRP) {
+            PYSIC_REQUIRE(!B.bond[g][e1][e2][e3].valid, PYSIC_EINVAL,
+                          "duplicate tersoff entry for one species triplet in a group");
+            B.bond[g][e1][e2][e3] = b;
+            if (!B.post[g][e1].valid) B.post[g][e1] = b;  // the group's first of the center
+        }
     }
     double rs = 0.0;
     for (int s = 0; s < PYSIC_MAXS; ++s) {
@@ -454,6 +495,14 @@ void compile_model(pysic_ctx *h)
     h->S.pmodel = P;
     h->S.dpmodel.ensure(1);
     HIP_TRY(hipMemcpyAsync(h->S.dpmodel.p, &h->S.pmodel, sizeof(PairModel),
+                           hipMemcpyHostToDevice, h->S.stream));
+    for (int a = 0; a < PYSIC_MAXS; ++a) {
+        B.maxbond[a] = M.maxbond[a];
+        if (B.on) B.radius = std::max(B.radius, M.maxbond[a]);
+    }
+    h->S.bmodel = B;
+    h->S.dbmodel.ensure(1);
+    HIP_TRY(hipMemcpyAsync(h->S.dbmodel.p, &h->S.bmodel, sizeof(BopModel),
                            hipMemcpyHostToDevice, h->S.stream));
     drop_graphs(h);  // captured sequences depend on which kernel families the model needs
     h->S.tables.ensure(htab.empty() ? 1 : htab.size());
@@ -710,7 +759,8 @@ std::vector<const void *> graph_signature(pysic_ctx *h, bool rebuild)
             S.cell_ev.p,
             S.frc.p, S.flags.p, S.dmodel.p, S.cells.p, S.first_bin.p, S.cell_of.p, S.atom0.p,
             S.dpmodel.p, S.tables.p, S.gpartial.p, S.gtotals.p, S.totals.p, S.bsj.p,
-            S.bsimg.p, S.bscnt.p, S.bG.p,
+            S.bsimg.p, S.bscnt.p, S.bG.p, S.dbmodel.p, S.osj.p, S.osimg.p, S.oscnt.p, S.oD.p,
+            S.oW.p, S.oown.p,
             (const void *)(long)S.cap, (const void *)(long)h->variant,
             (const void *)(long)S.site_grid, (const void *)(long)S.n, (const void *)(long)S.nbins,
             (const void *)(long)S.nb_method, (const void *)(long)(rebuild ? 1 : 0)};

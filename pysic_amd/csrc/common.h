@@ -129,6 +129,27 @@ struct PairModel {
     LJPair lj[PYSIC_MAXS][PYSIC_MAXS];
 };
 
+// Pair potentials (ProductPotentials) scaled by the 'tersoff' bond-order
+// factors of a coordinator (pair.hip, the bond-order pass): COMB's
+// attraction.  At most one such potential per ordered species pair; each
+// names its coordinator's group, whose 'tersoff' entries are keyed by the
+// (center, partner, third) species triplet.  post[g][s] is the group's
+// first entry with center species s, whose beta and eta turn the sum zeta
+// into the factor (Core.f90:2763-2807).  maxbond[s]: the largest bond-order
+// cutoff of center species s over every group (a third atom counts only
+// within it, Core.f90:2518).
+#define PYSIC_MAXGRP 4
+struct BopModel {
+    int on;
+    int nbop[PYSIC_MAXS][PYSIC_MAXS];
+    int group[PYSIC_MAXS][PYSIC_MAXS];
+    GenPair pair[PYSIC_MAXS][PYSIC_MAXS];
+    BondTerm bond[PYSIC_MAXGRP][PYSIC_MAXS][PYSIC_MAXS][PYSIC_MAXS];
+    BondTerm post[PYSIC_MAXGRP][PYSIC_MAXS];
+    double maxbond[PYSIC_MAXS];
+    double radius;        // short-list radius: every such cutoff and bond cutoff
+};
+
 // ---------------------------------------------------------------------------
 // packed periodic image offsets: three signed 10-bit fields
 // ---------------------------------------------------------------------------

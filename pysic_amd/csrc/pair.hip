@@ -659,6 +659,384 @@ void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt
 
 
 
+
+// ---------------------------------------------------------------------------
+// Bond-order-scaled pair potentials (COMB's attraction).  The reference
+// multiplies a coordinated potential's pair energy by
+// (b_i + b_j + B*_ij + B*_ji) / 2, b the per-atom (level-1) factors -- zero
+// for the level-2 'tersoff' factor -- and B*_ij the pair factor from the
+// sum zeta_ij over the center i's other neighbors k
+// (core_calculate_pair_bond_order_factor, Core.f90:2439-2637, post-processed
+// by the group's first factor of i's species, 2763-2807), and adds the
+// gradients of B*_ij and B*_ji to the forces (core_add_pair_bond_order_forces,
+// Core.f90:4952-5353) and the factor to the electronegativities
+// (Core.f90:5688-5729).  Here that pair energy is split between the two
+// atoms as site energies U_i = sum_j 1/2 f_c(r_ij) V_ij B*_ij, V_ij taken in
+// the reference's orientation (pick), so that E = sum_i U_i; U_i depends
+// on i's neighbor vectors r_ij and on the charges.  Pass 1 (one lane per
+// partner j) forms zeta_ij, B*_ij, U_i and, per short-list slot, dU_i/dr_ij
+// through V and through zeta's dependence on r_ij, and dU_i/dq_j; pass 2
+// (one lane per slot t as the third atom) adds dU_i/dr_it through the
+// zetas of i's other bonds.  The gathers form F_a = sum_t dU_a/dr_at -
+// sum_{i: a in list(i)} dU_i/dr_ia and chi_a = -dU_a/dq_a - sum_i dU_i/dq_a,
+// the reverse entries found in each neighbor's list at (a, -o).  No
+// atomics, fixed order; all arithmetic is FP64.
+// ---------------------------------------------------------------------------
+
+enum { BOP_CHI = 3 };
+
+// evaluate_pair_bond_order_factor_tersoff (Potentials.f90:5744-5795):
+// s1 = r_i - r_j (partner to center), s2 = r_k - r_i, r1 and r2 their norms
+__device__ __forceinline__ double bo_term(const double3 &s1, double r1, const double3 &s2,
+                                          double r2, const BondTerm &b)
+{
+    const double cosine = (-s1.x * s2.x + -s1.y * s2.y + -s1.z * s2.z) / (r1 * r2);
+    const double cc = b.c * b.c, dd = b.d * b.d, hc = b.h - cosine;
+    const double xi = fc64(r2, b.cut, b.soft) * exp(pow(b.a * (r1 - r2), b.mu));
+    return xi * (1.0 + cc / dd - cc / (dd + hc * hc));
+}
+
+// evaluate_pair_bond_order_gradient_tersoff (Potentials.f90:5806-5902): the
+// gradient of bo_term with respect to the partner's position (gp) and the
+// third atom's (gk); either may be null
+__device__ __forceinline__ void bo_grad(const double3 &s1, double r1, const double3 &s2,
+                                        double r2, const BondTerm &b, double3 *gp, double3 *gk)
+{
+    const double3 t1 = make_double3(-s1.x, -s1.y, -s1.z);
+    const double3 u1 = make_double3(t1.x / r1, t1.y / r1, t1.z / r1);
+    const double3 u2 = make_double3(s2.x / r2, s2.y / r2, s2.z / r2);
+    const double dot = t1.x * s2.x + t1.y * s2.y + t1.z * s2.z;
+    const double ratio = 1.0 / (r1 * r2);
+    const double cosine = dot * ratio;
+    const double cc = b.c * b.c, dd = b.d * b.d, hc = b.h - cosine;
+    const double decay = fc64(r2, b.cut, b.soft), dfc = dfc64(r2, b.cut, b.soft);
+    const double x = b.a * (r1 - r2);
+    const double e1a = pow(x, b.mu - 1.0);
+    const double ex = exp(x * e1a);
+    const double den = dd + hc * hc;
+    const double gee = 1.0 + cc / dd - cc / den;
+    const double dxi = decay * (ex * b.a * b.mu * e1a);   // d xi / d(r1 - r2) * decay
+    const double xi = ex * decay;
+    const double gc = -cc / (den * den) * 2.0 * hc * ratio;  // d gee / d cos, times ratio
+    if (gp) {
+        // d r1 / d r_partner = u1; d cos / d r_partner = t3 / (r1 r2)
+        const double k1 = dot / (r1 * r1);
+        const double3 t3 = make_double3(s2.x - k1 * t1.x, s2.y - k1 * t1.y, s2.z - k1 * t1.z);
+        *gp = make_double3(dxi * u1.x * gee + xi * gc * t3.x, dxi * u1.y * gee + xi * gc * t3.y,
+                           dxi * u1.z * gee + xi * gc * t3.z);
+    }
+    if (gk) {
+        // d r2 / d r_k = u2 (through f_c and through r1 - r2); d cos / d r_k = t4 / (r1 r2)
+        const double k2 = dot / (r2 * r2);
+        const double3 t4 = make_double3(t1.x - k2 * s2.x, t1.y - k2 * s2.y, t1.z - k2 * s2.z);
+        const double ex_ = ex;
+        *gk = make_double3((u2.x * dfc * ex_ - dxi * u2.x) * gee + xi * gc * t4.x,
+                           (u2.y * dfc * ex_ - dxi * u2.y) * gee + xi * gc * t4.y,
+                           (u2.z * dfc * ex_ - dxi * u2.z) * gee + xi * gc * t4.z);
+    }
+}
+
+// post_process_bond_order_factor_tersoff (Potentials.f90:5910-5921) and its
+// derivative (post_process_bond_order_gradient_tersoff on a unit gradient,
+// 5929-5961, with the zeta = 0 branch)
+__device__ __forceinline__ double bo_post(double zeta, const BondTerm &b)
+{
+    return pow(1.0 + pow(b.beta * zeta, b.eta), -1.0 / (2.0 * b.eta));
+}
+__device__ __forceinline__ double bo_dpost(double zeta, const BondTerm &b)
+{
+    if (b.eta == 0.0) return 0.0;
+    if (zeta == 0.0 && b.eta <= 1.0) return 0.0;
+    const double inv_eta = -1.0 / (2.0 * b.eta);
+    return inv_eta * pow(1.0 + pow(b.beta * zeta, b.eta), inv_eta - 1.0) * b.eta *
+           pow(b.beta * zeta, b.eta - 1.0) * b.beta;
+}
+
+// Each atom's neighbors within the bond-order pass's radius, widened by
+// 1e-10 relative so that a pair in one atom's list is in the other's.
+template <class Geo>
+__global__ __launch_bounds__(PB) void k_bop_list(int n, Geo geo, const double4 *__restrict__ pos,
+                                                 const int *__restrict__ nbr,
+                                                 const int *__restrict__ nimg,
+                                                 const int *__restrict__ ncnt, double radius,
+                                                 int *__restrict__ osj, int *__restrict__ osimg,
+                                                 int *__restrict__ oscnt)
+{
+    const int v = xcd_block(gridDim.x) * PB + threadIdx.x;
+    if (v >= n) return;
+    const double4 pv = pos[v];
+    const double *cv = geo.vec(v);
+    const size_t N = (size_t)n;
+    const double r2max = radius * radius * (1.0 + 1e-10);
+    const int nf = ncnt[v];
+    int c = 0;
+    for (int t = 0; t < nf; ++t) {
+        const int j = nbr[(size_t)t * N + v], img = nimg[(size_t)t * N + v];
+        if (!(dot64(separation64(pv, pos[j], img, cv)) < r2max)) continue;
+        osj[(size_t)c * N + v] = j;
+        osimg[(size_t)c * N + v] = img;
+        ++c;
+    }
+    oscnt[v] = c;
+}
+
+// Pass 1: G lanes per atom, lane l takes the partner slots l, l + G, ...
+template <class Geo, int MODE, int G>
+__global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 *__restrict__ pos,
+                                                  const int *__restrict__ id,
+                                                  const BopModel *__restrict__ BM,
+                                                  const PairModel *__restrict__ PM,
+                                                  const double2 *__restrict__ tab,
+                                                  const double *__restrict__ q,
+                                                  const int *__restrict__ osj,
+                                                  const int *__restrict__ osimg,
+                                                  const int *__restrict__ oscnt,
+                                                  double4 *__restrict__ oD, double *__restrict__ oW,
+                                                  double *__restrict__ oown,
+                                                  double *__restrict__ partial, int blk0)
+{
+    constexpr bool FORCE = MODE == PAIR_FORCES || MODE == PAIR_FORCES_ONLY;
+    constexpr bool CHI = MODE == BOP_CHI;
+    __shared__ double red[PB];
+    const int blk = xcd_block(gridDim.x);
+    const int gid = blk * PB + threadIdx.x;
+    const int i = gid / G, lane = gid % G;
+    const size_t N = (size_t)n;
+    double e = 0.0, own = 0.0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int si = (int)pi.w, idi = id[i];
+        const double qi = q[i];
+        const double *cv = geo.vec(i);
+        const double mb = BM->maxbond[si];
+        const int c = oscnt[i];
+        for (int t = lane; t < c; t += G) {
+            const int j = osj[(size_t)t * N + i], img = osimg[(size_t)t * N + i];
+            const double4 pj = pos[j];
+            const int sj = (int)pj.w;
+            const double3 d = separation64(pi, pj, img, cv);
+            const double r = sqrt(dot64(d));
+            double4 D = make_double4(0.0, 0.0, 0.0, 0.0);
+            double wk = 0.0;
+            if (BM->nbop[si][sj] && BM->pair[si][sj].cut > r && r > 0.0) {
+                const GenPair &T = BM->pair[si][sj];
+                const int g = BM->group[si][sj];
+                const double3 s1 = make_double3(-d.x, -d.y, -d.z);
+                double zeta = 0.0;
+                double3 gp = make_double3(0.0, 0.0, 0.0);
+                for (int u = 0; u < c; ++u) {
+                    const int k = osj[(size_t)u * N + i], imk = osimg[(size_t)u * N + i];
+                    const double4 pk = pos[k];
+                    const BondTerm &b = BM->bond[g][si][sj][(int)pk.w];
+                    if (!b.valid) continue;
+                    const double3 s2 = separation64(pi, pk, imk, cv);
+                    // the partner itself (the same vector) is not a third atom
+                    if (!(s1.x + s2.x != 0.0 || s1.y + s2.y != 0.0 || s1.z + s2.z != 0.0))
+                        continue;
+                    const double r2 = sqrt(dot64(s2));
+                    if (!(r2 < mb)) continue;
+                    zeta += bo_term(s1, r, s2, r2, b);
+                    if (FORCE) {
+                        double3 gpk;
+                        bo_grad(s1, r, s2, r2, b, &gpk, nullptr);
+                        gp.x += gpk.x;
+                        gp.y += gpk.y;
+                        gp.z += gpk.z;
+                    }
+                }
+                const BondTerm &pp = BM->post[g][si];
+                const double bt = pp.valid ? bo_post(zeta, pp) : zeta;
+                const bool first = pick_dev(idi, id[j], img);
+                const double qj = q[j];
+                const QPair qp = first ? QPair{qi, qj, si} : QPair{qj, qi, sj};
+                const double fc = T.smooth ? fc64(r, T.cut, T.soft) : 1.0;
+                if (CHI) {
+                    // potential_chi gives -dV/dq in the pair's orientation
+                    double xs[2];
+                    potential_chi(T, r, tab, PM->cf, qp, xs);
+                    own += -0.5 * fc * bt * (first ? xs[0] : xs[1]);
+                    D.w = -0.5 * fc * bt * (first ? xs[1] : xs[0]);
+                } else {
+                    const Radial pv = potential(T, r, tab, FORCE, PM->cf, qp);
+                    e += 0.5 * fc * pv.v * bt;
+                    if (FORCE) {
+                        // d(f_c V)/dr_ij = (G f_c + V f_c' / r) d: central (a
+                        // 'step' factor is refused for these potentials)
+                        double gs = pv.g * fc;
+                        if (T.smooth) gs += pv.v * dfc64(r, T.cut, T.soft) / r;
+                        const double dbt = pp.valid ? bo_dpost(zeta, pp) : 1.0;
+                        wk = 0.5 * fc * pv.v * dbt;
+                        D.x = 0.5 * bt * gs * d.x + wk * gp.x;
+                        D.y = 0.5 * bt * gs * d.y + wk * gp.y;
+                        D.z = 0.5 * bt * gs * d.z + wk * gp.z;
+                    }
+                }
+            }
+            if (FORCE || CHI) oD[(size_t)t * N + i] = D;
+            if (FORCE) oW[(size_t)t * N + i] = wk;
+        }
+    }
+    if (CHI) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) own += __shfl_xor(own, o);
+        if (i < n && lane == 0) oown[i] = own;
+        return;
+    }
+    if (MODE == PAIR_FORCES_ONLY) return;
+    const int tid = threadIdx.x;
+    red[tid] = e;
+    __syncthreads();
+    for (int w = PB / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid < 8) partial[(size_t)(blk0 + blk) * 8 + tid] = tid == 0 ? red[0] : 0.0;
+}
+
+// Pass 2: G lanes per atom, lane l takes the slots t = l, l + G, ... as the
+// third atom of every other bond of the atom; then the virial
+// -sum_t r_it (x) dU_i/dr_it.
+template <class Geo, int MODE, int G>
+__global__ __launch_bounds__(PB) void k_bop_third(int n, Geo geo, const double4 *__restrict__ pos,
+                                                  const BopModel *__restrict__ BM,
+                                                  const int *__restrict__ osj,
+                                                  const int *__restrict__ osimg,
+                                                  const int *__restrict__ oscnt,
+                                                  double4 *__restrict__ oD,
+                                                  const double *__restrict__ oW,
+                                                  double *__restrict__ partial, int blk0)
+{
+    constexpr bool VIR = MODE == PAIR_FORCES;
+    __shared__ double red[6][PB];
+    const int blk = xcd_block(gridDim.x);
+    const int gid = blk * PB + threadIdx.x;
+    const int i = gid / G, lane = gid % G;
+    const size_t N = (size_t)n;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int si = (int)pi.w;
+        const double *cv = geo.vec(i);
+        const double mb = BM->maxbond[si];
+        const int c = oscnt[i];
+        for (int t = lane; t < c; t += G) {
+            const int k = osj[(size_t)t * N + i], imk = osimg[(size_t)t * N + i];
+            const double4 pk = pos[k];
+            const int sk = (int)pk.w;
+            const double3 s2 = separation64(pi, pk, imk, cv);
+            const double r2 = sqrt(dot64(s2));
+            double4 D = oD[(size_t)t * N + i];
+            if (r2 < mb) {
+                for (int u = 0; u < c; ++u) {
+                    const double wj = oW[(size_t)u * N + i];
+                    if (wj == 0.0) continue;
+                    const int j = osj[(size_t)u * N + i], img = osimg[(size_t)u * N + i];
+                    const double4 pj = pos[j];
+                    const int sj = (int)pj.w;
+                    const BondTerm &b = BM->bond[BM->group[si][sj]][si][sj][sk];
+                    if (!b.valid) continue;
+                    const double3 d = separation64(pi, pj, img, cv);
+                    const double3 s1 = make_double3(-d.x, -d.y, -d.z);
+                    if (!(s1.x + s2.x != 0.0 || s1.y + s2.y != 0.0 || s1.z + s2.z != 0.0))
+                        continue;
+                    double3 gk;
+                    bo_grad(s1, sqrt(dot64(d)), s2, r2, b, nullptr, &gk);
+                    D.x += wj * gk.x;
+                    D.y += wj * gk.y;
+                    D.z += wj * gk.z;
+                }
+                oD[(size_t)t * N + i] = D;
+            }
+            if (VIR) {
+                w0 -= s2.x * D.x;
+                w1 -= s2.y * D.y;
+                w2 -= s2.z * D.z;
+                w3 -= s2.y * D.z;
+                w4 -= s2.x * D.z;
+                w5 -= s2.x * D.y;
+            }
+        }
+    }
+    if (MODE == PAIR_FORCES_ONLY) return;
+    const int tid = threadIdx.x;
+    red[0][tid] = w0; red[1][tid] = w1; red[2][tid] = w2;
+    red[3][tid] = w3; red[4][tid] = w4; red[5][tid] = w5;
+    __syncthreads();
+    for (int s = PB / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int q = 0; q < 6; ++q) red[q][tid] += red[q][tid + s];
+        __syncthreads();
+    }
+    if (tid < 8) partial[(size_t)(blk0 + blk) * 8 + tid] = (tid >= 1 && tid <= 6) ? red[tid - 1][0] : 0.0;
+}
+
+// F_a += sum_t dU_a/dr_at - sum of the reverse entries (F_a as in the
+// header); with CHI, chi_a += -dU_a/dq_a - sum of the reverse entries' w.
+// G lanes per atom split a's slots; shuffles sum them.
+template <int G, bool CHI>
+__global__ __launch_bounds__(PB) void k_bop_gather(int n, const int *__restrict__ osj,
+                                                   const int *__restrict__ osimg,
+                                                   const int *__restrict__ oscnt,
+                                                   const double4 *__restrict__ oD,
+                                                   const double *__restrict__ oown,
+                                                   double4 *__restrict__ frc,
+                                                   double4 *__restrict__ mom, double half_dt,
+                                                   int kick, double *__restrict__ chi)
+{
+    const int gid = xcd_block(gridDim.x) * PB + threadIdx.x;
+    const int a = gid / G, lane = gid % G;
+    const size_t N = (size_t)n;
+    double fx = 0.0, fy = 0.0, fz = 0.0, x = 0.0;
+    if (a < n) {
+        const int c = oscnt[a];
+        for (int t = lane; t < c; t += G) {
+            if (!CHI) {
+                const double4 own = oD[(size_t)t * N + a];
+                fx += own.x;
+                fy += own.y;
+                fz += own.z;
+            }
+            const int v = osj[(size_t)t * N + a];
+            const int want = neg_img(osimg[(size_t)t * N + a]);
+            const int cv = oscnt[v];
+            for (int u = 0; u < cv; ++u) {
+                if (osj[(size_t)u * N + v] == a && osimg[(size_t)u * N + v] == want) {
+                    const double4 g = oD[(size_t)u * N + v];
+                    fx -= g.x;
+                    fy -= g.y;
+                    fz -= g.z;
+                    x -= g.w;
+                    break;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o);
+        fy += __shfl_xor(fy, o);
+        fz += __shfl_xor(fz, o);
+        x += __shfl_xor(x, o);
+    }
+    if (a >= n || lane != 0) return;
+    if (CHI) {
+        chi[a] += x - oown[a];
+        return;
+    }
+    double4 f = frc[a];
+    f.x += fx;
+    f.y += fy;
+    f.z += fz;
+    frc[a] = f;
+    if (kick) {
+        double4 p = mom[a];
+        p.x += half_dt * f.x;
+        p.y += half_dt * f.y;
+        p.z += half_dt * f.z;
+        mom[a] = p;
+    }
+}
+
 }  // namespace
 
 void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
@@ -668,16 +1046,18 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
     const int kind = S.pmodel.lj_only ? (S.model.nspecies == 1 ? 2 : 1) : 0;
     const bool lanes = kind == 0 && S.n <= PAIR_LANES_MAX_ATOMS;
     const int nb = div_up(S.n * (lanes ? PAIR_LANES_SMALL : 1), PB);
-    // bond_bend angles add their partials after the pair blocks and their
-    // forces after the pair forces (then the half-kick, if asked)
+    // bond_bend angles, then the bond-order-scaled pairs, add their
+    // partials after the pair blocks and their forces after the pair forces
+    // (the last of them the half-kick, if asked)
     const int bb = bend_blocks(S);
-    S.gpartial.ensure((size_t)(nb + bb) * 8);
+    const int ob = bop_blocks(S, mode);
+    S.gpartial.ensure((size_t)(nb + bb + ob) * 8);
     S.gtotals.ensure(8);
-    S.gpartial_blocks = nb + bb;
+    S.gpartial_blocks = nb + bb + ob;
     hipEvent_t ev{};
     const char *name = mode == PAIR_ENERGY ? "pair_energy" : "pair_forces";
     S.timer.begin(name, S.stream, &ev);
-    const int pkick = bb ? 0 : kick;
+    const int pkick = (bb || ob) ? 0 : kick;
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         if (kind == 2) launch<Geo, 2, 1>(S, geo, mode, accumulate, half_dt, pkick);
@@ -686,9 +1066,10 @@ void pr_eval(System &S, int mode, bool accumulate, double half_dt, int kick)
         else launch<Geo, 0, 1>(S, geo, mode, accumulate, half_dt, pkick);
     });
     S.timer.end(name, S.stream, ev);
-    if (bb) bend_eval(S, mode, nb, half_dt, kick);
+    if (bb) bend_eval(S, mode, nb, half_dt, ob ? 0 : kick);
+    if (ob) bop_eval(S, mode, nb + bb, half_dt, kick);
     if (mode != PAIR_FORCES_ONLY) {
-        k_pair_totals<<<7, PB, 0, S.stream>>>(nb + bb, S.gpartial.p, S.gtotals.p);
+        k_pair_totals<<<7, PB, 0, S.stream>>>(nb + bb + ob, S.gpartial.p, S.gtotals.p);
         HIP_TRY(hipGetLastError());
     }
 }
@@ -704,5 +1085,90 @@ void pr_chi(System &S, double *chi)
                                                              S.nimg.p, S.ncnt.p, S.dpmodel.p,
                                                              S.tables.p, S.q.p, chi);
     });
+    HIP_TRY(hipGetLastError());
+    bop_chi(S, chi);
+}
+
+// lanes per atom of the bond-order passes: an oxide's few neighbors within
+// 3 A, one or two slots per lane
+constexpr int BOP_LANES = 8;
+
+int bop_blocks(const System &S, int mode)
+{
+    if (!S.bmodel.on) return 0;
+    const int nb = div_up(S.n * BOP_LANES, PB);
+    return mode == PAIR_ENERGY ? nb : 2 * nb;
+}
+
+static void bop_prepare(System &S)
+{
+    const int n = S.n;
+    const size_t cap = (size_t)S.cap * n;
+    S.osj.ensure(cap);
+    S.osimg.ensure(cap);
+    S.oscnt.ensure(n);
+    S.oD.ensure(cap);
+    S.oW.ensure(cap);
+    S.oown.ensure(n);
+    with_geo(S, [&](auto geo) {
+        k_bop_list<decltype(geo)><<<div_up(n, PB), PB, 0, S.stream>>>(
+            n, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.bmodel.radius, S.osj.p, S.osimg.p,
+            S.oscnt.p);
+    });
+    HIP_TRY(hipGetLastError());
+}
+
+void bop_eval(System &S, int mode, int blk0, double half_dt, int kick)
+{
+    if (!S.bmodel.on) return;
+    const int n = S.n, nb = div_up(n * BOP_LANES, PB);
+    hipEvent_t ev{};
+    S.timer.begin("bond_order_pairs", S.stream, &ev);
+    bop_prepare(S);
+    with_geo(S, [&](auto geo) {
+        using Geo = decltype(geo);
+        auto one = [&](auto kern) {
+            kern<<<nb, PB, 0, S.stream>>>(n, geo, S.pos.p, S.id.p, S.dbmodel.p, S.dpmodel.p,
+                                          S.tables.p, S.q.p, S.osj.p, S.osimg.p, S.oscnt.p,
+                                          S.oD.p, S.oW.p, S.oown.p, S.gpartial.p, blk0);
+        };
+        auto two = [&](auto kern) {
+            kern<<<nb, PB, 0, S.stream>>>(n, geo, S.pos.p, S.dbmodel.p, S.osj.p, S.osimg.p,
+                                          S.oscnt.p, S.oD.p, S.oW.p, S.gpartial.p, blk0 + nb);
+        };
+        if (mode == PAIR_ENERGY) {
+            one(k_bop_pairs<Geo, PAIR_ENERGY, BOP_LANES>);
+        } else if (mode == PAIR_FORCES) {
+            one(k_bop_pairs<Geo, PAIR_FORCES, BOP_LANES>);
+            two(k_bop_third<Geo, PAIR_FORCES, BOP_LANES>);
+        } else {
+            one(k_bop_pairs<Geo, PAIR_FORCES_ONLY, BOP_LANES>);
+            two(k_bop_third<Geo, PAIR_FORCES_ONLY, BOP_LANES>);
+        }
+    });
+    HIP_TRY(hipGetLastError());
+    if (mode != PAIR_ENERGY) {
+        k_bop_gather<BOP_LANES, false><<<div_up(n * BOP_LANES, PB), PB, 0, S.stream>>>(
+            n, S.osj.p, S.osimg.p, S.oscnt.p, S.oD.p, S.oown.p, S.frc.p, S.mom.p, half_dt, kick,
+            nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    S.timer.end("bond_order_pairs", S.stream, ev);
+}
+
+void bop_chi(System &S, double *chi)
+{
+    if (!S.bmodel.on) return;
+    const int n = S.n, nb = div_up(n * BOP_LANES, PB);
+    bop_prepare(S);
+    with_geo(S, [&](auto geo) {
+        k_bop_pairs<decltype(geo), BOP_CHI, BOP_LANES><<<nb, PB, 0, S.stream>>>(
+            n, geo, S.pos.p, S.id.p, S.dbmodel.p, S.dpmodel.p, S.tables.p, S.q.p, S.osj.p,
+            S.osimg.p, S.oscnt.p, S.oD.p, S.oW.p, S.oown.p, nullptr, 0);
+    });
+    HIP_TRY(hipGetLastError());
+    k_bop_gather<BOP_LANES, true><<<nb, PB, 0, S.stream>>>(n, S.osj.p, S.osimg.p, S.oscnt.p,
+                                                           S.oD.p, S.oown.p, nullptr, nullptr,
+                                                           0.0, 0, chi);
     HIP_TRY(hipGetLastError());
 }

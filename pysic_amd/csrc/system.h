@@ -106,6 +106,17 @@ struct System {
     // angles put on them
     DBuf<int> bsj, bsimg, bscnt;
     DBuf<double4> bG;
+    // bond-order-scaled pair potentials: the model, each atom's neighbors
+    // within its radius (column-major, capacity cap), per slot the
+    // derivative of the atom's site energy with respect to that bond vector
+    // (x, y, z) and to the neighbor's charge (w), and per atom the
+    // derivative with respect to its own charge
+    BopModel bmodel{};
+    DBuf<BopModel> dbmodel;
+    DBuf<int> osj, osimg, oscnt;
+    DBuf<double4> oD;
+    DBuf<double> oW;            // per slot: f_c V dB*/dzeta / 2 of the bond
+    DBuf<double> oown;
 
     // Batch of independent periodic cells (ncells > 0).  The atoms of cell c
     // occupy slots [hatom0[c], hatom0[c+1]) in every order: the bins of each
@@ -178,6 +189,11 @@ void pr_chi(System &S, double *chi);
 // [blk0, blk0 + blocks), forces added into frc (then p += half_dt F if kick)
 void bend_eval(System &S, int mode, int blk0, double half_dt, int kick);
 int bend_blocks(const System &S);
+// bond-order-scaled pair potentials: the same contract as bend_eval, and
+// their share of chi
+void bop_eval(System &S, int mode, int blk0, double half_dt, int kick);
+int bop_blocks(const System &S, int mode);
+void bop_chi(System &S, double *chi);
 
 // md.hip
 void md_drift(System &S, double dt, double half_dt);  // p += dt/2 F ; r += dt p/m (no rebuild)

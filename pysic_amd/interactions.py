@@ -688,8 +688,50 @@ def comb_bend_terms():
               parameters=[0.3122, 1.910611932, 1, 2])]
 
 
-def comb_sio2():
-    """COMB's local SiO2 terms with the reference builder's defaults, its
-    bond-order coordinators left out: the Si-O-Si and O-Si-O bending and
-    the charge-dependent repulsion and attraction (comb.py:630-990)."""
-    return comb_bend_terms() + comb_charge_model()
+# COMB's 'tersoff' bond-order parameters per center element with the
+# reference builder's defaults (comb.py:181-221, 279-319, 345-379): beta,
+# eta, mu, a, c, d, h; a Si-O bond takes the mean of the two a's.  Cutoffs
+# and margins per (center, partner, third) triplet.
+COMB_BOND = {"Si": dict(beta=1.0999e-6, eta=0.78734, mu=3.0, a=1.7322, c=100390.0, d=16.218,
+                        h=-0.59826),
+             "O": dict(beta=2.0, eta=1.0, mu=1.0, a=2.68, c=6.6, d=1.0, h=-0.229)}
+COMB_BOND_CUT = {("Si", "Si", "O"): (3.05, 0.5), ("Si", "Si", "Si"): (3.0, 0.2),
+                 ("O", "O", "Si"): (3.05, 0.5), ("O", "O", "O"): (3.0, 0.4),
+                 ("O", "Si", "Si"): (3.05, 0.5), ("O", "Si", "O"): (3.05, 0.5),
+                 ("Si", "O", "Si"): (3.05, 0.5), ("Si", "O", "O"): (3.05, 0.5)}
+
+
+def comb_coordinators():
+    """The three coordinators of COMB's attractive terms, built as
+    comb.py:703-729 (Si-Si: Si-Si-O, Si-Si-Si), 773-800 (O-O: O-O-Si,
+    O-O-O) and 846-895 (Si-O: O-Si-Si, O-Si-O, Si-O-O, Si-O-Si) do."""
+    B = COMB_BOND
+    a_sio = 0.5 * (B["Si"]["a"] + B["O"]["a"])
+
+    def bond(triplet, a=None):
+        p = B[triplet[0]]
+        cut, margin = COMB_BOND_CUT[triplet]
+        return BondOrderParameters("tersoff", cutoff=cut, cutoff_margin=margin,
+                                   parameters=[[p["beta"], p["eta"]],
+                                               [p["a"] if a is None else a, p["c"], p["d"],
+                                                p["h"], p["mu"]]],
+                                   symbols=[list(triplet)])
+    sisi = Coordinator([bond(("Si", "Si", "O")), bond(("Si", "Si", "Si"))])
+    oo = Coordinator([bond(("O", "O", "Si")), bond(("O", "O", "O"))])
+    sio = Coordinator([bond(("O", "Si", "Si"), a_sio), bond(("O", "Si", "O"), a_sio),
+                       bond(("Si", "O", "O"), a_sio), bond(("Si", "O", "Si"), a_sio)])
+    return sisi, oo, sio
+
+
+def comb_sio2(coordinators=True):
+    """COMB's local SiO2 terms with the reference builder's defaults
+    (comb.py:630-990): the Si-O-Si and O-Si-O bending, the charge-dependent
+    attraction -- each attractive product scaled by its coordinator's
+    bond-order factor -- and the charge-dependent repulsion.  With
+    coordinators=False the attraction is unscaled."""
+    pots = comb_bend_terms() + comb_charge_model()
+    if coordinators:
+        # comb_charge_model's first three products: attraction Si-Si, O-O, Si-O
+        for prod, coord in zip(pots[2:5], comb_coordinators()):
+            prod.set_coordinator(coord)
+    return pots

@@ -5,6 +5,7 @@ first with oracle/build_ref.sh, then
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py pairs    # the pair-potential fixtures
+    python tests/golden/make_golden.py comb     # COMB with its coordinators
 
 Every fixture is data only (inputs and the reference's outputs).  The
 system set-up replays pysic/calculator.py:1595-1637 through the reference's
@@ -204,6 +205,35 @@ def comb_bend():
         print(name, len(s), "E = %.12f" % e)
 
 
+def comb_full():
+    """COMB as the reference's builder makes it (comb.py:630-990): bending,
+    charge-dependent repulsion and attraction, each attractive product
+    scaled by its coordinator's 'tersoff' bond-order factors in three
+    groups, on amorphous SiO2 with random charges: energy, forces, stress
+    and electronegativities."""
+    from helpers import comb_charges, ref_bonds, ref_entries
+    from pysic_amd.interactions import comb_sio2
+    for name, n_si, seed in (("sio2_48_comb_full", 16, 27), ("sio2_96_comb_full", 32, 29)):
+        s = st.amorphous_sio2(n_si, 2.2, seed=seed)
+        q = comb_charges(s.symbols, seed=seed + 1)
+        names = sorted(set(s.symbols))
+        pd, bd, ng = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+        sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
+        cuts = np.full(len(s), 3.05 + 0.5)
+        R = O.RefCore()
+        R.setup(s.symbols, s.positions, s.cell, s.pbc, charges=q,
+                potentials=ref_entries(pd, names), tersoff=ref_bonds(bd, names), ngroups=ng,
+                cutoffs=cuts)
+        e = R.energy()
+        f, w = R.forces()
+        x = R.electronegativities()
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), positions=s.positions,
+                            cell=s.cell, pbc=s.pbc.astype(np.int32), species=sp,
+                            symbols=np.array(s.symbols), charges=q, potdesc=pd, bonddesc=bd,
+                            cutoffs=cuts, energy=e, forces=f, stress=w, enegs=x)
+        print(name, len(s), "E = %.12f" % e)
+
+
 def pair_cases():
     from helpers import MIXTURE_TABLE_ID, mixture_model, mixture_table
     from pysic_amd.interactions import lj_argon
@@ -280,6 +310,9 @@ def main():
     if sys.argv[1:] == ["pairs"]:
         pair_cases()
         return
+    if sys.argv[1:] == ["comb"]:
+        comb_full()
+        return
     # Tersoff Si, the parity configuration at reduced and full size
     run_tersoff("si_diamond_2x2x2_s005", st.diamond_si((2, 2, 2), sigma=0.05, seed=1))
     run_tersoff("si_diamond_4x4x4_s005", st.diamond_si((4, 4, 4), sigma=0.05, seed=0))
@@ -306,6 +339,7 @@ def main():
     comb_charge_pairs()
     comb_charge_pairs("sio2_48_charge_terms_asymmetric", asymmetric=True, seed=23)
     comb_bend()
+    comb_full()
     # EEM: small random SiO2 with charge_self + Ewald and random charges
     sio2 = st.amorphous_sio2(16, 2.2, seed=2)
     rng = np.random.default_rng(8)

@@ -169,6 +169,16 @@ def ref_entries(pd, names):
     return out
 
 
+def ref_bonds(bd, names):
+    """RefCore 'tersoff' entries (params7, cutoff, soft, symbols, group) from
+    engine bond-order descriptor rows (14 doubles each)."""
+    out = []
+    for r in np.asarray(bd, dtype=np.float64).reshape(-1, 14):
+        out.append((list(r[3:10]), float(r[10]), float(r[11]),
+                    tuple(names[int(e)] for e in r[0:3]), int(r[12])))
+    return out
+
+
 # Two-species Tersoff (Si, C; J. Tersoff, PRB 39, 5566 (1989) element
 # parameters).  Pair terms mix as A = sqrt(A_i A_j), lambda = (l_i + l_j)/2,
 # R = sqrt(R_i R_j), S = sqrt(S_i S_j); the bond-order factor of triplet

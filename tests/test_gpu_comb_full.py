@@ -1,0 +1,143 @@
+"""GPU parity of COMB as the reference's builder makes it (comb.py:630-990):
+the bond-bending and charge-dependent repulsion, and the charge-dependent
+attraction products scaled by their coordinators' 'tersoff' bond-order
+factors in three groups (Si-Si, O-O, Si-O; calculator.py:1217-1233,
+Core.f90:5469-5488, 5688-5729).
+
+Contract (FP64 on the device): |dE|/N <= 1e-9 eV; forces, virial and
+electronegativities max|d| / max|ref| <= 1e-9 against the reference core's
+golden vectors and the oracle; the pair set of the lists bit-exact; MD with the
+full model follows velocity Verlet on the oracle's forces to 1e-9.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from helpers import assert_same_pairs, comb_charges, rel_maxnorm
+from pysic_amd import structures as st
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from pysic_amd.engine import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def load(eng, sp, pos, q, cell, pd, bd, cuts, momenta=None, masses=None):
+    eng.create_atoms(sp, pos, charges=q, momenta=momenta, masses=masses)
+    eng.create_cell(cell, (True, True, True))
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(bd)
+    eng.ewald_off()
+    eng.generate_neighbor_lists(cuts)
+
+
+def compare(eng, ref, n, what):
+    e_ref, f_ref, w_ref, x_ref = ref
+    e = eng.energy()
+    f, w = eng.forces()
+    x = eng.electronegativities()
+    print("  %s: |dE|/N %.2e  rel|dF| %.2e  rel|dW| %.2e  rel|dchi| %.2e" % (
+        what, abs(e - e_ref) / n, rel_maxnorm(f, f_ref), rel_maxnorm(w, w_ref),
+        rel_maxnorm(x, x_ref)), flush=True)
+    assert abs(e - e_ref) / n <= 1e-9
+    assert rel_maxnorm(f, f_ref) <= 1e-9
+    assert rel_maxnorm(w, w_ref) <= 1e-9
+    assert rel_maxnorm(x, x_ref) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["sio2_48_comb_full", "sio2_96_comb_full"])
+def test_comb_full_golden(eng, oracle, name):
+    g = load_golden(name)
+    print("case: " + name, flush=True)
+    load(eng, g["species"], g["positions"], g["charges"], g["cell"], g["potdesc"], g["bonddesc"],
+         g["cutoffs"])
+    nl = oracle.neighbor_list(g["positions"], g["cell"], [1, 1, 1], g["cutoffs"])
+    assert_same_pairs(eng, nl)
+    compare(eng, (float(g["energy"]), g["forces"], g["stress"], g["enegs"]),
+            len(g["positions"]), "golden")
+
+
+def _comb_case(n_si, seed):
+    from pysic_amd.interactions import build_descriptors, comb_sio2
+    s = st.amorphous_sio2(n_si, 2.2, seed=seed)
+    q = comb_charges(s.symbols, seed=seed + 1)
+    names = sorted(set(s.symbols))
+    pd, bd, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+    sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
+    return s, q, sp, pd, bd
+
+
+@pytest.mark.parametrize("n_si,seed", [(256, 61), (2048, 63)])
+def test_comb_full_vs_oracle(eng, oracle, n_si, seed):
+    s, q, sp, pd, bd = _comb_case(n_si, seed)
+    n = len(s)
+    print("case: COMB with coordinators, %d atoms, seed %d" % (n, seed), flush=True)
+    cuts = np.full(n, 3.55)
+    load(eng, sp, s.positions, q, s.cell, pd, bd, cuts)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], cuts)
+    assert_same_pairs(eng, nl)
+    args = (sp, s.positions, q, s.cell, nl, pd, bd)
+    f, w = oracle.evaluate("forces", *args)
+    compare(eng, (oracle.evaluate("energy", *args), f, w, oracle.evaluate("enegs", *args)), n,
+            "oracle")
+
+
+def test_comb_full_through_the_calculator(oracle):
+    # pysic's construction, as a script writes it: Potential and
+    # ProductPotential objects with Coordinators on a Pysic calculator
+    from pysic_amd import Pysic
+    from pysic_amd.interactions import comb_sio2
+    s, q, sp, pd, bd = _comb_case(64, 71)
+    s.set_initial_charges(q)
+    pots = comb_sio2()
+    calc = Pysic(s, pots)
+    e = calc.get_potential_energy()
+    f = calc.get_forces()
+    x = calc.get_electronegativities()
+    n = len(s)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, 3.55))
+    args = (sp, s.positions, q, s.cell, nl, pd, bd)
+    e_ref = oracle.evaluate("energy", *args)
+    f_ref, _ = oracle.evaluate("forces", *args)
+    x_ref = oracle.evaluate("enegs", *args)
+    print("case: calculator, 192 atoms: |dE|/N %.2e rel|dF| %.2e rel|dchi| %.2e" % (
+        abs(e - e_ref) / n, rel_maxnorm(f, f_ref), rel_maxnorm(x, x_ref)), flush=True)
+    assert abs(e - e_ref) / n <= 1e-9
+    assert rel_maxnorm(f, f_ref) <= 1e-9 and rel_maxnorm(x, x_ref) <= 1e-9
+    # the coordinators' groups, as pysic numbers them (calculator.py:1217-1233)
+    groups = [p.get_coordinator().get_group_index() for p in pots
+              if p.get_coordinator() is not None]
+    assert groups == [0, 1, 2]
+
+
+def test_comb_full_md_matches_oracle(eng, oracle):
+    """The full model through the MD loop (10 steps of 0.5 fs, lists rebuilt
+    every step) against velocity Verlet on the oracle's forces."""
+    s, q, sp, pd, bd = _comb_case(32, 81)
+    st.maxwell_boltzmann(s, 300.0, seed=82)
+    n = len(s)
+    cuts = np.full(n, 3.55)
+    print("case: COMB with coordinators, %d atoms, 10 VV steps of 0.5 fs" % n, flush=True)
+    load(eng, sp, s.positions, q, s.cell, pd, bd, cuts, momenta=s.momenta, masses=s.masses)
+    dt, steps = 0.5 * st.ASE_FS, 10
+    eng.md_run(steps, dt, rebuild=1)
+    pos_g, mom_g = eng.state()
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cuts)
+    f, _ = oracle.evaluate("forces", sp, r, q, s.cell, nl, pd, bd)
+    for _ in range(steps):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cuts)
+        f, _ = oracle.evaluate("forces", sp, r, q, s.cell, nl, pd, bd)
+        p = p + 0.5 * dt * f
+    dr, dp = np.abs(pos_g - r).max(), np.abs(mom_g - p).max() / np.abs(p).max()
+    print("  |dr| %.2e A  rel|dp| %.2e" % (dr, dp), flush=True)
+    assert dr < 1e-9
+    assert dp < 1e-9
