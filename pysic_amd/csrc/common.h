@@ -209,17 +209,74 @@ __device__ __forceinline__ double recip64(double x)
     return fma(y, e, y);
 }
 
-// smoothening_factor and smoothening_derivative (Potentials.f90:442-509)
+// sin and cos of (pi/2) x for 0 <= x <= 1.  The angle is moved to
+// w = (pi/2) x - pi/4 in [-pi/4, pi/4], where the Taylor series of sin w
+// (through w^17) and cos w (through w^18) leave a truncation error below
+// 1e-19, and turned back by pi/4:
+//   sin(pi/4 + w) = (cos w + sin w) / sqrt(2),
+//   cos(pi/4 + w) = (cos w - sin w) / sqrt(2).
+// No range reduction is needed, unlike the library sin/cos.
+__device__ __forceinline__ void sincos_quarter64(double x, double *s, double *c)
+{
+    const double w = fma(x, 1.5707963267948966, -0.78539816339744831);
+    const double w2 = w * w;
+    double ps = 2.8114572543455206e-15;          // 1/17!
+    ps = fma(ps, w2, -7.647163731819816e-13);    // -1/15!
+    ps = fma(ps, w2, 1.6059043836821613e-10);    // 1/13!
+    ps = fma(ps, w2, -2.505210838544172e-08);    // -1/11!
+    ps = fma(ps, w2, 2.7557319223985893e-06);    // 1/9!
+    ps = fma(ps, w2, -0.0001984126984126984);    // -1/7!
+    ps = fma(ps, w2, 0.008333333333333333);      // 1/5!
+    ps = fma(ps, w2, -0.16666666666666666);      // -1/3!
+    const double sw = fma(ps * w2, w, w);
+    double pc = -1.5619206968586225e-16;         // -1/18!
+    pc = fma(pc, w2, 4.779477332387385e-14);     // 1/16!
+    pc = fma(pc, w2, -1.1470745597729725e-11);   // -1/14!
+    pc = fma(pc, w2, 2.08767569878681e-09);      // 1/12!
+    pc = fma(pc, w2, -2.755731922398589e-07);    // -1/10!
+    pc = fma(pc, w2, 2.48015873015873e-05);      // 1/8!
+    pc = fma(pc, w2, -0.001388888888888889);     // -1/6!
+    pc = fma(pc, w2, 4.1666666666666667e-02);    // 1/4!
+    pc = fma(pc, w2, -0.5);                      // -1/2!
+    const double cw = fma(pc, w2, 1.0);
+    *s = (cw + sw) * 0.70710678118654752;
+    *c = (cw - sw) * 0.70710678118654752;
+}
+
+// smoothening_factor and smoothening_derivative (Potentials.f90:442-509):
+// f = (1 + cos(pi t)) / 2 = cos^2(pi t / 2) and
+// df/dr = -(pi / (2 w)) sin(pi t) = -(pi / w) sin(pi t / 2) cos(pi t / 2),
+// t = (r - soft) / w, w = hard - soft; both from one sincos_quarter64.
+__device__ __forceinline__ void smooth64(double r, double hard, double soft, double *f,
+                                         double *df)
+{
+    if (r > hard) { *f = 0.0; *df = 0.0; return; }
+    if (r < soft) { *f = 1.0; *df = 0.0; return; }
+    const double iw = recip64(hard - soft);
+    double s, c;
+    sincos_quarter64((r - soft) * iw, &s, &c);
+    *f = c * c;
+    *df = -M_PI * iw * s * c;
+}
 __device__ __forceinline__ double fc64(double r, double hard, double soft)
 {
-    if (r > hard) return 0.0;
-    if (r < soft) return 1.0;
-    return 0.5 * (1.0 + cos(M_PI * (r - soft) / (hard - soft)));
+    double f, df;
+    smooth64(r, hard, soft, &f, &df);
+    return f;
 }
 __device__ __forceinline__ double dfc64(double r, double hard, double soft)
 {
-    if (r > hard || r < soft) return 0.0;
-    return -0.5 * M_PI / (hard - soft) * sin(M_PI * (r - soft) / (hard - soft));
+    double f, df;
+    smooth64(r, hard, soft, &f, &df);
+    return df;
+}
+
+// x^y for the bond-order post-processing, where x >= 0: exp(y log x) for
+// x > 0 (a few ulp for |y log x| of order one, a third of the library pow's
+// instructions); x <= 0 keeps pow's special cases.
+__device__ __forceinline__ double powpos64(double x, double y)
+{
+    return x > 0.0 ? exp(y * log(x)) : pow(x, y);
 }
 
 // F_i = sum_j (D_ij - D_ji) over atom i's short list: D_ij = dU_i/dr_ij is

@@ -63,13 +63,14 @@ __device__ __forceinline__ double zeta_term64(const double4 &u, const double4 &v
 __device__ __forceinline__ void bt64(double zeta, const BondTerm &p, double *bt, double *dbt)
 {
     double be = p.beta * zeta;
-    double pw = pow(be, p.eta);
+    double pw = powpos64(be, p.eta);
     double inv_eta = -1.0 / (2.0 * p.eta);
-    *bt = pow(1.0 + pw, inv_eta);
+    *bt = powpos64(1.0 + pw, inv_eta);
     if (p.eta == 0.0 || (zeta == 0.0 && p.eta <= 1.0))
         *dbt = 0.0;
     else
-        *dbt = inv_eta * pow(1.0 + pw, inv_eta - 1.0) * p.eta * pow(be, p.eta - 1.0) * p.beta;
+        *dbt = inv_eta * powpos64(1.0 + pw, inv_eta - 1.0) * p.eta * powpos64(be, p.eta - 1.0) *
+               p.beta;
 }
 
 template <int G, int SCAP>
@@ -188,8 +189,8 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
                 if (!(pt.cut > u.w)) continue;
                 double v = pt.eps * exp(-pt.zeta * u.w);
                 double dv = -pt.zeta * v;
-                double f = pt.smooth ? fc64(u.w, pt.cut, pt.soft) : 1.0;
-                double df = pt.smooth ? dfc64(u.w, pt.cut, pt.soft) : 0.0;
+                double f = 1.0, df = 0.0;
+                if (pt.smooth) smooth64(u.w, pt.cut, pt.soft, &f, &df);
                 if (pt.group >= 0) {
                     eh += 0.5 * v * f * bt;
                     dp += 0.5 * (dv * f + v * df) * bt;
@@ -385,7 +386,7 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
         if (keep && slot < SCAP) {
             bool inb = bond_on && d.w < maxbond;
             s_sep[slot][a] = d;
-            s_fi[slot][a] = make_double2(inb ? fc64(d.w, B.cut, B.soft) : 0.0, 1.0 / d.w);
+            s_fi[slot][a] = make_double2(inb ? fc64(d.w, B.cut, B.soft) : 0.0, recip64(d.w));
         }
         cnt += __popcll(gb);
     }
@@ -412,11 +413,11 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
                 double t2 = t * t;
                 double xi = fk.x;
                 if (!B.a_zero) xi *= exp(pow(B.a * (u.w - v.w), B.mu));
-                zeta += xi * (1.0 + cd2 * (t2 / (dd + t2)));
+                zeta += xi * (1.0 + cd2 * (t2 * recip64(dd + t2)));
             }
         }
         double bt = 1.0;
-        if (bond_on) bt = pow(1.0 + pow(B.beta * zeta, B.eta), -1.0 / (2.0 * B.eta));
+        if (bond_on) bt = powpos64(1.0 + powpos64(B.beta * zeta, B.eta), -1.0 / (2.0 * B.eta));
         if (u.w < maxpot) {
             for (int t = 0; t < npair; ++t) {
                 const PairTerm pt = M->pair[0][0][t];
@@ -732,17 +733,18 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                     const PairTerm pt = M->pair[0][0][t];
                     if (!(pt.cut > r)) continue;
                     double v = pt.eps * exp(-pt.zeta * r);
-                    double f = pt.smooth ? fc64(r, pt.cut, pt.soft) : 1.0;
-                    double df = pt.smooth ? dfc64(r, pt.cut, pt.soft) : 0.0;
+                    double f = 1.0, df = 0.0;
+                    if (pt.smooth) smooth64(r, pt.cut, pt.soft, &f, &df);
                     double dv = 0.5 * (-pt.zeta * v * f + v * df);
                     if (pt.group >= 0) { dA += dv; A += 0.5 * v * f; }
                     else dR += dv;
                 }
             }
             const double ir = recip64(r);  // r > 0: the list holds no atom at its own position
+            double fb = 0.0, dfb = 0.0;
+            if (inb) smooth64(r, B.cut, B.soft, &fb, &dfb);
             s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
-            s_aux[slot][a] = make_float4((float)ir, inb ? (float)fc64(r, B.cut, B.soft) : 0.f,
-                                         inb ? (float)dfc64(r, B.cut, B.soft) : 0.f, (float)A);
+            s_aux[slot][a] = make_float4((float)ir, (float)fb, (float)dfb, (float)A);
             s_pair[slot][a] = make_double2(dR * ir, dA * ir);  // divided by r once
             if (!unique) {
                 size_t o = (size_t)slot * n + i;
