@@ -744,7 +744,7 @@ void batch_sequence(pysic_ctx *h, bool rebuild)
     }
     ts_site(S, h->variant, SITE_FORCES);
     ts_gather(S, 0.0, 0);
-    ts_site(S, h->variant, SITE_ENERGY);
+    ts_site(S, h->variant, SITE_ENERGY_SHORT);  // over the force pass's short list
     ts_cell_sums(S);
 }
 

@@ -162,7 +162,9 @@ void nb_clear_flags(System &S);               // zero the flags (kernel; graph-s
 // SITE_ENERGY: energy only (FP64); SITE_FORCES: dU/dr_ij + virial (the
 // multi-species kernel computes all three in either mode)
 // SITE_FORCES_ONLY: dU/dr_ij without the virial (a velocity-Verlet step)
-enum { SITE_ENERGY = 1, SITE_FORCES = 2, SITE_FORCES_ONLY = 3 };
+// SITE_ENERGY_SHORT: SITE_ENERGY over the short list a SITE_FORCES pass on
+// the same positions and variant left (one species; else as SITE_ENERGY)
+enum { SITE_ENERGY = 1, SITE_FORCES = 2, SITE_FORCES_ONLY = 3, SITE_ENERGY_SHORT = 4 };
 void ts_site(System &S, int variant, int mode);
 void ts_gather(System &S, double half_dt, int kick);  // forces (+ optional momentum kick)
 void ts_reduce(System &S);                    // totals[0..6] = E, W(6)

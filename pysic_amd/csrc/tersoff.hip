@@ -333,12 +333,19 @@ __device__ __forceinline__ void site_tile(int blk, int n, const Geo &geo,
 //             virial in FP64.  A velocity-Verlet step needs exactly this, as
 //             calculate_forces returns forces and stress (PyInterface.f90:668).
 // ---------------------------------------------------------------------------
-template <int G, int SCAP, class Geo>
+//
+// k_site1e with SHORT reads the short list a k_site1f pass on the same
+// positions left (scnt, srow and the images in simg) instead of compacting
+// the full list again: the same slots in the same order, so the same sums.
+template <int G, int SCAP, class Geo, bool SHORT>
 __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
                                             const double4 *__restrict__ pos,
                                             const int *__restrict__ nbr,
                                             const int *__restrict__ nimg,
                                             const int *__restrict__ ncnt,
+                                            const int *__restrict__ scnt,
+                                            const int *__restrict__ srow,
+                                            const int *__restrict__ simg,
                                             const Model *__restrict__ M,
                                             double *__restrict__ partial,
                                             double *__restrict__ atom_ev,
@@ -366,32 +373,45 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
     const double dd = B.d * B.d, cd2 = B.c * B.c / dd;
 
     double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
-    int nf = valid ? ncnt[i] : 0;
-    int cnt = 0;
-    for (int base = 0; base < nf; base += G) {
-        int e = base + l;
-        bool keep = false;
-        double4 d = make_double4(0, 0, 0, 0);
-        if (e < nf) {
-            int j = nbr[(size_t)e * n + i];
-            int im = nimg[(size_t)e * n + i];
-            double3 sp3 = separation64(pi, pos[j], im, cv);
-            double r = sqrt(dot64(sp3));
-            d = make_double4(sp3.x, sp3.y, sp3.z, r);
-            keep = r < rshort;
+    int ns;
+    if constexpr (SHORT) {
+        ns = valid ? scnt[i] : 0;
+        for (int e = l; e < ns; e += G) {
+            const int j = srow[(size_t)i * SCAP + e];
+            const double3 sp3 = separation64(pi, pos[j], simg[(size_t)e * n + i], cv);
+            const double r = sqrt(dot64(sp3));
+            const bool inb = bond_on && r < maxbond;
+            s_sep[e][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
+            s_fi[e][a] = make_double2(inb ? fc64(r, B.cut, B.soft) : 0.0, recip64(r));
         }
-        unsigned long long b = __ballot(keep);
-        unsigned long long gb = (b >> gbase) & gmask;
-        int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
-        if (keep && slot < SCAP) {
-            bool inb = bond_on && d.w < maxbond;
-            s_sep[slot][a] = d;
-            s_fi[slot][a] = make_double2(inb ? fc64(d.w, B.cut, B.soft) : 0.0, recip64(d.w));
+    } else {
+        int nf = valid ? ncnt[i] : 0;
+        int cnt = 0;
+        for (int base = 0; base < nf; base += G) {
+            int e = base + l;
+            bool keep = false;
+            double4 d = make_double4(0, 0, 0, 0);
+            if (e < nf) {
+                int j = nbr[(size_t)e * n + i];
+                int im = nimg[(size_t)e * n + i];
+                double3 sp3 = separation64(pi, pos[j], im, cv);
+                double r = sqrt(dot64(sp3));
+                d = make_double4(sp3.x, sp3.y, sp3.z, r);
+                keep = r < rshort;
+            }
+            unsigned long long b = __ballot(keep);
+            unsigned long long gb = (b >> gbase) & gmask;
+            int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
+            if (keep && slot < SCAP) {
+                bool inb = bond_on && d.w < maxbond;
+                s_sep[slot][a] = d;
+                s_fi[slot][a] = make_double2(inb ? fc64(d.w, B.cut, B.soft) : 0.0, recip64(d.w));
+            }
+            cnt += __popcll(gb);
         }
-        cnt += __popcll(gb);
+        if (cnt > SCAP && l == 0) atomicMax(flags + 1, cnt);
+        ns = cnt < SCAP ? cnt : SCAP;
     }
-    if (cnt > SCAP && l == 0) atomicMax(flags + 1, cnt);
-    const int ns = cnt < SCAP ? cnt : SCAP;
     __syncthreads();
 
     double e_lane = 0.0;
@@ -746,11 +766,10 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
             s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
             s_aux[slot][a] = make_float4((float)ir, (float)fb, (float)dfb, (float)A);
             s_pair[slot][a] = make_double2(dR * ir, dA * ir);  // divided by r once
-            if (!unique) {
-                size_t o = (size_t)slot * n + i;
-                sj[o] = j;
-                simg[o] = im;
-            }
+            // the images also with unique images: an energy pass reads them
+            const size_t o = (size_t)slot * n + i;
+            if (!unique) sj[o] = j;
+            simg[o] = im;
             srow[(size_t)i * SCAP + slot] = j;
         }
         cnt += __popcll(gb);
@@ -923,12 +942,15 @@ __global__ __launch_bounds__(BLOCK) void k_site(int n, int ntiles, Geo geo,
     }
 }
 
-template <int G, int SCAP, class Geo>
+template <int G, int SCAP, class Geo, bool SHORT>
 __global__ __launch_bounds__(BLOCK) void k_site1e(int n, int ntiles, Geo geo,
                                                   const double4 *__restrict__ pos,
                                                   const int *__restrict__ nbr,
                                                   const int *__restrict__ nimg,
                                                   const int *__restrict__ ncnt,
+                                                  const int *__restrict__ scnt,
+                                                  const int *__restrict__ srow,
+                                                  const int *__restrict__ simg,
                                                   const Model *__restrict__ M,
                                                   double *__restrict__ partial,
                                                   double *__restrict__ atom_ev,
@@ -937,7 +959,8 @@ __global__ __launch_bounds__(BLOCK) void k_site1e(int n, int ntiles, Geo geo,
     int t0, t1;
     tile_range<Geo>(ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
-        site1e_tile<G, SCAP>(t, n, geo, pos, nbr, nimg, ncnt, M, partial, atom_ev, flags);
+        site1e_tile<G, SCAP, Geo, SHORT>(t, n, geo, pos, nbr, nimg, ncnt, scnt, srow, simg, M,
+                                         partial, atom_ev, flags);
         __syncthreads();
     }
 }
@@ -969,27 +992,36 @@ __global__ __launch_bounds__(BLOCK) void k_site1f(int n, int ntiles, Geo geo,
     }
 }
 
-// Per-cell sums of the per-atom energies and virials (one block per cell,
-// fixed order).
-__global__ __launch_bounds__(256) void k_cell_sums(int n, const int *__restrict__ atom0,
-                                                   const double *__restrict__ atom_ev, int ncomp0,
-                                                   int ncomp1, double *__restrict__ cell_ev)
+// Per-cell sums of the per-atom energies and virials: one block per cell,
+// one wave per component.  Each lane keeps 8 partial sums over a stride of
+// 512 atoms (8 loads in flight), adds them in a fixed order, and a butterfly
+// sums the wave: a fixed order throughout, no barriers.
+constexpr int CELL_SUM_COMPONENTS = 7;
+__global__ __launch_bounds__(64 * CELL_SUM_COMPONENTS) void k_cell_sums(
+    int n, const int *__restrict__ atom0, const double *__restrict__ atom_ev,
+    double *__restrict__ cell_ev)
 {
-    __shared__ double red[256];
+    constexpr int U = 8;
     const int c = blockIdx.x;
+    const int q = threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
     const int a0 = atom0[c], a1 = atom0[c + 1];
-    for (int q = ncomp0; q < ncomp1; ++q) {
-        double s = 0.0;
-        for (int i = a0 + threadIdx.x; i < a1; i += 256) s += atom_ev[(size_t)q * n + i];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+    const double *v = atom_ev + (size_t)q * n;
+    double p[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) p[u] = 0.0;
+    for (int b = a0 + lane; b < a1; b += 64 * U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = b + 64 * u;
+            p[u] += i < a1 ? v[i] : 0.0;
         }
-        if (threadIdx.x == 0) cell_ev[(size_t)c * 8 + q] = red[0];
-        __syncthreads();
     }
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += p[u];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) cell_ev[(size_t)c * 8 + q] = s;
 }
 
 // F_i = sum_j (D_ij - D_ji); D_ji found in j's short list at (i, -o).
@@ -1068,9 +1100,13 @@ void launch_site(System &S, int mode)
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
         if (S.model.nspecies == 1 && mode == SITE_ENERGY)
-            k_site1e<G, SCAP, Geo><<<grid, BLOCK, 0, S.stream>>>(
-                S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.partial.p,
-                aev, S.flags.p);
+            k_site1e<G, SCAP, Geo, false><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, nullptr, nullptr, nullptr,
+                S.dmodel.p, S.partial.p, aev, S.flags.p);
+        else if (S.model.nspecies == 1 && mode == SITE_ENERGY_SHORT)
+            k_site1e<G, SCAP, Geo, true><<<grid, BLOCK, 0, S.stream>>>(
+                S.n, ntiles, geo, S.pos.p, nullptr, nullptr, nullptr, S.scnt.p, S.srow.p,
+                S.simg.p, S.dmodel.p, S.partial.p, aev, S.flags.p);
         else if (S.model.nspecies == 1 && mode == SITE_FORCES)
             k_site1f<G, SCAP, Geo, true><<<grid, BLOCK, 0, S.stream>>>(
                 S.n, ntiles, geo, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, S.sj.p,
@@ -1111,7 +1147,8 @@ void ts_site(System &S, int variant, int mode)
     S.dU.ensure((size_t)S.scap * S.n);
     S.srow.ensure((size_t)S.scap * S.n);
     hipEvent_t ev;
-    const char *name = mode == SITE_ENERGY ? "tersoff_energy" : "tersoff_site";
+    const char *name =
+        (mode == SITE_ENERGY || mode == SITE_ENERGY_SHORT) ? "tersoff_energy" : "tersoff_site";
     S.timer.begin(name, S.stream, &ev);
     switch (variant) {
     case 0: launch_site<4, 8>(S, mode); break;
@@ -1139,9 +1176,9 @@ template <int G, int SCAP>
 void launch_energy_into(System &S, double *partial)
 {
     const int ntiles = div_up(S.n, BLOCK / G);
-    k_site1e<G, SCAP, GeoOne><<<ntiles, BLOCK, 0, S.stream>>>(
-        S.n, ntiles, GeoOne{S.geom}, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, S.dmodel.p, partial,
-        nullptr, S.flags.p);
+    k_site1e<G, SCAP, GeoOne, false><<<ntiles, BLOCK, 0, S.stream>>>(
+        S.n, ntiles, GeoOne{S.geom}, S.pos.p, S.nbr.p, S.nimg.p, S.ncnt.p, nullptr, nullptr,
+        nullptr, S.dmodel.p, partial, nullptr, S.flags.p);
 }
 }  // namespace
 
@@ -1186,7 +1223,8 @@ void ts_cell_sums(System &S)
     S.cell_ev.ensure((size_t)8 * S.ncells);
     // the energy kernel writes component 0, the force kernels 1..6; the
     // generic kernel writes all seven
-    k_cell_sums<<<S.ncells, 256, 0, S.stream>>>(S.n, S.atom0.p, S.atom_ev.p, 0, 7, S.cell_ev.p);
+    k_cell_sums<<<S.ncells, 64 * CELL_SUM_COMPONENTS, 0, S.stream>>>(S.n, S.atom0.p, S.atom_ev.p,
+                                                                    S.cell_ev.p);
     HIP_TRY(hipGetLastError());
 }
 
