@@ -443,7 +443,15 @@ __device__ __forceinline__ void site1e_tile(int blk, int n, const Geo &geo,
                 const PairTerm pt = M->pair[0][0][t];
                 if (!(pt.cut > u.w)) continue;
                 double v = pt.eps * exp(-pt.zeta * u.w);
-                double f = pt.smooth ? fc64(u.w, pt.cut, pt.soft) : 1.0;
+                double f = 1.0;
+                if (pt.smooth) {
+                    // a term with the bond factor's cutoffs (Tersoff's usual
+                    // form) takes the staged bond smoothing, the same value
+                    if (bond_on && pt.cut == B.cut && pt.soft == B.soft && u.w < maxbond)
+                        f = s_fi[jj][a].x;
+                    else
+                        f = fc64(u.w, pt.cut, pt.soft);
+                }
                 e_lane += 0.5 * v * f * (pt.group >= 0 ? bt : 1.0);
             }
         }
@@ -747,6 +755,8 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
         int slot = cnt + __popcll(gb & ((1ull << l) - 1ull));
         if (keep && slot < SCAP) {
             bool inb = bond_on && r < maxbond;
+            double fb = 0.0, dfb = 0.0;
+            if (inb) smooth64(r, B.cut, B.soft, &fb, &dfb);
             double dR = 0.0, dA = 0.0, A = 0.0;
             if (r < maxpot) {
                 for (int t = 0; t < npair; ++t) {
@@ -754,15 +764,17 @@ __device__ __forceinline__ void site1f_tile(int blk, int n, const Geo &geo,
                     if (!(pt.cut > r)) continue;
                     double v = pt.eps * exp(-pt.zeta * r);
                     double f = 1.0, df = 0.0;
-                    if (pt.smooth) smooth64(r, pt.cut, pt.soft, &f, &df);
+                    if (pt.smooth) {
+                        // the bond factor's cutoffs: its smoothing, the same values
+                        if (inb && pt.cut == B.cut && pt.soft == B.soft) { f = fb; df = dfb; }
+                        else smooth64(r, pt.cut, pt.soft, &f, &df);
+                    }
                     double dv = 0.5 * (-pt.zeta * v * f + v * df);
                     if (pt.group >= 0) { dA += dv; A += 0.5 * v * f; }
                     else dR += dv;
                 }
             }
             const double ir = recip64(r);  // r > 0: the list holds no atom at its own position
-            double fb = 0.0, dfb = 0.0;
-            if (inb) smooth64(r, B.cut, B.soft, &fb, &dfb);
             s_sep[slot][a] = make_double4(sp3.x, sp3.y, sp3.z, r);
             s_aux[slot][a] = make_float4((float)ir, (float)fb, (float)dfb, (float)A);
             s_pair[slot][a] = make_double2(dR * ir, dA * ir);  // divided by r once
