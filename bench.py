@@ -762,7 +762,7 @@ def main():
            "value": round(value, 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32 bond-order/three-body, f64 pair/accumulate",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 per-bond arithmetic, f64 positions/separations/force gather",
            "data": "synthetic (seeded diamond Si + Gaussian displacements, MB 600 K)",
            "config": {"workload": "Tersoff Si %dx%dx%d diamond cells, %d atoms, VV dt=1 fs, "
                                   "rebuild every step" % (tuple(args.cells) + (n,)),
