@@ -685,71 +685,104 @@ void launch(System &S, const Geo &geo, int mode, bool accumulate, double half_dt
 
 enum { BOP_CHI = 3 };
 
-// evaluate_pair_bond_order_factor_tersoff (Potentials.f90:5744-5795):
+// x^mu for the exponent of the Tersoff term: the integer exponents of the
+// usual parametrisations (mu = 1 or 3, and mu - 1 = 0 or 2) by products,
+// any other by the library pow
+__device__ __forceinline__ double mu_pow(double x, double mu)
+{
+    if (mu == 1.0) return x;
+    if (mu == 3.0) return x * x * x;
+    if (mu == 2.0) return x * x;
+    if (mu == 0.0) return 1.0;
+    return pow(x, mu);
+}
+
+// The pieces of one Tersoff zeta term shared by its value and gradients:
 // s1 = r_i - r_j (partner to center), s2 = r_k - r_i, r1 and r2 their norms
+struct BoTerm {
+    double xi, gee;    // f_c(r2) exp((a (r1 - r2))^mu) and g(theta)
+    double dxi;        // d xi / d(r1 - r2) (at fixed f_c)
+    double gc;         // d g / d cos, divided by r1 r2
+    double dot, ex, dfc;
+};
+
+__device__ __forceinline__ BoTerm bo_parts(const double3 &s1, double r1, const double3 &s2,
+                                           double r2, const BondTerm &b, bool grad)
+{
+    BoTerm t;
+    t.dot = -s1.x * s2.x + -s1.y * s2.y + -s1.z * s2.z;
+    const double ratio = recip64(r1 * r2);
+    const double cosine = t.dot * ratio;
+    const double cc = b.c * b.c, dd = b.d * b.d, hc = b.h - cosine;
+    const double den = dd + hc * hc, iden = recip64(den);
+    double decay, dfc;
+    smooth64(r2, b.cut, b.soft, &decay, &dfc);
+    t.dfc = dfc;
+    const double x = b.a * (r1 - r2);
+    t.gee = 1.0 + cc * recip64(dd) - cc * iden;
+    if (grad) {
+        const double e1a = mu_pow(x, b.mu - 1.0);
+        t.ex = exp(x * e1a);
+        t.dxi = decay * (t.ex * b.a * b.mu * e1a);
+        t.gc = -cc * iden * iden * 2.0 * hc * ratio;
+    } else {
+        t.ex = exp(mu_pow(x, b.mu));
+        t.dxi = 0.0;
+        t.gc = 0.0;
+    }
+    t.xi = decay * t.ex;
+    return t;
+}
+
+// evaluate_pair_bond_order_factor_tersoff (Potentials.f90:5744-5795)
 __device__ __forceinline__ double bo_term(const double3 &s1, double r1, const double3 &s2,
                                           double r2, const BondTerm &b)
 {
-    const double cosine = (-s1.x * s2.x + -s1.y * s2.y + -s1.z * s2.z) / (r1 * r2);
-    const double cc = b.c * b.c, dd = b.d * b.d, hc = b.h - cosine;
-    const double xi = fc64(r2, b.cut, b.soft) * exp(pow(b.a * (r1 - r2), b.mu));
-    return xi * (1.0 + cc / dd - cc / (dd + hc * hc));
+    const BoTerm t = bo_parts(s1, r1, s2, r2, b, false);
+    return t.xi * t.gee;
 }
 
 // evaluate_pair_bond_order_gradient_tersoff (Potentials.f90:5806-5902): the
 // gradient of bo_term with respect to the partner's position (gp) and the
-// third atom's (gk); either may be null
-__device__ __forceinline__ void bo_grad(const double3 &s1, double r1, const double3 &s2,
-                                        double r2, const BondTerm &b, double3 *gp, double3 *gk)
+// third atom's (gk); either may be null.  Returns bo_term's value.
+__device__ __forceinline__ double bo_grad(const double3 &s1, double r1, const double3 &s2,
+                                          double r2, const BondTerm &b, double3 *gp, double3 *gk)
 {
+    const BoTerm t = bo_parts(s1, r1, s2, r2, b, true);
     const double3 t1 = make_double3(-s1.x, -s1.y, -s1.z);
-    const double3 u1 = make_double3(t1.x / r1, t1.y / r1, t1.z / r1);
-    const double3 u2 = make_double3(s2.x / r2, s2.y / r2, s2.z / r2);
-    const double dot = t1.x * s2.x + t1.y * s2.y + t1.z * s2.z;
-    const double ratio = 1.0 / (r1 * r2);
-    const double cosine = dot * ratio;
-    const double cc = b.c * b.c, dd = b.d * b.d, hc = b.h - cosine;
-    const double decay = fc64(r2, b.cut, b.soft), dfc = dfc64(r2, b.cut, b.soft);
-    const double x = b.a * (r1 - r2);
-    const double e1a = pow(x, b.mu - 1.0);
-    const double ex = exp(x * e1a);
-    const double den = dd + hc * hc;
-    const double gee = 1.0 + cc / dd - cc / den;
-    const double dxi = decay * (ex * b.a * b.mu * e1a);   // d xi / d(r1 - r2) * decay
-    const double xi = ex * decay;
-    const double gc = -cc / (den * den) * 2.0 * hc * ratio;  // d gee / d cos, times ratio
     if (gp) {
-        // d r1 / d r_partner = u1; d cos / d r_partner = t3 / (r1 r2)
-        const double k1 = dot / (r1 * r1);
+        // d r1 / d r_partner = t1 / r1; d cos / d r_partner = t3 / (r1 r2)
+        const double ir1 = recip64(r1);
+        const double k1 = t.dot * ir1 * ir1;
         const double3 t3 = make_double3(s2.x - k1 * t1.x, s2.y - k1 * t1.y, s2.z - k1 * t1.z);
-        *gp = make_double3(dxi * u1.x * gee + xi * gc * t3.x, dxi * u1.y * gee + xi * gc * t3.y,
-                           dxi * u1.z * gee + xi * gc * t3.z);
+        const double a = t.dxi * t.gee * ir1, c = t.xi * t.gc;
+        *gp = make_double3(a * t1.x + c * t3.x, a * t1.y + c * t3.y, a * t1.z + c * t3.z);
     }
     if (gk) {
-        // d r2 / d r_k = u2 (through f_c and through r1 - r2); d cos / d r_k = t4 / (r1 r2)
-        const double k2 = dot / (r2 * r2);
+        // d r2 / d r_k = s2 / r2 (through f_c and through r1 - r2); d cos / d r_k = t4 / (r1 r2)
+        const double ir2 = recip64(r2);
+        const double k2 = t.dot * ir2 * ir2;
         const double3 t4 = make_double3(t1.x - k2 * s2.x, t1.y - k2 * s2.y, t1.z - k2 * s2.z);
-        const double ex_ = ex;
-        *gk = make_double3((u2.x * dfc * ex_ - dxi * u2.x) * gee + xi * gc * t4.x,
-                           (u2.y * dfc * ex_ - dxi * u2.y) * gee + xi * gc * t4.y,
-                           (u2.z * dfc * ex_ - dxi * u2.z) * gee + xi * gc * t4.z);
+        const double a = (t.dfc * t.ex - t.dxi) * t.gee * ir2, c = t.xi * t.gc;
+        *gk = make_double3(a * s2.x + c * t4.x, a * s2.y + c * t4.y, a * s2.z + c * t4.z);
     }
+    return t.xi * t.gee;
 }
 
 // post_process_bond_order_factor_tersoff (Potentials.f90:5910-5921) and its
 // derivative (post_process_bond_order_gradient_tersoff on a unit gradient,
-// 5929-5961, with the zeta = 0 branch)
+// 5929-5961, with the zeta = 0 branch); beta zeta >= 0
 __device__ __forceinline__ double bo_post(double zeta, const BondTerm &b)
 {
-    return pow(1.0 + pow(b.beta * zeta, b.eta), -1.0 / (2.0 * b.eta));
+    return powpos64(1.0 + powpos64(b.beta * zeta, b.eta), -1.0 / (2.0 * b.eta));
 }
 __device__ __forceinline__ double bo_dpost(double zeta, const BondTerm &b)
 {
     if (b.eta == 0.0) return 0.0;
     if (zeta == 0.0 && b.eta <= 1.0) return 0.0;
     const double inv_eta = -1.0 / (2.0 * b.eta);
-    return inv_eta * pow(1.0 + pow(b.beta * zeta, b.eta), inv_eta - 1.0) * b.eta *
-           pow(b.beta * zeta, b.eta - 1.0) * b.beta;
+    return inv_eta * powpos64(1.0 + powpos64(b.beta * zeta, b.eta), inv_eta - 1.0) * b.eta *
+           powpos64(b.beta * zeta, b.eta - 1.0) * b.beta;
 }
 
 // Each atom's neighbors within the bond-order pass's radius, widened by
@@ -835,13 +868,14 @@ __global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 
                         continue;
                     const double r2 = sqrt(dot64(s2));
                     if (!(r2 < mb)) continue;
-                    zeta += bo_term(s1, r, s2, r2, b);
                     if (FORCE) {
                         double3 gpk;
-                        bo_grad(s1, r, s2, r2, b, &gpk, nullptr);
+                        zeta += bo_grad(s1, r, s2, r2, b, &gpk, nullptr);
                         gp.x += gpk.x;
                         gp.y += gpk.y;
                         gp.z += gpk.z;
+                    } else {
+                        zeta += bo_term(s1, r, s2, r2, b);
                     }
                 }
                 const BondTerm &pp = BM->post[g][si];
