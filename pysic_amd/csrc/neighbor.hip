@@ -405,13 +405,26 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
     const float c2sure = (float)c2lim - 2.0f * margin2;
     int count = 0, nq = 0;
     constexpr int SHELL = 1 << 30;  // queue flag: exact test needed (images use bits 0-29)
-    // write out the queued entries, in queue order; shell entries first pass
-    // the exact test
+    // write out the queued entries, in queue order.  Sparse bins (a few hits
+    // among many candidates): an entry holds the candidate and its bin's
+    // periodic shift, and the pair's image, bin shift + wrap(t) - wrap(s), is
+    // formed here, once per accepted candidate rather than in the scan loop,
+    // where the branch runs whenever any lane of the wave has a hit; the atom
+    // itself at image 0 is dropped here.  Dense bins (most candidates hit):
+    // the image is formed at the hit, as the drain would otherwise reload
+    // each entry's wrap.  Shell entries first pass the exact test.
     auto drain = [&]() {
         for (int u = 0; u < nq; ++u) {
             const int t = q_slot[u][tid];
             const int q = q_shift[u][tid];
-            const int im = q & ~SHELL;
+            int im = q & ~SHELL;
+            if constexpr (!DENSE) {
+                const int wt = __float_as_int(wpos[t].w);
+                const int ox = img_x(im) - wsx + img_x(wt), oy = img_y(im) - wsy + img_y(wt),
+                          oz = img_z(im) - wsz + img_z(wt);
+                if (t == s && ox == 0 && oy == 0 && oz == 0) continue;
+                im = pack_img(ox, oy, oz);
+            }
             if ((q & SHELL) && !(dot64(separation64(ps, pos[t], im, g.v)) < c2lim)) continue;
             if (count < cap) {
                 nbr[(size_t)count * n + s] = t;
@@ -430,6 +443,7 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
         const float shx = sxf * v0x + syf * v1x + szf * v2x - qs.x;
         const float shy = sxf * v0y + syf * v1y + szf * v2y - qs.y;
         const float shz = sxf * v0z + syf * v1z + szf * v2z - qs.z;
+        const int sh = pack_img(sx, sy, sz);
         const int ox0 = sx - wsx, oy0 = sy - wsy, oz0 = sz - wsz;
         const int t1 = start[row + bx + 1];
         for (int t = start[row + bx]; t < t1; ++t) {
@@ -442,12 +456,18 @@ __global__ __launch_bounds__(256) void k_neighbors_pf(int n, Geo geo,
             }
             if (hit) {
                 if (!UNIFORM && nq == NB_QUEUE) drain();
-                // image of the pair: bin shift + wrap(t) - wrap(s)
-                const int wt = __float_as_int(qt.w);
-                const int ox = ox0 + img_x(wt), oy = oy0 + img_y(wt), oz = oz0 + img_z(wt);
-                if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
+                const int flag = d2f < c2sure ? 0 : SHELL;
+                if constexpr (DENSE) {
+                    const int wt = __float_as_int(qt.w);
+                    const int ox = ox0 + img_x(wt), oy = oy0 + img_y(wt), oz = oz0 + img_z(wt);
+                    if (!(t == s && ox == 0 && oy == 0 && oz == 0)) {
+                        q_slot[nq][tid] = t;
+                        q_shift[nq][tid] = pack_img(ox, oy, oz) | flag;
+                        ++nq;
+                    }
+                } else {
                     q_slot[nq][tid] = t;
-                    q_shift[nq][tid] = pack_img(ox, oy, oz) | (d2f < c2sure ? 0 : SHELL);
+                    q_shift[nq][tid] = sh | flag;
                     ++nq;
                 }
             }
