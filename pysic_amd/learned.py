@@ -70,10 +70,10 @@ DESC_ETA = np.full(16, 1.0 / (2.0 * 0.3 ** 2))
 NPARAM = 6532
 # Residual corrector (include/pysic_learned.h): coefficients per stage and
 # the number of stages trained.  Each stage cuts the charge error of the
-# held-out cells by ~2.6x for one evaluation of chi, where one step of the
+# held-out cells by ~2.7x for one evaluation of chi, where one step of the
 # damped dynamics cuts it by ~1.06x.
 CORR_NCOEF = 97
-CORR_STAGES = 8
+CORR_STAGES = 16
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 2048, 2304, 6400, 6464, 6528
 
 
