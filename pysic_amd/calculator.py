@@ -517,7 +517,16 @@ class Pysic:
             self.charge_relaxation.charge_relaxation()
             self._invalidate()
 
+    def _empty(self):
+        # a structure without atoms: the reference's loops over zero atoms
+        # give zero energy and stress and empty forces and electronegativities;
+        # nothing is sent to the engine
+        return self.structure is not None and len(self.structure) == 0
+
     def calculate_forces(self, skip_charge_relaxation=False):
+        if self._empty():
+            self.forces, self.stress = np.zeros((0, 3)), np.zeros(6)
+            return
         self.set_core()
         self._relax_if_needed(skip_charge_relaxation)
         self.set_core()
@@ -528,6 +537,9 @@ class Pysic:
         self.forces, self.stress = f, st
 
     def calculate_energy(self, skip_charge_relaxation=False):
+        if self._empty():
+            self.energy = 0.0
+            return
         self.set_core()
         self._relax_if_needed(skip_charge_relaxation)
         self.set_core()
@@ -540,6 +552,9 @@ class Pysic:
         self.calculate_forces(skip_charge_relaxation)
 
     def calculate_electronegativities(self):
+        if self._empty():
+            self.electronegativities = np.zeros(0)
+            return
         self.set_core()
         self.electronegativities = self.engine.electronegativities()
 

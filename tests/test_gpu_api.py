@@ -250,3 +250,57 @@ def test_comb_terms_through_the_calculator(oracle):
     assert abs(e - e_ref) / n <= 1e-9
     assert rel_maxnorm(f, f_ref) <= 1e-9
     assert rel_maxnorm(x, x_ref) <= 1e-9
+
+
+def test_empty_single_atom_and_dimer_closed_forms(oracle):
+    """Edge cases, with answers fixed by the mathematics: no atoms (zero
+    energy and stress, empty forces, as the reference's loops over zero atoms
+    give); one atom (no neighbors: zero energy, force and stress); a Si dimer,
+    where zeta = 0 so b = 1 and E = A exp(-l1 r) + B exp(-l2 r) times the
+    smoothing f_c(r) of tersoff_si (3.0 A cutoff, 0.3 A margin), F = -dE/dr
+    along the bond; and a dimer beyond the cutoff (all zero)."""
+    from pysic_amd import Pysic
+    print("case: empty, single atom and dimer cells", flush=True)
+    L = 20.0
+    cell = np.eye(3) * L
+    calc = Pysic(st.Structure([], np.zeros((0, 3)), cell), tersoff_si())
+    assert calc.get_potential_energy() == 0.0
+    assert calc.get_forces().shape == (0, 3)
+    assert np.array_equal(calc.get_stress(), np.zeros(6))
+    assert calc.get_electronegativities().shape == (0,)
+
+    one = st.Structure(["Si"], np.array([[3.1, 4.7, 9.2]]), cell)
+    calc = Pysic(one, tersoff_si())
+    assert calc.get_potential_energy() == 0.0
+    assert np.array_equal(calc.get_forces(), np.zeros((1, 3)))
+    assert np.array_equal(calc.get_stress(), np.zeros(6))
+
+    pots = tersoff_si()
+    (A, l1), (B, l2) = [p.get_parameter_values() for p in pots]
+    hard, soft = 3.0, 2.7
+    for r in (2.35, 2.85, 3.4):
+        d = np.array([0.6, -0.48, 0.64])
+        d = d / np.linalg.norm(d) * r
+        pos = np.array([[7.0, 8.0, 9.0], [7.0, 8.0, 9.0] + d])
+        dimer = st.Structure(["Si", "Si"], pos, cell)
+        calc = Pysic(dimer, tersoff_si())
+        e, f = calc.get_potential_energy(), calc.get_forces()
+        if r < soft:
+            fc, dfc = 1.0, 0.0
+        elif r < hard:
+            x = math.pi * (r - soft) / (hard - soft)
+            fc, dfc = 0.5 * (1 + math.cos(x)), -0.5 * math.pi / (hard - soft) * math.sin(x)
+        else:
+            fc, dfc = 0.0, 0.0
+        v = A * math.exp(-l1 * r) + B * math.exp(-l2 * r)
+        dv = -l1 * A * math.exp(-l1 * r) - l2 * B * math.exp(-l2 * r)
+        e_ref = fc * v
+        g = dfc * v + fc * dv            # dE/dr
+        f_ref = np.array([g * d / r, -g * d / r])
+        e_o, f_o, _ = oracle_tersoff(oracle, dimer)
+        print("  dimer r=%.2f: E %.12f closed form %.12f oracle %.12f; |dF| %.2e" % (
+            r, e, e_ref, e_o, np.abs(f - f_ref).max()), flush=True)
+        assert abs(e - e_ref) <= 1e-12 * max(1.0, abs(e_ref))
+        assert abs(e_o - e_ref) <= 1e-12 * max(1.0, abs(e_ref))
+        assert np.abs(f - f_ref).max() <= 1e-5 * max(np.abs(f_ref).max(), 0.01)
+        assert np.allclose(f.sum(axis=0), 0.0, atol=1e-12)
