@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libpysic_ref.so")
 # the reference core built with -DMPI (build_ref.sh), for the multi-rank
-# CPU baseline (ref_mpi_md.py); selected by use_mpi_reference()
+# CPU baseline (ref_md.py); selected by use_mpi_reference()
 REF_MPI_PATH = os.path.join(HERE, "_ref", "libpysic_ref_mpi.so")
 
 
