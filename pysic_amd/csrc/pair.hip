@@ -813,6 +813,63 @@ __global__ __launch_bounds__(PB) void k_bop_list(int n, Geo geo, const double4 *
     oscnt[v] = c;
 }
 
+// The first BOP_STAGE entries of each atom's bond-order list, staged once per
+// atom in LDS for the triplet loops of both passes: the vector r_k - r_i
+// with its length, and the species of k.  Entries beyond are formed from
+// global memory where they are used, by the same functions, so the values
+// are the same either way.
+constexpr int BOP_STAGE = 16;
+
+template <int G>
+struct BopStage {
+    double4 v[BOP_STAGE][PB / G];
+    int sp[BOP_STAGE][PB / G];
+};
+
+template <int G, class Geo>
+__device__ __forceinline__ void bop_stage(BopStage<G> &st, int i, int n, int lane, int a,
+                                          const Geo &geo, const double4 *__restrict__ pos,
+                                          const int *__restrict__ osj,
+                                          const int *__restrict__ osimg,
+                                          const int *__restrict__ oscnt)
+{
+    if (i < n) {
+        const size_t N = (size_t)n;
+        const double4 pi = pos[i];
+        const double *cv = geo.vec(i);
+        const int c = oscnt[i] < BOP_STAGE ? oscnt[i] : BOP_STAGE;
+        for (int u = lane; u < c; u += G) {
+            const double4 pk = pos[osj[(size_t)u * N + i]];
+            const double3 s2 = separation64(pi, pk, osimg[(size_t)u * N + i], cv);
+            st.v[u][a] = make_double4(s2.x, s2.y, s2.z, sqrt(dot64(s2)));
+            st.sp[u][a] = (int)pk.w;
+        }
+    }
+    __syncthreads();
+}
+
+// Entry u of atom i's list: r_k - r_i, its length, the species of k.
+template <int G>
+__device__ __forceinline__ void bop_entry(const BopStage<G> &st, int u, int i, int a, size_t N,
+                                          const double4 &pi, const double *cv,
+                                          const double4 *__restrict__ pos,
+                                          const int *__restrict__ osj,
+                                          const int *__restrict__ osimg, double3 &s, double &r,
+                                          int &sk)
+{
+    if (u < BOP_STAGE) {
+        const double4 v = st.v[u][a];
+        s = make_double3(v.x, v.y, v.z);
+        r = v.w;
+        sk = st.sp[u][a];
+    } else {
+        const double4 pk = pos[osj[(size_t)u * N + i]];
+        s = separation64(pi, pk, osimg[(size_t)u * N + i], cv);
+        r = sqrt(dot64(s));
+        sk = (int)pk.w;
+    }
+}
+
 // Pass 1: G lanes per atom, lane l takes the partner slots l, l + G, ...
 template <class Geo, int MODE, int G>
 __global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 *__restrict__ pos,
@@ -831,10 +888,12 @@ __global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 
     constexpr bool FORCE = MODE == PAIR_FORCES || MODE == PAIR_FORCES_ONLY;
     constexpr bool CHI = MODE == BOP_CHI;
     __shared__ double red[PB];
+    __shared__ BopStage<G> stg;
     const int blk = xcd_block(gridDim.x);
     const int gid = blk * PB + threadIdx.x;
-    const int i = gid / G, lane = gid % G;
+    const int i = gid / G, lane = gid % G, a = threadIdx.x / G;
     const size_t N = (size_t)n;
+    bop_stage<G>(stg, i, n, lane, a, geo, pos, osj, osimg, oscnt);
     double e = 0.0, own = 0.0;
     if (i < n) {
         const double4 pi = pos[i];
@@ -845,10 +904,10 @@ __global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 
         const int c = oscnt[i];
         for (int t = lane; t < c; t += G) {
             const int j = osj[(size_t)t * N + i], img = osimg[(size_t)t * N + i];
-            const double4 pj = pos[j];
-            const int sj = (int)pj.w;
-            const double3 d = separation64(pi, pj, img, cv);
-            const double r = sqrt(dot64(d));
+            double3 d;
+            double r;
+            int sj;
+            bop_entry<G>(stg, t, i, a, N, pi, cv, pos, osj, osimg, d, r, sj);
             double4 D = make_double4(0.0, 0.0, 0.0, 0.0);
             double wk = 0.0;
             if (BM->nbop[si][sj] && BM->pair[si][sj].cut > r && r > 0.0) {
@@ -858,15 +917,15 @@ __global__ __launch_bounds__(PB) void k_bop_pairs(int n, Geo geo, const double4 
                 double zeta = 0.0;
                 double3 gp = make_double3(0.0, 0.0, 0.0);
                 for (int u = 0; u < c; ++u) {
-                    const int k = osj[(size_t)u * N + i], imk = osimg[(size_t)u * N + i];
-                    const double4 pk = pos[k];
-                    const BondTerm &b = BM->bond[g][si][sj][(int)pk.w];
+                    double3 s2;
+                    double r2;
+                    int sk;
+                    bop_entry<G>(stg, u, i, a, N, pi, cv, pos, osj, osimg, s2, r2, sk);
+                    const BondTerm &b = BM->bond[g][si][sj][sk];
                     if (!b.valid) continue;
-                    const double3 s2 = separation64(pi, pk, imk, cv);
                     // the partner itself (the same vector) is not a third atom
                     if (!(s1.x + s2.x != 0.0 || s1.y + s2.y != 0.0 || s1.z + s2.z != 0.0))
                         continue;
-                    const double r2 = sqrt(dot64(s2));
                     if (!(r2 < mb)) continue;
                     if (FORCE) {
                         double3 gpk;
@@ -942,10 +1001,12 @@ __global__ __launch_bounds__(PB) void k_bop_third(int n, Geo geo, const double4 
 {
     constexpr bool VIR = MODE == PAIR_FORCES;
     __shared__ double red[6][PB];
+    __shared__ BopStage<G> stg;
     const int blk = xcd_block(gridDim.x);
     const int gid = blk * PB + threadIdx.x;
-    const int i = gid / G, lane = gid % G;
+    const int i = gid / G, lane = gid % G, a = threadIdx.x / G;
     const size_t N = (size_t)n;
+    bop_stage<G>(stg, i, n, lane, a, geo, pos, osj, osimg, oscnt);
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     if (i < n) {
         const double4 pi = pos[i];
@@ -954,27 +1015,26 @@ __global__ __launch_bounds__(PB) void k_bop_third(int n, Geo geo, const double4 
         const double mb = BM->maxbond[si];
         const int c = oscnt[i];
         for (int t = lane; t < c; t += G) {
-            const int k = osj[(size_t)t * N + i], imk = osimg[(size_t)t * N + i];
-            const double4 pk = pos[k];
-            const int sk = (int)pk.w;
-            const double3 s2 = separation64(pi, pk, imk, cv);
-            const double r2 = sqrt(dot64(s2));
+            double3 s2;
+            double r2;
+            int sk;
+            bop_entry<G>(stg, t, i, a, N, pi, cv, pos, osj, osimg, s2, r2, sk);
             double4 D = oD[(size_t)t * N + i];
             if (r2 < mb) {
                 for (int u = 0; u < c; ++u) {
                     const double wj = oW[(size_t)u * N + i];
                     if (wj == 0.0) continue;
-                    const int j = osj[(size_t)u * N + i], img = osimg[(size_t)u * N + i];
-                    const double4 pj = pos[j];
-                    const int sj = (int)pj.w;
+                    double3 d;
+                    double r1;
+                    int sj;
+                    bop_entry<G>(stg, u, i, a, N, pi, cv, pos, osj, osimg, d, r1, sj);
                     const BondTerm &b = BM->bond[BM->group[si][sj]][si][sj][sk];
                     if (!b.valid) continue;
-                    const double3 d = separation64(pi, pj, img, cv);
                     const double3 s1 = make_double3(-d.x, -d.y, -d.z);
                     if (!(s1.x + s2.x != 0.0 || s1.y + s2.y != 0.0 || s1.z + s2.z != 0.0))
                         continue;
                     double3 gk;
-                    bo_grad(s1, sqrt(dot64(d)), s2, r2, b, nullptr, &gk);
+                    bo_grad(s1, r1, s2, r2, b, nullptr, &gk);
                     D.x += wj * gk.x;
                     D.y += wj * gk.y;
                     D.z += wj * gk.z;
