@@ -3,7 +3,7 @@ reference Fortran core: serially in the calling process (si_md_serial), or
 as one rank of an mpiexec launch of this file, which loads the core built
 with -DMPI.
 
-BASELINE / TEST INFRASTRUCTURE ONLY: bench.py's CPU baseline runs it; nothing
+CPU-BASELINE AND TEST INFRASTRUCTURE ONLY: bench.py's CPU baseline runs it; nothing
 in the product path imports it.
 
 Under MPI every rank runs this same script, as pysic itself runs under
