@@ -87,6 +87,33 @@ def test_comb_full_vs_oracle(eng, oracle, n_si, seed):
             "oracle")
 
 
+def test_comb_full_dense_lists_past_the_staged_entries(eng, oracle):
+    """A compressed cell (4.0 g/cm^3, 384 atoms) where 34 atoms have more than
+    16 partners within 3.05 A, the smallest bond-order list radius of the
+    model: the bond-order passes stage each atom's first 16 list entries in
+    LDS and read the rest from global memory, so both paths are taken, in
+    energy, force and electronegativity modes."""
+    from pysic_amd.interactions import build_descriptors, comb_sio2
+    s = st.amorphous_sio2(128, 4.0, seed=91, dmin=1.3)
+    n = len(s)
+    q = comb_charges(s.symbols, seed=92)
+    names = sorted(set(s.symbols))
+    pd, bd, _ = build_descriptors(comb_sio2(), {x: k for k, x in enumerate(names)})
+    sp = np.array([names.index(x) for x in s.symbols], dtype=np.int32)
+    near = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, 3.05)).counts
+    print("case: dense COMB, %d atoms, max %d partners within 3.05 A (%d atoms above 16)" % (
+        n, near.max(), int((near > 16).sum())), flush=True)
+    assert near.max() > 16
+    cuts = np.full(n, 3.55)
+    load(eng, sp, s.positions, q, s.cell, pd, bd, cuts)
+    nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], cuts)
+    assert_same_pairs(eng, nl)
+    args = (sp, s.positions, q, s.cell, nl, pd, bd)
+    f, w = oracle.evaluate("forces", *args)
+    compare(eng, (oracle.evaluate("energy", *args), f, w, oracle.evaluate("enegs", *args)), n,
+            "oracle")
+
+
 def test_comb_full_through_the_calculator(oracle):
     # pysic's construction, as a script writes it: Potential and
     # ProductPotential objects with Coordinators on a Pysic calculator
