@@ -147,6 +147,34 @@ def test_neighbor_capacity_growth(eng, oracle):
     check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
 
 
+def test_short_list_capacity_switch(eng, oracle):
+    """A compressed random Si cell (4.5 g/cm^3; up to 15 atoms within the
+    3.0 A cutoff) with the 8-entry kernel shape selected: the evaluation
+    finds the overflow, switches to a shape that holds it and runs again,
+    so energy, forces and virial match the oracle.  A cell with 34 atoms
+    within the cutoff is more than any shape holds (32): the call fails
+    with that message, and the engine then evaluates a sound cell."""
+    from pysic_amd.engine import PysicEngineError
+    pd, bd = tersoff_descriptors()
+    s = st.amorphous_si(256, 4.5, seed=5, dmin=1.8)
+    near = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(len(s), 3.0)).counts
+    print("case: compressed a-Si, max %d atoms within 3.0 A" % near.max(), flush=True)
+    assert 8 < near.max() <= 16
+    eng.set_variant(eng.variants().index("g4_s8"))
+    check_tersoff(eng, oracle, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
+    d = st.amorphous_si(128, 12.0, seed=5, dmin=1.0)
+    near = oracle.neighbor_list(d.positions, d.cell, [1, 1, 1], np.full(len(d), 3.0)).counts
+    print("case: over-dense Si, max %d atoms within 3.0 A" % near.max(), flush=True)
+    assert near.max() > 32
+    eng.set_variant(0)
+    load_system(eng, d.positions, d.cell, [1, 1, 1], pd, bd, np.full(len(d), 3.5))
+    with pytest.raises(PysicEngineError, match="more short-range neighbors"):
+        eng.forces()
+    eng.set_variant(0)
+    g = st.diamond_si((3, 3, 3), sigma=0.05, seed=6)
+    check_tersoff(eng, oracle, g.positions, g.cell, [1, 1, 1], pd, bd, np.full(len(g), 3.5))
+
+
 def test_md_matches_oracle_velocity_verlet(eng, oracle):
     # ASE VelocityVerlet semantics, list rebuilt every step; 20 steps of 1 fs
     s = st.diamond_si((4, 4, 4), sigma=0.1, seed=3)
