@@ -974,11 +974,18 @@ int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *
 {
     API_BEGIN(h)
     PYSIC_REQUIRE(n > 0 && positions && species, PYSIC_EINVAL, "create_atoms: bad arguments");
+    for (int i = 0; i < n; ++i)
+        PYSIC_REQUIRE(species[i] >= 0 && species[i] < PYSIC_MAXS, PYSIC_EINVAL,
+                      "species must be 0..3");
+    // arguments checked: from here the handle holds no usable system until
+    // the call completes, so a failure part-way (an allocation) leaves
+    // evaluations refused rather than running on buffers of another size
+    h->atoms_ready = false;
+    h->lists_valid = false;
+    h->forces_valid = false;
     System &S = h->S;
     S.n = n;
     h->species.assign(species, species + n);
-    for (int s : h->species)
-        PYSIC_REQUIRE(s >= 0 && s < PYSIC_MAXS, PYSIC_EINVAL, "species must be 0..3");
     h->masses.assign(n, 1.0);
     if (masses) h->masses.assign(masses, masses + n);
     S.ncells = 0;
@@ -1114,6 +1121,10 @@ int pysic_generate_neighbor_lists(pysic_ctx *h, int n, const double *cutoffs)
         mc = std::max(mc, cutoffs[i]);
         lo = std::min(lo, cutoffs[i]);
     }
+    PYSIC_REQUIRE(mc > 0.0, PYSIC_EINVAL, "neighbor cutoff must be positive");
+    // the old lists no longer match the cutoffs: refused until this build ends
+    h->lists_valid = false;
+    h->forces_valid = false;
     h->min_cut = lo;
     upload_scalar(h->S, cutoffs, h->S.cut);
     nb_configure_grid(h->S, mc);
@@ -1482,6 +1493,9 @@ int pysic_set_cells(pysic_ctx *h, int ncells, const int *natoms, const double *v
     }
     PYSIC_REQUIRE(a0[ncells] == S.n, PYSIC_EINVAL,
                   "set_cells: the cells' atom counts must add up to the atoms created");
+    // the atoms go back to their input order below: the lists no longer apply
+    h->lists_valid = false;
+    h->forces_valid = false;
     // atoms keep their cell's slot range: start from the input order
     restore_original_order(S);
     S.hcells.assign(ncells, CellGeom{});
