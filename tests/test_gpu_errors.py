@@ -169,3 +169,60 @@ def test_bad_descriptors(eng, sound, case):
         load(eng, s, bad, bd)
         eng.energy()
     still_sound(eng, sound)
+
+
+def test_learned_model_refusals(eng, sound):
+    """The warm-start network (include/pysic_learned.h) refuses use before
+    it is configured, sample indices past its training set, species it was
+    not configured for, systems whose lists are shorter than its cutoff,
+    and more corrector stages than it holds."""
+    from pysic_amd.engine import Mlp, PysicEngineError
+    s, pd, bd, _ = sound
+    load(eng, s, pd, bd)
+    m = Mlp(0)
+    try:
+        print("case: network not configured", flush=True)
+        with pytest.raises(PysicEngineError, match="network not configured"):
+            m.predict_charges(eng)
+        mu, eta = np.linspace(0.5, 3.0, 16), np.full(16, 4.0)
+        m.configure(1, 3.0, mu, eta, np.zeros(32), np.ones(32))
+        with pytest.raises(PysicEngineError, match="network parameters not set"):
+            m.warm_start(eng, 0)
+        m.set_params(np.zeros(m.nparams))
+        print("case: training samples and species", flush=True)
+        with pytest.raises(PysicEngineError, match="training species out of range"):
+            m.set_training_data(np.zeros((4, 32)), [0, 0, 0, 9], np.zeros(4))
+        m.set_training_data(np.zeros((4, 32)), [0, 0, 0, 0], np.zeros(4))
+        with pytest.raises(PysicEngineError, match="sample index out of range: 4"):
+            m.evaluate([0, 4])
+        with pytest.raises(PysicEngineError, match="at most 262144 samples"):
+            m.gradients(np.zeros((1 << 18) + 1, np.int32))
+        assert np.isfinite(m.evaluate([0, 3])).all()
+        print("case: corrector stages", flush=True)
+        with pytest.raises(PysicEngineError, match="set_corrector: bad arguments"):
+            m.set_corrector(np.zeros((17, m.CORRECTOR_NCOEF)))
+        bad = np.zeros((2, m.CORRECTOR_NCOEF))
+        bad[1, 5] = np.nan
+        with pytest.raises(PysicEngineError, match="non-finite coefficient"):
+            m.set_corrector(bad)
+        assert m.corrector_stages() == 0
+        with pytest.raises(PysicEngineError, match="more stages asked for"):
+            m.warm_start(eng, 2)
+        print("case: lists shorter than the cutoff, and a second species", flush=True)
+        m.configure(1, 4.0, mu, eta, np.zeros(32), np.ones(32))
+        with pytest.raises(PysicEngineError, match="shorter than the descriptor cutoff"):
+            m.predict_charges(eng)
+        m.configure(1, 3.0, mu, eta, np.zeros(32), np.ones(32))
+        sp = np.zeros(len(s), np.int32)
+        sp[::2] = 1
+        eng.create_atoms(sp, s.positions)
+        eng.generate_neighbor_lists(np.full(len(s), 3.5))
+        with pytest.raises(PysicEngineError, match="more species than the network"):
+            m.predict_charges(eng)
+        print("case: configured for the system: a prediction", flush=True)
+        load(eng, s, pd, bd)
+        q = m.predict_charges(eng)
+        assert q.shape == (len(s),) and np.isfinite(q).all()
+    finally:
+        m.close()
+    still_sound(eng, sound)
