@@ -5,7 +5,9 @@
  * (fortran/PyInterface.f90) for the Tersoff / Ewald / charge-relaxation
  * path.  Plain pointers and sizes only; all host buffers are copied, none is
  * retained.  Every call returns 0 on success or a negative PYSIC_E* code,
- * with a message available from pysic_last_error().  Indices are 0-based,
+ * with a message available from pysic_last_error().  Arguments are checked
+ * before the handle changes: a call refused for its arguments leaves the
+ * loaded system as it was.  Indices are 0-based,
  * arrays of 3-vectors are row-major (N,3), cells are given as three vectors
  * in rows, and species are small integers 0..3 chosen by the caller.
  *
