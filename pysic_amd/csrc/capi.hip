@@ -922,6 +922,23 @@ void batch_evaluate(pysic_ctx *h, bool use_graph, bool rebuild)
     }
 }
 
+// A setter has just put its new rows in `now` (the previous ones in
+// `old`): compile them if the atoms are there; if they are refused, put the
+// previous rows back, so that the handle keeps the model it had.
+void recompile_or_restore(pysic_ctx *h, std::vector<double> &now, std::vector<double> &old)
+{
+    h->S.model_dirty = true;
+    h->forces_valid = false;
+    if (!h->atoms_ready) return;
+    try {
+        compile_model(h);
+    } catch (...) {
+        now.swap(old);
+        h->S.model_dirty = true;   // recompiled from the previous rows when next used
+        throw;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1053,10 +1070,9 @@ int pysic_set_potentials(pysic_ctx *h, int npot, const double *desc)
 {
     API_BEGIN(h)
     PYSIC_REQUIRE(npot >= 0 && (npot == 0 || desc), PYSIC_EINVAL, "set_potentials: bad arguments");
-    h->potdesc.assign(desc, desc + POT_STRIDE * (size_t)npot);
-    h->S.model_dirty = true;
-    h->forces_valid = false;
-    if (h->atoms_ready) compile_model(h);
+    std::vector<double> old(desc, desc + POT_STRIDE * (size_t)npot);
+    h->potdesc.swap(old);
+    recompile_or_restore(h, h->potdesc, old);
     API_END(h)
 }
 
@@ -1065,10 +1081,15 @@ int pysic_set_potential_table(pysic_ctx *h, int id, int nrows, const double *row
     API_BEGIN(h)
     PYSIC_REQUIRE(nrows >= 2 && rows, PYSIC_EINVAL,
                   "set_potential_table: a table needs at least 2 rows");
-    h->tables[id].assign(rows, rows + 2 * (size_t)nrows);
-    h->S.model_dirty = true;
-    h->forces_valid = false;
-    if (h->atoms_ready) compile_model(h);
+    const bool had = h->tables.count(id) > 0;
+    std::vector<double> old(rows, rows + 2 * (size_t)nrows);
+    h->tables[id].swap(old);
+    try {
+        recompile_or_restore(h, h->tables[id], old);
+    } catch (...) {
+        if (!had) h->tables.erase(id);   // as before the call: no such table
+        throw;
+    }
     API_END(h)
 }
 
@@ -1076,10 +1097,9 @@ int pysic_set_bond_order_factors(pysic_ctx *h, int nbond, const double *desc)
 {
     API_BEGIN(h)
     PYSIC_REQUIRE(nbond >= 0 && (nbond == 0 || desc), PYSIC_EINVAL, "set_bond_order_factors: bad arguments");
-    h->bonddesc.assign(desc, desc + 14 * (size_t)nbond);
-    h->S.model_dirty = true;
-    h->forces_valid = false;
-    if (h->atoms_ready) compile_model(h);
+    std::vector<double> old(desc, desc + 14 * (size_t)nbond);
+    h->bonddesc.swap(old);
+    recompile_or_restore(h, h->bonddesc, old);
     API_END(h)
 }
 

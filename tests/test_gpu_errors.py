@@ -152,7 +152,7 @@ def test_bad_arguments(eng, sound):
 @pytest.mark.parametrize("case", ["type", "species", "exponentials", "factors"])
 def test_bad_descriptors(eng, sound, case):
     from pysic_amd.engine import PysicEngineError
-    s, pd, bd, _ = sound
+    s, pd, bd, e_ref = sound
     bad = pd.copy()
     if case == "type":
         bad[0, 0], text = 99, "unsupported potential type"
@@ -165,9 +165,16 @@ def test_bad_descriptors(eng, sound, case):
         mult[11] = 1.0
         bad, text = np.vstack([pd[0]] + [mult] * 4), "at most 4 factors in a product"
     print("case: descriptor with %s out of range" % case, flush=True)
+    load(eng, s, pd, bd)
     with pytest.raises(PysicEngineError, match=text):
-        load(eng, s, bad, bd)
-        eng.energy()
+        eng.set_potentials(bad)
+    # the refused rows are not kept: the model is the one loaded before
+    assert abs(eng.energy() - e_ref) <= 1e-12 * abs(e_ref)
+    with pytest.raises(PysicEngineError, match="bond-order species out of range"):
+        b = bd.copy()
+        b[0, 0] = 9
+        eng.set_bond_order_factors(b)
+    assert abs(eng.energy() - e_ref) <= 1e-12 * abs(e_ref)
     still_sound(eng, sound)
 
 
