@@ -922,6 +922,22 @@ void batch_evaluate(pysic_ctx *h, bool use_graph, bool rebuild)
     }
 }
 
+// Three cell vectors (rows of v) must be finite and span a volume: the bin
+// grid divides by each axis' width.
+void require_cell_vectors(const double *v, const std::string &who)
+{
+    for (int k = 0; k < 9; ++k)
+        PYSIC_REQUIRE(std::isfinite(v[k]), PYSIC_EINVAL, who + ": non-finite cell vector");
+    const double det = v[0] * (v[4] * v[8] - v[5] * v[7]) - v[1] * (v[3] * v[8] - v[5] * v[6]) +
+                       v[2] * (v[3] * v[7] - v[4] * v[6]);
+    double len = 1.0;
+    for (int i = 0; i < 3; ++i)
+        len *= std::sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                         v[3 * i + 2] * v[3 * i + 2]);
+    PYSIC_REQUIRE(std::fabs(det) > 1e-9 * len, PYSIC_EINVAL,
+                  who + ": the cell vectors span no volume");
+}
+
 // A setter has just put its new rows in `now` (the previous ones in
 // `old`): compile them if the atoms are there; if they are refused, put the
 // previous rows back, so that the handle keeps the model it had.
@@ -1027,6 +1043,7 @@ int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse
 {
     API_BEGIN(h)
     PYSIC_REQUIRE(vectors && inverse && pbc, PYSIC_EINVAL, "create_cell: bad arguments");
+    require_cell_vectors(vectors, "create_cell");
     std::memcpy(h->S.geom.v, vectors, sizeof(double) * 9);
     std::memcpy(h->S.geom.inv, inverse, sizeof(double) * 9);
     for (int a = 0; a < 3; ++a) h->S.geom.pbc[a] = pbc[a] ? 1 : 0;
@@ -1513,6 +1530,8 @@ int pysic_set_cells(pysic_ctx *h, int ncells, const int *natoms, const double *v
     }
     PYSIC_REQUIRE(a0[ncells] == S.n, PYSIC_EINVAL,
                   "set_cells: the cells' atom counts must add up to the atoms created");
+    for (int c = 0; c < ncells; ++c)
+        require_cell_vectors(vectors + 9 * c, "set_cells: cell " + std::to_string(c));
     // the atoms go back to their input order below: the lists no longer apply
     h->lists_valid = false;
     h->forces_valid = false;

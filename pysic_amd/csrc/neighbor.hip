@@ -952,6 +952,8 @@ void nb_configure_grid(System &S, double max_cut)
                    nz = a[0] * b[1] - a[1] * b[0];
             w[i] = std::fabs(v[0] * nx + v[1] * ny + v[2] * nz) /
                    std::sqrt(nx * nx + ny * ny + nz * nz);
+            PYSIC_REQUIRE(w[i] > 0.0 && std::isfinite(w[i]), PYSIC_EINVAL,
+                          "the cell has no width along one of its axes");
             // a cell thinner than the list radius gets one bin on that axis
             // and a scan over several bins (its periodic images)
             g.nc[i] = std::max(1, (int)std::floor(w[i] / max_cut));

@@ -140,6 +140,16 @@ def test_bad_arguments(eng, sound):
     refused(eng, L.pysic_generate_neighbor_lists(h, n - 1, zero.ctypes.data_as(dbl)), EINVAL,
             "generate_neighbor_lists: bad arguments")
     unchanged()
+    print("case: cell vectors that span no volume, or are not finite", flush=True)
+    flat = np.array([[5.0, 0.0, 0.0], [0.0, 5.0, 0.0], [5.0, 5.0, 0.0]])
+    inv = np.zeros((3, 3))
+    pb = np.ones(3, np.int32)
+    refused(eng, L.pysic_create_cell(h, flat.ctypes.data_as(dbl), inv.ctypes.data_as(dbl),
+                                     pb.ctypes.data_as(ip)), EINVAL, "span no volume")
+    flat[2] = [0.0, 0.0, np.inf]
+    refused(eng, L.pysic_create_cell(h, flat.ctypes.data_as(dbl), inv.ctypes.data_as(dbl),
+                                     pb.ctypes.data_as(ip)), EINVAL, "non-finite cell vector")
+    unchanged()
     print("case: run length, kernel shape, grid size, batch evaluation", flush=True)
     refused(eng, L.pysic_md_run(h, -1, 0.1, 1, None), EINVAL, "md_run: bad arguments")
     refused(eng, L.pysic_set_kernel_variant(h, 99), EINVAL, "no such kernel variant")
