@@ -922,6 +922,16 @@ void batch_evaluate(pysic_ctx *h, bool use_graph, bool rebuild)
     }
 }
 
+// Host arrays that become device state are checked before anything changes:
+// a NaN position would bin as a valid one and drop out of every cutoff test.
+void require_finite(const double *x, size_t count, const std::string &what)
+{
+    if (!x) return;
+    for (size_t k = 0; k < count; ++k)
+        PYSIC_REQUIRE(std::isfinite(x[k]), PYSIC_EINVAL,
+                      "non-finite " + what + " (entry " + std::to_string(k) + ")");
+}
+
 // Three cell vectors (rows of v) must be finite and span a volume: the bin
 // grid divides by each axis' width.
 void require_cell_vectors(const double *v, const std::string &who)
@@ -1010,6 +1020,13 @@ int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *
     for (int i = 0; i < n; ++i)
         PYSIC_REQUIRE(species[i] >= 0 && species[i] < PYSIC_MAXS, PYSIC_EINVAL,
                       "species must be 0..3");
+    require_finite(positions, 3 * (size_t)n, "position");
+    require_finite(momenta, 3 * (size_t)n, "momentum");
+    require_finite(charges, (size_t)n, "charge");
+    if (masses)
+        for (int i = 0; i < n; ++i)
+            PYSIC_REQUIRE(masses[i] > 0.0 && std::isfinite(masses[i]), PYSIC_EINVAL,
+                          "masses must be positive and finite");
     // arguments checked: from here the handle holds no usable system until
     // the call completes, so a failure part-way (an allocation) leaves
     // evaluations refused rather than running on buffers of another size
@@ -1061,6 +1078,8 @@ int pysic_update_atom_coordinates(pysic_ctx *h, int n, const double *positions, 
     API_BEGIN(h)
     PYSIC_REQUIRE(h->atoms_ready && n == h->S.n && positions, PYSIC_EINVAL,
                   "update_atom_coordinates: atom count does not match");
+    require_finite(positions, 3 * (size_t)n, "position");
+    require_finite(momenta, 3 * (size_t)n, "momentum");
     upload_state(h, positions, momenta, true);
     h->forces_valid = false;
     API_END(h)
@@ -1071,6 +1090,7 @@ int pysic_update_atom_charges(pysic_ctx *h, int n, const double *charges)
     API_BEGIN(h)
     PYSIC_REQUIRE(h->atoms_ready && n == h->S.n && charges, PYSIC_EINVAL,
                   "update_atom_charges: atom count does not match");
+    require_finite(charges, (size_t)n, "charge");
     upload_scalar(h->S, charges, h->S.q);
     h->forces_valid = false;
     API_END(h)
@@ -1391,6 +1411,8 @@ int pysic_relax_charges(pysic_ctx *h, int n, double *charges, double *rates, int
     System &S = h->S;
     PYSIC_REQUIRE(n == S.n && charges && rates && steps && error, PYSIC_EINVAL,
                   "relax_charges: bad arguments");
+    require_finite(charges, (size_t)n, "charge");
+    require_finite(rates, (size_t)n, "charge rate");
     upload_scalar(S, charges, S.q);
     h->rates.ensure(n);
     upload_scalar(S, rates, h->rates);
@@ -1407,7 +1429,8 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
     API_BEGIN(h)
     require_ready(h);
     System &S = h->S;
-    PYSIC_REQUIRE(nsteps >= 0 && rebuild >= 0, PYSIC_EINVAL, "md_run: bad arguments");
+    PYSIC_REQUIRE(nsteps >= 0 && rebuild >= 0 && std::isfinite(dt), PYSIC_EINVAL,
+                  "md_run: bad arguments");
     PYSIC_REQUIRE(!h->ewald.on && (S.model.has_pairs || has_gen(S)), PYSIC_EINVAL,
                   "md_run drives the Tersoff and pair potentials (no Ewald)");
     HIP_TRY(hipMemsetAsync(S.flags.p, 0, sizeof(int) * 4, S.stream));

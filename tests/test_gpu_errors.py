@@ -140,6 +140,32 @@ def test_bad_arguments(eng, sound):
     refused(eng, L.pysic_generate_neighbor_lists(h, n - 1, zero.ctypes.data_as(dbl)), EINVAL,
             "generate_neighbor_lists: bad arguments")
     unchanged()
+    print("case: a NaN position, an infinite momentum, a zero mass, a NaN charge, a NaN "
+          "time step", flush=True)
+    sp = np.zeros(n, np.int32)
+    bad = np.ascontiguousarray(s.positions.copy())
+    bad[5, 1] = np.nan
+    refused(eng, L.pysic_create_atoms(h, n, None, None, bad.ctypes.data_as(dbl), None,
+                                      sp.ctypes.data_as(ip)), EINVAL, "non-finite position")
+    refused(eng, L.pysic_update_atom_coordinates(h, n, bad.ctypes.data_as(dbl), None), EINVAL,
+            "non-finite position")
+    good = np.ascontiguousarray(s.positions)
+    mom = np.zeros((n, 3))
+    mom[3, 0] = np.inf
+    refused(eng, L.pysic_update_atom_coordinates(h, n, good.ctypes.data_as(dbl),
+                                                 mom.ctypes.data_as(dbl)), EINVAL,
+            "non-finite momentum")
+    mass = np.ones(n)
+    mass[2] = 0.0
+    refused(eng, L.pysic_create_atoms(h, n, mass.ctypes.data_as(dbl), None,
+                                      good.ctypes.data_as(dbl), None, sp.ctypes.data_as(ip)),
+            EINVAL, "masses must be positive")
+    q = np.zeros(n)
+    q[1] = np.nan
+    refused(eng, L.pysic_update_atom_charges(h, n, q.ctypes.data_as(dbl)), EINVAL,
+            "non-finite charge")
+    refused(eng, L.pysic_md_run(h, 1, float("nan"), 1, None), EINVAL, "md_run: bad arguments")
+    unchanged()
     print("case: cell vectors that span no volume, or are not finite", flush=True)
     flat = np.array([[5.0, 0.0, 0.0], [0.0, 5.0, 0.0], [5.0, 5.0, 0.0]])
     inv = np.zeros((3, 3))
