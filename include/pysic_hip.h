@@ -74,7 +74,8 @@ int pysic_create_atoms(pysic_ctx *h, int n, const double *masses, const double *
  * row-major where M has the vectors as columns */
 int pysic_create_cell(pysic_ctx *h, const double *vectors, const double *inverse,
                       const int *pbc);
-/* update_atom_coordinates (PyInterface.f90:277) */
+/* update_atom_coordinates (PyInterface.f90:277); momenta may be NULL, which
+ * keeps the momenta the atoms have */
 int pysic_update_atom_coordinates(pysic_ctx *h, int n, const double *positions,
                                   const double *momenta);
 /* update_atom_charges (PyInterface.f90:295) */

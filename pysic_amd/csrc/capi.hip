@@ -532,8 +532,11 @@ void upload_state(pysic_ctx *h, const double *positions, const double *momenta, 
         S.out4.ensure(n);
         HIP_TRY(hipMemcpyAsync(S.out4.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
         k_gather4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.out4.p, S.pos.p);
-        HIP_TRY(hipMemcpyAsync(S.out4.p, hm.data(), sizeof(double4) * n, hipMemcpyHostToDevice, S.stream));
-        k_gather4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.out4.p, S.mom.p);
+        if (momenta) {   // an update without momenta keeps them
+            HIP_TRY(hipMemcpyAsync(S.out4.p, hm.data(), sizeof(double4) * n, hipMemcpyHostToDevice,
+                                   S.stream));
+            k_gather4<<<div_up(n, 256), 256, 0, S.stream>>>(n, S.id.p, S.out4.p, S.mom.p);
+        }
     }
     HIP_TRY(hipStreamSynchronize(S.stream));
 }

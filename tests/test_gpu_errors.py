@@ -269,3 +269,29 @@ def test_learned_model_refusals(eng, sound):
     finally:
         m.close()
     still_sound(eng, sound)
+
+
+def test_coordinate_update_without_momenta_keeps_them(eng, sound):
+    """update_atom_coordinates (PyInterface.f90:277) with no momenta moves
+    the atoms and keeps their momenta; with momenta it sets both.  Checked
+    after a list build has reordered the atoms on the device."""
+    s, pd, bd, _ = sound
+    n = len(s)
+    rng = np.random.default_rng(13)
+    mom0 = rng.normal(size=(n, 3))
+    eng.create_atoms(np.zeros(n, np.int32), s.positions, momenta=mom0)
+    eng.create_cell(s.cell, (True, True, True))
+    eng.set_potentials(pd)
+    eng.set_bond_order_factors(bd)
+    eng.ewald_off()
+    eng.generate_neighbor_lists(np.full(n, 3.5))
+    print("case: positions moved, momenta not given", flush=True)
+    moved = s.positions + rng.normal(scale=0.01, size=(n, 3))
+    eng.update_coordinates(moved)
+    pos, mom = eng.state()
+    assert np.array_equal(pos, moved) and np.array_equal(mom, mom0)
+    print("case: positions and momenta given", flush=True)
+    mom1 = rng.normal(size=(n, 3))
+    eng.update_coordinates(s.positions, mom1)
+    pos, mom = eng.state()
+    assert np.array_equal(pos, s.positions) and np.array_equal(mom, mom1)
