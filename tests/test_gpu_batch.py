@@ -145,3 +145,40 @@ def test_mixed_species_cells(oracle):
             assert de <= 1e-7 and df <= 1e-5 and dw <= 1e-5
     finally:
         b.close()
+
+
+def test_sheared_cells(oracle):
+    """Three a-Si cells, each under its own shear, in one batch: each cell's
+    bins and image shifts follow its own three vectors.  Same per-cell
+    bounds as the cubic batch."""
+    from pysic_amd import structures as st
+    shears = ([[0, 0, 0], [0.2, 0, 0], [-0.1, 0.15, 0]],
+              [[0, 0, 0], [-0.25, 0, 0], [0.1, 0.1, 0]],
+              [[0, 0, 0], [0.0, 0, 0], [0.3, -0.2, 0]])
+    cells = []
+    for k, sh in enumerate(shears):
+        s = st.amorphous_si(200 + 60 * k, 2.3, seed=40 + k)
+        frac = np.linalg.solve(s.cell.T, s.positions.T).T
+        s.cell = s.cell + np.array(sh) * s.cell[0, 0]
+        s.set_positions(frac @ s.cell)
+        cells.append(s)
+    print("case: %d sheared a-Si cells in one batch" % len(cells), flush=True)
+    b = BatchedCells(cells, tersoff_si())
+    try:
+        b.evaluate()
+        E, W, F = b.results()
+        pd, bd = tersoff_descriptors()
+        for c, s in enumerate(cells):
+            sl = b.cell_slice(c)
+            n = len(s)
+            sp = np.zeros(n, np.int32)
+            nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, b.list_radius))
+            e_ref = oracle.evaluate("energy", sp, s.positions, None, s.cell, nl, pd, bd)
+            f_ref, w_ref = oracle.evaluate("forces", sp, s.positions, None, s.cell, nl, pd, bd)
+            de, df = abs(E[c] - e_ref) / n, rel_maxnorm(F[sl], f_ref, 1e-2)
+            dw = np.abs(W[c] - w_ref).max() / n
+            print("  cell %d n=%d |dE|/N %.2e rel|dF| %.2e |dW|/N %.2e" % (c, n, de, df, dw),
+                  flush=True)
+            assert de <= 1e-7 and df <= 1e-5 and dw <= 1e-6
+    finally:
+        b.close()

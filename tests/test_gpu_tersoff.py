@@ -214,6 +214,49 @@ def test_md_matches_oracle_velocity_verlet(eng, oracle):
     assert np.abs(np.array(e_gpu) - np.array(e_or)).max() / n <= 1e-7
 
 
+def test_md_triclinic_matches_oracle(eng, oracle):
+    """The MD loop in a sheared cell (b and c tilted by 0.2 and (-0.1,
+    0.15) of the cell length), 20 steps of 1 fs at 1200 K with the lists
+    rebuilt every step: atoms cross the sheared periodic faces, and the
+    drift's binning wraps them along all three cell vectors.  Same bounds as
+    the cubic case."""
+    s = st.diamond_si((4, 4, 4), sigma=0.1, seed=23)
+    frac = np.linalg.solve(s.cell.T, s.positions.T).T
+    L = s.cell[0, 0]
+    s.cell = s.cell + np.array([[0, 0, 0], [0.2, 0, 0], [-0.1, 0.15, 0]]) * L
+    s.set_positions(frac @ s.cell)
+    st.maxwell_boltzmann(s, 1200.0, seed=24)
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    cut = np.full(n, 3.5)
+    print("case: triclinic Si, %d atoms, 20 VV steps at 1200 K" % n, flush=True)
+    load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, cut, masses=s.masses,
+                momenta=s.momenta)
+    dt, steps = st.ASE_FS, 20
+    eng.md_run(steps, dt, rebuild=1)
+    pos_g, mom_g = eng.state()
+    sp = np.zeros(n, np.int32)
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+    for _ in range(steps):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        nl = oracle.neighbor_list(r, s.cell, [1, 1, 1], cut)
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+        p = p + 0.5 * dt * f
+    inv = np.linalg.inv(s.cell)
+    crossed = np.any(np.floor(r @ inv) != np.floor(s.positions @ inv), axis=1).sum()
+    travel = np.abs(r - s.positions).max()
+    dr, dp = np.abs(pos_g - r).max(), np.abs(mom_g - p).max() / np.abs(p).max()
+    print("  %d atoms crossed a face  travel %.3f A  |dr| %.2e A  rel|dp| %.2e" % (
+        crossed, travel, dr, dp), flush=True)
+    assert crossed > 0
+    assert dr < 1e-5 * travel
+    assert dp < 1e-5
+
+
 @pytest.mark.parametrize("cells,sigma", [((8, 8, 8), 0.05), ((6, 6, 6), 0.1)])
 def test_md_force_pass_matches_oracle(eng, oracle, cells, sigma):
     """The force pass of a device MD step (the forces-only site kernel, FP32
