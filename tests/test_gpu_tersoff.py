@@ -257,6 +257,50 @@ def test_md_triclinic_matches_oracle(eng, oracle):
     assert dp < 1e-5
 
 
+def test_md_open_boundaries_matches_oracle(eng, oracle):
+    """A 512-atom Si cluster with no periodic axis in a box that fits it
+    tightly: 47 atoms start outside the box and about as many stay outside
+    while the cluster vibrates (20 steps of 1 fs at 600 K, lists rebuilt
+    every step).  The lists equal the all-pairs search, and the trajectory
+    follows velocity Verlet on the oracle's forces within the cubic case's
+    bounds."""
+    s = st.diamond_si((4, 4, 4), sigma=0.1, seed=31)
+    st.maxwell_boltzmann(s, 600.0, seed=32)
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    cut = np.full(n, 3.5)
+    print("case: open-boundary Si cluster, %d atoms, 20 VV steps" % n, flush=True)
+    load_system(eng, s.positions, s.cell, [0, 0, 0], pd, bd, cut, masses=s.masses,
+                momenta=s.momenta)
+    counts, idx, off = eng.neighbor_lists()
+    d = np.linalg.norm(s.positions[:, None, :] - s.positions[None, :, :], axis=2)
+    assert np.array_equal(counts, (d < 3.5).sum(axis=1) - 1)
+    assert not off.any()
+    dt, steps = st.ASE_FS, 20
+    eng.md_run(steps, dt, rebuild=1)
+    pos_g, mom_g = eng.state()
+    sp = np.zeros(n, np.int32)
+    m = s.masses[:, None]
+    r, p = s.positions.copy(), s.momenta.copy()
+    nl = oracle.neighbor_list(r, s.cell, [0, 0, 0], cut)
+    f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+    for _ in range(steps):
+        p = p + 0.5 * dt * f
+        r = r + dt * p / m
+        nl = oracle.neighbor_list(r, s.cell, [0, 0, 0], cut)
+        f, _ = oracle.evaluate("forces", sp, r, None, s.cell, nl, pd, bd)
+        p = p + 0.5 * dt * f
+    L = s.cell[0, 0]
+    outside = ((r < 0) | (r >= L)).any(axis=1).sum()
+    travel = np.abs(r - s.positions).max()
+    dr, dp = np.abs(pos_g - r).max(), np.abs(mom_g - p).max() / np.abs(p).max()
+    print("  %d atoms outside the box  travel %.3f A  |dr| %.2e A  rel|dp| %.2e" % (
+        outside, travel, dr, dp), flush=True)
+    assert outside > 0
+    assert dr < 1e-5 * travel
+    assert dp < 1e-5
+
+
 @pytest.mark.parametrize("cells,sigma", [((8, 8, 8), 0.05), ((6, 6, 6), 0.1)])
 def test_md_force_pass_matches_oracle(eng, oracle, cells, sigma):
     """The force pass of a device MD step (the forces-only site kernel, FP32
