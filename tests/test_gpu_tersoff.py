@@ -499,3 +499,44 @@ def test_coincident_atoms_report_non_finite_forces(eng):
     load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(len(s), 3.5))
     f, _ = eng.forces()
     assert np.isfinite(f).all()
+
+
+def test_full_size_md_conserves_energy():
+    """The 1,048,576-atom workload run NVE for 400 velocity-Verlet steps at
+    1 fs and again at 0.5 fs from the same state, lists rebuilt every step.
+    The potential energy moves by ~0.16 eV/atom as the displaced lattice
+    relaxes, yet the total energy (device FP64 potential energy + kinetic
+    energy of the returned momenta) stays within 1e-3 eV/atom, and its error
+    shrinks by more than 3x when the step halves: the integrator's O(dt^2)
+    error (4x), not a mismatch between forces and energies, which would give
+    a drift in proportion to the simulated time (2x)."""
+    import bench
+    from pysic_amd.engine import Engine
+    s = bench.build_workload((64, 64, 32))
+    n = len(s)
+    pd, bd = tersoff_descriptors()
+    m = s.masses[:, None]
+    eng = Engine(0)
+    try:
+        drift = {}
+        for dt_fs in (1.0, 0.5):
+            eng.create_atoms(np.zeros(n, np.int32), s.positions, masses=s.masses,
+                             momenta=s.momenta)
+            eng.create_cell(s.cell, s.pbc)
+            eng.set_potentials(pd)
+            eng.set_bond_order_factors(bd)
+            eng.ewald_off()
+            eng.set_neighbor_capacity(16)
+            eng.generate_neighbor_lists(np.full(n, 3.5))
+            eng.set_variant(0)
+            e0 = eng.energy() + 0.5 * (s.momenta ** 2 / m).sum()
+            print("case: full-size NVE, 400 steps of %.1f fs" % dt_fs, flush=True)
+            ep = eng.md_run(400, dt_fs * st.ASE_FS, rebuild=1, energies=True)
+            _, mom = eng.state()
+            drift[dt_fs] = (ep[-1] + 0.5 * (mom ** 2 / m).sum() - e0) / n
+            print("  drift %.3e eV/atom, potential energy range %.3f eV/atom" % (
+                drift[dt_fs], (ep.max() - ep.min()) / n), flush=True)
+        assert abs(drift[1.0]) < 1e-3
+        assert abs(drift[1.0]) > 3.0 * abs(drift[0.5])
+    finally:
+        eng.close()
