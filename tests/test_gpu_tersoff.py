@@ -540,3 +540,37 @@ def test_full_size_md_conserves_energy():
         assert abs(drift[1.0]) > 3.0 * abs(drift[0.5])
     finally:
         eng.close()
+
+
+def test_full_size_md_is_bitwise_reproducible():
+    """Two 50-step runs of the 1,048,576-atom workload from the same state,
+    in two handles, end on the same bits: no force, energy or sort path
+    depends on the order atomics land in (in-bin order is sorted by the
+    original index; every sum is a fixed tree)."""
+    import bench
+    from pysic_amd.engine import Engine
+    s = bench.build_workload((64, 64, 32))
+    n = len(s)
+    pd, bd = tersoff_descriptors()
+    out = []
+    for run in range(2):
+        eng = Engine(0)
+        try:
+            eng.create_atoms(np.zeros(n, np.int32), s.positions, masses=s.masses,
+                             momenta=s.momenta)
+            eng.create_cell(s.cell, s.pbc)
+            eng.set_potentials(pd)
+            eng.set_bond_order_factors(bd)
+            eng.ewald_off()
+            eng.set_neighbor_capacity(16)
+            eng.generate_neighbor_lists(np.full(n, 3.5))
+            eng.set_variant(0)
+            print("case: full-size MD, run %d, 50 steps" % run, flush=True)
+            ep = eng.md_run(50, st.ASE_FS, rebuild=1, energies=True)
+            pos, mom = eng.state()
+            out.append((ep, pos, mom))
+        finally:
+            eng.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
