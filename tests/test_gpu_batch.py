@@ -182,3 +182,42 @@ def test_sheared_cells(oracle):
             assert de <= 1e-7 and df <= 1e-5 and dw <= 1e-6
     finally:
         b.close()
+
+
+def test_short_list_overflow_is_reported(oracle):
+    """A batch with one compressed cell (4.5 g/cm^3, up to 15 atoms within
+    the 3.0 A cutoff) under the 8-entry kernel shape: the results call
+    reports the overflow instead of returning truncated sums; under a
+    16-entry shape the same batch matches the oracle cell by cell."""
+    from pysic_amd import structures as st
+    from pysic_amd.engine import PysicEngineError
+    cells = [st.amorphous_si(150, 2.3, seed=50), st.amorphous_si(256, 4.5, seed=5, dmin=1.8)]
+    near = oracle.neighbor_list(cells[1].positions, cells[1].cell, [1, 1, 1],
+                                np.full(len(cells[1]), 3.0)).counts
+    print("case: batch with a compressed cell, max %d atoms within 3.0 A" % near.max(),
+          flush=True)
+    assert 8 < near.max() <= 16
+    b = BatchedCells(cells, tersoff_si())
+    try:
+        b.set_variant(b.eng.variants().index("g4_s8"))
+        b.evaluate()
+        with pytest.raises(PysicEngineError, match="short list overflow"):
+            b.results()
+        b.set_variant(b.eng.variants().index("g8_s16"))
+        b.evaluate()
+        E, W, F = b.results()
+        pd, bd = tersoff_descriptors()
+        for c, s in enumerate(cells):
+            sl = b.cell_slice(c)
+            n = len(s)
+            sp = np.zeros(n, np.int32)
+            nl = oracle.neighbor_list(s.positions, s.cell, [1, 1, 1], np.full(n, b.list_radius))
+            e_ref = oracle.evaluate("energy", sp, s.positions, None, s.cell, nl, pd, bd)
+            f_ref, w_ref = oracle.evaluate("forces", sp, s.positions, None, s.cell, nl, pd, bd)
+            de, df = abs(E[c] - e_ref) / n, rel_maxnorm(F[sl], f_ref, 1e-2)
+            dw = np.abs(W[c] - w_ref).max() / n
+            print("  cell %d n=%d |dE|/N %.2e rel|dF| %.2e |dW|/N %.2e" % (c, n, de, df, dw),
+                  flush=True)
+            assert de <= 1e-7 and df <= 1e-5 and dw <= 1e-6
+    finally:
+        b.close()
