@@ -574,3 +574,28 @@ def test_full_size_md_is_bitwise_reproducible():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_md_switches_shape_at_its_start(eng):
+    """md_run's first force evaluation is the API's: with the 8-entry shape
+    selected on a compressed cell (up to 15 atoms within 3.0 A) it switches
+    to a shape that holds them, and the run ends where the same run with
+    the 32-entry shape selected from the start ends.  (An overflow later in
+    a run, after the shape is fixed, is reported by md_run.)"""
+    s = st.amorphous_si(256, 4.5, seed=5, dmin=1.8)
+    pd, bd = tersoff_descriptors()
+    n = len(s)
+    print("case: MD on a compressed a-Si cell, shapes g4_s8 and g16_s32", flush=True)
+    out = []
+    for v in ("g4_s8", "g16_s32"):
+        load_system(eng, s.positions, s.cell, [1, 1, 1], pd, bd, np.full(n, 3.5),
+                    masses=s.masses)
+        eng.set_variant(eng.variants().index(v))
+        eng.md_run(4, 0.5 * st.ASE_FS, rebuild=1)
+        out.append(eng.state())
+    eng.set_variant(0)
+    dr = np.abs(out[0][0] - out[1][0]).max()
+    dp = np.abs(out[0][1] - out[1][1]).max() / np.abs(out[1][1]).max()
+    print("  |dr| %.2e A  rel|dp| %.2e" % (dr, dp), flush=True)
+    assert np.isfinite(out[0][0]).all()
+    assert dr < 1e-9 and dp < 1e-6
