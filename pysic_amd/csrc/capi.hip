@@ -122,7 +122,7 @@ void System::release_all()
     DBuf<int> *bi[] = {&id, &id2, &key, &key2, &wrap, &wrap2, &perm, &cellcount, &cellstart, &cellfill, &scanblk,
                        &prevstart, &dirty, &scanloc, &srow,
                        &nbr, &nimg, &ncnt, &flags, &sj, &simg, &scnt, &cell_of, &first_bin, &atom0,
-                       &bsj, &bsimg, &bscnt, &osj, &osimg, &oscnt};
+                       &bsj, &bsimg, &bscnt, &osj, &osimg, &oscnt, &qc_check};
     for (auto *b : bi) b->release();
     bG.release();
     oD.release();
@@ -850,6 +850,7 @@ std::vector<const void *> md_signature(pysic_ctx *h, double dt)
     sig.push_back(d1);
     sig.push_back(d2);
     sig.push_back((const void *)(long)(fused_md(h) ? 1 : 0));
+    sig.push_back((const void *)(long)(h->S.qc_uniform ? 1 : 0));
     return sig;
 }
 
@@ -1444,8 +1445,17 @@ int pysic_md_run(pysic_ctx *h, int nsteps, double dt, int rebuild, double *energ
     const double half = 0.5 * dt;
     if (rebuild == 1 && !energies && !S.timer.on && nsteps >= 3) {
         // the launch-bound step loop from graphs (per-kernel timing wants
-        // the eager path, where each launch is bracketed by events)
-        md_run_graph(h, nsteps, dt);
+        // the eager path, where each launch is bracketed by events).  Nothing
+        // in the loop writes charges or cutoffs: when each is one value for
+        // every atom, the sorts leave them in place.
+        nb_begin_uniform_qc(S);
+        try {
+            md_run_graph(h, nsteps, dt);
+        } catch (...) {
+            S.qc_uniform = false;
+            throw;
+        }
+        S.qc_uniform = false;
     } else {
         for (int step = 0; step < nsteps; ++step) {
             if (rebuild > 0 && (step + 1) % rebuild == 0) {

@@ -314,7 +314,10 @@ __global__ void k_cellsort(int ncell, int n, const int *__restrict__ loc,
 }
 
 // Forces are not carried: after the drift they are dead until recomputed.
-template <class Geo>
+// QC: move the charges and cutoffs too.  Without it they stay where they
+// are, which is right only while both buffers of each hold one value for
+// every atom (checked before a run that uses it).
+template <class Geo, bool QC>
 __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__restrict__ pos,
                           const double4 *__restrict__ mom, const double *__restrict__ q,
                           const double *__restrict__ cut, const int *__restrict__ id,
@@ -336,8 +339,10 @@ __global__ void k_permute(int n, const int *__restrict__ perm, const double4 *__
     int w = wrap[o];
     pos2[s] = p;
     mom2[s] = mom[o];
-    q2[s] = q[o];
-    cut2[s] = cut[o];
+    if constexpr (QC) {
+        q2[s] = q[o];
+        cut2[s] = cut[o];
+    }
     id2[s] = id[o];
     key2[s] = key[o];
     wrap2[s] = w;
@@ -940,6 +945,38 @@ __global__ __launch_bounds__(256) void k_neighbors_group(int n, CellGeom g,
 // get_optimal_splitting (Geometry.f90:679-707) without the 40 cap; the total
 // is kept below ~2 bins per atom by coarsening (bins only grow, so the
 // 27-bin stencil stays complete).
+// Every charge equals the first and every cutoff the first: bit 0 of
+// *differs is set otherwise.
+__global__ void k_qc_differs(int n, const double *__restrict__ q, const double *__restrict__ cut,
+                             int *__restrict__ differs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (q[i] != q[0] || cut[i] != cut[0]) atomicOr(differs, 1);
+}
+
+bool nb_begin_uniform_qc(System &S)
+{
+    S.qc_uniform = false;
+    if (S.n < 1) return false;
+    S.qc_check.ensure(1);
+    HIP_TRY(hipMemsetAsync(S.qc_check.p, 0, sizeof(int), S.stream));   // never under capture
+    k_qc_differs<<<div_up(S.n, 256), 256, 0, S.stream>>>(S.n, S.q.p, S.cut.p, S.qc_check.p);
+    int differs = 1;
+    HIP_TRY(hipMemcpyAsync(&differs, S.qc_check.p, sizeof(int), hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    if (differs) return false;
+    // both buffers of each hold the one value, whichever a sort leaves current
+    S.q2.ensure(S.n);
+    S.cut2.ensure(S.n);
+    HIP_TRY(hipMemcpyAsync(S.q2.p, S.q.p, sizeof(double) * S.n, hipMemcpyDeviceToDevice, S.stream));
+    HIP_TRY(hipMemcpyAsync(S.cut2.p, S.cut.p, sizeof(double) * S.n, hipMemcpyDeviceToDevice,
+                           S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    S.qc_uniform = true;
+    return true;
+}
+
 void nb_configure_grid(System &S, double max_cut)
 {
     PYSIC_REQUIRE(max_cut > 0.0, PYSIC_EINVAL, "neighbor cutoff must be positive");
@@ -1067,11 +1104,11 @@ static void sort_binned(System &S)
                                                    S.id.p, S.dirty.p, all_dirty, S.prevstart.p);
     with_geo(S, [&](auto geo) {
         using Geo = decltype(geo);
-        k_permute<Geo><<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p,
-                                                       S.cut.p, S.id.p, S.key.p, S.wrap.p,
-                                                       S.pos2.p, S.mom2.p, S.q2.p, S.cut2.p,
-                                                       S.id2.p, S.key2.p, S.wrap2.p, geo, S.wpos.p,
-                                                       S.flags.p);
+        auto kern = S.qc_uniform ? k_permute<Geo, false> : k_permute<Geo, true>;
+        kern<<<div_up(n, 256), 256, 0, st>>>(n, S.perm.p, S.pos.p, S.mom.p, S.q.p, S.cut.p, S.id.p,
+                                             S.key.p, S.wrap.p, S.pos2.p, S.mom2.p, S.q2.p,
+                                             S.cut2.p, S.id2.p, S.key2.p, S.wrap2.p, geo,
+                                             S.wpos.p, S.flags.p);
     });
     HIP_TRY(hipGetLastError());
     // the new starts went to prevstart, which nb_swap_state makes cellstart

@@ -68,6 +68,9 @@ struct System {
     bool order_valid = false;   // the slot order is the last sort's (key[] holds its bins)
     bool unique_images = false; // no atom reaches two images of another within the list radius
     bool bins_clean = false;    // cellcount and dirty are all zero between sorts
+    bool qc_uniform = false;    // for this MD run: q, q2 all one value, cut, cut2 too
+                                // (the sort then leaves them in place)
+    DBuf<int> qc_check;
     double max_cut = 0.0;
 
     // neighbor lists (column-major: entry e of atom i at e*n + i)
@@ -157,6 +160,9 @@ void nb_build_lists(System &S);               // cell-list search, reference sem
 void nb_swap_state(System &S);                // the buffers a sort alternates
 int nb_read_flags(System &S, int *out4);      // copies the 4 flags to host
 void nb_clear_flags(System &S);               // zero the flags (kernel; graph-safe)
+// before an MD run: if every charge and every cutoff is one value, copy them
+// to the swap buffers and set S.qc_uniform (clear it after the run)
+bool nb_begin_uniform_qc(System &S);
 
 // tersoff.hip
 // SITE_ENERGY: energy only (FP64); SITE_FORCES: dU/dr_ij + virial (the

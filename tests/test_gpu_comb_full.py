@@ -168,3 +168,41 @@ def test_comb_full_md_matches_oracle(eng, oracle):
     print("  |dr| %.2e A  rel|dp| %.2e" % (dr, dp), flush=True)
     assert dr < 1e-9
     assert dp < 1e-9
+
+
+def test_comb_md_with_uniform_charges_then_new_charges(oracle):
+    """With one charge on every atom, the graph-replayed MD loop leaves the
+    charges and cutoffs in place at each sort (both buffers hold the one
+    value): 6 steps replayed from graphs end on the bits of 6 eager steps
+    (the energy observer selects the eager loop).  Charges set afterwards,
+    one per atom, then evaluate as in a fresh handle at those positions."""
+    from pysic_amd.engine import Engine
+    s, q, sp, pd, bd = _comb_case(32, 85)
+    st.maxwell_boltzmann(s, 300.0, seed=86)
+    n = len(s)
+    cuts = np.full(n, 3.55)
+    q1 = np.full(n, 0.4)
+    print("case: COMB MD, uniform charges, graph against eager, then new charges", flush=True)
+    out = []
+    for eager in (False, True):
+        e = Engine(0)
+        try:
+            load(e, sp, s.positions, q1, s.cell, pd, bd, cuts, momenta=s.momenta,
+                 masses=s.masses)
+            e.md_run(6, 0.5 * st.ASE_FS, rebuild=1, energies=eager)
+            pos, mom = e.state()
+            e.update_charges(q)
+            out.append((pos, mom, e.energy(), e.forces()[0], e.electronegativities()))
+        finally:
+            e.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    fresh = Engine(0)
+    try:
+        load(fresh, sp, out[0][0], q, s.cell, pd, bd, cuts)
+        ref = (fresh.energy(), fresh.forces()[0], fresh.electronegativities())
+    finally:
+        fresh.close()
+    for got in out:
+        assert abs(got[2] - ref[0]) <= 1e-12 * abs(ref[0])
+        assert rel_maxnorm(got[3], ref[1]) <= 1e-12
+        assert rel_maxnorm(got[4], ref[2]) <= 1e-12
